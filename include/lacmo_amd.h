@@ -1,0 +1,178 @@
+/*
+ * lacmo_amd.h -- C ABI of the gfx950 super-droplet (SIP) time step.
+ *
+ * Plain C: device pointers, sizes and a hipStream_t passed as void*.  The
+ * Python mirror of the reference interface (lacmopy_amd.integration,
+ * lacmopy_amd.collision.all_or_nothing) fills LmEngine with pointers to
+ * device buffers it allocates and calls these entry points through ctypes.
+ * Every entry point returns 0 on success or a nonzero LM_ERR_* / hipError_t
+ * code; lm_last_error_message() describes the last failure.  Calls are
+ * stream-ordered and make no host synchronisation of their own (except
+ * lm_engine_check, which copies the device error word back).
+ *
+ * Engine layout: SIPs of all runs in one structure-of-arrays, sorted by
+ * (run, home cell, SIP id) at the last reindex; seg[] holds, per run, the
+ * nc cell segments followed by one "graveyard" segment of inactive SIPs.
+ * Cells are linear i*nz + j (x-major, like the reference's loops); the
+ * current cell of a SIP is the global cell r*nc + c, or -(2 + i) when the SIP
+ * is inactive (i = the x index the reference keeps in cells[0]).
+ */
+#ifndef LACMO_AMD_H
+#define LACMO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LM_ERR_OK 0
+#define LM_ERR_ARG 1001        /* invalid argument / shape */
+#define LM_ERR_CELL_CAP 1002   /* a cell holds more SIPs than the kernels' LDS capacity */
+#define LM_ERR_LOST_SIP 1003   /* a SIP moved more than one cell between reindexes */
+#define LM_ERR_REM_CAP 1004    /* too many SIPs reached the ground in one substep */
+#define LM_ERR_OUT_OF_DOMAIN 1005 /* a SIP landed on x == x_max (cell index nx) */
+
+typedef struct LmEngine {
+  /* geometry */
+  int32_t n_runs, nx, nz, nc;
+  int64_t n_tot;
+  double x_min, x_max, z_min, z_max, dx, dz, V_cell, p_ref, p_ref_inv;
+  /* physics parameters (config keys of integration.py:1908-1923) */
+  int32_t solute;         /* 0 = 'AS', 1 = 'NaCl' */
+  int32_t no_iter;        /* no_iter_impl_mass */
+  int32_t scale_dt_cond;  /* no_cond_per_adv // 2 */
+  int32_t no_col_per_adv;
+  int32_t act_collisions, act_relaxation;
+  double dt, dt_sub, dt_sub_half, dt_col_over_dV, g_set;
+  /* collection efficiency: table E[nb*nb] (Ecol_grid_R) or constant */
+  const double *E;
+  int32_t nb, e_const_mode;
+  double E_const, lnR0, lnf;
+  uint64_t seed;
+  uint32_t col_step; /* collision-step counter (Philox counter word) */
+  int32_t cur;       /* current ping-pong buffer of the SIP state and seg */
+  int32_t ccur;      /* current ping-pong buffer of cell[] */
+  int32_t pad1;
+  /* SIP state, ping-pong buffers [2] */
+  double *x[2], *z[2], *u[2], *w[2], *m_w[2], *m_s[2], *xi[2];
+  int32_t *id[2];
+  int32_t *cell[2]; /* cell[ccur] = current cell; cell[1-ccur] = next */
+  int32_t *seg[2];  /* n_runs*(nc+1)+1 segment starts */
+  double *dmxi;     /* n_tot scratch: dm * xi per SIP */
+  /* grid, per run */
+  double *scal; /* n_runs*10*nc: T, p, Theta, rho_d, r_v, r_l, S, e_s, m_dry_inv, rho_d_inv */
+  double *mat;  /* n_runs*7*nc: K, D_v, L_v, sigma_w, c_p, mu, rho_f */
+  double *cc;   /* n_runs*nc*16 per-cell constants for the particle kernel */
+  double *dTh, *dRv; /* n_runs*nc advective increments of Theta and r_v */
+  double *dml;       /* n_runs*nc Delta m_l of the last substep (kg) */
+  const double *grid_vel; /* 2*(nx+1)*(nz+1), shared by all runs */
+  const double *flux;     /* 2*(nx+1)*(nz+1) dry-air mass flux */
+  const double *prof_rv, *prof_Th; /* n_runs*nz initial profiles (relaxation) */
+  const double *t_relax;           /* nz */
+  double *colsum;                  /* n_runs*nz*2 scratch */
+  /* per-run accumulators */
+  int64_t *no_cols;      /* n_runs*2 */
+  double *water_removed; /* n_runs */
+  int64_t *n_equal_xi;   /* n_runs: near-equal-xi collision events (the reference prints them) */
+  int32_t *rem_cnt;      /* n_runs */
+  int32_t *rem_slot;     /* n_runs*rem_cap */
+  int32_t rem_cap, pad0;
+  /* scratch */
+  int32_t *scan_buf;  /* n_runs*(2nc+1)+1 */
+  int32_t *scan_part; /* scan partials */
+  int32_t *err;       /* 4 error words on the device */
+  int32_t *ovf_list;  /* cells over the fast-path LDS capacity */
+  int32_t *ovf_cnt;   /* 1 */
+  /* injected uniform stream (parity mode; NULL = Philox) */
+  const double *rnd_stream;
+  int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */
+  int64_t rnd_pos;  /* stream position consumed so far */
+} LmEngine;
+
+/* Library identity: returns a static version string. */
+const char *lm_version(void);
+/* Message of the last error on this thread. */
+const char *lm_last_error_message(void);
+/* sizeof(LmEngine) and byte offsets of selected fields (for binding checks). */
+int64_t lm_engine_sizeof(void);
+void lm_engine_offsets(int64_t *out, int32_t n);
+/* Physical constants as the kernels use them (for the constant-parity test). */
+void lm_get_constants(double *out, int32_t n);
+int32_t lm_num_constants(void);
+
+/* Builds cc[] from scal/mat (after initialisation or a host-side edit of the grid). */
+int lm_engine_init_cells(LmEngine *e, void *stream);
+
+/*
+ * One all-cells collision step (collision/all_or_nothing.py:754-846,
+ * collision_step_Ecol_grid_R_all_cells_2D_multicomp_np): every cell of every
+ * run, pairs in the reference's row-major order over ascending SIP id, kernel
+ * E_col(R_i,R_j) from the table or constant, radii at the cell temperature.
+ * Draws come from rnd_stream (consumed in x-major cell order) or Philox keyed
+ * by (seed, run, col_step, cell, pair).  Increments col_step.
+ */
+int lm_collision_step(LmEngine *e, void *stream);
+
+/*
+ * Position/cell update of all active SIPs by dt (integration.py:552-630 and
+ * 509-550: update_pos_from_vel_BC_PS + update_cells_and_rel_pos), with the
+ * water_removed bookkeeping in ascending SIP id.
+ */
+int lm_position_step(LmEngine *e, double dt, void *stream);
+
+/* Full reindex: counting sort of the SIPs by (run, current cell, id). */
+int lm_reindex(LmEngine *e, void *stream);
+
+/*
+ * Advective increments of r_v and Theta (integration.py:1483-1508 with
+ * second=0; 1536-1561 with second=1): Hundsdorfer-limited upwind divergence
+ * of the dry-air mass flux (integration.py:150-286) plus relaxation
+ * (integration.py:325-354).
+ */
+int lm_advection_increments(LmEngine *e, int32_t second, void *stream);
+
+/*
+ * One particle substep followed by one grid substep
+ * (integration.py:944-1089 propagate_particles_subloop_step_np and
+ * 402-483 propagate_grid_subloop_step_np): implicit Newton condensation,
+ * per-cell Delta m_l summed in ascending id, drag update, position and cell
+ * update, then the cell thermodynamics with latent heating.
+ */
+int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream);
+
+/* n_steps advection steps (integration.py:1298-1611, integrate_adv_step_np). */
+int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream);
+
+/* Synchronises the stream and reports the device error word (0 = fine). */
+int lm_engine_check(LmEngine *e, void *stream);
+
+/*
+ * Test hook (no reference counterpart): elementwise device math of the
+ * kernels -- which = 0 exp, 1 log, 2 pow(x,y), 3 x % y (Python float modulo),
+ * 4 sqrt, 5 x / y, 6 floor(x / y) -- for the host/device bit-parity test.
+ */
+int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, double *out, void *stream);
+
+/*
+ * Batched box model: one collision step in each of n_boxes independent boxes
+ * (collision/all_or_nothing.py:109-185 collision_step_Golovin_np).  Box b
+ * holds SIPs [off[b], off[b+1]) of xis/masses; counters no_cols[2*b..].
+ */
+int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
+                    int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream);
+
+/*
+ * Batched box model with the E_col(R1,R2) table and 1-D Beard velocities
+ * (collision/all_or_nothing.py:269-390 collision_step_Ecol_grid_R_np followed
+ * by collision/kernel.py:193-210 update_velocity_Beard).
+ */
+int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double *radii, double *vel,
+                     const double *rho, double dt_over_dV, const double *E, int32_t nb, double lnR0, double lnf,
+                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LACMO_AMD_H */

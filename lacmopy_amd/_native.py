@@ -1,0 +1,113 @@
+"""ctypes binding of the C ABI declared in include/lacmo_amd.h.
+
+The shared library liblacmo_amd.so is built in-tree by lacmopy_amd.build
+(hipcc, gfx950).  Loading fails loudly: there is no CPU fallback anywhere in
+the product.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblacmo_amd.so")
+
+LM_ERR_NAMES = {
+    1001: "LM_ERR_ARG",
+    1002: "LM_ERR_CELL_CAP",
+    1003: "LM_ERR_LOST_SIP",
+    1004: "LM_ERR_REM_CAP",
+    1005: "LM_ERR_OUT_OF_DOMAIN",
+}
+
+_p = C.c_void_p
+_d = C.c_double
+_i32 = C.c_int32
+_i64 = C.c_int64
+
+
+class LmEngine(C.Structure):
+    """Mirror of struct LmEngine (include/lacmo_amd.h), same field order."""
+
+    _fields_ = [
+        ("n_runs", _i32), ("nx", _i32), ("nz", _i32), ("nc", _i32),
+        ("n_tot", _i64),
+        ("x_min", _d), ("x_max", _d), ("z_min", _d), ("z_max", _d),
+        ("dx", _d), ("dz", _d), ("V_cell", _d), ("p_ref", _d), ("p_ref_inv", _d),
+        ("solute", _i32), ("no_iter", _i32), ("scale_dt_cond", _i32),
+        ("no_col_per_adv", _i32), ("act_collisions", _i32), ("act_relaxation", _i32),
+        ("dt", _d), ("dt_sub", _d), ("dt_sub_half", _d), ("dt_col_over_dV", _d), ("g_set", _d),
+        ("E", _p), ("nb", _i32), ("e_const_mode", _i32),
+        ("E_const", _d), ("lnR0", _d), ("lnf", _d),
+        ("seed", C.c_uint64),
+        ("col_step", C.c_uint32), ("cur", _i32), ("ccur", _i32), ("pad1", _i32),
+        ("x", _p * 2), ("z", _p * 2), ("u", _p * 2), ("w", _p * 2),
+        ("m_w", _p * 2), ("m_s", _p * 2), ("xi", _p * 2),
+        ("id", _p * 2), ("cell", _p * 2), ("seg", _p * 2),
+        ("dmxi", _p),
+        ("scal", _p), ("mat", _p), ("cc", _p), ("dTh", _p), ("dRv", _p), ("dml", _p),
+        ("grid_vel", _p), ("flux", _p), ("prof_rv", _p), ("prof_Th", _p), ("t_relax", _p),
+        ("colsum", _p),
+        ("no_cols", _p), ("water_removed", _p), ("n_equal_xi", _p),
+        ("rem_cnt", _p), ("rem_slot", _p),
+        ("rem_cap", _i32), ("pad0", _i32),
+        ("scan_buf", _p), ("scan_part", _p), ("err", _p), ("ovf_list", _p), ("ovf_cnt", _p),
+        ("rnd_stream", _p), ("rnd_off", _p), ("rnd_pos", _i64),
+    ]
+
+
+OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
+                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_pos"]
+
+EXPORTED = [
+    "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",
+    "lm_get_constants", "lm_num_constants", "lm_engine_init_cells", "lm_collision_step",
+    "lm_position_step", "lm_reindex", "lm_advection_increments", "lm_substep",
+    "lm_adv_steps", "lm_engine_check", "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe",
+]
+
+_lib = None
+
+
+class LacmoError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load liblacmo_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LacmoError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    E = C.POINTER(LmEngine)
+    L.lm_version.restype = C.c_char_p
+    L.lm_last_error_message.restype = C.c_char_p
+    L.lm_engine_sizeof.restype = _i64
+    L.lm_engine_offsets.argtypes = [C.POINTER(_i64), _i32]
+    L.lm_get_constants.argtypes = [C.POINTER(_d), _i32]
+    L.lm_num_constants.restype = _i32
+    for name in ["lm_engine_init_cells", "lm_collision_step", "lm_reindex", "lm_engine_check"]:
+        getattr(L, name).argtypes = [E, _p]
+        getattr(L, name).restype = C.c_int
+    L.lm_position_step.argtypes = [E, _d, _p]
+    L.lm_advection_increments.argtypes = [E, _i32, _p]
+    L.lm_substep.argtypes = [E, _d, _d, _p]
+    L.lm_adv_steps.argtypes = [E, _i32, _p]
+    L.lm_golovin_step.argtypes = [_i32, _p, _p, _p, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
+    L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
+                                   C.c_uint64, C.c_uint32, _p, _p]
+    L.lm_math_probe.argtypes = [_i32, _i64, _p, _p, _p, _p]
+    for name in ["lm_position_step", "lm_advection_increments", "lm_substep", "lm_adv_steps",
+                 "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe"]:
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    """Raise LacmoError for a nonzero return code of an lm_* entry point."""
+    if rc != 0:
+        msg = lib().lm_last_error_message().decode(errors="replace")
+        raise LacmoError(f"lacmo_amd error {rc} ({LM_ERR_NAMES.get(rc, 'hip')}): {msg}")

@@ -1,0 +1,1405 @@
+/*
+ * lacmo_kernels.hip -- gfx950 kernels and C ABI of the super-droplet step.
+ *
+ * Layout and schedule (one GPU, FP64 throughout):
+ *  - SIPs of all runs live in one structure-of-arrays sorted by
+ *    (run, home cell, id) at the last reindex.  A full reindex runs once per
+ *    advection step; between reindexes a SIP moves at most one cell, so the
+ *    members of cell c are the SIPs of the 3x3 home segments around c whose
+ *    current cell is c ("neighbourhood gather").  Ordering by id inside a
+ *    cell is recovered by ranking in LDS.
+ *  - Per-SIP work (condensation Newton, drag, position) is one thread per SIP
+ *    over the home-ordered arrays (coalesced).  Per-cell work (ordered
+ *    Delta m_l sums, collisions) is one 64-lane wavefront per cell; cell
+ *    thermodynamics is one thread per cell.
+ *  - Collisions use the anti-diagonal schedule: pairs (i,j) of the
+ *    row-major sequential loop grouped by d = i + j are mutually independent
+ *    and every dependence points to a smaller d, so processing d = 1..2N-3
+ *    in order reproduces the sequential result exactly.
+ * All kernels are compiled with -ffp-contract=off.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/lacmo_amd.h"
+#include "lm_philox.h"
+#include "lm_physics.h"
+
+#define WAVE 64
+#define DML_CAP 1024 /* member capacity of the gather kernels (Delta m_l, reindex) */
+
+/* ------------------------------------------------------------------------ */
+/* host-side constants (same expressions as the reference modules)          */
+/* ------------------------------------------------------------------------ */
+static LmPhys make_phys() {
+  LmPhys P;
+  memset(&P, 0, sizeof(P));
+  const double pi = 3.141592653589793;
+  P.pi = pi;
+  P.one_third = 1.0 / 3.0;                /* constants.py:27 */
+  P.four_pi_over_three_inv = 0.75 / pi;   /* constants.py:29 */
+  P.R_v = 461.53;                         /* constants.py:90 */
+  P.R_d = 287.06;                         /* constants.py:95 */
+  P.c_pd = 1004.71;                       /* constants.py:109 */
+  const double c_pv = 1906.0;             /* constants.py:114 */
+  P.eps_gc = P.R_d / P.R_v;               /* atmosphere.py:99 */
+  P.kappa_dry = P.R_d / P.c_pd;           /* atmosphere.py:74 */
+  P.kappa_factor = 1.0 / (1.0 - P.kappa_dry);      /* atmosphere.py:370 */
+  P.kappa_factor2 = -P.kappa_dry * P.kappa_factor; /* atmosphere.py:371 */
+  P.heat_factor_r_v = (c_pv - P.R_v) / (P.c_pd - P.R_d); /* atmosphere.py:365 */
+  const double adi_inv = 0.7142857142857143, T0 = 289.0; /* microphysics.py:971-1026 */
+  P.T_alpha_0 = T0;
+  P.c_alpha_1 = 1.0E6 * sqrt(2.0 * pi * P.R_d * T0) / (1.0 * (P.c_pd * adi_inv + 0.5 * P.R_d));
+  P.c_alpha_2 = 0.5E6 * sqrt(2.0 * pi * P.R_d / T0) / (1.0 * (P.c_pd * adi_inv + 0.5 * P.R_d));
+  P.c_beta_1 = 1.0E6 * sqrt(2.0 * pi * 18.015E-3 / (8.3144598 * T0)) / 0.0415;
+  P.c_beta_2 = -0.5 / T0;
+  P.w_s_max_AS_inv = 1. / 0.78;                        /* integration.py:724 */
+  P.par_sigma_AS = 2.362E-3 / 132.14E-3 / 0.072;       /* material_properties.py:344 */
+  P.par_sigma_NaCl = 1.62E-3 / 58.4428E-3 / 0.072;     /* material_properties.py:272 */
+  const double rho_raw[4] = {1.024E1, -5.036E1, 592., 997.1}; /* material_properties.py:87 */
+  for (int i = 0; i < 4; ++i) P.par_rho_AS[i] = rho_raw[i] / 997.1;
+  for (int i = 0; i < 3; ++i) P.par_rho_deriv_AS[i] = P.par_rho_AS[i] * (double)(3 - i); /* microphysics.py:1355 */
+  const double aw_as[5] = {1.412, -2.336, 3.113E-1, -2.715E-1, 1.0}; /* material_properties.py:430 */
+  const double aw_nacl[5] = {1.518E1, -1.158E1, 8.642E-1, -6.366E-1, 1.0}; /* material_properties.py:400 */
+  for (int i = 0; i < 5; ++i) {
+    P.par_wat_act_AS[i] = aw_as[i];
+    P.par_wat_act_NaCl[i] = aw_nacl[i];
+  }
+  for (int i = 0; i < 4; ++i) {
+    P.par_wat_act_deriv_AS[i] = aw_as[i] * (double)(4 - i);
+    P.par_wat_act_deriv_NaCl[i] = aw_nacl[i] * (double)(4 - i);
+  }
+  const double sd[6] = {7.619443952135e+02, 1.021264281453e+03, 1.828970151543e+00,
+                        2.405352122804e+02, -1.080547892416e+00, -3.492805028749e-03};
+  for (int i = 0; i < 6; ++i) P.par_sol_dens_NaCl[i] = sd[i]; /* material_properties.py:56 */
+  P.par_water_dens[0] = 1.00013502e+03;                        /* material_properties.py:29 */
+  P.par_water_dens[1] = -4.68112708e-03;
+  P.par_water_dens[2] = 2.72389977e+02;
+  return P;
+}
+
+static const LmPhys &phys() {
+  static LmPhys P = make_phys();
+  return P;
+}
+
+static thread_local char g_errmsg[512] = "";
+
+static int set_err(int code, const char *msg) {
+  snprintf(g_errmsg, sizeof(g_errmsg), "%s", msg);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      snprintf(g_errmsg, sizeof(g_errmsg), "%s: %s", #expr, hipGetErrorString(_e));          \
+      return (int)_e;                                                                        \
+    }                                                                                        \
+  } while (0)
+
+#define LAUNCH_CHECK()                                                                       \
+  do {                                                                                       \
+    hipError_t _e = hipGetLastError();                                                       \
+    if (_e != hipSuccess) {                                                                  \
+      snprintf(g_errmsg, sizeof(g_errmsg), "kernel launch: %s", hipGetErrorString(_e));      \
+      return (int)_e;                                                                        \
+    }                                                                                        \
+  } while (0)
+
+/* ------------------------------------------------------------------------ */
+/* device helpers                                                           */
+/* ------------------------------------------------------------------------ */
+__device__ static inline void raise_err(int32_t *err, int code, int detail) {
+  if (atomicCAS(&err[0], 0, code) == 0) err[1] = detail;
+}
+
+/* distinct cells of the 3x3 neighbourhood (x periodic, z bounded) */
+__device__ static inline int nbr_cells(int i, int j, int nx, int nz, int *out) {
+  int n = 0;
+  for (int di = -1; di <= 1; ++di) {
+    int ii = i + di;
+    if (ii < 0) ii += nx;
+    if (ii >= nx) ii -= nx;
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int jj = j + dj;
+      if (jj < 0 || jj >= nz) continue;
+      const int c = ii * nz + jj;
+      bool dup = false;
+      for (int t = 0; t < n; ++t) dup |= (out[t] == c);
+      if (!dup) out[n++] = c;
+    }
+  }
+  return n;
+}
+
+__device__ static inline unsigned long long lane_mask_lt() {
+  const int lane = threadIdx.x & (WAVE - 1);
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+/*
+ * Wavefront gather of the members of global cell gc: slots k of the 3x3 home
+ * segments with cell[k] == gc.  Stores up to cap (slot, id) pairs in LDS and
+ * returns the member count (all lanes).  Must be called by a full wave.
+ */
+__device__ static int gather_members(int gc, int nc, int nx, int nz, const int32_t *seg_run, const int32_t *cell,
+                                     const int32_t *ids, int *s_slot, int *s_id, int cap) {
+  const int c = gc % nc;
+  const int i = c / nz, j = c % nz;
+  int nb[9];
+  const int nn = nbr_cells(i, j, nx, nz, nb);
+  const int lane = threadIdx.x;
+  int cnt = 0;
+  for (int t = 0; t < nn; ++t) {
+    const int s0 = seg_run[nb[t]], s1 = seg_run[nb[t] + 1];
+    for (int base = s0; base < s1; base += WAVE) {
+      const int k = base + lane;
+      const bool m = (k < s1) && (cell[k] == gc);
+      const unsigned long long bal = __ballot(m);
+      if (m) {
+        const int pos = cnt + __popcll(bal & lane_mask_lt());
+        if (pos < cap) {
+          s_slot[pos] = k;
+          s_id[pos] = ids[k];
+        }
+      }
+      cnt += __popcll(bal);
+    }
+  }
+  return cnt;
+}
+
+/* rank of every member by id (ids unique within a run) */
+__device__ static void rank_members(int n, const int *s_id, int *s_rank) {
+  for (int m = threadIdx.x; m < n; m += WAVE) {
+    const int idm = s_id[m];
+    int rk = 0;
+    for (int q = 0; q < n; ++q) rk += (s_id[q] < idm);
+    s_rank[m] = rk;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* kernel view: what the device needs from LmEngine                         */
+/* ------------------------------------------------------------------------ */
+struct KView {
+  int n_runs, nx, nz, nc;
+  double x_min, x_max, z_min, z_max, dx, dz, V_cell, p_ref, p_ref_inv;
+  int solute, no_iter;
+  double g_set, dt_col_over_dV;
+  const double *E;
+  int nb, e_const_mode;
+  double E_const, lnR0, lnf;
+  unsigned long long seed;
+  unsigned col_step;
+  double *x, *z, *u, *w, *m_w, *m_s, *xi;
+  int32_t *id, *cell, *cell_next, *seg;
+  double *x2, *z2, *u2, *w2, *m_w2, *m_s2, *xi2;
+  int32_t *id2, *cell2, *seg2;
+  double *dmxi, *scal, *mat, *cc, *dTh, *dRv, *dml;
+  const double *grid_vel, *flux, *prof_rv, *prof_Th, *t_relax;
+  double *colsum;
+  long long *no_cols;
+  double *water_removed;
+  long long *n_equal_xi;
+  int32_t *rem_cnt, *rem_slot;
+  int rem_cap;
+  int32_t *scan_buf, *scan_part, *err, *ovf_list, *ovf_cnt;
+  const double *rnd_stream;
+  long long *rnd_off;
+};
+
+static KView make_view(const LmEngine *e) {
+  KView v;
+  memset(&v, 0, sizeof(v));
+  v.n_runs = e->n_runs;
+  v.nx = e->nx;
+  v.nz = e->nz;
+  v.nc = e->nc;
+  v.x_min = e->x_min;
+  v.x_max = e->x_max;
+  v.z_min = e->z_min;
+  v.z_max = e->z_max;
+  v.dx = e->dx;
+  v.dz = e->dz;
+  v.V_cell = e->V_cell;
+  v.p_ref = e->p_ref;
+  v.p_ref_inv = e->p_ref_inv;
+  v.solute = e->solute;
+  v.no_iter = e->no_iter;
+  v.g_set = e->g_set;
+  v.dt_col_over_dV = e->dt_col_over_dV;
+  v.E = e->E;
+  v.nb = e->nb;
+  v.e_const_mode = e->e_const_mode;
+  v.E_const = e->E_const;
+  v.lnR0 = e->lnR0;
+  v.lnf = e->lnf;
+  v.seed = e->seed;
+  v.col_step = e->col_step;
+  const int c = e->cur, o = 1 - e->cur;
+  v.x = e->x[c];
+  v.z = e->z[c];
+  v.u = e->u[c];
+  v.w = e->w[c];
+  v.m_w = e->m_w[c];
+  v.m_s = e->m_s[c];
+  v.xi = e->xi[c];
+  v.id = e->id[c];
+  v.seg = e->seg[c];
+  v.x2 = e->x[o];
+  v.z2 = e->z[o];
+  v.u2 = e->u[o];
+  v.w2 = e->w[o];
+  v.m_w2 = e->m_w[o];
+  v.m_s2 = e->m_s[o];
+  v.xi2 = e->xi[o];
+  v.id2 = e->id[o];
+  v.seg2 = e->seg[o];
+  v.cell = e->cell[e->ccur];
+  v.cell_next = e->cell[1 - e->ccur];
+  v.cell2 = e->cell[1 - e->ccur];
+  v.dmxi = e->dmxi;
+  v.scal = e->scal;
+  v.mat = e->mat;
+  v.cc = e->cc;
+  v.dTh = e->dTh;
+  v.dRv = e->dRv;
+  v.dml = e->dml;
+  v.grid_vel = e->grid_vel;
+  v.flux = e->flux;
+  v.prof_rv = e->prof_rv;
+  v.prof_Th = e->prof_Th;
+  v.t_relax = e->t_relax;
+  v.colsum = e->colsum;
+  v.no_cols = (long long *)e->no_cols;
+  v.water_removed = e->water_removed;
+  v.n_equal_xi = (long long *)e->n_equal_xi;
+  v.rem_cnt = e->rem_cnt;
+  v.rem_slot = e->rem_slot;
+  v.rem_cap = e->rem_cap;
+  v.scan_buf = e->scan_buf;
+  v.scan_part = e->scan_part;
+  v.err = e->err;
+  v.ovf_list = e->ovf_list;
+  v.ovf_cnt = e->ovf_cnt;
+  v.rnd_stream = e->rnd_stream;
+  v.rnd_off = (long long *)e->rnd_off;
+  return v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-cell constants                                                       */
+/* ------------------------------------------------------------------------ */
+__device__ static void fill_cc(const LmPhys &P, const double *scal_r, const double *mat_r, int c, int nc, double *C) {
+  const double T = scal_r[0 * nc + c], p = scal_r[1 * nc + c];
+  const double K = mat_r[0 * nc + c], D_v = mat_r[1 * nc + c], L_v = mat_r[2 * nc + c];
+  const double e_s = scal_r[7 * nc + c];
+  C[CC_T] = T;
+  C[CC_P] = p;
+  C[CC_S] = scal_r[6 * nc + c];
+  C[CC_ES] = e_s;
+  C[CC_LV] = L_v;
+  C[CC_K] = K;
+  C[CC_DV] = D_v;
+  C[CC_MU] = mat_r[5 * nc + c];
+  C[CC_RHOF] = mat_r[6 * nc + c];
+  C[CC_LALPHA] = lmp_l_alpha(P, T, p, K);
+  C[CC_LBETA] = lmp_l_beta(P, T, D_v);
+  C[CC_C1] = L_v * L_v / (P.R_v * K * T * T);  /* microphysics.py:1458 */
+  C[CC_C2] = P.R_v * T / (D_v * e_s);          /* microphysics.py:1459 */
+  C[CC_RHOW] = lmp_density_water(P, T);
+  C[CC_SIGW] = lmp_sigma_water(T);
+  C[15] = 0.0;
+}
+
+__global__ void k_init_cc(KView v, LmPhys P) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= v.n_runs * v.nc) return;
+  const int r = gc / v.nc, c = gc % v.nc;
+  fill_cc(P, v.scal + (size_t)r * 10 * v.nc, v.mat + (size_t)r * 7 * v.nc, c, v.nc, v.cc + (size_t)gc * CC_N);
+}
+
+/* ------------------------------------------------------------------------ */
+/* collision step: one wavefront per cell, anti-diagonal pair schedule       */
+/* ------------------------------------------------------------------------ */
+template <int CAP>
+__global__ __launch_bounds__(WAVE) void k_collision(KView v, LmPhys P, int from_ovf) {
+  int gc;
+  if (from_ovf) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && *v.ovf_cnt > (int)gridDim.x) raise_err(v.err, LM_ERR_CELL_CAP, -1);
+    if ((int)blockIdx.x >= *v.ovf_cnt) return;
+    gc = v.ovf_list[blockIdx.x];
+  } else {
+    gc = blockIdx.x;
+  }
+  const int nc = v.nc;
+  const int r = gc / nc, c = gc % nc;
+  const int lane = threadIdx.x;
+
+  __shared__ double sX[CAP], sMW[CAP], sMS[CAP], sR[CAP], sRHO[CAP], sU[CAP], sW[CAP];
+  __shared__ int sIND[CAP], sSL[CAP], sSlot[CAP], sId[CAP], sRank[CAP];
+
+  const int n = gather_members(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.cell, v.id, sSlot, sId, CAP);
+  if (n > CAP) {
+    if (lane == 0) {
+      if (from_ovf) {
+        raise_err(v.err, LM_ERR_CELL_CAP, gc);
+      } else {
+        const int q = atomicAdd(v.ovf_cnt, 1);
+        v.ovf_list[q] = gc;
+      }
+    }
+    return;
+  }
+  if (n < 2) return;
+  __syncthreads();
+  rank_members(n, sId, sRank);
+  __syncthreads();
+
+  const double *C = v.cc + (size_t)gc * CC_N;
+  const double T = C[CC_T], rho_w_T = C[CC_RHOW];
+  for (int m = lane; m < n; m += WAVE) {
+    const int k = sSlot[m], p = sRank[m];
+    const double mw = v.m_w[k], ms = v.m_s[k];
+    sX[p] = v.xi[k];
+    sMW[p] = mw;
+    sMS[p] = ms;
+    sU[p] = v.u[k];
+    sW[p] = v.w[k];
+    sSL[p] = k;
+    double R, ws, rho;
+    lmp_R_ws_rho(P, v.solute, mw, ms, T, rho_w_T, R, ws, rho);
+    sR[p] = R;
+    sRHO[p] = rho;
+    sIND[p] = v.e_const_mode ? 0 : lmp_kernel_index(R, v.lnR0, v.lnf, v.nb);
+  }
+  __syncthreads();
+
+  const double dtdV = v.dt_col_over_dV;
+  const long long nn = n;
+  const long long roff = v.rnd_stream ? v.rnd_off[gc] : 0;
+  long long c_single = 0, c_multi = 0, c_eq = 0;
+  for (int d = 1; d <= 2 * n - 3; ++d) {
+    const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
+    const int i_hi = (d - 1) >> 1;
+    for (int i = i_lo + lane; i <= i_hi; i += WAVE) {
+      const int j = d - i;
+      int imin, imax;
+      if (sX[i] <= sX[j]) {
+        imin = i;
+        imax = j;
+      } else {
+        imin = j;
+        imax = i;
+      }
+      const double xi_min = sX[imin], xi_max = sX[imax];
+      const double mw_min = sMW[imin], mw_max = sMW[imax];
+      const double ms_min = sMS[imin], ms_max = sMS[imax];
+      const double du = sU[i] - sU[j], dw = sW[i] - sW[j];
+      const double Ec = v.e_const_mode ? v.E_const : v.E[sIND[i] * v.nb + sIND[j]];
+      const double p_crit = xi_max * dtdV * lmp_kernel_hydro(P, sR[i], sR[j], Ec, sqrt(du * du + dw * dw));
+      if (p_crit > 1.0) {
+        const double xi_col = p_crit * xi_min;
+        const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
+        sMW[imin] = mwn;
+        sMS[imin] = msn;
+        const double Rn = lmp_radius(P, (mwn + msn), sRHO[imin]);
+        sR[imin] = Rn;
+        sX[imax] = xi_max - xi_col;
+        if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+        ++c_multi;
+      } else if (p_crit > 0.0) {
+        const long long cnt = (long long)i * nn - (long long)i * (i + 1) / 2 + (j - i - 1);
+        const double rnd = v.rnd_stream ? v.rnd_stream[roff + cnt]
+                                        : lm_pair_uniform(v.seed, (uint32_t)r, v.col_step, (uint32_t)c,
+                                                          (uint64_t)cnt);
+        if (p_crit > rnd) {
+          ++c_single;
+          const double xi_rel_dev = (xi_max - xi_min) / xi_max;
+          if (xi_rel_dev < 1.0E-5) {
+            ++c_eq;
+            const double xi_ges = xi_min + xi_max;
+            const double mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
+            const double msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
+            sMW[imin] = mwn;
+            sMS[imin] = msn;
+            sMW[imax] = mwn;
+            sMS[imax] = msn;
+            const double Rn = lmp_radius(P, (mwn + msn), sRHO[imin]);
+            sR[imin] = Rn;
+            sR[imax] = Rn;
+            const double xmax_new = 0.5 * 0.7 * xi_ges;
+            sX[imax] = xmax_new;
+            sX[imin] = 0.5 * xi_ges - xmax_new;
+            if (!v.e_const_mode) {
+              const int ind = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+              sIND[imin] = ind;
+              sIND[imax] = ind;
+            }
+          } else {
+            const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
+            sMW[imin] = mwn;
+            sMS[imin] = msn;
+            const double Rn = lmp_radius(P, (mwn + msn), sRHO[imin]);
+            sR[imin] = Rn;
+            sX[imax] = xi_max - xi_min;
+            if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int m = lane; m < n; m += WAVE) {
+    const int k = sSL[m];
+    v.xi[k] = sX[m];
+    v.m_w[k] = sMW[m];
+    v.m_s[k] = sMS[m];
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    c_single += __shfl_down(c_single, off);
+    c_multi += __shfl_down(c_multi, off);
+    c_eq += __shfl_down(c_eq, off);
+  }
+  if (lane == 0) {
+    if (c_single) atomicAdd((unsigned long long *)&v.no_cols[2 * r], (unsigned long long)c_single);
+    if (c_multi) atomicAdd((unsigned long long *)&v.no_cols[2 * r + 1], (unsigned long long)c_multi);
+    if (c_eq) atomicAdd((unsigned long long *)&v.n_equal_xi[r], (unsigned long long)c_eq);
+  }
+}
+
+/* member counts -> pair counts for the injected stream offsets */
+__global__ __launch_bounds__(WAVE) void k_count_pairs(KView v) {
+  const int gc = blockIdx.x, nc = v.nc, r = gc / nc;
+  __shared__ int sSlot[1], sId[1];
+  const int n = gather_members(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.cell, v.id, sSlot, sId, 0);
+  if (threadIdx.x == 0) v.rnd_off[gc] = (long long)n * (n - 1) / 2;
+}
+
+/* exclusive scan of rnd_off[0..L) in one block, offset by the running stream position rnd_off[L] */
+__global__ __launch_bounds__(1024) void k_scan_pairs(long long *a, int L) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const int per = (L + 1023) / 1024;
+  const int b = t * per, e = b + per < L ? b + per : L;
+  long long s = 0;
+  for (int k = b; k < e; ++k) s += a[k];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    long long acc = a[L];
+    for (int q = 0; q < 1024; ++q) {
+      const long long x = part[q];
+      part[q] = acc;
+      acc += x;
+    }
+    a[L] = acc;
+  }
+  __syncthreads();
+  long long acc = part[t];
+  for (int k = b; k < e; ++k) {
+    const long long x = a[k];
+    a[k] = acc;
+    acc += x;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* particle substep: one thread per SIP slot                                */
+/* ------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void k_particles(KView v, LmPhys P, long long n_tot, double dt_sub,
+                                                   double dt_pos, int pos_only) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_tot) return;
+  const int gc = v.cell[k];
+  if (gc < 0) {
+    v.cell_next[k] = gc;
+    return;
+  }
+  const int nc = v.nc;
+  const int r = gc / nc, c = gc % nc;
+  const int i = c / v.nz, j = c % v.nz;
+  double x = v.x[k], z = v.z[k], u = v.u[k], w = v.w[k];
+  if (!pos_only) {
+    const double *C = v.cc + (size_t)gc * CC_N;
+    const double T = C[CC_T];
+    double m_w = v.m_w[k];
+    const double m_s = v.m_s[k], xi = v.xi[k];
+    double R, w_s, rho;
+    lmp_R_ws_rho(P, v.solute, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
+    const double sigma_p = lmp_sigma_solution(P, v.solute, w_s, C[CC_SIGW]);
+    const double dm = lmp_newton_dm(P, v.solute, dt_sub, v.no_iter, m_w, m_s, w_s, R, T, rho, C, sigma_p);
+    m_w += dm;
+    v.m_w[k] = m_w;
+    v.dmxi[k] = dm * xi;
+    /* integration.py:1075: AS radius and density after the mass update */
+    lmp_R_ws_rho(P, 0, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
+    /* grid.py:257-323 with rel_pos of integration.py:545-549 */
+    const double wx = (x - v.x_min) / v.dx - (double)i;
+    const double wy = (z - v.z_min) / v.dz - (double)j;
+    const double *U = v.grid_vel, *V = v.grid_vel + (size_t)(v.nx + 1) * (v.nz + 1);
+    double uf, vf;
+    lmp_interp_vel(i, j, wx, wy, U, V, v.nx, v.nz, uf, vf);
+    /* integration.py:699-718 */
+    const double mu = C[CC_MU], rho_f = C[CC_RHOF];
+    const double dv0 = uf - u, dv1 = vf - w;
+    const double vel_dev = sqrt(dv0 * dv0 + dv1 * dv1);
+    const double Re = 2.0E-6 * rho_f * R * vel_dev / mu;
+    double k_dt = 4.5E12 * dt_sub * mu / (rho * R * R);
+    if (Re > 0.5) {
+      if (Re < 1000.0)
+        k_dt *= (1 + 0.15 * lm_pow(Re, 0.687));
+      else
+        k_dt *= 0.0183333333333 * Re;
+    }
+    u = (u + k_dt * uf) / (1.0 + k_dt);
+    w = (w + k_dt * vf - dt_sub * v.g_set) / (1.0 + k_dt);
+  }
+  /* integration.py:552-630 */
+  x += dt_pos * u;
+  z += dt_pos * w;
+  x = lm_pymod(x, v.x_max);
+  z = lm_np_min(z, v.z_max * 0.999999);
+  int gc_new;
+  if (z <= v.z_min) {
+    z = v.z_min - 0.01 * v.dz;
+    u = 0.0;
+    w = 0.0;
+    gc_new = -(2 + i); /* inactive; keeps cells[0] as the reference does */
+    const int q = atomicAdd(&v.rem_cnt[r], 1);
+    if (q < v.rem_cap)
+      v.rem_slot[(size_t)r * v.rem_cap + q] = (int)k;
+    else
+      raise_err(v.err, LM_ERR_REM_CAP, r);
+  } else {
+    /* integration.py:545-546 */
+    const long long ii = (long long)floor(x / v.dx);
+    const long long jj = (long long)floor(z / v.dz);
+    if (ii < 0 || ii >= v.nx || jj < 0 || jj >= v.nz) {
+      raise_err(v.err, LM_ERR_OUT_OF_DOMAIN, (int)k);
+      gc_new = gc;
+    } else {
+      gc_new = r * nc + (int)ii * v.nz + (int)jj;
+    }
+  }
+  v.x[k] = x;
+  v.z[k] = z;
+  v.u[k] = u;
+  v.w[k] = w;
+  v.cell_next[k] = gc_new;
+}
+
+/* water_removed[r] += xi*m_w of this substep's removed SIPs, ascending id (integration.py:620-628) */
+#define REM_LDS 1024
+__global__ __launch_bounds__(WAVE) void k_water(KView v) {
+  const int r = blockIdx.x;
+  __shared__ int sId[REM_LDS], sSlot[REM_LDS], sRank[REM_LDS], sOrd[REM_LDS];
+  const int n0 = v.rem_cnt[r];
+  if (n0 == 0) return;
+  const int n = n0 < REM_LDS ? n0 : REM_LDS;
+  if (n0 > REM_LDS && threadIdx.x == 0) raise_err(v.err, LM_ERR_REM_CAP, r);
+  for (int t = threadIdx.x; t < n; t += WAVE) {
+    const int k = v.rem_slot[(size_t)r * v.rem_cap + t];
+    sSlot[t] = k;
+    sId[t] = v.id[k];
+  }
+  __syncthreads();
+  rank_members(n, sId, sRank);
+  __syncthreads();
+  for (int t = threadIdx.x; t < n; t += WAVE) sOrd[sRank[t]] = sSlot[t];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = v.water_removed[r];
+    for (int p = 0; p < n; ++p) {
+      const int k = sOrd[p];
+      acc += v.xi[k] * v.m_w[k];
+    }
+    v.water_removed[r] = acc;
+    v.rem_cnt[r] = 0;
+  }
+}
+
+/*
+ * Delta m_l per cell: sum of dm*xi over the cell's members in ascending id
+ * (integration.py:915-938), then *1e-18.
+ */
+__global__ __launch_bounds__(WAVE) void k_dml(KView v) {
+  const int nc = v.nc;
+  const int gc = blockIdx.x, r = gc / nc;
+  __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
+  __shared__ double sVal[DML_CAP];
+  const int n = gather_members(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.cell, v.id, sSlot, sId, DML_CAP);
+  if (n > DML_CAP) {
+    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+    return;
+  }
+  __syncthreads();
+  rank_members(n, sId, sRank);
+  __syncthreads();
+  for (int m = threadIdx.x; m < n; m += WAVE) sVal[sRank[m]] = v.dmxi[sSlot[m]];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int p = 0; p < n; ++p) s += sVal[p];
+    v.dml[gc] = s * 1.0E-18;
+  }
+}
+
+/* integration.py:402-483 + 359-400 for one cell; refreshes cc */
+__global__ __launch_bounds__(256) void k_grid(KView v, LmPhys P) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= v.n_runs * v.nc) return;
+  const int nc = v.nc, r = gc / nc, c = gc % nc;
+  double *S = v.scal + (size_t)r * 10 * nc;
+  double *M = v.mat + (size_t)r * 7 * nc;
+  const double rho_d = S[3 * nc + c];
+  const double dml = v.dml[gc];
+  double Th = S[2 * nc + c] + v.dTh[gc];
+  double r_v = S[4 * nc + c];
+  r_v += v.dRv[gc] - dml * S[8 * nc + c];
+  const double Theta_over_T = lm_pow(v.p_ref_inv * P.R_d * rho_d * Th, P.kappa_factor2);
+  Th += Theta_over_T * (M[2 * nc + c] * dml) / (P.c_pd * rho_d * v.V_cell * (1.0 + r_v * P.heat_factor_r_v));
+  const double pdr = lm_pow(rho_d * Th * P.R_d * v.p_ref_inv, P.kappa_factor);
+  const double T = Th * lm_pow(pdr, P.kappa_dry);
+  const double p = pdr * v.p_ref * (1 + r_v / P.eps_gc);
+  const double e_s = 2.53E11 * lm_exp(-5420.0 / T);
+  const double Sat = (rho_d * r_v) * T * P.R_v / e_s;
+  S[0 * nc + c] = T;
+  S[1 * nc + c] = p;
+  S[2 * nc + c] = Th;
+  S[4 * nc + c] = r_v;
+  S[6 * nc + c] = Sat;
+  S[7 * nc + c] = e_s;
+  M[0 * nc + c] = 4.1868E-3 * (5.69 + 0.017 * (T - 273.15));
+  M[1 * nc + c] = 4.01218E-5 * lm_pow(T, 1.94) / p;
+  M[2 * nc + c] = 1.0E3 * (2500.82 - 2.358 * (T - 273.0));
+  M[3 * nc + c] = lmp_sigma_water(T);
+  M[4 * nc + c] = P.c_pd * (1 + 0.897 * r_v);
+  M[5 * nc + c] = 1.458E-6 * lm_pow(T, 1.5) / (T + 110.4);
+  M[6 * nc + c] = rho_d * (1 + r_v);
+  fill_cc(P, S, M, c, nc, v.cc + (size_t)gc * CC_N);
+}
+
+/* ------------------------------------------------------------------------ */
+/* advection increments                                                     */
+/* ------------------------------------------------------------------------ */
+/* column sums over x in index order -> relaxation terms (integration.py:325-354) */
+__global__ void k_colsum(KView v, double dt_sub, int act_relax) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= v.n_runs * v.nz) return;
+  const int r = t / v.nz, j = t % v.nz, nc = v.nc;
+  const double *S = v.scal + (size_t)r * 10 * nc;
+  double srv = 0.0, sth = 0.0;
+  for (int i = 0; i < v.nx; ++i) {
+    srv += S[4 * nc + i * v.nz + j];
+    sth += S[2 * nc + i * v.nz + j];
+  }
+  if (act_relax) {
+    v.colsum[2 * t + 0] = dt_sub * (v.prof_rv[t] - srv / (double)v.nx) / v.t_relax[j];
+    v.colsum[2 * t + 1] = dt_sub * (v.prof_Th[t] - sth / (double)v.nx) / v.t_relax[j];
+  }
+}
+
+__device__ static inline double lim_fn(double r, double one_third) { /* integration.py:80-102 */
+  const double Kv = (1.0 + 2.0 * r) * one_third;
+  return lm_np_max(0.0, lm_np_fmin(2.0 * r, lm_np_fmin(Kv, 2.0)));
+}
+__device__ static inline double lim_up(double a0, double a1, double a2, double one_third) { /* :113-148 */
+  double r;
+  if (fabs(a1 - a2) > 1.0E-8 * fabs(a0 - a1))
+    r = (a0 - a1) / (a1 - a2);
+  else
+    r = (a0 - a1) * 1.0E8 * lm_np_sign(a1 - a2);
+  return lim_fn(r, one_third);
+}
+__device__ static inline double face_flux(double a0, double a1, double a2, double a3, double u, double ot) {
+  const double f_pos = a1 + 0.5 * lim_up(a0, a1, a2, ot) * (a1 - a2);
+  const double f_neg = a0 + 0.5 * lim_up(a1, a0, a3, ot) * (a0 - a3);
+  return u >= 0.0 ? u * f_pos : u * f_neg;
+}
+
+/* integration.py:150-286, periodic x / solid z, at cell (i,j) */
+__device__ static double divergence_at(const double *f, const double *flux, int i, int j, int nx, int nz, double dx,
+                                       double dz, double ot) {
+  const int nfz = nz + 1;
+  const double *FU = flux, *FW = flux + (size_t)(nx + 1) * nfz;
+  double Fx[2];
+  for (int s = 0; s < 2; ++s) {
+    const int fi = i + s;
+    const int i0 = fi % nx, i1 = (fi - 1 + nx) % nx, i2 = (fi - 2 + 2 * nx) % nx, i3 = (fi + 1) % nx;
+    Fx[s] = face_flux(f[i0 * nz + j], f[i1 * nz + j], f[i2 * nz + j], f[i3 * nz + j], FU[fi * nfz + j], ot);
+  }
+  double Fz[2];
+  for (int s = 0; s < 2; ++s) {
+    const int fj = j + s;
+    if (fj == 0 || fj == nz) {
+      Fz[s] = 0.0;
+      continue;
+    }
+    const int j0 = fj > nz - 1 ? nz - 1 : fj;
+    const int j1 = fj - 1;
+    const int j2 = fj - 2 < 0 ? 0 : fj - 2;
+    const int j3 = fj + 1 > nz - 1 ? nz - 1 : fj + 1;
+    Fz[s] = face_flux(f[i * nz + j0], f[i * nz + j1], f[i * nz + j2], f[i * nz + j3], FW[i * nfz + fj], ot);
+  }
+  double out = (Fx[1] - Fx[0]) / dx;
+  out = out + (Fz[1] - Fz[0]) / dz;
+  return out;
+}
+
+__global__ void k_adv_incr(KView v, LmPhys P, double dt_sub, int second, int act_relax) {
+  const int gc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gc >= v.n_runs * v.nc) return;
+  const int nc = v.nc, r = gc / nc, c = gc % nc, i = c / v.nz, j = c % v.nz;
+  const double *S = v.scal + (size_t)r * 10 * nc;
+  const double rho_inv = S[9 * nc + c];
+  const double div_rv = divergence_at(S + 4 * nc, v.flux, i, j, v.nx, v.nz, v.dx, v.dz, P.one_third);
+  const double div_th = divergence_at(S + 2 * nc, v.flux, i, j, v.nx, v.nz, v.dx, v.dz, P.one_third);
+  double drv, dth;
+  if (!second) {
+    drv = -dt_sub * div_rv * rho_inv;
+    dth = -dt_sub * div_th * rho_inv;
+  } else {
+    drv = -2.0 * dt_sub * div_rv * rho_inv - v.dRv[gc];
+    dth = -2.0 * dt_sub * div_th * rho_inv - v.dTh[gc];
+  }
+  if (act_relax) {
+    const double *cs = v.colsum + (size_t)2 * (r * v.nz + j);
+    if (!second) {
+      drv += cs[0];
+      dth += cs[1];
+    } else {
+      drv += 2. * cs[0];
+      dth += 2. * cs[1];
+    }
+  }
+  v.dRv[gc] = drv;
+  v.dTh[gc] = dth;
+}
+
+/* ------------------------------------------------------------------------ */
+/* reindex: counting sort by (run, cell, id)                                */
+/* ------------------------------------------------------------------------ */
+/* scan layout per run: [cnt_0..cnt_{nc-1}, graveyard_old, dead_0..dead_{nc-1}] */
+__global__ __launch_bounds__(WAVE) void k_ri_count(KView v) {
+  const int nc = v.nc, L = 2 * nc + 1;
+  const int lane = threadIdx.x;
+  if ((int)blockIdx.x >= v.n_runs * nc) {
+    const int r = blockIdx.x - v.n_runs * nc;
+    if (lane == 0) {
+      const int32_t *sr = v.seg + (size_t)r * (nc + 1);
+      v.scan_buf[(size_t)r * L + nc] = sr[nc + 1] - sr[nc];
+    }
+    return;
+  }
+  const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
+  const int32_t *sr = v.seg + (size_t)r * (nc + 1);
+  const int n = gather_members(gc, nc, v.nx, v.nz, sr, v.cell, v.id, nullptr, nullptr, 0);
+  int dead = 0;
+  for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
+    const int k = base + lane;
+    dead += __popcll(__ballot(k < sr[c + 1] && v.cell[k] < 0));
+  }
+  if (lane == 0) {
+    v.scan_buf[(size_t)r * L + c] = n;
+    v.scan_buf[(size_t)r * L + nc + 1 + c] = dead;
+  }
+}
+
+#define SCAN_BLOCK 256
+#define SCAN_PER 16
+#define SCAN_CHUNK (SCAN_BLOCK * SCAN_PER)
+
+__device__ static int block_exclusive_scan(int val, int *tmp) {
+  const int t = threadIdx.x;
+  tmp[t] = val;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BLOCK; off <<= 1) {
+    const int add = t >= off ? tmp[t - off] : 0;
+    __syncthreads();
+    tmp[t] += add;
+    __syncthreads();
+  }
+  const int incl = tmp[t];
+  __syncthreads();
+  return incl - val;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_local(int32_t *a, int32_t *part, int L) {
+  __shared__ int tmp[SCAN_BLOCK];
+  const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER;
+  int loc[SCAN_PER];
+  int s = 0;
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int k = base + q;
+    loc[q] = k < L ? a[k] : 0;
+    s += loc[q];
+  }
+  const int ex = block_exclusive_scan(s, tmp);
+  int acc = ex;
+  for (int q = 0; q < SCAN_PER; ++q) {
+    const int k = base + q;
+    if (k < L) a[k] = acc;
+    acc += loc[q];
+  }
+  if (threadIdx.x == SCAN_BLOCK - 1) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_part(int32_t *part, int nparts, int32_t *a, int L) {
+  __shared__ int tmp[SCAN_BLOCK];
+  int carry = 0;
+  for (int b0 = 0; b0 < nparts; b0 += SCAN_BLOCK) {
+    const int k = b0 + threadIdx.x;
+    const int val = k < nparts ? part[k] : 0;
+    const int ex = block_exclusive_scan(val, tmp);
+    if (k < nparts) part[k] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) tmp[0] = ex + val;
+    __syncthreads();
+    carry += tmp[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a[L] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(int32_t *a, const int32_t *part, int L) {
+  const int base = blockIdx.x * SCAN_CHUNK;
+  const int add = part[blockIdx.x];
+  for (int q = threadIdx.x; q < SCAN_CHUNK; q += SCAN_BLOCK) {
+    const int k = base + q;
+    if (k < L) a[k] += add;
+  }
+}
+
+__device__ static inline void copy_sip(const KView &v, int src, int dst, int cellv) {
+  v.x2[dst] = v.x[src];
+  v.z2[dst] = v.z[src];
+  v.u2[dst] = v.u[src];
+  v.w2[dst] = v.w[src];
+  v.m_w2[dst] = v.m_w[src];
+  v.m_s2[dst] = v.m_s[src];
+  v.xi2[dst] = v.xi[src];
+  v.id2[dst] = v.id[src];
+  v.cell2[dst] = cellv;
+}
+
+__global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
+  const int nc = v.nc, L = 2 * nc + 1;
+  const int lane = threadIdx.x;
+  if ((int)blockIdx.x >= v.n_runs * nc) {
+    /* graveyard of run r: old graveyard first, in slot order */
+    const int r = blockIdx.x - v.n_runs * nc;
+    const int32_t *sr = v.seg + (size_t)r * (nc + 1);
+    const int32_t *sc = v.scan_buf + (size_t)r * L;
+    const int g0 = sr[nc], g1 = sr[nc + 1];
+    const int dst0 = sc[nc];
+    for (int k = g0 + lane; k < g1; k += WAVE) copy_sip(v, k, dst0 + (k - g0), v.cell[k]);
+    if (lane == 0) {
+      v.seg2[(size_t)r * (nc + 1) + nc] = dst0;
+      const int new_size = v.scan_buf[(size_t)(r + 1) * L] - sc[0];
+      const int old_size = sr[nc + 1] - sr[0];
+      if (new_size != old_size) raise_err(v.err, LM_ERR_LOST_SIP, r);
+      if (r == v.n_runs - 1) v.seg2[(size_t)v.n_runs * (nc + 1)] = v.scan_buf[(size_t)v.n_runs * L];
+    }
+    return;
+  }
+  const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
+  const int32_t *sr = v.seg + (size_t)r * (nc + 1);
+  const int32_t *sc = v.scan_buf + (size_t)r * L;
+  __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
+  const int n = gather_members(gc, nc, v.nx, v.nz, sr, v.cell, v.id, sSlot, sId, DML_CAP);
+  if (n > DML_CAP) {
+    if (lane == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+    return;
+  }
+  __syncthreads();
+  rank_members(n, sId, sRank);
+  __syncthreads();
+  const int dst0 = sc[c];
+  for (int m = lane; m < n; m += WAVE) copy_sip(v, sSlot[m], dst0 + sRank[m], gc);
+  /* SIPs that died in this home segment since the last reindex, in slot order */
+  int dead = 0;
+  const int ddst = sc[nc + 1 + c];
+  for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
+    const int k = base + lane;
+    const bool d = k < sr[c + 1] && v.cell[k] < 0;
+    const unsigned long long bal = __ballot(d);
+    if (d) copy_sip(v, k, ddst + dead + __popcll(bal & lane_mask_lt()), v.cell[k]);
+    dead += __popcll(bal);
+  }
+  if (lane == 0) v.seg2[(size_t)r * (nc + 1) + c] = dst0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* box models: one wavefront per box                                        */
+/* ------------------------------------------------------------------------ */
+#define BOX_CAP 1024
+
+__device__ static long long box_stream_offset(const int32_t *off, int b) {
+  long long s = 0;
+  for (int q = 0; q < b; ++q) {
+    const long long n = off[q + 1] - off[q];
+    s += n * (n - 1) / 2;
+  }
+  return s;
+}
+
+/* all_or_nothing.py:109-185 */
+__global__ __launch_bounds__(WAVE) void k_golovin(int nbox, const int32_t *off, double *xis, double *masses,
+                                                  double dt_over_dV, long long *no_cols, unsigned long long seed,
+                                                  unsigned step, const double *rnd_stream, int32_t *err) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int s0 = off[b], n = off[b + 1] - off[b];
+  if (n > BOX_CAP) {
+    if (lane == 0) raise_err(err, LM_ERR_CELL_CAP, b);
+    return;
+  }
+  __shared__ double sX[BOX_CAP], sM[BOX_CAP];
+  for (int m = lane; m < n; m += WAVE) {
+    sX[m] = xis[s0 + m];
+    sM[m] = masses[s0 + m];
+  }
+  __syncthreads();
+  const long long roff = rnd_stream ? box_stream_offset(off, b) : 0;
+  long long cs = 0, cm = 0;
+  for (int d = 1; d <= 2 * n - 3; ++d) {
+    const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
+    const int i_hi = (d - 1) >> 1;
+    for (int i = i_lo + lane; i <= i_hi; i += WAVE) {
+      const int j = d - i;
+      int imin, imax;
+      if (sX[i] <= sX[j]) {
+        imin = i;
+        imax = j;
+      } else {
+        imin = j;
+        imax = i;
+      }
+      const double xi_min = sX[imin], xi_max = sX[imax];
+      const double m_min = sM[imin], m_max = sM[imax];
+      const double p_crit = xi_max * ((m_min + m_max) * 1.5E-18) * dt_over_dV; /* kernel.py:461-480 */
+      if (p_crit > 1.0) {
+        ++cm;
+        const double xi_col = p_crit * xi_min;
+        sM[imin] = (xi_min * m_min + xi_col * m_max) / xi_min;
+        sX[imax] = xi_max - xi_col;
+      } else if (p_crit > 0.0) {
+        const long long cnt = (long long)i * n - (long long)i * (i + 1) / 2 + (j - i - 1);
+        const double rnd = rnd_stream ? rnd_stream[roff + cnt]
+                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint64_t)cnt);
+        if (p_crit > rnd) {
+          ++cs;
+          const double xi_rel_dev = (xi_max - xi_min) / xi_max;
+          if (xi_rel_dev < 1.0E-5) {
+            const double xi_ges = xi_min + xi_max;
+            const double mn = 2.0 * (xi_min * m_min + xi_max * m_max) / xi_ges;
+            sM[imin] = mn;
+            sM[imax] = mn;
+            const double xmax_new = 0.5 * 0.7 * xi_ges;
+            sX[imax] = xmax_new;
+            sX[imin] = 0.5 * xi_ges - xmax_new;
+          } else {
+            sM[imin] = m_min + m_max;
+            sX[imax] = xi_max - xi_min;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int m = lane; m < n; m += WAVE) {
+    xis[s0 + m] = sX[m];
+    masses[s0 + m] = sM[m];
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    cs += __shfl_down(cs, o);
+    cm += __shfl_down(cm, o);
+  }
+  if (lane == 0) {
+    no_cols[2 * b] += cs;
+    no_cols[2 * b + 1] += cm;
+  }
+}
+
+/* collision/kernel.py:107-167 */
+__device__ static double beard_velocity(double R) {
+  const double rho_w = 1.0E3, rho_a = 1.225, visc = 1.818E-5, sigma_w = 73.0E-3;
+  const double drho = rho_w - rho_a, g = 9.80665;
+  double v;
+  if (R < 10.0) {
+    const double l0 = 6.62E-2;
+    const double C1 = drho * g / (4.5 * visc);
+    const double C_sc = 1.0 + 1.257 * l0 / R;
+    v = C1 * C_sc * R * R * 1.0E-12;
+  } else if (R < 535.0) {
+    const double p1[7] = {-0.327815e-5, 0.855176e-4, -0.578878e-3, -0.987059e-3, -0.153193e-2, 0.992696, -0.318657e1};
+    const double N_Da = 32.0E-18 * R * R * R * rho_a * drho * g / (3.0 * visc * visc);
+    const double Y = lmp_poly(p1, 7, lm_log(N_Da));
+    const double C_sc = 1.0 + 1.257 * 6.62E-2 / R;
+    v = visc * C_sc * lm_exp(Y) / (rho_a * R * 2.0E-6);
+  } else if (R < 3500.0) {
+    const double p2[6] = {0.238449e-2, -0.542819e-1, 0.475294, -0.204914e1, 0.523778e1, -0.500015e1};
+    const double N_Bo = 16.0E-12 * R * R * drho * g / (3.0 * sigma_w);
+    const double visc2 = visc * visc;
+    const double N_P16 =
+        lm_pow(sigma_w * sigma_w * sigma_w * rho_a * rho_a / ((visc2 * visc2) * drho * g), 1.0 / 6.0);
+    const double Y = lmp_poly(p2, 6, lm_log(N_Bo * N_P16));
+    v = visc * N_P16 * lm_exp(Y) / (rho_a * R * 2.0E-6);
+  } else {
+    v = 9.04929248;
+  }
+  return v;
+}
+
+/* all_or_nothing.py:269-390 followed by kernel.py:193-210 */
+__global__ __launch_bounds__(WAVE) void k_ecol_box(int nbox, const int32_t *off, double *xis, double *masses,
+                                                   double *radii, double *vel, const double *rho, double dt_over_dV,
+                                                   const double *E, int nb, double lnR0, double lnf,
+                                                   long long *no_cols, unsigned long long seed, unsigned step,
+                                                   const double *rnd_stream, int32_t *err, LmPhys P) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int s0 = off[b], n = off[b + 1] - off[b];
+  if (n > BOX_CAP) {
+    if (lane == 0) raise_err(err, LM_ERR_CELL_CAP, b);
+    return;
+  }
+  __shared__ double sX[BOX_CAP], sM[BOX_CAP], sR[BOX_CAP], sV[BOX_CAP], sRho[BOX_CAP];
+  __shared__ int sI[BOX_CAP];
+  for (int m = lane; m < n; m += WAVE) {
+    sX[m] = xis[s0 + m];
+    sM[m] = masses[s0 + m];
+    sR[m] = radii[s0 + m];
+    sV[m] = vel[s0 + m];
+    sRho[m] = rho[s0 + m];
+    sI[m] = lmp_kernel_index(sR[m], lnR0, lnf, nb);
+  }
+  __syncthreads();
+  const long long roff = rnd_stream ? box_stream_offset(off, b) : 0;
+  long long cs = 0, cm = 0;
+  for (int d = 1; d <= 2 * n - 3; ++d) {
+    const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
+    const int i_hi = (d - 1) >> 1;
+    for (int i = i_lo + lane; i <= i_hi; i += WAVE) {
+      const int j = d - i;
+      int imin, imax;
+      if (sX[i] <= sX[j]) {
+        imin = i;
+        imax = j;
+      } else {
+        imin = j;
+        imax = i;
+      }
+      const double xi_min = sX[imin], xi_max = sX[imax];
+      const double m_min = sM[imin], m_max = sM[imax];
+      const double p_crit =
+          xi_max * dt_over_dV * lmp_kernel_hydro(P, sR[i], sR[j], E[sI[i] * nb + sI[j]], fabs(sV[i] - sV[j]));
+      if (p_crit > 1.0) {
+        ++cm;
+        const double xi_col = p_crit * xi_min;
+        const double mn = (xi_min * m_min + xi_col * m_max) / xi_min;
+        sM[imin] = mn;
+        sR[imin] = lmp_radius(P, mn, sRho[imin]);
+        sX[imax] = xi_max - xi_col;
+        sI[imin] = lmp_kernel_index(sR[imin], lnR0, lnf, nb);
+      } else if (p_crit > 0.0) {
+        const long long cnt = (long long)i * n - (long long)i * (i + 1) / 2 + (j - i - 1);
+        const double rnd = rnd_stream ? rnd_stream[roff + cnt]
+                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint64_t)cnt);
+        if (p_crit > rnd) {
+          ++cs;
+          const double xi_rel_dev = (xi_max - xi_min) / xi_max;
+          if (xi_rel_dev < 1.0E-5) {
+            const double xi_ges = xi_min + xi_max;
+            const double mn = 2.0 * (xi_min * m_min + xi_max * m_max) / xi_ges;
+            sM[imin] = mn;
+            sM[imax] = mn;
+            const double Rn = lmp_radius(P, mn, sRho[imin]);
+            sR[imin] = Rn;
+            sR[imax] = Rn;
+            const double xmax_new = 0.5 * 0.7 * xi_ges;
+            sX[imax] = xmax_new;
+            sX[imin] = 0.5 * xi_ges - xmax_new;
+            const int ind = lmp_kernel_index(Rn, lnR0, lnf, nb);
+            sI[imin] = ind;
+            sI[imax] = ind;
+          } else {
+            const double mn = m_min + m_max;
+            sM[imin] = mn;
+            sR[imin] = lmp_radius(P, mn, sRho[imin]);
+            sX[imax] = xi_max - xi_min;
+            sI[imin] = lmp_kernel_index(sR[imin], lnR0, lnf, nb);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int m = lane; m < n; m += WAVE) {
+    xis[s0 + m] = sX[m];
+    masses[s0 + m] = sM[m];
+    radii[s0 + m] = sR[m];
+    vel[s0 + m] = beard_velocity(sR[m]);
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    cs += __shfl_down(cs, o);
+    cm += __shfl_down(cm, o);
+  }
+  if (lane == 0) {
+    no_cols[2 * b] += cs;
+    no_cols[2 * b + 1] += cm;
+  }
+}
+
+/* elementwise FP64 math of the kernels, for the host/device parity test */
+__global__ void k_math_probe(int which, long long n, const double *x, const double *y, double *out) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double a = x[k], b = y[k];
+  double r;
+  switch (which) {
+    case 0: r = lm_exp(a); break;
+    case 1: r = lm_log(a); break;
+    case 2: r = lm_pow(a, b); break;
+    case 3: r = lm_pymod(a, b); break;
+    case 4: r = sqrt(a); break;
+    case 5: r = a / b; break;
+    default: r = floor(a / b); break;
+  }
+  out[k] = r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* C ABI                                                                    */
+/* ------------------------------------------------------------------------ */
+static int check_engine(const LmEngine *e) {
+  if (!e) return set_err(LM_ERR_ARG, "engine is NULL");
+  if (e->n_runs <= 0 || e->nx <= 0 || e->nz <= 0 || e->nc != e->nx * e->nz)
+    return set_err(LM_ERR_ARG, "bad grid shape (need nc == nx*nz > 0, n_runs > 0)");
+  if (e->cur < 0 || e->cur > 1 || e->ccur < 0 || e->ccur > 1) return set_err(LM_ERR_ARG, "bad buffer index");
+  if (!e->x[0] || !e->x[1] || !e->cell[0] || !e->cell[1] || !e->seg[0] || !e->seg[1] || !e->cc || !e->err)
+    return set_err(LM_ERR_ARG, "engine buffer pointer is NULL");
+  if ((int64_t)e->n_runs * e->nc > 0x7fffffff / 2) return set_err(LM_ERR_ARG, "too many cells");
+  if (e->n_tot > 0x7ffffff0) return set_err(LM_ERR_ARG, "too many SIPs for 32-bit slots");
+  return 0;
+}
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+extern "C" {
+
+const char *lm_version(void) { return "lacmopy_amd 0.1 gfx950"; }
+
+const char *lm_last_error_message(void) { return g_errmsg; }
+
+int64_t lm_engine_sizeof(void) { return (int64_t)sizeof(LmEngine); }
+
+void lm_engine_offsets(int64_t *out, int32_t n) {
+  const int64_t o[] = {
+      (int64_t)offsetof(LmEngine, n_tot),     (int64_t)offsetof(LmEngine, solute),
+      (int64_t)offsetof(LmEngine, dt),        (int64_t)offsetof(LmEngine, E),
+      (int64_t)offsetof(LmEngine, seed),      (int64_t)offsetof(LmEngine, x),
+      (int64_t)offsetof(LmEngine, id),        (int64_t)offsetof(LmEngine, dmxi),
+      (int64_t)offsetof(LmEngine, scal),      (int64_t)offsetof(LmEngine, grid_vel),
+      (int64_t)offsetof(LmEngine, no_cols),   (int64_t)offsetof(LmEngine, rem_cap),
+      (int64_t)offsetof(LmEngine, scan_buf),  (int64_t)offsetof(LmEngine, rnd_stream),
+      (int64_t)offsetof(LmEngine, rnd_pos),
+  };
+  const int m = (int)(sizeof(o) / sizeof(o[0]));
+  for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
+}
+
+int32_t lm_num_constants(void) { return (int32_t)(sizeof(LmPhys) / sizeof(double)); }
+
+void lm_get_constants(double *out, int32_t n) {
+  const LmPhys &P = phys();
+  const double *p = (const double *)&P;
+  const int m = lm_num_constants();
+  for (int i = 0; i < n && i < m; ++i) out[i] = p[i];
+}
+
+int lm_engine_init_cells(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  const int ncell = e->n_runs * e->nc;
+  hipLaunchKernelGGL(k_init_cc, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), make_view(e), phys());
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_collision_step(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (!e->e_const_mode && (!e->E || e->nb <= 0)) return set_err(LM_ERR_ARG, "collision table missing");
+  const int ncell = e->n_runs * e->nc;
+  if (e->rnd_stream) {
+    if (!e->rnd_off) return set_err(LM_ERR_ARG, "rnd_off scratch missing");
+    KView v = make_view(e);
+    hipLaunchKernelGGL(k_count_pairs, dim3(ncell), dim3(WAVE), 0, S(stream), v);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, S(stream), (long long *)e->rnd_off, ncell);
+    LAUNCH_CHECK();
+  }
+  HIP_TRY(hipMemsetAsync(e->ovf_cnt, 0, sizeof(int32_t), S(stream)));
+  KView v = make_view(e);
+  const LmPhys &P = phys();
+  hipLaunchKernelGGL(k_collision<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, P, 0);
+  hipLaunchKernelGGL(k_collision<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, P, 1);
+  LAUNCH_CHECK();
+  e->col_step += 1;
+  return 0;
+}
+
+int lm_position_step(LmEngine *e, double dt, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  KView v = make_view(e);
+  const long long n = e->n_tot;
+  if (n > 0) hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, phys(), n, 0.0, dt, 1);
+  hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
+  LAUNCH_CHECK();
+  e->ccur = 1 - e->ccur;
+  return 0;
+}
+
+int lm_reindex(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  KView v = make_view(e);
+  const int nc = e->nc, R = e->n_runs;
+  const int L = R * (2 * nc + 1);
+  hipLaunchKernelGGL(k_ri_count, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
+  const int nchunk = (L + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  hipLaunchKernelGGL(k_scan_local, dim3(nchunk), dim3(SCAN_BLOCK), 0, S(stream), e->scan_buf, e->scan_part, L);
+  hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(SCAN_BLOCK), 0, S(stream), e->scan_part, nchunk, e->scan_buf, L);
+  hipLaunchKernelGGL(k_scan_add, dim3(nchunk), dim3(SCAN_BLOCK), 0, S(stream), e->scan_buf, e->scan_part, L);
+  hipLaunchKernelGGL(k_ri_scatter, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
+  LAUNCH_CHECK();
+  e->cur = 1 - e->cur;
+  e->ccur = 1 - e->ccur;
+  return 0;
+}
+
+int lm_advection_increments(LmEngine *e, int32_t second, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  KView v = make_view(e);
+  const int ncell = e->n_runs * e->nc;
+  if (e->act_relaxation) {
+    const int nt = e->n_runs * e->nz;
+    hipLaunchKernelGGL(k_colsum, dim3((nt + 127) / 128), dim3(128), 0, S(stream), v, e->dt_sub, 1);
+  }
+  hipLaunchKernelGGL(k_adv_incr, dim3((ncell + 127) / 128), dim3(128), 0, S(stream), v, phys(), e->dt_sub, second,
+                     e->act_relaxation);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  KView v = make_view(e);
+  const LmPhys &P = phys();
+  const long long n = e->n_tot;
+  const int ncell = e->n_runs * e->nc;
+  if (n > 0)
+    hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, P, n, dt_sub,
+                       dt_sub_pos, 0);
+  hipLaunchKernelGGL(k_dml, dim3(ncell), dim3(WAVE), 0, S(stream), v);
+  hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
+  hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v, P);
+  LAUNCH_CHECK();
+  e->ccur = 1 - e->ccur;
+  return 0;
+}
+
+static int subloop(LmEngine *e, int no_cond_steps, int no_col_steps, void *stream) {
+  int rc;
+  if (e->act_collisions && no_col_steps == 1)
+    if ((rc = lm_collision_step(e, stream))) return rc;
+  for (int n = 0; n < no_cond_steps; ++n) {
+    if (e->act_collisions && no_col_steps > 1)
+      if ((rc = lm_collision_step(e, stream))) return rc;
+    if ((rc = lm_substep(e, e->dt_sub, e->dt_sub, stream))) return rc;
+  }
+  if (e->act_collisions && no_col_steps > no_cond_steps)
+    if ((rc = lm_collision_step(e, stream))) return rc;
+  return 0;
+}
+
+int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  for (int s = 0; s < n_steps; ++s) {
+    if ((rc = lm_position_step(e, e->dt_sub_half, stream))) return rc;
+    if ((rc = lm_reindex(e, stream))) return rc;
+    if ((rc = lm_advection_increments(e, 0, stream))) return rc;
+    int ncol = e->no_col_per_adv == 1 ? 0 : (e->no_col_per_adv == 2 ? 1 : e->scale_dt_cond);
+    if ((rc = subloop(e, e->scale_dt_cond, ncol, stream))) return rc;
+    if ((rc = lm_advection_increments(e, 1, stream))) return rc;
+    ncol = (e->no_col_per_adv == 1 || e->no_col_per_adv == 2) ? 1 : e->scale_dt_cond;
+    if ((rc = subloop(e, e->scale_dt_cond - 1, ncol, stream))) return rc;
+    if ((rc = lm_substep(e, e->dt_sub, e->dt_sub_half, stream))) return rc;
+  }
+  return 0;
+}
+
+int lm_engine_check(LmEngine *e, void *stream) {
+  if (!e || !e->err) return set_err(LM_ERR_ARG, "engine/err NULL");
+  int32_t h[2] = {0, 0};
+  HIP_TRY(hipMemcpyAsync(h, e->err, sizeof(h), hipMemcpyDeviceToHost, S(stream)));
+  HIP_TRY(hipStreamSynchronize(S(stream)));
+  if (h[0]) {
+    const char *what = h[0] == LM_ERR_CELL_CAP   ? "cell holds more SIPs than the kernel capacity"
+                       : h[0] == LM_ERR_LOST_SIP ? "SIP moved more than one cell between reindexes"
+                       : h[0] == LM_ERR_REM_CAP  ? "too many SIPs removed in one substep"
+                       : h[0] == LM_ERR_OUT_OF_DOMAIN ? "SIP position outside the grid"
+                                                       : "device error";
+    snprintf(g_errmsg, sizeof(g_errmsg), "%s (code %d, detail %d)", what, h[0], h[1]);
+    return h[0];
+  }
+  return 0;
+}
+
+int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, double *out, void *stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_math_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), which, (long long)n,
+                     x, y, out);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
+                    int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
+  if (n_boxes <= 0 || !off || !xis || !masses || !no_cols) return set_err(LM_ERR_ARG, "bad box arguments");
+  static int32_t *err_dev = nullptr;
+  if (!err_dev) {
+    HIP_TRY(hipMalloc(&err_dev, 16));
+    HIP_TRY(hipMemset(err_dev, 0, 16));
+  }
+  hipLaunchKernelGGL(k_golovin, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, dt_over_dV,
+                     (long long *)no_cols, (unsigned long long)seed, step, rnd_stream, err_dev);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double *radii, double *vel,
+                     const double *rho, double dt_over_dV, const double *E, int32_t nb, double lnR0, double lnf,
+                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
+  if (n_boxes <= 0 || !off || !xis || !masses || !radii || !vel || !rho || !E || !no_cols)
+    return set_err(LM_ERR_ARG, "bad box arguments");
+  static int32_t *err_dev = nullptr;
+  if (!err_dev) {
+    HIP_TRY(hipMalloc(&err_dev, 16));
+    HIP_TRY(hipMemset(err_dev, 0, 16));
+  }
+  hipLaunchKernelGGL(k_ecol_box, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, radii, vel,
+                     rho, dt_over_dV, E, nb, lnR0, lnf, (long long *)no_cols, (unsigned long long)seed, step,
+                     rnd_stream, err_dev, phys());
+  LAUNCH_CHECK();
+  return 0;
+}
+
+} /* extern "C" */

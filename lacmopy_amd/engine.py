@@ -1,0 +1,298 @@
+"""Device-resident SIP state of one or many independent runs on one GPU.
+
+The Engine owns torch tensors on the GPU (plumbing only) and an LmEngine
+struct that points into them; every numerical operation is a HIP kernel of
+liblacmo_amd.so.  SIP arrays are kept as structure-of-arrays sorted by
+(run, home cell, SIP id); the reference's ID-ordered layout (pos (2,N),
+vel (2,N), cells (2,N) int64, m_w, m_s, xi, active_ids) is only produced on
+export.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from ._native import LmEngine, check, lib
+
+SOLUTES = {"AS": 0, "NaCl": 1}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Engine:
+    """Batched SIP engine: n_runs runs of the same grid geometry."""
+
+    def __init__(self, nx, nz, n_per_run, ranges, steps, volume_cell, p_ref=1.0E5, p_ref_inv=1.0E-5,
+                 device="cuda", rem_cap=1024):
+        if not torch.cuda.is_available():
+            raise RuntimeError("lacmopy_amd.Engine needs a GPU (HIP device); there is no CPU fallback")
+        self.L = lib()
+        self.device = torch.device(device)
+        self.nx, self.nz = int(nx), int(nz)
+        self.nc = self.nx * self.nz
+        self.n_per_run = [int(n) for n in n_per_run]
+        self.n_runs = len(self.n_per_run)
+        self.base = np.concatenate([[0], np.cumsum(self.n_per_run)]).astype(np.int64)
+        self.n_tot = int(self.base[-1])
+        self.ranges = np.asarray(ranges, dtype=np.float64)
+        self.steps = np.asarray(steps, dtype=np.float64)
+        nr, nc, dev = self.n_runs, self.nc, self.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        i64 = dict(dtype=torch.int64, device=dev)
+        n = max(self.n_tot, 1)
+        self.buf = {}
+
+        def mk(name, shape, kw):
+            t = torch.zeros(shape, **kw)
+            self.buf[name] = t
+            return t
+
+        for b in (0, 1):
+            for f in ("x", "z", "u", "w", "m_w", "m_s", "xi"):
+                mk(f"{f}{b}", (n,), f64)
+            mk(f"id{b}", (n,), i32)
+            mk(f"cell{b}", (n,), i32)
+            mk(f"seg{b}", (nr * (nc + 1) + 1,), i32)
+        mk("dmxi", (n,), f64)
+        mk("scal", (nr, 10, nc), f64)
+        mk("mat", (nr, 7, nc), f64)
+        mk("cc", (nr * nc, 16), f64)
+        mk("dTh", (nr * nc,), f64)
+        mk("dRv", (nr * nc,), f64)
+        mk("dml", (nr * nc,), f64)
+        mk("grid_vel", (2, self.nx + 1, self.nz + 1), f64)
+        mk("flux", (2, self.nx + 1, self.nz + 1), f64)
+        mk("prof_rv", (nr, self.nz), f64)
+        mk("prof_Th", (nr, self.nz), f64)
+        mk("t_relax", (self.nz,), f64)
+        mk("colsum", (nr * self.nz * 2,), f64)
+        mk("no_cols", (nr, 2), i64)
+        mk("water_removed", (nr,), f64)
+        mk("n_equal_xi", (nr,), i64)
+        mk("rem_cnt", (nr,), i32)
+        mk("rem_slot", (nr * rem_cap,), i32)
+        L = nr * (2 * nc + 1)
+        mk("scan_buf", (L + 1,), i32)
+        mk("scan_part", ((L + 4095) // 4096 + 1,), i32)
+        mk("err", (4,), i32)
+        mk("ovf_list", (nr * nc,), i32)
+        mk("ovf_cnt", (1,), i32)
+        mk("rnd_off", (nr * nc + 2,), i64)
+        self.E_tab = None
+        self.rnd_stream = None
+
+        e = self.e = LmEngine()
+        e.n_runs, e.nx, e.nz, e.nc = nr, self.nx, self.nz, nc
+        e.n_tot = self.n_tot
+        e.x_min, e.x_max = float(self.ranges[0, 0]), float(self.ranges[0, 1])
+        e.z_min, e.z_max = float(self.ranges[1, 0]), float(self.ranges[1, 1])
+        e.dx, e.dz = float(self.steps[0]), float(self.steps[1])
+        e.V_cell, e.p_ref, e.p_ref_inv = float(volume_cell), float(p_ref), float(p_ref_inv)
+        e.cur, e.ccur = 0, 0
+        for b in (0, 1):
+            for f in ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "cell", "seg"):
+                getattr(e, f)[b] = self.buf[f"{f}{b}"].data_ptr()
+        for f in ("dmxi", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
+                  "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
+                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off"):
+            setattr(e, f, self.buf[f].data_ptr())
+        e.rem_cap = rem_cap
+        e.e_const_mode = 1
+        e.E_const = 0.5
+        self.set_params()
+
+    # ------------------------------------------------------------------ setup
+    @property
+    def stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def set_params(self, solute="AS", no_iter_impl_mass=3, scale_dt_cond=5, no_col_per_adv=2,
+                   act_collisions=True, act_relaxation=True, dt=1.0, dt_col=None, g_set=9.80665, seed=0):
+        """Step parameters as config.py / init.set_config derive them (init.py:181-206, integration.py:1992)."""
+        e = self.e
+        e.solute = SOLUTES[solute]
+        e.no_iter = int(no_iter_impl_mass)
+        e.scale_dt_cond = int(scale_dt_cond)
+        e.no_col_per_adv = int(no_col_per_adv)
+        e.act_collisions = int(bool(act_collisions))
+        e.act_relaxation = int(bool(act_relaxation))
+        e.dt = float(dt)
+        e.dt_sub = dt / (2 * scale_dt_cond)
+        e.dt_sub_half = 0.5 * e.dt_sub
+        if dt_col is None:
+            dt_col = dt / no_col_per_adv
+        e.dt_col_over_dV = dt_col / e.V_cell
+        e.g_set = float(g_set)
+        e.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+
+    def set_collision_kernel(self, E_col_grid, R_kernel_low_log=None, bin_factor_R_log=None):
+        """Table E_col(R_i,R_j) (Ecol_grid_R) or a float (Ecol_const), as passed to the reference."""
+        e = self.e
+        if isinstance(E_col_grid, float):
+            e.e_const_mode, e.E_const, e.nb = 1, float(E_col_grid), 0
+            self.E_tab = None
+            e.E = None
+        else:
+            E = np.ascontiguousarray(E_col_grid, dtype=np.float64)
+            self.E_tab = torch.from_numpy(E).to(self.device)
+            e.E = self.E_tab.data_ptr()
+            e.nb = E.shape[0]
+            e.e_const_mode = 0
+            e.lnR0 = float(R_kernel_low_log)
+            e.lnf = float(bin_factor_R_log)
+
+    def set_flow(self, velocity, mass_flux_air_dry):
+        self.buf["grid_vel"].copy_(torch.from_numpy(np.ascontiguousarray(velocity, dtype=np.float64)))
+        self.buf["flux"].copy_(torch.from_numpy(np.ascontiguousarray(mass_flux_air_dry, dtype=np.float64)))
+
+    def set_relaxation(self, run, init_profile_r_v, init_profile_Theta, relaxation_time_profile):
+        self.buf["prof_rv"][run].copy_(torch.from_numpy(np.asarray(init_profile_r_v, dtype=np.float64)))
+        self.buf["prof_Th"][run].copy_(torch.from_numpy(np.asarray(init_profile_Theta, dtype=np.float64)))
+        self.buf["t_relax"].copy_(torch.from_numpy(np.asarray(relaxation_time_profile, dtype=np.float64)))
+
+    def set_grid(self, run, grid_scalar_fields, grid_mat_prop):
+        """(10,nx,nz) and (7,nx,nz) in the index order of integration.py:366-386."""
+        s = np.ascontiguousarray(grid_scalar_fields, dtype=np.float64).reshape(10, self.nc)
+        m = np.ascontiguousarray(grid_mat_prop, dtype=np.float64).reshape(7, self.nc)
+        self.buf["scal"][run].copy_(torch.from_numpy(s))
+        self.buf["mat"][run].copy_(torch.from_numpy(m))
+
+    def init_cells(self):
+        check(self.L.lm_engine_init_cells(C.byref(self.e), self.stream))
+
+    def load_particles(self, runs):
+        """runs: list of (pos, vel, cells, m_w, m_s, xi, active_ids) in the reference layout."""
+        assert len(runs) == self.n_runs
+        nc, nz = self.nc, self.nz
+        host = {f: np.zeros(max(self.n_tot, 1), dtype=np.float64) for f in ("x", "z", "u", "w", "m_w", "m_s", "xi")}
+        hid = np.zeros(max(self.n_tot, 1), dtype=np.int32)
+        hcell = np.zeros(max(self.n_tot, 1), dtype=np.int32)
+        seg = np.zeros(self.n_runs * (nc + 1) + 1, dtype=np.int32)
+        for r, (pos, vel, cells, m_w, m_s, xi, active) in enumerate(runs):
+            N = self.n_per_run[r]
+            assert len(xi) == N, "SIP count differs from the engine's allocation"
+            active = np.asarray(active, dtype=bool)
+            cells = np.asarray(cells, dtype=np.int64)
+            ids = np.arange(N)
+            key = cells[0] * nz + cells[1]
+            act = ids[active]
+            if np.any((cells[0][active] < 0) | (cells[0][active] >= self.nx) |
+                      (cells[1][active] < 0) | (cells[1][active] >= nz)):
+                raise ValueError("active SIP outside the grid")
+            order = act[np.lexsort((act, key[active]))]
+            dead = ids[~active]
+            perm = np.concatenate([order, dead])
+            b = int(self.base[r])
+            sl = slice(b, b + N)
+            host["x"][sl], host["z"][sl] = pos[0][perm], pos[1][perm]
+            host["u"][sl], host["w"][sl] = vel[0][perm], vel[1][perm]
+            host["m_w"][sl], host["m_s"][sl], host["xi"][sl] = m_w[perm], m_s[perm], xi[perm]
+            hid[sl] = perm
+            hcell[b:b + len(order)] = r * nc + key[order]
+            hcell[b + len(order):b + N] = -(2 + cells[0][dead])
+            counts = np.bincount(key[active], minlength=nc)
+            starts = b + np.concatenate([[0], np.cumsum(counts)])
+            seg[r * (nc + 1):r * (nc + 1) + nc + 1] = starts
+        seg[-1] = self.n_tot
+        cur = self.e.cur
+        for f, a in host.items():
+            self.buf[f"{f}{cur}"].copy_(torch.from_numpy(a[:max(self.n_tot, 1)]))
+        self.buf[f"id{cur}"].copy_(torch.from_numpy(hid))
+        self.buf[f"cell{self.e.ccur}"].copy_(torch.from_numpy(hcell))
+        self.buf[f"seg{cur}"].copy_(torch.from_numpy(seg))
+
+    def set_counters(self, run, no_cols=None, water_removed=None):
+        if no_cols is not None:
+            self.buf["no_cols"][run].copy_(torch.as_tensor(np.asarray(no_cols, dtype=np.int64)))
+        if water_removed is not None:
+            self.buf["water_removed"][run] = float(np.asarray(water_removed).reshape(-1)[0])
+
+    def set_rnd_stream(self, stream_values):
+        """Inject the uniform stream (1-D float64, consumed in the reference's order) or None for Philox."""
+        if stream_values is None:
+            self.rnd_stream = None
+            self.e.rnd_stream = None
+            return
+        self.rnd_stream = torch.as_tensor(np.asarray(stream_values, dtype=np.float64), device=self.device)
+        self.e.rnd_stream = self.rnd_stream.data_ptr()
+        self.buf["rnd_off"].zero_()
+
+    # ------------------------------------------------------------- stepping
+    def adv_steps(self, n):
+        check(self.L.lm_adv_steps(C.byref(self.e), int(n), self.stream))
+
+    def collision_step(self):
+        check(self.L.lm_collision_step(C.byref(self.e), self.stream))
+
+    def position_step(self, dt):
+        check(self.L.lm_position_step(C.byref(self.e), float(dt), self.stream))
+
+    def reindex(self):
+        check(self.L.lm_reindex(C.byref(self.e), self.stream))
+
+    def advection_increments(self, second):
+        check(self.L.lm_advection_increments(C.byref(self.e), int(second), self.stream))
+
+    def substep(self, dt_sub, dt_sub_pos):
+        check(self.L.lm_substep(C.byref(self.e), float(dt_sub), float(dt_sub_pos), self.stream))
+
+    def check(self):
+        check(self.L.lm_engine_check(C.byref(self.e), self.stream))
+
+    # --------------------------------------------------------------- export
+    def export_particles(self, run):
+        """Reference layout arrays of one run (pos, vel, cells, m_w, m_s, xi, active_ids)."""
+        self.check()
+        cur, ccur = self.e.cur, self.e.ccur
+        b, N = int(self.base[run]), self.n_per_run[run]
+        g = {f: self.buf[f"{f}{cur}"][b:b + N].cpu().numpy() for f in ("x", "z", "u", "w", "m_w", "m_s", "xi")}
+        ids = self.buf[f"id{cur}"][b:b + N].cpu().numpy().astype(np.int64)
+        cell = self.buf[f"cell{ccur}"][b:b + N].cpu().numpy().astype(np.int64)
+        pos = np.zeros((2, N))
+        vel = np.zeros((2, N))
+        cells = np.zeros((2, N), dtype=np.int64)
+        m_w, m_s, xi = np.zeros(N), np.zeros(N), np.zeros(N)
+        active = np.zeros(N, dtype=bool)
+        pos[0, ids], pos[1, ids] = g["x"], g["z"]
+        vel[0, ids], vel[1, ids] = g["u"], g["w"]
+        m_w[ids], m_s[ids], xi[ids] = g["m_w"], g["m_s"], g["xi"]
+        act = cell >= 0
+        loc = np.where(act, cell - run * self.nc, 0)
+        cells[0, ids] = np.where(act, loc // self.nz, -cell - 2)
+        cells[1, ids] = np.where(act, loc % self.nz, -1)
+        active[ids] = act
+        return pos, vel, cells, m_w, m_s, xi, active
+
+    def export_grid(self, run):
+        self.check()
+        s = self.buf["scal"][run].cpu().numpy().reshape(10, self.nx, self.nz).copy()
+        m = self.buf["mat"][run].cpu().numpy().reshape(7, self.nx, self.nz).copy()
+        return s, m
+
+    def counters(self, run):
+        self.check()
+        return (self.buf["no_cols"][run].cpu().numpy().copy(),
+                np.array([float(self.buf["water_removed"][run].item())]))
+
+
+def host_material_properties(T, p, r_v, rho_d):
+    """integration.py:359-400 on the host with the reference's libm (Python math), elementwise."""
+    T = np.asarray(T, dtype=np.float64)
+    out = np.zeros((7,) + T.shape)
+    flatT, flatp, flatr, flatd = T.ravel(), np.asarray(p).ravel(), np.asarray(r_v).ravel(), np.asarray(rho_d).ravel()
+    o = out.reshape(7, -1)
+    for k in range(flatT.size):
+        t, pp, rv, rd = float(flatT[k]), float(flatp[k]), float(flatr[k]), float(flatd[k])
+        o[0, k] = 4.1868E-3 * (5.69 + 0.017 * (t - 273.15))
+        o[1, k] = 4.01218E-5 * math.pow(t, 1.94) / pp
+        o[2, k] = 1.0E3 * (2500.82 - 2.358 * (t - 273.0))
+        tau = 1 - t / 647.096
+        o[3, k] = 0.2358 * math.pow(tau, 1.256) * (1 - 0.625 * tau)
+        o[4, k] = 1004.71 * (1 + 0.897 * rv)
+        o[5, k] = 1.458E-6 * math.pow(t, 1.5) / (t + 110.4)
+        o[6, k] = rd * (1 + rv)
+    return out

@@ -1,0 +1,152 @@
+"""GPU parity: gfx950 kernels (through the C ABI) vs the CPU oracle, bit-exact.
+
+Inputs are seeded synthetic ICMW cases small enough for the oracle to finish
+in seconds.  Integer / index results (cells, active flags, collision counts,
+ID order) and every FP64 state value must be identical.
+"""
+import numpy as np
+import pytest
+
+from lacmopy_amd import load_kernel_data_Ecol
+from lacmopy_amd.setup_case import init_profiles, make_case
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _config(E, lnR0, lnf, nb, **kw):
+    c = dict(solute_type="AS", dt_adv=1.0, dt_col=0.5, scale_dt_cond=5, no_col_per_adv=2, no_iter_impl_mass=3,
+             g_set=9.80665, act_collisions=True, act_relaxation=True, no_kernel_bins=nb, R_kernel_low_log=lnR0,
+             bin_factor_R_log=lnf)
+    c.update(kw)
+    return c
+
+
+def _case(nx=10, nz=10, spc=24, seed=5, **kw):
+    kw.setdefault("R_drop", 22.0)
+    kw.setdefault("sigma_drop", 1.6)
+    return make_case(nx=nx, nz=nz, spc=spc, seed=seed, cloud_base=nz * 20.0 * 0.4, cloud_top=nz * 20.0 * 0.7, **kw)
+
+
+def _engine(grid, parts, config, E, lnR0, lnf, seed, n_runs=1, rnd=None):
+    from lacmopy_amd.engine import Engine
+    prof = init_profiles(grid)
+    N = len(parts[3])
+    eng = Engine(int(grid.no_cells[0]), int(grid.no_cells[1]), [N] * n_runs, grid.ranges, grid.steps,
+                 grid.volume_cell)
+    eng.set_params(solute=config["solute_type"], no_iter_impl_mass=config["no_iter_impl_mass"],
+                   scale_dt_cond=config["scale_dt_cond"], no_col_per_adv=config["no_col_per_adv"],
+                   act_collisions=config["act_collisions"], act_relaxation=config["act_relaxation"],
+                   dt=config["dt_adv"], dt_col=config["dt_col"], g_set=config["g_set"], seed=seed)
+    eng.set_collision_kernel(E, lnR0, lnf)
+    eng.set_flow(grid.velocity, grid.mass_flux_air_dry)
+    for r in range(n_runs):
+        eng.set_relaxation(r, *prof)
+        eng.set_grid(r, grid.scalar_fields(), grid.mat_prop())
+    eng.init_cells()
+    eng.load_particles([parts] * n_runs)
+    eng.set_rnd_stream(rnd)
+    return eng
+
+
+def _oracle_run(grid, parts, config, E, seed, n_steps, rnd=None, libm=False):
+    pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
+    N = len(xi)
+    scal, mat = grid.scalar_fields(), grid.mat_prop()
+    prof_rv, prof_Th, t_relax = init_profiles(grid)
+    rel_pos = np.zeros((2, N))
+    T_p = np.ones(N)
+    wr = np.zeros(1)
+    no_cols = np.zeros(2, dtype=np.int64)
+    p = orc.adv_params(grid, N, config, seed=seed, E_col_grid=E)
+    orc.integrate_adv_steps(p, n_steps, pos, vel, cells, rel_pos, m_w, m_s, xi, active, T_p, wr, no_cols, scal,
+                            mat, grid.velocity, grid.mass_flux_air_dry, E, prof_rv, prof_Th, t_relax,
+                            rnd_stream=rnd, libm=libm)
+    return (pos, vel, cells, m_w, m_s, xi, active), scal, mat, wr, no_cols
+
+
+def _prep(grid, parts):
+    """simulate() starts with update_cells_and_rel_pos (integration.py:1932-1933)."""
+    pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
+    N = len(xi)
+    p = orc.adv_params(grid, N, _config(0.5, 0, 1, 0))
+    rel = np.zeros((2, N))
+    orc.lib().or_update_pos_and_cells(orc.C.byref(p), 0.0, orc.P(pos), orc.P(vel), orc.P(cells), orc.P(rel),
+                                      orc.P(m_w), orc.P(xi), orc.P(active.view(np.uint8)), orc.P(np.zeros(1)))
+    return pos, vel, cells, m_w, m_s, xi, active
+
+
+def _assert_same(a, b, name):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, name
+    if a.dtype.kind == "f":
+        bad = ~((a == b) | (np.isnan(a) & np.isnan(b)))
+        if bad.any():
+            k = np.argwhere(bad)[0]
+            rel = np.max(np.abs(a[bad] - b[bad]) / np.maximum(np.abs(b[bad]), 1e-300))
+            raise AssertionError(f"{name}: {bad.sum()} values differ (first at {k}: {a[tuple(k)]!r} vs "
+                                 f"{b[tuple(k)]!r}; max rel {rel:.3e})")
+    else:
+        assert np.array_equal(a, b), f"{name}: integer arrays differ"
+
+
+@pytest.mark.parametrize("n_steps", [1, 3])
+def test_adv_steps_bit_exact(gpu, n_steps):
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case()
+    parts = _prep(grid, parts)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 11, n_steps)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
+    eng.adv_steps(n_steps)
+    got = eng.export_particles(0)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], got, ref_parts):
+        _assert_same(a, b, nm)
+    s, m = eng.export_grid(0)
+    _assert_same(s[:8], scal[:8], "grid_scalar_fields")
+    _assert_same(m, mat, "grid_mat_prop")
+    nc_got, wr_got = eng.counters(0)
+    _assert_same(nc_got, no_cols, "no_cols")
+    _assert_same(wr_got, wr, "water_removed")
+    assert no_cols.sum() > 0, "case should produce collisions"
+
+
+def test_collision_step_injected_stream(gpu):
+    """All-cells collision step with the reference's draw stream injected (x-major cells, row-major pairs)."""
+    from lacmopy_amd.engine import Engine  # noqa: F401
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Hall_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(nx=6, nz=12, spc=40, seed=9)
+    parts = _prep(grid, parts)
+    pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
+    rng = np.random.default_rng(3)
+    rnd = rng.random(2_000_000)
+    no_cols = np.zeros(2, dtype=np.int64)
+    orc.collision_all_cells(xi, m_w, m_s, vel, grid.temperature.copy(), cells, grid.no_cells, 0.5 / grid.volume_cell,
+                            E, nb, lnR0, lnf, no_cols, rnd_stream=rnd)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 0, rnd=rnd)
+    eng.collision_step()
+    got = eng.export_particles(0)
+    _assert_same(got[5], xi, "xi")
+    _assert_same(got[3], m_w, "m_w")
+    _assert_same(got[4], m_s, "m_s")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    assert no_cols.sum() > 0
+
+
+def test_batched_runs_match_single(gpu):
+    """Runs of a batch are independent: run r of a batch == a single run with the same run index."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(nx=8, nz=8, spc=20, seed=21)
+    parts = _prep(grid, parts)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 7, n_runs=3)
+    eng.adv_steps(2)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 7, 2)
+    # run 0 of the batch uses Philox run word 0, like the oracle
+    got = eng.export_particles(0)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], got, ref_parts):
+        _assert_same(a, b, nm)
+    # other runs differ only by their draws
+    nc1 = eng.counters(1)[0]
+    assert nc1.sum() > 0
