@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: SIP-steps/s of the 2-D ICMW 75x75 case on one MI355X.
+
+Workload (synthetic inputs, lacmopy_amd.setup_case): 75x75 cells, 20 m,
+Poisson(128) SIPs per cell, bimodal AS aerosol, Long E_col table, dt_adv = 1 s,
+10 condensation substeps, 2 collision steps per advection step, relaxation on.
+  * batch: 64 independent seeds in one engine (the headline `value`)
+  * plain: one run (reported next to it; ratio = batch / plain)
+A "step" is one advection step of every SIP of every run.
+SIP-steps/s = sum over runs of active SIPs x timed steps / wall seconds, with
+all inputs resident in HBM; the timed region is bracketed by
+torch.cuda.synchronize().
+
+Also measured: the dominant kernel's duration with HIP events (roofline vs
+HBM 8 TB/s, algorithmic bytes per SIP), and the CPU oracle on a bounded sample
+of the plain case (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_SUBSTEP = 104.0    # algorithmic bytes per SIP per particle substep (60 read + 44 write)
+B_COLLISION = 64.0   # per SIP per collision step (40 read + 24 write)
+B_REINDEX = 140.0    # per SIP per reindex
+B_STEP = 10 * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX   # 1308 B per SIP-step
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_engine(n_runs, spc, seed0, nx=75, nz=75, kernel="Long_Bott"):
+    import torch  # noqa: F401
+    from lacmopy_amd import load_kernel_data_Ecol
+    from lacmopy_amd.engine import Engine
+    from lacmopy_amd.setup_case import init_profiles, make_case
+    E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
+    cases = []
+    for r in range(n_runs):
+        grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=nx, nz=nz, spc=spc, seed=seed0 + 2 * r)
+        cases.append((grid, (pos, vel, cells, m_w, m_s, xi, act)))
+    grid0 = cases[0][0]
+    eng = Engine(nx, nz, [len(c[1][5]) for c in cases], grid0.ranges, grid0.steps, grid0.volume_cell)
+    eng.set_params(solute="AS", no_iter_impl_mass=3, scale_dt_cond=5, no_col_per_adv=2, act_collisions=True,
+                   act_relaxation=True, dt=1.0, dt_col=0.5, g_set=9.80665, seed=seed0)
+    eng.set_collision_kernel(E, lnR0, lnf)
+    eng.set_flow(grid0.velocity, grid0.mass_flux_air_dry)
+    for r, (g, _) in enumerate(cases):
+        eng.set_relaxation(r, *init_profiles(g))
+        eng.set_grid(r, g.scalar_fields(), g.mat_prop())
+    eng.init_cells()
+    eng.load_particles([c[1] for c in cases])
+    n_sips = sum(len(c[1][5]) for c in cases)
+    return eng, n_sips, cases
+
+
+def time_steps(eng, steps, warmup):
+    import torch
+    eng.adv_steps(warmup)
+    torch.cuda.synchronize()
+    eng.check()
+    t0 = time.perf_counter()
+    eng.adv_steps(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.check()
+    return t1 - t0
+
+
+def kernel_times(eng, steps):
+    """HIP-event durations of the step's kernels over `steps` profiled advection steps.
+
+    The events are recorded by liblacmo_amd on the stream the kernels run on.
+    Returns {kind: (mean ms per launch, launches)}.
+    """
+    import torch
+    eng.prof_enable(max_events=64 * steps + 64)
+    eng.adv_steps(steps)
+    torch.cuda.synchronize()
+    res = eng.prof_collect()
+    eng.prof_disable()
+    eng.check()
+    return {k: (ms / max(n, 1), n) for k, (ms, n) in res.items()}
+
+
+def cpu_baseline(seed0, spc):
+    """Oracle (C restatement, 1 thread) on the plain 75x75 case: bounded sample of adv steps."""
+    from lacmopy_amd import load_kernel_data_Ecol
+    from lacmopy_amd.setup_case import init_profiles, make_case
+    from oracle import oracle as orc
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=75, nz=75, spc=spc, seed=seed0)
+    N = len(xi)
+    cfg = dict(solute_type="AS", dt_adv=1.0, dt_col=0.5, scale_dt_cond=5, no_col_per_adv=2, no_iter_impl_mass=3,
+               g_set=9.80665, act_collisions=True, act_relaxation=True, no_kernel_bins=nb, R_kernel_low_log=lnR0,
+               bin_factor_R_log=lnf)
+    p = orc.adv_params(grid, N, cfg, seed=seed0)
+    scal, mat = grid.scalar_fields(), grid.mat_prop()
+    prof = init_profiles(grid)
+    rel = np.zeros((2, N))
+    T_p = np.ones(N)
+    wr = np.zeros(1)
+    nco = np.zeros(2, dtype=np.int64)
+    q = orc.adv_params(grid, N, cfg)
+    orc.lib().or_update_pos_and_cells(orc.C.byref(q), 0.0, orc.P(pos), orc.P(vel), orc.P(cells), orc.P(rel),
+                                      orc.P(m_w), orc.P(xi), orc.P(act.view(np.uint8)), orc.P(wr))
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.integrate_adv_steps(p, 1, pos, vel, cells, rel, m_w, m_s, xi, act, T_p, wr, nco, scal, mat,
+                                grid.velocity, grid.mass_flux_air_dry, E, prof[0], prof[1], prof[2])
+        steps += 1
+        if time.perf_counter() - t0 > 10.0 or steps >= 3:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": N * steps / dt, "unit": "SIP-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} advection steps of the plain 75x75 case ({N} SIPs), oracle C restatement, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--runs", type=int, default=64)
+    ap.add_argument("--spc", type=float, default=128.0)
+    ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-kernels", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): independent seeds shard "
+              "trivially across GPUs and no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "needs a GPU"
+
+    log(f"building batch of {args.runs} runs ...")
+    t = time.perf_counter()
+    eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed)
+    log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
+    secs = time_steps(eng, args.steps, args.warmup)
+    batch_rate = n_sips * args.steps / secs
+    log(f"batch: {secs / args.steps * 1e3:.2f} ms/step, {batch_rate:.4g} SIP-steps/s")
+    ktimes = {} if args.no_kernels else kernel_times(eng, max(1, min(args.steps, 3)))
+    log(f"kernel times (ms): {ktimes}")
+    n_batch_sips = n_sips
+    del eng
+    torch.cuda.empty_cache()
+
+    plain = None
+    if not args.no_plain:
+        eng1, n1, _ = build_engine(1, args.spc, args.seed)
+        s1 = time_steps(eng1, max(args.steps, 10), args.warmup)
+        plain = {"sip_steps_per_s": n1 * max(args.steps, 10) / s1, "ms_per_step": s1 / max(args.steps, 10) * 1e3,
+                 "sips": n1}
+        log(f"plain: {plain}")
+        del eng1
+
+    roof = None
+    if ktimes:
+        # dominant per-SIP kernel: the particle substep kernel (one thread per SIP)
+        ms = ktimes["particles"][0]
+        achieved = n_batch_sips * B_SUBSTEP / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_particles",
+                "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
+
+    cpu = None if args.no_cpu else cpu_baseline(args.seed, args.spc)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    res = {
+        "metric": "SIP-steps/s, 2D ICMW 75x75: 64-seed batch vs plain GPU",
+        "value": batch_rate,
+        "unit": "SIP-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"ICMW 2D kinematic Sc, 75x75 cells, Poisson({args.spc:g}) SIPs/cell, "
+                               f"{args.runs} seeds batched, Long E_col, dt=1 s, 10 substeps, 2 collision steps",
+                   "runs": args.runs, "sips_total": n_batch_sips, "cells": 75 * 75,
+                   "plain_single_run": plain,
+                   "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
+                   "kernel_ms": ktimes},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -88,6 +88,8 @@ typedef struct LmEngine {
   const double *rnd_stream;
   int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */
   int64_t rnd_pos;  /* stream position consumed so far */
+  /* optional kernel timing (lm_prof_*); NULL = off */
+  void *prof;
 } LmEngine;
 
 /* Library identity: returns a static version string. */
@@ -143,6 +145,18 @@ int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream);
 
 /* n_steps advection steps (integration.py:1298-1611, integrate_adv_step_np). */
 int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream);
+
+/*
+ * Kernel timing with HIP events on the launch stream (no reference
+ * counterpart; used by bench.py).  While enabled, every launch of the
+ * particle kernel (kind 0), the collision kernel (1), the Delta m_l kernel (2)
+ * and the reindex scatter (3) is bracketed by events; lm_prof_collect
+ * synchronises and returns the summed milliseconds and launch counts per
+ * kind, then resets.  At most max_events launches are recorded.
+ */
+int lm_prof_enable(LmEngine *e, int32_t max_events);
+int lm_prof_collect(LmEngine *e, double *ms, int32_t *count, int32_t nkinds);
+int lm_prof_disable(LmEngine *e);
 
 /* Synchronises the stream and reports the device error word (0 = fine). */
 int lm_engine_check(LmEngine *e, void *stream);

@@ -51,6 +51,7 @@ class LmEngine(C.Structure):
         ("rem_cap", _i32), ("pad0", _i32),
         ("scan_buf", _p), ("scan_part", _p), ("err", _p), ("ovf_list", _p), ("ovf_cnt", _p),
         ("rnd_stream", _p), ("rnd_off", _p), ("rnd_pos", _i64),
+        ("prof", _p),
     ]
 
 
@@ -62,6 +63,7 @@ EXPORTED = [
     "lm_get_constants", "lm_num_constants", "lm_engine_init_cells", "lm_collision_step",
     "lm_position_step", "lm_reindex", "lm_advection_increments", "lm_substep",
     "lm_adv_steps", "lm_engine_check", "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe",
+    "lm_prof_enable", "lm_prof_collect", "lm_prof_disable",
 ]
 
 _lib = None
@@ -99,8 +101,12 @@ def lib():
     L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
                                    C.c_uint64, C.c_uint32, _p, _p]
     L.lm_math_probe.argtypes = [_i32, _i64, _p, _p, _p, _p]
+    L.lm_prof_enable.argtypes = [E, _i32]
+    L.lm_prof_collect.argtypes = [E, C.POINTER(_d), C.POINTER(_i32), _i32]
+    L.lm_prof_disable.argtypes = [E]
     for name in ["lm_position_step", "lm_advection_increments", "lm_substep", "lm_adv_steps",
-                 "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe"]:
+                 "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe", "lm_prof_enable", "lm_prof_collect",
+                 "lm_prof_disable"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -1175,6 +1175,29 @@ __global__ void k_math_probe(int which, long long n, const double *x, const doub
 /* ------------------------------------------------------------------------ */
 /* C ABI                                                                    */
 /* ------------------------------------------------------------------------ */
+/* event-pair timing of selected kernels (bench.py) */
+struct LmProf {
+  int max_ev, n;
+  hipEvent_t *a, *b;
+  int *kind;
+};
+
+struct ProfScope {
+  LmProf *p;
+  int idx;
+  hipStream_t s;
+  ProfScope(const LmEngine *e, int kind, hipStream_t st) : p((LmProf *)e->prof), idx(-1), s(st) {
+    if (p && p->n < p->max_ev) {
+      idx = p->n++;
+      p->kind[idx] = kind;
+      hipEventRecord(p->a[idx], s);
+    }
+  }
+  ~ProfScope() {
+    if (idx >= 0) hipEventRecord(p->b[idx], s);
+  }
+};
+
 static int check_engine(const LmEngine *e) {
   if (!e) return set_err(LM_ERR_ARG, "engine is NULL");
   if (e->n_runs <= 0 || e->nx <= 0 || e->nz <= 0 || e->nc != e->nx * e->nz)
@@ -1245,7 +1268,10 @@ int lm_collision_step(LmEngine *e, void *stream) {
   HIP_TRY(hipMemsetAsync(e->ovf_cnt, 0, sizeof(int32_t), S(stream)));
   KView v = make_view(e);
   const LmPhys &P = phys();
-  hipLaunchKernelGGL(k_collision<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, P, 0);
+  {
+    ProfScope ps(e, 1, S(stream));
+    hipLaunchKernelGGL(k_collision<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, P, 0);
+  }
   hipLaunchKernelGGL(k_collision<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, P, 1);
   LAUNCH_CHECK();
   e->col_step += 1;
@@ -1275,7 +1301,10 @@ int lm_reindex(LmEngine *e, void *stream) {
   hipLaunchKernelGGL(k_scan_local, dim3(nchunk), dim3(SCAN_BLOCK), 0, S(stream), e->scan_buf, e->scan_part, L);
   hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(SCAN_BLOCK), 0, S(stream), e->scan_part, nchunk, e->scan_buf, L);
   hipLaunchKernelGGL(k_scan_add, dim3(nchunk), dim3(SCAN_BLOCK), 0, S(stream), e->scan_buf, e->scan_part, L);
-  hipLaunchKernelGGL(k_ri_scatter, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
+  {
+    ProfScope ps(e, 3, S(stream));
+    hipLaunchKernelGGL(k_ri_scatter, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
+  }
   LAUNCH_CHECK();
   e->cur = 1 - e->cur;
   e->ccur = 1 - e->ccur;
@@ -1304,10 +1333,15 @@ int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
   const LmPhys &P = phys();
   const long long n = e->n_tot;
   const int ncell = e->n_runs * e->nc;
-  if (n > 0)
+  if (n > 0) {
+    ProfScope ps(e, 0, S(stream));
     hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, P, n, dt_sub,
                        dt_sub_pos, 0);
-  hipLaunchKernelGGL(k_dml, dim3(ncell), dim3(WAVE), 0, S(stream), v);
+  }
+  {
+    ProfScope ps(e, 2, S(stream));
+    hipLaunchKernelGGL(k_dml, dim3(ncell), dim3(WAVE), 0, S(stream), v);
+  }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v, P);
   LAUNCH_CHECK();
@@ -1343,6 +1377,58 @@ int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream) {
     if ((rc = subloop(e, e->scale_dt_cond - 1, ncol, stream))) return rc;
     if ((rc = lm_substep(e, e->dt_sub, e->dt_sub_half, stream))) return rc;
   }
+  return 0;
+}
+
+int lm_prof_enable(LmEngine *e, int32_t max_events) {
+  if (!e || max_events <= 0) return set_err(LM_ERR_ARG, "bad prof arguments");
+  lm_prof_disable(e);
+  LmProf *p = new LmProf;
+  p->max_ev = max_events;
+  p->n = 0;
+  p->a = new hipEvent_t[max_events];
+  p->b = new hipEvent_t[max_events];
+  p->kind = new int[max_events];
+  for (int i = 0; i < max_events; ++i) {
+    HIP_TRY(hipEventCreate(&p->a[i]));
+    HIP_TRY(hipEventCreate(&p->b[i]));
+  }
+  e->prof = p;
+  return 0;
+}
+
+int lm_prof_collect(LmEngine *e, double *ms, int32_t *count, int32_t nkinds) {
+  if (!e || !e->prof) return set_err(LM_ERR_ARG, "profiling not enabled");
+  LmProf *p = (LmProf *)e->prof;
+  for (int k = 0; k < nkinds; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
+  }
+  for (int i = 0; i < p->n; ++i) {
+    HIP_TRY(hipEventSynchronize(p->b[i]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, p->a[i], p->b[i]));
+    if (p->kind[i] < nkinds) {
+      ms[p->kind[i]] += t;
+      count[p->kind[i]] += 1;
+    }
+  }
+  p->n = 0;
+  return 0;
+}
+
+int lm_prof_disable(LmEngine *e) {
+  if (!e || !e->prof) return 0;
+  LmProf *p = (LmProf *)e->prof;
+  for (int i = 0; i < p->max_ev; ++i) {
+    hipEventDestroy(p->a[i]);
+    hipEventDestroy(p->b[i]);
+  }
+  delete[] p->a;
+  delete[] p->b;
+  delete[] p->kind;
+  delete p;
+  e->prof = nullptr;
   return 0;
 }
 

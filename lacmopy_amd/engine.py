@@ -243,6 +243,20 @@ class Engine:
     def check(self):
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
 
+    KINDS = ("particles", "collision", "dml", "reindex_scatter")
+
+    def prof_enable(self, max_events=4096):
+        check(self.L.lm_prof_enable(C.byref(self.e), int(max_events)))
+
+    def prof_collect(self):
+        ms = (C.c_double * 4)()
+        cnt = (C.c_int32 * 4)()
+        check(self.L.lm_prof_collect(C.byref(self.e), ms, cnt, 4))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KINDS)}
+
+    def prof_disable(self):
+        check(self.L.lm_prof_disable(C.byref(self.e)))
+
     # --------------------------------------------------------------- export
     def export_particles(self, run):
         """Reference layout arrays of one run (pos, vel, cells, m_w, m_s, xi, active_ids)."""
