@@ -27,7 +27,7 @@
 #include "lm_physics.h"
 
 #define WAVE 64
-#define DML_CAP 1024 /* member capacity of the gather kernels (Delta m_l, reindex) */
+#define DML_CAP 512 /* member capacity of the gather kernels (Delta m_l, reindex) */
 
 /* ------------------------------------------------------------------------ */
 /* host-side constants (same expressions as the reference modules)          */
@@ -84,6 +84,21 @@ static const LmPhys &phys() {
   return P;
 }
 
+/* physics constants in constant memory (scalar loads, no per-thread copies) */
+__constant__ LmPhys c_P;
+
+/* uploads c_P once per device (before the first kernel launch on it) */
+static int ensure_phys() {
+  static unsigned done_mask = 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (dev < 32 && (done_mask & (1u << dev))) return 0;
+  const LmPhys &P = phys();
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_P), &P, sizeof(LmPhys)) != hipSuccess) return -1;
+  if (dev < 32) done_mask |= (1u << dev);
+  return 0;
+}
+
 static thread_local char g_errmsg[512] = "";
 
 static int set_err(int code, const char *msg) {
@@ -116,54 +131,89 @@ __device__ static inline void raise_err(int32_t *err, int code, int detail) {
   if (atomicCAS(&err[0], 0, code) == 0) err[1] = detail;
 }
 
-/* distinct cells of the 3x3 neighbourhood (x periodic, z bounded) */
-__device__ static inline int nbr_cells(int i, int j, int nx, int nz, int *out) {
-  int n = 0;
-  for (int di = -1; di <= 1; ++di) {
-    int ii = i + di;
-    if (ii < 0) ii += nx;
-    if (ii >= nx) ii -= nx;
-    for (int dj = -1; dj <= 1; ++dj) {
-      const int jj = j + dj;
-      if (jj < 0 || jj >= nz) continue;
-      const int c = ii * nz + jj;
-      bool dup = false;
-      for (int t = 0; t < n; ++t) dup |= (out[t] == c);
-      if (!dup) out[n++] = c;
-    }
-  }
-  return n;
-}
-
 __device__ static inline unsigned long long lane_mask_lt() {
   const int lane = threadIdx.x & (WAVE - 1);
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-/*
- * Wavefront gather of the members of global cell gc: slots k of the 3x3 home
- * segments with cell[k] == gc.  Stores up to cap (slot, id) pairs in LDS and
- * returns the member count (all lanes).  Must be called by a full wave.
- */
-__device__ static int gather_members(int gc, int nc, int nx, int nz, const int32_t *seg_run, const int32_t *cell,
-                                     const int32_t *ids, int *s_slot, int *s_id, int cap) {
+/* Segment table of the 3x3 neighbourhood of a cell, in LDS (x periodic, z bounded). */
+struct NbrTab {
+  int start[9];
+  int pre[10]; /* prefix of segment lengths */
+  int off[10]; /* members before sub-list t (after the gather) */
+};
+
+/* Lanes 0..8 fill the table; the caller synchronises afterwards. */
+__device__ static inline void nbr_table(int gc, int nc, int nx, int nz, const int32_t *seg_run, NbrTab *tb) {
+  const int lane = threadIdx.x;
   const int c = gc % nc;
   const int i = c / nz, j = c % nz;
-  int nb[9];
-  const int nn = nbr_cells(i, j, nx, nz, nb);
+  if (lane < 9) {
+    const int di = lane / 3 - 1, dj = lane % 3 - 1;
+    int ii = i + di;
+    if (ii < 0) ii += nx;
+    if (ii >= nx) ii -= nx;
+    const int jj = j + dj;
+    /* duplicates of the periodic wrap when nx < 3 are dropped */
+    const bool valid = jj >= 0 && jj < nz && !(nx == 1 && di != 0) && !(nx == 2 && di == 1);
+    const int cc = ii * nz + jj;
+    const int s0 = valid ? seg_run[cc] : 0;
+    tb->start[lane] = s0;
+    tb->pre[lane + 1] = valid ? seg_run[cc + 1] - s0 : 0;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    tb->pre[0] = 0;
+    for (int t = 0; t < 9; ++t) tb->pre[t + 1] += tb->pre[t];
+  }
+  __syncthreads();
+}
+
+/*
+ * Wavefront gather of the members of global cell gc: slots k of the 3x3 home
+ * segments with cell[k] == gc.  All segment loads are issued together
+ * (flat index over the concatenated segments, 4 per lane per round).  Stores
+ * up to cap (slot, id, sub-list) triples in LDS in segment order, so each
+ * sub-list is ascending in id.  Returns the member count (all lanes).
+ */
+__device__ static int gather_members(int gc, const int32_t *cell, const int32_t *ids, const NbrTab *tb, int *s_slot,
+                                     int *s_id, unsigned char *s_sub, int cap) {
   const int lane = threadIdx.x;
+  int pre[10];
+#pragma unroll
+  for (int t = 0; t < 10; ++t) pre[t] = tb->pre[t];
+  const int T = pre[9];
   int cnt = 0;
-  for (int t = 0; t < nn; ++t) {
-    const int s0 = seg_run[nb[t]], s1 = seg_run[nb[t] + 1];
-    for (int base = s0; base < s1; base += WAVE) {
-      const int k = base + lane;
-      const bool m = (k < s1) && (cell[k] == gc);
-      const unsigned long long bal = __ballot(m);
-      if (m) {
+  for (int base = 0; base < T; base += 4 * WAVE) {
+    int kk[4];
+    int tt[4];
+    bool m[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = base + q * WAVE + lane;
+      int t = 0, pt = 0;
+#pragma unroll
+      for (int u = 1; u < 9; ++u) {
+        const bool ge = f >= pre[u];
+        t += ge;
+        pt = ge ? pre[u] : pt;
+      }
+      tt[q] = t;
+      kk[q] = f < T ? tb->start[t] + (f - pt) : -1;
+    }
+    int cv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cv[q] = kk[q] >= 0 ? cell[kk[q]] : -1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      m[q] = kk[q] >= 0 && cv[q] == gc;
+      const unsigned long long bal = __ballot(m[q]);
+      if (m[q]) {
         const int pos = cnt + __popcll(bal & lane_mask_lt());
         if (pos < cap) {
-          s_slot[pos] = k;
-          s_id[pos] = ids[k];
+          s_slot[pos] = kk[q];
+          s_id[pos] = ids[kk[q]];
+          s_sub[pos] = (unsigned char)tt[q];
         }
       }
       cnt += __popcll(bal);
@@ -172,12 +222,78 @@ __device__ static int gather_members(int gc, int nc, int nx, int nz, const int32
   return cnt;
 }
 
-/* rank of every member by id (ids unique within a run) */
-__device__ static void rank_members(int n, const int *s_id, int *s_rank) {
-  for (int m = threadIdx.x; m < n; m += WAVE) {
+/* Member count only (no LDS output). */
+__device__ static int count_members(int gc, const int32_t *cell, const NbrTab *tb) {
+  const int lane = threadIdx.x;
+  int pre[10];
+#pragma unroll
+  for (int t = 0; t < 10; ++t) pre[t] = tb->pre[t];
+  const int T = pre[9];
+  int cnt = 0;
+  for (int base = 0; base < T; base += 4 * WAVE) {
+    int cv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int f = base + q * WAVE + lane;
+      int t = 0, pt = 0;
+#pragma unroll
+      for (int u = 1; u < 9; ++u) {
+        const bool ge = f >= pre[u];
+        t += ge;
+        pt = ge ? pre[u] : pt;
+      }
+      cv[q] = f < T ? cell[tb->start[t] + (f - pt)] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cnt += __popcll(__ballot(cv[q] == gc));
+  }
+  return cnt;
+}
+
+/*
+ * Rank of every member by id (ids unique within a run).  Each sub-list is
+ * ascending, so rank = index in its own sub-list + lower bounds in the
+ * others (binary search): O(n log n) instead of O(n^2).
+ */
+__device__ static void rank_members(int n, const int *s_id, const unsigned char *s_sub, NbrTab *tb, int *s_rank) {
+  const int lane = threadIdx.x;
+  if (lane < 10) {
+    /* off[t] = first member whose sub-list >= t (s_sub is non-decreasing) */
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)s_sub[mid] < lane)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    tb->off[lane] = lane == 9 ? n : lo;
+  }
+  __syncthreads();
+  int off[10];
+#pragma unroll
+  for (int t = 0; t < 10; ++t) off[t] = tb->off[t];
+  for (int m = lane; m < n; m += WAVE) {
     const int idm = s_id[m];
+    const int tm = s_sub[m];
     int rk = 0;
-    for (int q = 0; q < n; ++q) rk += (s_id[q] < idm);
+#pragma unroll
+    for (int u = 0; u < 9; ++u) {
+      int lo = off[u], hi = off[u + 1];
+      if (u == tm) {
+        rk += m - off[u];
+        continue;
+      }
+      const int l0 = lo;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_id[mid] < idm)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      rk += lo - l0;
+    }
     s_rank[m] = rk;
   }
 }
@@ -316,7 +432,8 @@ __device__ static void fill_cc(const LmPhys &P, const double *scal_r, const doub
   C[15] = 0.0;
 }
 
-__global__ void k_init_cc(KView v, LmPhys P) {
+__global__ void k_init_cc(KView v) {
+  const LmPhys &P = c_P;
   const int gc = blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= v.n_runs * v.nc) return;
   const int r = gc / v.nc, c = gc % v.nc;
@@ -326,8 +443,16 @@ __global__ void k_init_cc(KView v, LmPhys P) {
 /* ------------------------------------------------------------------------ */
 /* collision step: one wavefront per cell, anti-diagonal pair schedule       */
 /* ------------------------------------------------------------------------ */
+/*
+ * LDS holds what every pair reads (xi, R, u, w, kernel index) in id order;
+ * m_w, m_s and the pre-step density live in global memory (slot-indexed) and
+ * are touched only by collision events.  Lane l takes the rows i = l (mod 64)
+ * of the pair loop, so a lane walks one row through consecutive diagonals
+ * and consecutive pair numbers: one Philox block serves two diagonals.
+ */
 template <int CAP>
-__global__ __launch_bounds__(WAVE) void k_collision(KView v, LmPhys P, int from_ovf) {
+__global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
+  const LmPhys &P = c_P;
   int gc;
   if (from_ovf) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && *v.ovf_cnt > (int)gridDim.x) raise_err(v.err, LM_ERR_CELL_CAP, -1);
@@ -340,10 +465,13 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, LmPhys P, int from_
   const int r = gc / nc, c = gc % nc;
   const int lane = threadIdx.x;
 
-  __shared__ double sX[CAP], sMW[CAP], sMS[CAP], sR[CAP], sRHO[CAP], sU[CAP], sW[CAP];
+  __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
   __shared__ int sIND[CAP], sSL[CAP], sSlot[CAP], sId[CAP], sRank[CAP];
+  __shared__ unsigned char sSub[CAP];
+  __shared__ NbrTab tb;
 
-  const int n = gather_members(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.cell, v.id, sSlot, sId, CAP);
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
   if (n > CAP) {
     if (lane == 0) {
       if (from_ovf) {
@@ -357,24 +485,22 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, LmPhys P, int from_
   }
   if (n < 2) return;
   __syncthreads();
-  rank_members(n, sId, sRank);
+  rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
 
   const double *C = v.cc + (size_t)gc * CC_N;
   const double T = C[CC_T], rho_w_T = C[CC_RHOW];
+  double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
   for (int m = lane; m < n; m += WAVE) {
     const int k = sSlot[m], p = sRank[m];
-    const double mw = v.m_w[k], ms = v.m_s[k];
     sX[p] = v.xi[k];
-    sMW[p] = mw;
-    sMS[p] = ms;
     sU[p] = v.u[k];
     sW[p] = v.w[k];
     sSL[p] = k;
     double R, ws, rho;
-    lmp_R_ws_rho(P, v.solute, mw, ms, T, rho_w_T, R, ws, rho);
+    lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
     sR[p] = R;
-    sRHO[p] = rho;
+    rho_g[k] = rho;
     sIND[p] = v.e_const_mode ? 0 : lmp_kernel_index(R, v.lnR0, v.lnf, v.nb);
   }
   __syncthreads();
@@ -382,84 +508,110 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, LmPhys P, int from_
   const double dtdV = v.dt_col_over_dV;
   const long long nn = n;
   const long long roff = v.rnd_stream ? v.rnd_off[gc] : 0;
+  const uint32_t key_run = (uint32_t)r, key_step = v.col_step, key_cell = (uint32_t)c;
   long long c_single = 0, c_multi = 0, c_eq = 0;
+  /* Philox block cache, one per row class (i >> 6) & 1 */
+  long long tag0 = -1, tag1 = -1;
+  double u00 = 0.0, u01 = 0.0, u10 = 0.0, u11 = 0.0;
   for (int d = 1; d <= 2 * n - 3; ++d) {
     const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
     const int i_hi = (d - 1) >> 1;
-    for (int i = i_lo + lane; i <= i_hi; i += WAVE) {
+    for (int i = i_lo + ((lane - i_lo) & (WAVE - 1)); i <= i_hi; i += WAVE) {
       const int j = d - i;
+      const long long cnt = (long long)i * nn - (long long)i * (i + 1) / 2 + (j - i - 1);
+      /* uniform of this pair (independent of the state: issued before the state reads) */
+      double rnd;
+      if (v.rnd_stream) {
+        rnd = v.rnd_stream[roff + cnt];
+      } else {
+        const long long tag = cnt >> 1;
+        const bool cls = (i >> 6) & 1;
+        if (tag != (cls ? tag1 : tag0)) {
+          lm_u32x4 cw;
+          cw.v[0] = (uint32_t)tag;
+          cw.v[1] = key_cell;
+          cw.v[2] = key_step;
+          cw.v[3] = key_run;
+          const lm_u32x4 o = lm_philox4x32_10(cw, (uint32_t)v.seed, (uint32_t)(v.seed >> 32));
+          const double a = lm_res53(o.v[0], o.v[1]), b = lm_res53(o.v[2], o.v[3]);
+          if (cls) {
+            tag1 = tag;
+            u10 = a;
+            u11 = b;
+          } else {
+            tag0 = tag;
+            u00 = a;
+            u01 = b;
+          }
+        }
+        rnd = cls ? ((cnt & 1) ? u11 : u10) : ((cnt & 1) ? u01 : u00);
+      }
+      const double Ec = v.e_const_mode ? v.E_const : v.E[sIND[i] * v.nb + sIND[j]];
+      const double xi_i = sX[i], xi_j = sX[j];
       int imin, imax;
-      if (sX[i] <= sX[j]) {
+      if (xi_i <= xi_j) {
         imin = i;
         imax = j;
       } else {
         imin = j;
         imax = i;
       }
-      const double xi_min = sX[imin], xi_max = sX[imax];
-      const double mw_min = sMW[imin], mw_max = sMW[imax];
-      const double ms_min = sMS[imin], ms_max = sMS[imax];
+      const double xi_min = imin == i ? xi_i : xi_j, xi_max = imin == i ? xi_j : xi_i;
       const double du = sU[i] - sU[j], dw = sW[i] - sW[j];
-      const double Ec = v.e_const_mode ? v.E_const : v.E[sIND[i] * v.nb + sIND[j]];
       const double p_crit = xi_max * dtdV * lmp_kernel_hydro(P, sR[i], sR[j], Ec, sqrt(du * du + dw * dw));
       if (p_crit > 1.0) {
+        const int kmin = sSL[imin], kmax = sSL[imax];
+        const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
+        const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
         const double xi_col = p_crit * xi_min;
         const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
-        sMW[imin] = mwn;
-        sMS[imin] = msn;
-        const double Rn = lmp_radius(P, (mwn + msn), sRHO[imin]);
+        v.m_w[kmin] = mwn;
+        v.m_s[kmin] = msn;
+        const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
         sR[imin] = Rn;
         sX[imax] = xi_max - xi_col;
         if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
         ++c_multi;
-      } else if (p_crit > 0.0) {
-        const long long cnt = (long long)i * nn - (long long)i * (i + 1) / 2 + (j - i - 1);
-        const double rnd = v.rnd_stream ? v.rnd_stream[roff + cnt]
-                                        : lm_pair_uniform(v.seed, (uint32_t)r, v.col_step, (uint32_t)c,
-                                                          (uint64_t)cnt);
-        if (p_crit > rnd) {
-          ++c_single;
-          const double xi_rel_dev = (xi_max - xi_min) / xi_max;
-          if (xi_rel_dev < 1.0E-5) {
-            ++c_eq;
-            const double xi_ges = xi_min + xi_max;
-            const double mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
-            const double msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
-            sMW[imin] = mwn;
-            sMS[imin] = msn;
-            sMW[imax] = mwn;
-            sMS[imax] = msn;
-            const double Rn = lmp_radius(P, (mwn + msn), sRHO[imin]);
-            sR[imin] = Rn;
-            sR[imax] = Rn;
-            const double xmax_new = 0.5 * 0.7 * xi_ges;
-            sX[imax] = xmax_new;
-            sX[imin] = 0.5 * xi_ges - xmax_new;
-            if (!v.e_const_mode) {
-              const int ind = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
-              sIND[imin] = ind;
-              sIND[imax] = ind;
-            }
-          } else {
-            const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
-            sMW[imin] = mwn;
-            sMS[imin] = msn;
-            const double Rn = lmp_radius(P, (mwn + msn), sRHO[imin]);
-            sR[imin] = Rn;
-            sX[imax] = xi_max - xi_min;
-            if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+      } else if (p_crit > rnd) {
+        ++c_single;
+        const int kmin = sSL[imin], kmax = sSL[imax];
+        const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
+        const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
+        const double xi_rel_dev = (xi_max - xi_min) / xi_max;
+        if (xi_rel_dev < 1.0E-5) {
+          ++c_eq;
+          const double xi_ges = xi_min + xi_max;
+          const double mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
+          const double msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
+          v.m_w[kmin] = mwn;
+          v.m_s[kmin] = msn;
+          v.m_w[kmax] = mwn;
+          v.m_s[kmax] = msn;
+          const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+          sR[imin] = Rn;
+          sR[imax] = Rn;
+          const double xmax_new = 0.5 * 0.7 * xi_ges;
+          sX[imax] = xmax_new;
+          sX[imin] = 0.5 * xi_ges - xmax_new;
+          if (!v.e_const_mode) {
+            const int ind = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+            sIND[imin] = ind;
+            sIND[imax] = ind;
           }
+        } else {
+          const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
+          v.m_w[kmin] = mwn;
+          v.m_s[kmin] = msn;
+          const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+          sR[imin] = Rn;
+          sX[imax] = xi_max - xi_min;
+          if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
         }
       }
     }
     __syncthreads();
   }
-  for (int m = lane; m < n; m += WAVE) {
-    const int k = sSL[m];
-    v.xi[k] = sX[m];
-    v.m_w[k] = sMW[m];
-    v.m_s[k] = sMS[m];
-  }
+  for (int m = lane; m < n; m += WAVE) v.xi[sSL[m]] = sX[m];
   for (int off = WAVE / 2; off > 0; off >>= 1) {
     c_single += __shfl_down(c_single, off);
     c_multi += __shfl_down(c_multi, off);
@@ -475,8 +627,9 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, LmPhys P, int from_
 /* member counts -> pair counts for the injected stream offsets */
 __global__ __launch_bounds__(WAVE) void k_count_pairs(KView v) {
   const int gc = blockIdx.x, nc = v.nc, r = gc / nc;
-  __shared__ int sSlot[1], sId[1];
-  const int n = gather_members(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.cell, v.id, sSlot, sId, 0);
+  __shared__ NbrTab tb;
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  const int n = count_members(gc, v.cell, &tb);
   if (threadIdx.x == 0) v.rnd_off[gc] = (long long)n * (n - 1) / 2;
 }
 
@@ -511,8 +664,9 @@ __global__ __launch_bounds__(1024) void k_scan_pairs(long long *a, int L) {
 /* ------------------------------------------------------------------------ */
 /* particle substep: one thread per SIP slot                                */
 /* ------------------------------------------------------------------------ */
-__global__ __launch_bounds__(256) void k_particles(KView v, LmPhys P, long long n_tot, double dt_sub,
+__global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, double dt_sub,
                                                    double dt_pos, int pos_only) {
+  const LmPhys &P = c_P;
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_tot) return;
   const int gc = v.cell[k];
@@ -608,7 +762,12 @@ __global__ __launch_bounds__(WAVE) void k_water(KView v) {
     sId[t] = v.id[k];
   }
   __syncthreads();
-  rank_members(n, sId, sRank);
+  for (int m = threadIdx.x; m < n; m += WAVE) {
+    const int idm = sId[m];
+    int rk = 0;
+    for (int q = 0; q < n; ++q) rk += (sId[q] < idm);
+    sRank[m] = rk;
+  }
   __syncthreads();
   for (int t = threadIdx.x; t < n; t += WAVE) sOrd[sRank[t]] = sSlot[t];
   __syncthreads();
@@ -631,14 +790,17 @@ __global__ __launch_bounds__(WAVE) void k_dml(KView v) {
   const int nc = v.nc;
   const int gc = blockIdx.x, r = gc / nc;
   __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
+  __shared__ unsigned char sSub[DML_CAP];
   __shared__ double sVal[DML_CAP];
-  const int n = gather_members(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.cell, v.id, sSlot, sId, DML_CAP);
+  __shared__ NbrTab tb;
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   if (n > DML_CAP) {
     if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
     return;
   }
   __syncthreads();
-  rank_members(n, sId, sRank);
+  rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
   for (int m = threadIdx.x; m < n; m += WAVE) sVal[sRank[m]] = v.dmxi[sSlot[m]];
   __syncthreads();
@@ -650,7 +812,8 @@ __global__ __launch_bounds__(WAVE) void k_dml(KView v) {
 }
 
 /* integration.py:402-483 + 359-400 for one cell; refreshes cc */
-__global__ __launch_bounds__(256) void k_grid(KView v, LmPhys P) {
+__global__ __launch_bounds__(256) void k_grid(KView v) {
+  const LmPhys &P = c_P;
   const int gc = blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= v.n_runs * v.nc) return;
   const int nc = v.nc, r = gc / nc, c = gc % nc;
@@ -751,7 +914,8 @@ __device__ static double divergence_at(const double *f, const double *flux, int 
   return out;
 }
 
-__global__ void k_adv_incr(KView v, LmPhys P, double dt_sub, int second, int act_relax) {
+__global__ void k_adv_incr(KView v, double dt_sub, int second, int act_relax) {
+  const LmPhys &P = c_P;
   const int gc = blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= v.n_runs * v.nc) return;
   const int nc = v.nc, r = gc / nc, c = gc % nc, i = c / v.nz, j = c % v.nz;
@@ -798,7 +962,9 @@ __global__ __launch_bounds__(WAVE) void k_ri_count(KView v) {
   }
   const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
   const int32_t *sr = v.seg + (size_t)r * (nc + 1);
-  const int n = gather_members(gc, nc, v.nx, v.nz, sr, v.cell, v.id, nullptr, nullptr, 0);
+  __shared__ NbrTab tb;
+  nbr_table(gc, nc, v.nx, v.nz, sr, &tb);
+  const int n = count_members(gc, v.cell, &tb);
   int dead = 0;
   for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
     const int k = base + lane;
@@ -911,13 +1077,16 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
   const int32_t *sr = v.seg + (size_t)r * (nc + 1);
   const int32_t *sc = v.scan_buf + (size_t)r * L;
   __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
-  const int n = gather_members(gc, nc, v.nx, v.nz, sr, v.cell, v.id, sSlot, sId, DML_CAP);
+  __shared__ unsigned char sSub[DML_CAP];
+  __shared__ NbrTab tb;
+  nbr_table(gc, nc, v.nx, v.nz, sr, &tb);
+  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   if (n > DML_CAP) {
     if (lane == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
     return;
   }
   __syncthreads();
-  rank_members(n, sId, sRank);
+  rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
   const int dst0 = sc[c];
   for (int m = lane; m < n; m += WAVE) copy_sip(v, sSlot[m], dst0 + sRank[m], gc);
@@ -1060,7 +1229,8 @@ __global__ __launch_bounds__(WAVE) void k_ecol_box(int nbox, const int32_t *off,
                                                    double *radii, double *vel, const double *rho, double dt_over_dV,
                                                    const double *E, int nb, double lnR0, double lnf,
                                                    long long *no_cols, unsigned long long seed, unsigned step,
-                                                   const double *rnd_stream, int32_t *err, LmPhys P) {
+                                                   const double *rnd_stream, int32_t *err) {
+  const LmPhys &P = c_P;
   const int b = blockIdx.x, lane = threadIdx.x;
   const int s0 = off[b], n = off[b + 1] - off[b];
   if (n > BOX_CAP) {
@@ -1190,16 +1360,17 @@ struct ProfScope {
     if (p && p->n < p->max_ev) {
       idx = p->n++;
       p->kind[idx] = kind;
-      hipEventRecord(p->a[idx], s);
+      (void)hipEventRecord(p->a[idx], s);
     }
   }
   ~ProfScope() {
-    if (idx >= 0) hipEventRecord(p->b[idx], s);
+    if (idx >= 0) (void)hipEventRecord(p->b[idx], s);
   }
 };
 
 static int check_engine(const LmEngine *e) {
   if (!e) return set_err(LM_ERR_ARG, "engine is NULL");
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
   if (e->n_runs <= 0 || e->nx <= 0 || e->nz <= 0 || e->nc != e->nx * e->nz)
     return set_err(LM_ERR_ARG, "bad grid shape (need nc == nx*nz > 0, n_runs > 0)");
   if (e->cur < 0 || e->cur > 1 || e->ccur < 0 || e->ccur > 1) return set_err(LM_ERR_ARG, "bad buffer index");
@@ -1238,8 +1409,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
 int32_t lm_num_constants(void) { return (int32_t)(sizeof(LmPhys) / sizeof(double)); }
 
 void lm_get_constants(double *out, int32_t n) {
-  const LmPhys &P = phys();
-  const double *p = (const double *)&P;
+  const double *p = (const double *)&phys();
   const int m = lm_num_constants();
   for (int i = 0; i < n && i < m; ++i) out[i] = p[i];
 }
@@ -1248,7 +1418,7 @@ int lm_engine_init_cells(LmEngine *e, void *stream) {
   int rc = check_engine(e);
   if (rc) return rc;
   const int ncell = e->n_runs * e->nc;
-  hipLaunchKernelGGL(k_init_cc, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), make_view(e), phys());
+  hipLaunchKernelGGL(k_init_cc, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), make_view(e));
   LAUNCH_CHECK();
   return 0;
 }
@@ -1267,12 +1437,11 @@ int lm_collision_step(LmEngine *e, void *stream) {
   }
   HIP_TRY(hipMemsetAsync(e->ovf_cnt, 0, sizeof(int32_t), S(stream)));
   KView v = make_view(e);
-  const LmPhys &P = phys();
   {
     ProfScope ps(e, 1, S(stream));
-    hipLaunchKernelGGL(k_collision<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, P, 0);
+    hipLaunchKernelGGL(k_collision<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
   }
-  hipLaunchKernelGGL(k_collision<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, P, 1);
+  hipLaunchKernelGGL(k_collision<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, 1);
   LAUNCH_CHECK();
   e->col_step += 1;
   return 0;
@@ -1283,7 +1452,7 @@ int lm_position_step(LmEngine *e, double dt, void *stream) {
   if (rc) return rc;
   KView v = make_view(e);
   const long long n = e->n_tot;
-  if (n > 0) hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, phys(), n, 0.0, dt, 1);
+  if (n > 0) hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n, 0.0, dt, 1);
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();
   e->ccur = 1 - e->ccur;
@@ -1320,7 +1489,7 @@ int lm_advection_increments(LmEngine *e, int32_t second, void *stream) {
     const int nt = e->n_runs * e->nz;
     hipLaunchKernelGGL(k_colsum, dim3((nt + 127) / 128), dim3(128), 0, S(stream), v, e->dt_sub, 1);
   }
-  hipLaunchKernelGGL(k_adv_incr, dim3((ncell + 127) / 128), dim3(128), 0, S(stream), v, phys(), e->dt_sub, second,
+  hipLaunchKernelGGL(k_adv_incr, dim3((ncell + 127) / 128), dim3(128), 0, S(stream), v, e->dt_sub, second,
                      e->act_relaxation);
   LAUNCH_CHECK();
   return 0;
@@ -1330,12 +1499,11 @@ int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
   int rc = check_engine(e);
   if (rc) return rc;
   KView v = make_view(e);
-  const LmPhys &P = phys();
   const long long n = e->n_tot;
   const int ncell = e->n_runs * e->nc;
   if (n > 0) {
     ProfScope ps(e, 0, S(stream));
-    hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, P, n, dt_sub,
+    hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n, dt_sub,
                        dt_sub_pos, 0);
   }
   {
@@ -1343,7 +1511,7 @@ int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
     hipLaunchKernelGGL(k_dml, dim3(ncell), dim3(WAVE), 0, S(stream), v);
   }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
-  hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v, P);
+  hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
   LAUNCH_CHECK();
   e->ccur = 1 - e->ccur;
   return 0;
@@ -1421,8 +1589,8 @@ int lm_prof_disable(LmEngine *e) {
   if (!e || !e->prof) return 0;
   LmProf *p = (LmProf *)e->prof;
   for (int i = 0; i < p->max_ev; ++i) {
-    hipEventDestroy(p->a[i]);
-    hipEventDestroy(p->b[i]);
+    (void)hipEventDestroy(p->a[i]);
+    (void)hipEventDestroy(p->b[i]);
   }
   delete[] p->a;
   delete[] p->b;
@@ -1460,6 +1628,7 @@ int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, do
 int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
   if (n_boxes <= 0 || !off || !xis || !masses || !no_cols) return set_err(LM_ERR_ARG, "bad box arguments");
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
   static int32_t *err_dev = nullptr;
   if (!err_dev) {
     HIP_TRY(hipMalloc(&err_dev, 16));
@@ -1476,6 +1645,7 @@ int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *m
                      int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
   if (n_boxes <= 0 || !off || !xis || !masses || !radii || !vel || !rho || !E || !no_cols)
     return set_err(LM_ERR_ARG, "bad box arguments");
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
   static int32_t *err_dev = nullptr;
   if (!err_dev) {
     HIP_TRY(hipMalloc(&err_dev, 16));
@@ -1483,7 +1653,7 @@ int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *m
   }
   hipLaunchKernelGGL(k_ecol_box, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, radii, vel,
                      rho, dt_over_dV, E, nb, lnR0, lnf, (long long *)no_cols, (unsigned long long)seed, step,
-                     rnd_stream, err_dev, phys());
+                     rnd_stream, err_dev);
   LAUNCH_CHECK();
   return 0;
 }
