@@ -450,6 +450,16 @@ __global__ void k_init_cc(KView v) {
  * of the pair loop, so a lane walks one row through consecutive diagonals
  * and consecutive pair numbers: one Philox block serves two diagonals.
  */
+/* p > u(cnt), deciding from the A word when possible (see lm_philox.h) */
+__device__ static inline bool pair_fires(double p, uint32_t aw, unsigned long long seed, uint32_t run, uint32_t step,
+                                         uint32_t cell, long long cnt) {
+  const double lo = (double)(aw >> 5) * (1.0 / 134217728.0); /* (A>>5) * 2^-27 */
+  if (p < lo) return false;
+  if (p >= lo + (1.0 / 134217728.0)) return true;
+  const lm_u32x4 b = lm_pair_block(seed, run, step, cell, (uint64_t)(cnt >> 2), 1);
+  return p > lm_res53(aw, lm_word(b, (int)(cnt & 3)));
+}
+
 template <int CAP>
 __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   const LmPhys &P = c_P;
@@ -510,41 +520,36 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   const long long roff = v.rnd_stream ? v.rnd_off[gc] : 0;
   const uint32_t key_run = (uint32_t)r, key_step = v.col_step, key_cell = (uint32_t)c;
   long long c_single = 0, c_multi = 0, c_eq = 0;
-  /* Philox block cache, one per row class (i >> 6) & 1 */
+  /* Philox A-word block cache, one per row class (i >> 6) & 1 */
   long long tag0 = -1, tag1 = -1;
-  double u00 = 0.0, u01 = 0.0, u10 = 0.0, u11 = 0.0;
+  lm_u32x4 blk0, blk1;
+  blk0.v[0] = blk0.v[1] = blk0.v[2] = blk0.v[3] = 0;
+  blk1 = blk0;
   for (int d = 1; d <= 2 * n - 3; ++d) {
     const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
     const int i_hi = (d - 1) >> 1;
     for (int i = i_lo + ((lane - i_lo) & (WAVE - 1)); i <= i_hi; i += WAVE) {
       const int j = d - i;
       const long long cnt = (long long)i * nn - (long long)i * (i + 1) / 2 + (j - i - 1);
-      /* uniform of this pair (independent of the state: issued before the state reads) */
-      double rnd;
+      /* A word of this pair's uniform (independent of the state: issued before the state reads) */
+      uint32_t aw = 0;
+      double rnd_inj = 0.0;
       if (v.rnd_stream) {
-        rnd = v.rnd_stream[roff + cnt];
+        rnd_inj = v.rnd_stream[roff + cnt];
       } else {
-        const long long tag = cnt >> 1;
+        const long long tag = cnt >> 2;
         const bool cls = (i >> 6) & 1;
         if (tag != (cls ? tag1 : tag0)) {
-          lm_u32x4 cw;
-          cw.v[0] = (uint32_t)tag;
-          cw.v[1] = key_cell;
-          cw.v[2] = key_step;
-          cw.v[3] = key_run;
-          const lm_u32x4 o = lm_philox4x32_10(cw, (uint32_t)v.seed, (uint32_t)(v.seed >> 32));
-          const double a = lm_res53(o.v[0], o.v[1]), b = lm_res53(o.v[2], o.v[3]);
+          const lm_u32x4 o = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint64_t)tag, 0);
           if (cls) {
             tag1 = tag;
-            u10 = a;
-            u11 = b;
+            blk1 = o;
           } else {
             tag0 = tag;
-            u00 = a;
-            u01 = b;
+            blk0 = o;
           }
         }
-        rnd = cls ? ((cnt & 1) ? u11 : u10) : ((cnt & 1) ? u01 : u00);
+        aw = lm_word(cls ? blk1 : blk0, (int)(cnt & 3));
       }
       const double Ec = v.e_const_mode ? v.E_const : v.E[sIND[i] * v.nb + sIND[j]];
       const double xi_i = sX[i], xi_j = sX[j];
@@ -572,7 +577,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         sX[imax] = xi_max - xi_col;
         if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
         ++c_multi;
-      } else if (p_crit > rnd) {
+      } else if (v.rnd_stream ? (p_crit > rnd_inj) : pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, cnt)) {
         ++c_single;
         const int kmin = sSL[imin], kmax = sSL[imax];
         const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
