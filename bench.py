@@ -38,11 +38,15 @@ def log(*a):
 
 
 def build_engine(n_runs, spc, seed0, nx=75, nz=75, kernel="Long_Bott"):
+    """kernel: 'Long_Bott' / 'Hall_Bott' table, or 'const' (Hydro_E_const, E = 0.5)."""
     import torch  # noqa: F401
     from lacmopy_amd import load_kernel_data_Ecol
     from lacmopy_amd.engine import Engine
     from lacmopy_amd.setup_case import init_profiles, make_case
-    E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
+    if kernel == "const":
+        E, lnR0, lnf = 0.5, None, None
+    else:
+        E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
     cases = []
     for r in range(n_runs):
         grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=nx, nz=nz, spc=spc, seed=seed0 + 2 * r)
@@ -136,6 +140,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
+    ap.add_argument("--kernel", default="Long_Bott", choices=["Long_Bott", "Hall_Bott", "const"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): independent seeds shard "
@@ -146,7 +151,7 @@ def main():
 
     log(f"building batch of {args.runs} runs ...")
     t = time.perf_counter()
-    eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed)
+    eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, kernel=args.kernel)
     log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
     secs = time_steps(eng, args.steps, args.warmup)
     batch_rate = n_sips * args.steps / secs
@@ -159,7 +164,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        eng1, n1, _ = build_engine(1, args.spc, args.seed)
+        eng1, n1, _ = build_engine(1, args.spc, args.seed, kernel=args.kernel)
         s1 = time_steps(eng1, max(args.steps, 10), args.warmup)
         plain = {"sip_steps_per_s": n1 * max(args.steps, 10) / s1, "ms_per_step": s1 / max(args.steps, 10) * 1e3,
                  "sips": n1}
