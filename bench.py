@@ -156,6 +156,8 @@ def main():
     secs = time_steps(eng, args.steps, args.warmup)
     batch_rate = n_sips * args.steps / secs
     log(f"batch: {secs / args.steps * 1e3:.2f} ms/step, {batch_rate:.4g} SIP-steps/s")
+    ncols = eng.buf["no_cols"].sum(0).cpu().tolist()
+    log(f"collision events so far (single, multiple): {ncols}")
     ktimes = {} if args.no_kernels else kernel_times(eng, max(1, min(args.steps, 3)))
     log(f"kernel times (ms): {ktimes}")
     n_batch_sips = n_sips
@@ -201,7 +203,7 @@ def main():
                    "runs": args.runs, "sips_total": n_batch_sips, "cells": 75 * 75,
                    "plain_single_run": plain,
                    "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
-                   "kernel_ms": ktimes},
+                   "kernel_ms": ktimes, "collision_events": ncols},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -450,14 +450,14 @@ __global__ void k_init_cc(KView v) {
  * of the pair loop, so a lane walks one row through consecutive diagonals
  * and consecutive pair numbers: one Philox block serves two diagonals.
  */
-/* p > u(cnt), deciding from the A word when possible (see lm_philox.h) */
+/* p > u(i, j), deciding from the A word when possible (see lm_philox.h) */
 __device__ static inline bool pair_fires(double p, uint32_t aw, unsigned long long seed, uint32_t run, uint32_t step,
-                                         uint32_t cell, long long cnt) {
+                                         uint32_t cell, uint32_t i, uint32_t d) {
   const double lo = (double)(aw >> 5) * (1.0 / 134217728.0); /* (A>>5) * 2^-27 */
   if (p < lo) return false;
   if (p >= lo + (1.0 / 134217728.0)) return true;
-  const lm_u32x4 b = lm_pair_block(seed, run, step, cell, (uint64_t)(cnt >> 2), 1);
-  return p > lm_res53(aw, lm_word(b, (int)(cnt & 3)));
+  const lm_u32x4 b = lm_pair_block(seed, run, step, cell, i, d, 1);
+  return p > lm_res53(aw, lm_word(b, d & 3));
 }
 
 template <int CAP>
@@ -516,12 +516,11 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   __syncthreads();
 
   const double dtdV = v.dt_col_over_dV;
-  const long long nn = n;
   const long long roff = v.rnd_stream ? v.rnd_off[gc] : 0;
   const uint32_t key_run = (uint32_t)r, key_step = v.col_step, key_cell = (uint32_t)c;
   long long c_single = 0, c_multi = 0, c_eq = 0;
-  /* Philox A-word block cache, one per row class (i >> 6) & 1 */
-  long long tag0 = -1, tag1 = -1;
+  /* Philox A-word blocks, one per row class (i >> 6) & 1, refreshed when d % 4 == 0 (or first use) */
+  int tag0 = -1, tag1 = -1;
   lm_u32x4 blk0, blk1;
   blk0.v[0] = blk0.v[1] = blk0.v[2] = blk0.v[3] = 0;
   blk1 = blk0;
@@ -530,17 +529,17 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
     const int i_hi = (d - 1) >> 1;
     for (int i = i_lo + ((lane - i_lo) & (WAVE - 1)); i <= i_hi; i += WAVE) {
       const int j = d - i;
-      const long long cnt = (long long)i * nn - (long long)i * (i + 1) / 2 + (j - i - 1);
       /* A word of this pair's uniform (independent of the state: issued before the state reads) */
       uint32_t aw = 0;
       double rnd_inj = 0.0;
       if (v.rnd_stream) {
+        const int cnt = i * n - i * (i + 1) / 2 + (j - i - 1);
         rnd_inj = v.rnd_stream[roff + cnt];
       } else {
-        const long long tag = cnt >> 2;
+        const int tag = (i << 16) | (d >> 2);
         const bool cls = (i >> 6) & 1;
         if (tag != (cls ? tag1 : tag0)) {
-          const lm_u32x4 o = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint64_t)tag, 0);
+          const lm_u32x4 o = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d, 0);
           if (cls) {
             tag1 = tag;
             blk1 = o;
@@ -549,7 +548,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
             blk0 = o;
           }
         }
-        aw = lm_word(cls ? blk1 : blk0, (int)(cnt & 3));
+        aw = lm_word(cls ? blk1 : blk0, (uint32_t)d & 3);
       }
       const double Ec = v.e_const_mode ? v.E_const : v.E[sIND[i] * v.nb + sIND[j]];
       const double xi_i = sX[i], xi_j = sX[j];
@@ -577,7 +576,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         sX[imax] = xi_max - xi_col;
         if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
         ++c_multi;
-      } else if (v.rnd_stream ? (p_crit > rnd_inj) : pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, cnt)) {
+      } else if (v.rnd_stream ? (p_crit > rnd_inj) : pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d)) {
         ++c_single;
         const int kmin = sSL[imin], kmax = sSL[imax];
         const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
@@ -1164,7 +1163,7 @@ __global__ __launch_bounds__(WAVE) void k_golovin(int nbox, const int32_t *off, 
       } else if (p_crit > 0.0) {
         const long long cnt = (long long)i * n - (long long)i * (i + 1) / 2 + (j - i - 1);
         const double rnd = rnd_stream ? rnd_stream[roff + cnt]
-                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint64_t)cnt);
+                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint32_t)i, (uint32_t)j);
         if (p_crit > rnd) {
           ++cs;
           const double xi_rel_dev = (xi_max - xi_min) / xi_max;
@@ -1283,7 +1282,7 @@ __global__ __launch_bounds__(WAVE) void k_ecol_box(int nbox, const int32_t *off,
       } else if (p_crit > 0.0) {
         const long long cnt = (long long)i * n - (long long)i * (i + 1) / 2 + (j - i - 1);
         const double rnd = rnd_stream ? rnd_stream[roff + cnt]
-                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint64_t)cnt);
+                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint32_t)i, (uint32_t)j);
         if (p_crit > rnd) {
           ++cs;
           const double xi_rel_dev = (xi_max - xi_min) / xi_max;

@@ -6,16 +6,18 @@
  * known-answer vectors of Random123 (tests/test_oracle_cpu.py).
  *
  * Consumption structure follows the reference: one uniform per unordered SIP
- * pair (i<j) of a cell, in row-major order, pair number
- * cnt = i*N - i*(i+1)/2 + (j-i-1) (collision/all_or_nothing.py:545-633).
- * The uniform of pair cnt is res53(A, B) (NumPy's random_sample construction
- * from two 32-bit words: (A>>5)*2^26 + (B>>6), scaled by 2^-53) with
- *   A = word (cnt & 3) of Philox(ctr = (cnt >> 2, cell, step, run)),
- *   B = word (cnt & 3) of Philox(ctr = (cnt >> 2, cell, step, run | 2^31)),
- * key = (seed lo, seed hi).  One block of A words serves four consecutive
- * pairs, and since (A>>5)*2^-27 <= u < (A>>5)*2^-27 + 2^-27, the comparison
- * p > u is decided by A alone unless p falls in that 2^-27-wide interval;
- * only then is the B block needed.
+ * pair (i<j) of a cell (collision/all_or_nothing.py:545-633), none for cells
+ * with fewer than two SIPs.  The uniform of pair (i, j), d = i + j, is
+ * res53(A, B) -- NumPy's random_sample construction from two 32-bit words,
+ * ((A>>5)*2^26 + (B>>6)) * 2^-53 -- with
+ *   A = word (d & 3) of Philox(ctr = (i<<16 | d>>2, cell, step, run)),
+ *   B = word (d & 3) of Philox(ctr = (i<<16 | d>>2, cell, step, run | 2^31)),
+ * key = (seed lo, seed hi).  Blocks follow the anti-diagonals d, so the
+ * gfx950 collision kernel (one row per lane, diagonals in order) refreshes
+ * its A block on the same diagonal in every lane, once per four diagonals.
+ * Since (A>>5)*2^-27 <= u < (A>>5)*2^-27 + 2^-27, the comparison p > u is
+ * decided by A alone unless p falls inside that interval; only then is the
+ * B block needed.  Valid for cells of up to 65536 SIPs.
  */
 #ifndef LM_PHILOX_H
 #define LM_PHILOX_H
@@ -60,23 +62,27 @@ LMP_FN double lm_res53(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-/* Philox block of pair words: half = 0 gives the A words, 1 the B words. */
-LMP_FN lm_u32x4 lm_pair_block(uint64_t seed, uint32_t run, uint32_t step, uint32_t cell, uint64_t blk, int half) {
+/* Philox block of the pair words of row i, diagonals 4*(d>>2) .. +3: half 0 = A words, 1 = B words. */
+LMP_FN lm_u32x4 lm_pair_block(uint64_t seed, uint32_t run, uint32_t step, uint32_t cell, uint32_t i, uint32_t d,
+                              int half) {
   lm_u32x4 c;
-  c.v[0] = (uint32_t)blk;
+  c.v[0] = (i << 16) | (d >> 2);
   c.v[1] = cell;
   c.v[2] = step;
   c.v[3] = half ? (run | 0x80000000u) : run;
   return lm_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
-LMP_FN uint32_t lm_word(lm_u32x4 o, int k) { return k == 0 ? o.v[0] : (k == 1 ? o.v[1] : (k == 2 ? o.v[2] : o.v[3])); }
+LMP_FN uint32_t lm_word(lm_u32x4 o, uint32_t k) {
+  return k == 0 ? o.v[0] : (k == 1 ? o.v[1] : (k == 2 ? o.v[2] : o.v[3]));
+}
 
-/* Uniform in [0,1) for pair number cnt of cell `cell` in collision step `step` of run `run`. */
-LMP_FN double lm_pair_uniform(uint64_t seed, uint32_t run, uint32_t step, uint32_t cell, uint64_t cnt) {
-  const lm_u32x4 a = lm_pair_block(seed, run, step, cell, cnt >> 2, 0);
-  const lm_u32x4 b = lm_pair_block(seed, run, step, cell, cnt >> 2, 1);
-  return lm_res53(lm_word(a, (int)(cnt & 3)), lm_word(b, (int)(cnt & 3)));
+/* Uniform in [0,1) of pair (i, j), i < j, of cell `cell` in collision step `step` of run `run`. */
+LMP_FN double lm_pair_uniform(uint64_t seed, uint32_t run, uint32_t step, uint32_t cell, uint32_t i, uint32_t j) {
+  const uint32_t d = i + j;
+  const lm_u32x4 a = lm_pair_block(seed, run, step, cell, i, d, 0);
+  const lm_u32x4 b = lm_pair_block(seed, run, step, cell, i, d, 1);
+  return lm_res53(lm_word(a, d & 3), lm_word(b, d & 3));
 }
 
 #endif /* LM_PHILOX_H */

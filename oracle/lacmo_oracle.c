@@ -326,9 +326,9 @@ typedef struct {
   uint32_t run, step, cell;
 } OrRng;
 
-static double or_draw(OrRng *g, int64_t cnt) {
+static double or_draw(OrRng *g, int64_t cnt, int64_t i, int64_t j) {
   if (g->stream) return g->stream[g->pos + cnt];
-  return lm_pair_uniform(g->seed, g->run, g->step, g->cell, (uint64_t)cnt);
+  return lm_pair_uniform(g->seed, g->run, g->step, g->cell, (uint32_t)i, (uint32_t)j);
 }
 
 /*
@@ -365,7 +365,7 @@ static void coll_cell_2D(int64_t n, double *xis, double *m_w, double *m_s, doubl
         xis[imax] -= xi_col;
         if (E) ind[imin] = or_kernel_index(radii[imin], lnR0, lnf, nb);
         no_cols[1] += 1;
-      } else if (p_crit > or_draw(g, cnt)) {
+      } else if (p_crit > or_draw(g, cnt, i, j)) {
         no_cols[0] += 1;
         const double xi_rel_dev = (xi_max - xi_min) / xi_max;
         if (xi_rel_dev < 1.0E-5) {
@@ -502,7 +502,7 @@ EXPORT void or_collision_golovin(int64_t n, double *xis, double *masses, double 
         const double xi_col = p_crit * xi_min;
         masses[imin] = (xi_min * m_min + xi_col * m_max) / xi_min;
         xis[imax] -= xi_col;
-      } else if (p_crit > or_draw(&g, cnt)) {
+      } else if (p_crit > or_draw(&g, cnt, i, j)) {
         no_cols[0] += 1;
         const double xi_rel_dev = (xi_max - xi_min) / xi_max;
         if (xi_rel_dev < 1.0E-5) {
@@ -552,7 +552,7 @@ EXPORT void or_collision_ecol_box(int64_t n, double *xis, double *masses, double
         xis[imax] -= xi_col;
         ind[imin] = or_kernel_index(radii[imin], lnR0, lnf, nb);
         no_cols[1] += 1;
-      } else if (p_crit > or_draw(&g, cnt)) {
+      } else if (p_crit > or_draw(&g, cnt, i, j)) {
         no_cols[0] += 1;
         const double xi_rel_dev = (xi_max - xi_min) / xi_max;
         if (xi_rel_dev < 1.0E-5) {
@@ -1128,8 +1128,8 @@ EXPORT void or_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
   for (int i = 0; i < 4; ++i) out[i] = o.v[i];
 }
 
-EXPORT double or_pair_uniform(uint64_t seed, uint32_t run, uint32_t step, uint32_t cell, uint64_t cnt) {
-  return lm_pair_uniform(seed, run, step, cell, cnt);
+EXPORT double or_pair_uniform(uint64_t seed, uint32_t run, uint32_t step, uint32_t cell, uint32_t i, uint32_t j) {
+  return lm_pair_uniform(seed, run, step, cell, i, j);
 }
 
 /* lm_math access for tests */

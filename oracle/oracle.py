@@ -48,7 +48,7 @@ def lib(libm=False):
     L.or_surface_tension.restype = _d
     L.or_surface_tension.argtypes = [C.c_int, _d, _d]
     L.or_pair_uniform.restype = _d
-    L.or_pair_uniform.argtypes = [_u64, _u32, _u32, _u32, _u64]
+    L.or_pair_uniform.argtypes = [_u64, _u32, _u32, _u32, _u32, _u32]
     L.or_num_consts.restype = C.c_int
     L.or_interpolate_velocity.argtypes = [_i64, _i64, _d, _d, _p, _i64, _i64, _p, _p]
     L.or_collision_cell_2D.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _p, _d, _p, _d, _i64, _d, _d, _p, _p,
