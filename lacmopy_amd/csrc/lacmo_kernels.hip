@@ -475,10 +475,16 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   const int r = gc / nc, c = gc % nc;
   const int lane = threadIdx.x;
 
+  /* LDS: xi, R, u, w, kernel index and slot of each member, in id order.  The
+     gather scratch (slot, id, rank, sub-list) lives inside the u/w arrays,
+     which are only filled after the ranks are known. */
   __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
-  __shared__ int sIND[CAP], sSL[CAP], sSlot[CAP], sId[CAP], sRank[CAP];
-  __shared__ unsigned char sSub[CAP];
+  __shared__ int sIND[CAP], sSL[CAP];
   __shared__ NbrTab tb;
+  int *sSlot = (int *)sU;
+  int *sId = sSlot + CAP;
+  int *sRank = (int *)sW;
+  unsigned char *sSub = (unsigned char *)(sRank + CAP);
 
   nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
@@ -497,16 +503,17 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   __syncthreads();
   rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
+  for (int m = lane; m < n; m += WAVE) sSL[sRank[m]] = sSlot[m];
+  __syncthreads();
 
   const double *C = v.cc + (size_t)gc * CC_N;
   const double T = C[CC_T], rho_w_T = C[CC_RHOW];
   double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
-  for (int m = lane; m < n; m += WAVE) {
-    const int k = sSlot[m], p = sRank[m];
+  for (int p = lane; p < n; p += WAVE) {
+    const int k = sSL[p];
     sX[p] = v.xi[k];
     sU[p] = v.u[k];
     sW[p] = v.w[k];
-    sSL[p] = k;
     double R, ws, rho;
     lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
     sR[p] = R;
@@ -524,9 +531,33 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   lm_u32x4 blk0, blk1;
   blk0.v[0] = blk0.v[1] = blk0.v[2] = blk0.v[3] = 0;
   blk1 = blk0;
+  /* E_col of the lane's first pair of the next diagonal, loaded one diagonal ahead and
+     revalidated against the kernel indices when it is used (a collision may change them) */
+  const bool use_tab = !v.e_const_mode;
+  int pf_i = -1, pf_ii = 0, pf_jj = 0;
+  double pf_E = 0.0;
+  if (use_tab && lane == 0) { /* diagonal 1: pair (0, 1) */
+    pf_i = 0;
+    pf_ii = sIND[0];
+    pf_jj = sIND[1];
+    pf_E = v.E[pf_ii * v.nb + pf_jj];
+  }
   for (int d = 1; d <= 2 * n - 3; ++d) {
     const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
     const int i_hi = (d - 1) >> 1;
+    const int cur_i = pf_i, cur_ii = pf_ii, cur_jj = pf_jj;
+    const double cur_E = pf_E;
+    pf_i = -1;
+    if (use_tab && d + 1 <= 2 * n - 3) {
+      const int lo1 = d + 1 - (n - 1) > 0 ? d + 1 - (n - 1) : 0;
+      const int i1 = lo1 + ((lane - lo1) & (WAVE - 1));
+      if (i1 <= (d >> 1)) {
+        pf_i = i1;
+        pf_ii = sIND[i1];
+        pf_jj = sIND[d + 1 - i1];
+        pf_E = v.E[pf_ii * v.nb + pf_jj];
+      }
+    }
     for (int i = i_lo + ((lane - i_lo) & (WAVE - 1)); i <= i_hi; i += WAVE) {
       const int j = d - i;
       /* A word of this pair's uniform (independent of the state: issued before the state reads) */
@@ -550,7 +581,13 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         }
         aw = lm_word(cls ? blk1 : blk0, (uint32_t)d & 3);
       }
-      const double Ec = v.e_const_mode ? v.E_const : v.E[sIND[i] * v.nb + sIND[j]];
+      double Ec;
+      if (!use_tab) {
+        Ec = v.E_const;
+      } else {
+        const int ii = sIND[i], jj = sIND[j];
+        Ec = (i == cur_i && ii == cur_ii && jj == cur_jj) ? cur_E : v.E[ii * v.nb + jj];
+      }
       const double xi_i = sX[i], xi_j = sX[j];
       int imin, imax;
       if (xi_i <= xi_j) {
