@@ -460,6 +460,110 @@ __device__ static inline bool pair_fires(double p, uint32_t aw, unsigned long lo
   return p > lm_res53(aw, lm_word(b, d & 3));
 }
 
+/* State of one cell's collision step in LDS (id order) plus slot-indexed globals. */
+struct CollCell {
+  double *sX, *sR, *sU, *sW;
+  int *sIND, *sSL;
+  int n;
+};
+
+/* p_crit of pair (i, j) with the current state (all_or_nothing.py:569-575) */
+__device__ static inline double pair_p(const LmPhys &P, const KView &v, const CollCell &cc, int i, int j, double Ec) {
+  const double xi_i = cc.sX[i], xi_j = cc.sX[j];
+  const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
+  const double du = cc.sU[i] - cc.sU[j], dw = cc.sW[i] - cc.sW[j];
+  return xi_max * v.dt_col_over_dV * lmp_kernel_hydro(P, cc.sR[i], cc.sR[j], Ec, sqrt(du * du + dw * dw));
+}
+
+/*
+ * Apply the event of a firing pair (all_or_nothing.py:577-632): multiple
+ * collection if p_crit > 1, else a single collection (with the near-equal xi
+ * split).  m_w, m_s live in global memory (slot-indexed), the pre-step
+ * density in rho_g.  Returns 1 for a multiple, 2 for a single, 3 for a
+ * near-equal single event.
+ */
+__device__ static inline int apply_event(const LmPhys &P, const KView &v, const CollCell &cc, const double *rho_g,
+                                         int i, int j, double p_crit) {
+  int imin, imax;
+  if (cc.sX[i] <= cc.sX[j]) {
+    imin = i;
+    imax = j;
+  } else {
+    imin = j;
+    imax = i;
+  }
+  const double xi_min = cc.sX[imin], xi_max = cc.sX[imax];
+  const int kmin = cc.sSL[imin], kmax = cc.sSL[imax];
+  const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
+  const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
+  const bool table = !v.e_const_mode;
+  if (p_crit > 1.0) {
+    const double xi_col = p_crit * xi_min;
+    const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
+    v.m_w[kmin] = mwn;
+    v.m_s[kmin] = msn;
+    const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+    cc.sR[imin] = Rn;
+    cc.sX[imax] = xi_max - xi_col;
+    if (table) cc.sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+    return 1;
+  }
+  const double xi_rel_dev = (xi_max - xi_min) / xi_max;
+  if (xi_rel_dev < 1.0E-5) {
+    const double xi_ges = xi_min + xi_max;
+    const double mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
+    const double msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
+    v.m_w[kmin] = mwn;
+    v.m_s[kmin] = msn;
+    v.m_w[kmax] = mwn;
+    v.m_s[kmax] = msn;
+    const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+    cc.sR[imin] = Rn;
+    cc.sR[imax] = Rn;
+    const double xmax_new = 0.5 * 0.7 * xi_ges;
+    cc.sX[imax] = xmax_new;
+    cc.sX[imin] = 0.5 * xi_ges - xmax_new;
+    if (table) {
+      const int ind = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+      cc.sIND[imin] = ind;
+      cc.sIND[imax] = ind;
+    }
+    return 3;
+  }
+  const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
+  v.m_w[kmin] = mwn;
+  v.m_s[kmin] = msn;
+  const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+  cc.sR[imin] = Rn;
+  cc.sX[imax] = xi_max - xi_min;
+  if (table) cc.sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+  return 2;
+}
+
+/* pair number of (i, j) in the reference's row-major loop */
+__device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
+
+#define SPEC_MAX_EVENTS 4
+
+/*
+ * One all-cells collision step, one wavefront per cell.
+ *
+ * Exactness argument.  The reference walks the pairs of a cell in row-major
+ * order and changes the state only when a pair fires.  So, given the state
+ * after the last applied event, every later pair up to the first one that
+ * fires under that state is evaluated by the reference on that same state.
+ * The kernel therefore evaluates all remaining pairs in parallel on the
+ * current state (full lane use, no barriers), takes the first firing pair in
+ * row-major order (wave minimum), applies it and repeats from the next pair.
+ * Cells with more than SPEC_MAX_EVENTS events finish with the anti-diagonal
+ * schedule: pairs grouped by d = i + j are mutually independent and every
+ * dependence points to a smaller d, so processing the remaining pairs by
+ * increasing d reproduces the sequential loop too.
+ *
+ * Work items of the parallel pass are quads: row i and one Philox block
+ * (diagonals 4b..4b+3), i.e. four consecutive pairs of the row sharing one
+ * block of A words.
+ */
 template <int CAP>
 __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   const LmPhys &P = c_P;
@@ -475,11 +579,10 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   const int r = gc / nc, c = gc % nc;
   const int lane = threadIdx.x;
 
-  /* LDS: xi, R, u, w, kernel index and slot of each member, in id order.  The
-     gather scratch (slot, id, rank, sub-list) lives inside the u/w arrays,
-     which are only filled after the ranks are known. */
+  /* LDS: xi, R, u, w, kernel index and slot of each member, in id order; the
+     gather scratch lives inside the u/w arrays until the ranks are known. */
   __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
-  __shared__ int sIND[CAP], sSL[CAP];
+  __shared__ int sIND[CAP], sSL[CAP], sQ[CAP + 1];
   __shared__ NbrTab tb;
   int *sSlot = (int *)sU;
   int *sId = sSlot + CAP;
@@ -508,7 +611,6 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
 
   const double *C = v.cc + (size_t)gc * CC_N;
   const double T = C[CC_T], rho_w_T = C[CC_RHOW];
-  double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
   for (int p = lane; p < n; p += WAVE) {
     const int k = sSL[p];
     sX[p] = v.xi[k];
@@ -517,142 +619,172 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
     double R, ws, rho;
     lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
     sR[p] = R;
-    rho_g[k] = rho;
     sIND[p] = v.e_const_mode ? 0 : lmp_kernel_index(R, v.lnR0, v.lnf, v.nb);
+  }
+  /* quads per row: row i holds diagonals d = 2i+1 .. i+n-1, blocks (2i+1)>>2 .. (i+n-1)>>2 */
+  if (lane == 0) {
+    int acc = 0;
+    for (int i = 0; i < n - 1; ++i) {
+      sQ[i] = acc;
+      acc += ((i + n - 1) >> 2) - ((2 * i + 1) >> 2) + 1;
+    }
+    sQ[n - 1] = acc;
   }
   __syncthreads();
 
-  const double dtdV = v.dt_col_over_dV;
+  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n};
+  const bool use_tab = !v.e_const_mode;
   const long long roff = v.rnd_stream ? v.rnd_off[gc] : 0;
   const uint32_t key_run = (uint32_t)r, key_step = v.col_step, key_cell = (uint32_t)c;
+  double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
+  const int nQ = sQ[n - 1];
   long long c_single = 0, c_multi = 0, c_eq = 0;
-  /* Philox A-word blocks, one per row class (i >> 6) & 1, refreshed when d % 4 == 0 (or first use) */
-  int tag0 = -1, tag1 = -1;
-  lm_u32x4 blk0, blk1;
-  blk0.v[0] = blk0.v[1] = blk0.v[2] = blk0.v[3] = 0;
-  blk1 = blk0;
-  /* E_col of the lane's first pair of the next diagonal, loaded one diagonal ahead and
-     revalidated against the kernel indices when it is used (a collision may change them) */
-  const bool use_tab = !v.e_const_mode;
-  int pf_i = -1, pf_ii = 0, pf_jj = 0;
-  double pf_E = 0.0;
-  if (use_tab && lane == 0) { /* diagonal 1: pair (0, 1) */
-    pf_i = 0;
-    pf_ii = sIND[0];
-    pf_jj = sIND[1];
-    pf_E = v.E[pf_ii * v.nb + pf_jj];
-  }
-  for (int d = 1; d <= 2 * n - 3; ++d) {
-    const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
-    const int i_hi = (d - 1) >> 1;
-    const int cur_i = pf_i, cur_ii = pf_ii, cur_jj = pf_jj;
-    const double cur_E = pf_E;
-    pf_i = -1;
-    if (use_tab && d + 1 <= 2 * n - 3) {
-      const int lo1 = d + 1 - (n - 1) > 0 ? d + 1 - (n - 1) : 0;
-      const int i1 = lo1 + ((lane - lo1) & (WAVE - 1));
-      if (i1 <= (d >> 1)) {
-        pf_i = i1;
-        pf_ii = sIND[i1];
-        pf_jj = sIND[d + 1 - i1];
-        pf_E = v.E[pf_ii * v.nb + pf_jj];
+  int k_next = 0;  /* first pair (row-major number) not yet decided */
+  int n_events = 0;
+  bool rho_ready = false;
+  bool finished = false;
+
+  while (n_events < SPEC_MAX_EVENTS) {
+    /* parallel pass: first firing pair with cnt >= k_next under the current state */
+    unsigned long long best = ~0ull;
+    for (int q = lane; q < nQ; q += WAVE) {
+      int lo = 0, hi = n - 2; /* row: largest i with sQ[i] <= q */
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sQ[mid] <= q)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int i = lo;
+      const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
+      const int dlast = (4 * b + 3 < i + n - 1) ? 4 * b + 3 : i + n - 1;
+      if (pair_cnt(i, dlast - i, n) < k_next) continue;
+      lm_u32x4 blk = {{0u, 0u, 0u, 0u}};
+      if (!v.rnd_stream) blk = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)(4 * b), 0);
+      const double xi_i = sX[i], R_i = sR[i], u_i = sU[i], w_i = sW[i];
+      const int ind_i = sIND[i];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int d = 4 * b + t;
+        const int j = d - i;
+        if (j <= i || j >= n) continue;
+        const int cnt = pair_cnt(i, j, n);
+        if (cnt < k_next) continue;
+        const double Ec = use_tab ? v.E[ind_i * v.nb + sIND[j]] : v.E_const;
+        const double xi_j = sX[j];
+        const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
+        const double du = u_i - sU[j], dw = w_i - sW[j];
+        const double p_crit = xi_max * v.dt_col_over_dV * lmp_kernel_hydro(P, R_i, sR[j], Ec, sqrt(du * du + dw * dw));
+        bool fires;
+        if (p_crit > 1.0)
+          fires = true;
+        else if (v.rnd_stream)
+          fires = p_crit > v.rnd_stream[roff + cnt];
+        else
+          fires = pair_fires(p_crit, lm_word(blk, (uint32_t)t), v.seed, key_run, key_step, key_cell, (uint32_t)i,
+                             (uint32_t)d);
+        if (fires) {
+          const unsigned long long key = ((unsigned long long)cnt << 32) | ((unsigned)i << 16) | (unsigned)j;
+          best = key < best ? key : best;
+          break;
+        }
       }
     }
-    for (int i = i_lo + ((lane - i_lo) & (WAVE - 1)); i <= i_hi; i += WAVE) {
-      const int j = d - i;
-      /* A word of this pair's uniform (independent of the state: issued before the state reads) */
-      uint32_t aw = 0;
-      double rnd_inj = 0.0;
-      if (v.rnd_stream) {
-        const int cnt = i * n - i * (i + 1) / 2 + (j - i - 1);
-        rnd_inj = v.rnd_stream[roff + cnt];
-      } else {
-        const int tag = (i << 16) | (d >> 2);
-        const bool cls = (i >> 6) & 1;
-        if (tag != (cls ? tag1 : tag0)) {
-          const lm_u32x4 o = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d, 0);
-          if (cls) {
-            tag1 = tag;
-            blk1 = o;
-          } else {
-            tag0 = tag;
-            blk0 = o;
-          }
-        }
-        aw = lm_word(cls ? blk1 : blk0, (uint32_t)d & 3);
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(best, off);
+      best = o < best ? o : best;
+    }
+    if (best == ~0ull) {
+      finished = true;
+      break;
+    }
+    if (!rho_ready) {
+      /* first event of the cell: the pre-step densities from the still unchanged masses */
+      for (int p = lane; p < n; p += WAVE) {
+        const int k = sSL[p];
+        double R, ws, rho;
+        lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
+        rho_g[k] = rho;
       }
-      double Ec;
-      if (!use_tab) {
-        Ec = v.E_const;
-      } else {
-        const int ii = sIND[i], jj = sIND[j];
-        Ec = (i == cur_i && ii == cur_ii && jj == cur_jj) ? cur_E : v.E[ii * v.nb + jj];
-      }
-      const double xi_i = sX[i], xi_j = sX[j];
-      int imin, imax;
-      if (xi_i <= xi_j) {
-        imin = i;
-        imax = j;
-      } else {
-        imin = j;
-        imax = i;
-      }
-      const double xi_min = imin == i ? xi_i : xi_j, xi_max = imin == i ? xi_j : xi_i;
-      const double du = sU[i] - sU[j], dw = sW[i] - sW[j];
-      const double p_crit = xi_max * dtdV * lmp_kernel_hydro(P, sR[i], sR[j], Ec, sqrt(du * du + dw * dw));
-      if (p_crit > 1.0) {
-        const int kmin = sSL[imin], kmax = sSL[imax];
-        const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
-        const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
-        const double xi_col = p_crit * xi_min;
-        const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
-        v.m_w[kmin] = mwn;
-        v.m_s[kmin] = msn;
-        const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
-        sR[imin] = Rn;
-        sX[imax] = xi_max - xi_col;
-        if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+      rho_ready = true;
+      __threadfence_block();
+      __syncthreads();
+    }
+    const int ei = (int)((best >> 16) & 0xffff), ej = (int)(best & 0xffff);
+    const int ecnt = (int)(best >> 32);
+    if (lane == 0) {
+      const double Ec = use_tab ? v.E[sIND[ei] * v.nb + sIND[ej]] : v.E_const;
+      const int kind = apply_event(P, v, cc, rho_g, ei, ej, pair_p(P, v, cc, ei, ej, Ec));
+      if (kind == 1)
         ++c_multi;
-      } else if (v.rnd_stream ? (p_crit > rnd_inj) : pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d)) {
+      else
         ++c_single;
-        const int kmin = sSL[imin], kmax = sSL[imax];
-        const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
-        const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
-        const double xi_rel_dev = (xi_max - xi_min) / xi_max;
-        if (xi_rel_dev < 1.0E-5) {
-          ++c_eq;
-          const double xi_ges = xi_min + xi_max;
-          const double mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
-          const double msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
-          v.m_w[kmin] = mwn;
-          v.m_s[kmin] = msn;
-          v.m_w[kmax] = mwn;
-          v.m_s[kmax] = msn;
-          const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
-          sR[imin] = Rn;
-          sR[imax] = Rn;
-          const double xmax_new = 0.5 * 0.7 * xi_ges;
-          sX[imax] = xmax_new;
-          sX[imin] = 0.5 * xi_ges - xmax_new;
-          if (!v.e_const_mode) {
-            const int ind = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
-            sIND[imin] = ind;
-            sIND[imax] = ind;
-          }
+      if (kind == 3) ++c_eq;
+    }
+    __threadfence_block();
+    __syncthreads();
+    ++n_events;
+    k_next = ecnt + 1;
+  }
+
+  if (!finished) {
+    /* many events: anti-diagonal schedule over the pairs with cnt >= k_next */
+    int tag0 = -1, tag1 = -1;
+    lm_u32x4 blk0, blk1;
+    blk0.v[0] = blk0.v[1] = blk0.v[2] = blk0.v[3] = 0;
+    blk1 = blk0;
+    for (int d = 1; d <= 2 * n - 3; ++d) {
+      const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
+      const int i_hi = (d - 1) >> 1;
+      for (int i = i_lo + ((lane - i_lo) & (WAVE - 1)); i <= i_hi; i += WAVE) {
+        const int j = d - i;
+        const int cnt = pair_cnt(i, j, n);
+        if (cnt < k_next) continue;
+        uint32_t aw = 0;
+        double rnd_inj = 0.0;
+        if (v.rnd_stream) {
+          rnd_inj = v.rnd_stream[roff + cnt];
         } else {
-          const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
-          v.m_w[kmin] = mwn;
-          v.m_s[kmin] = msn;
-          const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
-          sR[imin] = Rn;
-          sX[imax] = xi_max - xi_min;
-          if (!v.e_const_mode) sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+          const int tag = (i << 16) | (d >> 2);
+          const bool cls = (i >> 6) & 1;
+          if (tag != (cls ? tag1 : tag0)) {
+            const lm_u32x4 o = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d, 0);
+            if (cls) {
+              tag1 = tag;
+              blk1 = o;
+            } else {
+              tag0 = tag;
+              blk0 = o;
+            }
+          }
+          aw = lm_word(cls ? blk1 : blk0, (uint32_t)d & 3);
+        }
+        const double Ec = use_tab ? v.E[sIND[i] * v.nb + sIND[j]] : v.E_const;
+        const double p_crit = pair_p(P, v, cc, i, j, Ec);
+        bool fires;
+        if (p_crit > 1.0)
+          fires = true;
+        else if (v.rnd_stream)
+          fires = p_crit > rnd_inj;
+        else
+          fires = pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d);
+        if (fires) {
+          const int kind = apply_event(P, v, cc, rho_g, i, j, p_crit);
+          if (kind == 1)
+            ++c_multi;
+          else
+            ++c_single;
+          if (kind == 3) ++c_eq;
         }
       }
+      __threadfence_block();
+      __syncthreads();
     }
-    __syncthreads();
   }
-  for (int m = lane; m < n; m += WAVE) v.xi[sSL[m]] = sX[m];
+
+  if (n_events > 0)
+    for (int m = lane; m < n; m += WAVE) v.xi[sSL[m]] = sX[m];
   for (int off = WAVE / 2; off > 0; off >>= 1) {
     c_single += __shfl_down(c_single, off);
     c_multi += __shfl_down(c_multi, off);

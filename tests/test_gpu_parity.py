@@ -150,3 +150,36 @@ def test_batched_runs_match_single(gpu):
     # other runs differ only by their draws
     nc1 = eng.counters(1)[0]
     assert nc1.sum() > 0
+
+
+@pytest.mark.parametrize("kernel", ["Long_Bott", "const"])
+def test_collision_many_events(gpu, kernel):
+    """Dense rain-like cells (many events per cell): exercises the first-event passes and the
+    anti-diagonal continuation of the collision kernel; Philox draws, bit-exact vs the oracle."""
+    if kernel == "const":
+        E, lnR0, lnf, nb = 0.9, 0.0, 1.0, 0
+    else:
+        E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = make_case(nx=6, nz=6, spc=60, seed=17, cloud_base=0.0, cloud_top=120.0, R_drop=25.0,
+                             sigma_drop=1.6)
+    parts = list(_prep(grid, parts))
+    rng = np.random.default_rng(5)
+    parts[1] = np.array([parts[1][0], -rng.uniform(0.0, 3.0, parts[1].shape[1])])  # fall speeds
+    parts[5] = parts[5] * rng.uniform(3.0, 8.0, parts[5].size)                     # dense multiplicities
+    parts = tuple(parts)
+    pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 99)
+    no_cols = np.zeros(2, dtype=np.int64)
+    for step in range(3):
+        orc.collision_all_cells(xi, m_w, m_s, vel, grid.temperature.copy(), cells, grid.no_cells,
+                                cfg["dt_col"] / grid.volume_cell, E, nb, lnR0, lnf, no_cols, seed=99, run=0,
+                                step=step)
+        eng.collision_step()
+    got = eng.export_particles(0)
+    _assert_same(got[5], xi, "xi")
+    _assert_same(got[3], m_w, "m_w")
+    _assert_same(got[4], m_s, "m_s")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    assert no_cols[0] > 36 * 3 * 20, f"expected many events per cell, got {no_cols}"
+    assert np.isfinite(xi).all()
