@@ -167,40 +167,10 @@ class Engine:
     def load_particles(self, runs):
         """runs: list of (pos, vel, cells, m_w, m_s, xi, active_ids) in the reference layout."""
         assert len(runs) == self.n_runs
-        nc, nz = self.nc, self.nz
-        host = {f: np.zeros(max(self.n_tot, 1), dtype=np.float64) for f in ("x", "z", "u", "w", "m_w", "m_s", "xi")}
-        hid = np.zeros(max(self.n_tot, 1), dtype=np.int32)
-        hcell = np.zeros(max(self.n_tot, 1), dtype=np.int32)
-        seg = np.zeros(self.n_runs * (nc + 1) + 1, dtype=np.int32)
-        for r, (pos, vel, cells, m_w, m_s, xi, active) in enumerate(runs):
-            N = self.n_per_run[r]
-            assert len(xi) == N, "SIP count differs from the engine's allocation"
-            active = np.asarray(active, dtype=bool)
-            cells = np.asarray(cells, dtype=np.int64)
-            ids = np.arange(N)
-            key = cells[0] * nz + cells[1]
-            act = ids[active]
-            if np.any((cells[0][active] < 0) | (cells[0][active] >= self.nx) |
-                      (cells[1][active] < 0) | (cells[1][active] >= nz)):
-                raise ValueError("active SIP outside the grid")
-            order = act[np.lexsort((act, key[active]))]
-            dead = ids[~active]
-            perm = np.concatenate([order, dead])
-            b = int(self.base[r])
-            sl = slice(b, b + N)
-            host["x"][sl], host["z"][sl] = pos[0][perm], pos[1][perm]
-            host["u"][sl], host["w"][sl] = vel[0][perm], vel[1][perm]
-            host["m_w"][sl], host["m_s"][sl], host["xi"][sl] = m_w[perm], m_s[perm], xi[perm]
-            hid[sl] = perm
-            hcell[b:b + len(order)] = r * nc + key[order]
-            hcell[b + len(order):b + N] = -(2 + cells[0][dead])
-            counts = np.bincount(key[active], minlength=nc)
-            starts = b + np.concatenate([[0], np.cumsum(counts)])
-            seg[r * (nc + 1):r * (nc + 1) + nc + 1] = starts
-        seg[-1] = self.n_tot
+        host, hid, hcell, seg = host_layout(runs, self.nx, self.nz, self.n_per_run)
         cur = self.e.cur
         for f, a in host.items():
-            self.buf[f"{f}{cur}"].copy_(torch.from_numpy(a[:max(self.n_tot, 1)]))
+            self.buf[f"{f}{cur}"].copy_(torch.from_numpy(a))
         self.buf[f"id{cur}"].copy_(torch.from_numpy(hid))
         self.buf[f"cell{self.e.ccur}"].copy_(torch.from_numpy(hcell))
         self.buf[f"seg{cur}"].copy_(torch.from_numpy(seg))
@@ -264,22 +234,9 @@ class Engine:
         cur, ccur = self.e.cur, self.e.ccur
         b, N = int(self.base[run]), self.n_per_run[run]
         g = {f: self.buf[f"{f}{cur}"][b:b + N].cpu().numpy() for f in ("x", "z", "u", "w", "m_w", "m_s", "xi")}
-        ids = self.buf[f"id{cur}"][b:b + N].cpu().numpy().astype(np.int64)
-        cell = self.buf[f"cell{ccur}"][b:b + N].cpu().numpy().astype(np.int64)
-        pos = np.zeros((2, N))
-        vel = np.zeros((2, N))
-        cells = np.zeros((2, N), dtype=np.int64)
-        m_w, m_s, xi = np.zeros(N), np.zeros(N), np.zeros(N)
-        active = np.zeros(N, dtype=bool)
-        pos[0, ids], pos[1, ids] = g["x"], g["z"]
-        vel[0, ids], vel[1, ids] = g["u"], g["w"]
-        m_w[ids], m_s[ids], xi[ids] = g["m_w"], g["m_s"], g["xi"]
-        act = cell >= 0
-        loc = np.where(act, cell - run * self.nc, 0)
-        cells[0, ids] = np.where(act, loc // self.nz, -cell - 2)
-        cells[1, ids] = np.where(act, loc % self.nz, -1)
-        active[ids] = act
-        return pos, vel, cells, m_w, m_s, xi, active
+        ids = self.buf[f"id{cur}"][b:b + N].cpu().numpy()
+        cell = self.buf[f"cell{ccur}"][b:b + N].cpu().numpy()
+        return host_unlayout(g, ids, cell, run * self.nc, self.nz)
 
     def export_grid(self, run):
         self.check()
@@ -310,3 +267,70 @@ def host_material_properties(T, p, r_v, rho_d):
         o[5, k] = 1.458E-6 * math.pow(t, 1.5) / (t + 110.4)
         o[6, k] = rd * (1 + rv)
     return out
+
+
+def host_layout(runs, nx, nz, n_per_run):
+    """Engine layout of reference-layout runs (host side, pure numpy).
+
+    SIPs of each run are ordered by (current cell, id) -- the reference visits
+    cells x-major and the SIPs of a cell in ascending id -- with the inactive
+    SIPs (cells[1] == -1) after the cells in ascending id.  Returns the SoA
+    arrays, ids, engine cell codes (global cell, or -(2 + cells[0]) when
+    inactive) and the segment table (per run: nc cell starts + graveyard start,
+    then the total).
+    """
+    nc = nx * nz
+    n_tot = int(sum(n_per_run))
+    n = max(n_tot, 1)
+    host = {f: np.zeros(n, dtype=np.float64) for f in ("x", "z", "u", "w", "m_w", "m_s", "xi")}
+    hid = np.zeros(n, dtype=np.int32)
+    hcell = np.zeros(n, dtype=np.int32)
+    seg = np.zeros(len(runs) * (nc + 1) + 1, dtype=np.int32)
+    base = 0
+    for r, (pos, vel, cells, m_w, m_s, xi, active) in enumerate(runs):
+        N = int(n_per_run[r])
+        assert len(xi) == N, "SIP count differs from the engine's allocation"
+        active = np.asarray(active, dtype=bool)
+        cells = np.asarray(cells, dtype=np.int64)
+        ids = np.arange(N)
+        key = cells[0] * nz + cells[1]
+        ca = cells[:, active]
+        if np.any((ca[0] < 0) | (ca[0] >= nx) | (ca[1] < 0) | (ca[1] >= nz)):
+            raise ValueError("active SIP outside the grid")
+        act = ids[active]
+        order = act[np.lexsort((act, key[active]))]
+        dead = ids[~active]
+        perm = np.concatenate([order, dead])
+        sl = slice(base, base + N)
+        host["x"][sl], host["z"][sl] = pos[0][perm], pos[1][perm]
+        host["u"][sl], host["w"][sl] = vel[0][perm], vel[1][perm]
+        host["m_w"][sl], host["m_s"][sl], host["xi"][sl] = m_w[perm], m_s[perm], xi[perm]
+        hid[sl] = perm
+        hcell[base:base + len(order)] = r * nc + key[order]
+        hcell[base + len(order):base + N] = -(2 + cells[0][dead])
+        counts = np.bincount(key[active], minlength=nc)
+        seg[r * (nc + 1):r * (nc + 1) + nc + 1] = base + np.concatenate([[0], np.cumsum(counts)])
+        base += N
+    seg[-1] = n_tot
+    return host, hid, hcell, seg
+
+
+def host_unlayout(g, ids, cell, cell0, nz):
+    """Inverse of host_layout for one run: engine arrays -> reference layout (ID order)."""
+    N = len(ids)
+    ids = ids.astype(np.int64)
+    cell = cell.astype(np.int64)
+    pos = np.zeros((2, N))
+    vel = np.zeros((2, N))
+    cells = np.zeros((2, N), dtype=np.int64)
+    m_w, m_s, xi = np.zeros(N), np.zeros(N), np.zeros(N)
+    active = np.zeros(N, dtype=bool)
+    pos[0, ids], pos[1, ids] = g["x"], g["z"]
+    vel[0, ids], vel[1, ids] = g["u"], g["w"]
+    m_w[ids], m_s[ids], xi[ids] = g["m_w"], g["m_s"], g["xi"]
+    act = cell >= 0
+    loc = np.where(act, cell - cell0, 0)
+    cells[0, ids] = np.where(act, loc // nz, -cell - 2)
+    cells[1, ids] = np.where(act, loc % nz, -1)
+    active[ids] = act
+    return pos, vel, cells, m_w, m_s, xi, active

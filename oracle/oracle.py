@@ -118,8 +118,8 @@ def adv_params(grid, N, config, seed=0, run=0, col_step=0, rnd_pos=0, E_col_grid
     p.no_iter = config["no_iter_impl_mass"]
     p.solute = 0 if config["solute_type"] == "AS" else 1
     p.g_set = config["g_set"]
-    if isinstance(E_col_grid, float):
-        p.nb, p.E_const = 0, E_col_grid
+    if isinstance(E_col_grid, float) or E_col_grid is None:
+        p.nb, p.E_const = 0, (E_col_grid if E_col_grid is not None else 0.5)
     else:
         p.nb = config["no_kernel_bins"]
         p.lnR0 = config["R_kernel_low_log"]
