@@ -179,11 +179,13 @@ int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *ma
 /*
  * Batched box model with the E_col(R1,R2) table and 1-D Beard velocities
  * (collision/all_or_nothing.py:269-390 collision_step_Ecol_grid_R_np followed
- * by collision/kernel.py:193-210 update_velocity_Beard).
+ * by collision/kernel.py:193-210 update_velocity_Beard when update_beard != 0,
+ * as collision/box_model.py:164-176 does after every step).
  */
 int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double *radii, double *vel,
                      const double *rho, double dt_over_dV, const double *E, int32_t nb, double lnR0, double lnf,
-                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream);
+                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, int32_t update_beard,
+                     void *stream);
 
 #ifdef __cplusplus
 }

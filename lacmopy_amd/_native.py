@@ -99,7 +99,7 @@ def lib():
     L.lm_adv_steps.argtypes = [E, _i32, _p]
     L.lm_golovin_step.argtypes = [_i32, _p, _p, _p, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
     L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
-                                   C.c_uint64, C.c_uint32, _p, _p]
+                                   C.c_uint64, C.c_uint32, _p, _i32, _p]
     L.lm_math_probe.argtypes = [_i32, _i64, _p, _p, _p, _p]
     L.lm_prof_enable.argtypes = [E, _i32]
     L.lm_prof_collect.argtypes = [E, C.POINTER(_d), C.POINTER(_i32), _i32]

@@ -429,7 +429,6 @@ __device__ static void fill_cc(const LmPhys &P, const double *scal_r, const doub
   C[CC_C2] = P.R_v * T / (D_v * e_s);          /* microphysics.py:1459 */
   C[CC_RHOW] = lmp_density_water(P, T);
   C[CC_SIGW] = lmp_sigma_water(T);
-  C[15] = 0.0;
 }
 
 __global__ void k_init_cc(KView v) {
@@ -437,7 +436,9 @@ __global__ void k_init_cc(KView v) {
   const int gc = blockIdx.x * blockDim.x + threadIdx.x;
   if (gc >= v.n_runs * v.nc) return;
   const int r = gc / v.nc, c = gc % v.nc;
-  fill_cc(P, v.scal + (size_t)r * 10 * v.nc, v.mat + (size_t)r * 7 * v.nc, c, v.nc, v.cc + (size_t)gc * CC_N);
+  double *C = v.cc + (size_t)gc * CC_N;
+  fill_cc(P, v.scal + (size_t)r * 10 * v.nc, v.mat + (size_t)r * 7 * v.nc, c, v.nc, C);
+  C[CC_TP] = C[CC_T];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -994,6 +995,7 @@ __global__ __launch_bounds__(256) void k_grid(KView v) {
   double *M = v.mat + (size_t)r * 7 * nc;
   const double rho_d = S[3 * nc + c];
   const double dml = v.dml[gc];
+  v.cc[(size_t)gc * CC_N + CC_TP] = S[0 * nc + c]; /* particle temperature of this substep (integration.py:489-507) */
   double Th = S[2 * nc + c] + v.dTh[gc];
   double r_v = S[4 * nc + c];
   r_v += v.dRv[gc] - dml * S[8 * nc + c];
@@ -1402,7 +1404,7 @@ __global__ __launch_bounds__(WAVE) void k_ecol_box(int nbox, const int32_t *off,
                                                    double *radii, double *vel, const double *rho, double dt_over_dV,
                                                    const double *E, int nb, double lnR0, double lnf,
                                                    long long *no_cols, unsigned long long seed, unsigned step,
-                                                   const double *rnd_stream, int32_t *err) {
+                                                   const double *rnd_stream, int32_t *err, int update_beard) {
   const LmPhys &P = c_P;
   const int b = blockIdx.x, lane = threadIdx.x;
   const int s0 = off[b], n = off[b + 1] - off[b];
@@ -1485,7 +1487,7 @@ __global__ __launch_bounds__(WAVE) void k_ecol_box(int nbox, const int32_t *off,
     xis[s0 + m] = sX[m];
     masses[s0 + m] = sM[m];
     radii[s0 + m] = sR[m];
-    vel[s0 + m] = beard_velocity(sR[m]);
+    if (update_beard) vel[s0 + m] = beard_velocity(sR[m]);
   }
   for (int o = WAVE / 2; o > 0; o >>= 1) {
     cs += __shfl_down(cs, o);
@@ -1815,7 +1817,8 @@ int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *ma
 
 int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double *radii, double *vel,
                      const double *rho, double dt_over_dV, const double *E, int32_t nb, double lnR0, double lnf,
-                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
+                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, int32_t update_beard,
+                     void *stream) {
   if (n_boxes <= 0 || !off || !xis || !masses || !radii || !vel || !rho || !E || !no_cols)
     return set_err(LM_ERR_ARG, "bad box arguments");
   if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
@@ -1826,7 +1829,7 @@ int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *m
   }
   hipLaunchKernelGGL(k_ecol_box, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, radii, vel,
                      rho, dt_over_dV, E, nb, lnR0, lnf, (long long *)no_cols, (unsigned long long)seed, step,
-                     rnd_stream, err_dev);
+                     rnd_stream, err_dev, update_beard);
   LAUNCH_CHECK();
   return 0;
 }

@@ -6,7 +6,7 @@
  * so the results are bit-identical to the reference's arithmetic given the
  * same transcendentals (lm_math.h).  Per-cell quantities that the reference
  * recomputes for every SIP (l_alpha, l_beta, c1, c2, rho_w(T), sigma_w(T))
- * are evaluated once per cell into LmCell by the grid kernel: same
+ * are evaluated once per cell into the per-cell table by the grid kernel: same
  * expression, same inputs, same value.
  */
 #ifndef LM_PHYSICS_H
@@ -30,7 +30,9 @@ struct LmPhys {
 /* Per-cell constants read by the particle kernel (16 doubles, 128 B). */
 enum {
   CC_T = 0, CC_P, CC_S, CC_ES, CC_LV, CC_K, CC_DV, CC_MU, CC_RHOF,
-  CC_LALPHA, CC_LBETA, CC_C1, CC_C2, CC_RHOW, CC_SIGW, CC_N = 16
+  CC_LALPHA, CC_LBETA, CC_C1, CC_C2, CC_RHOW, CC_SIGW,
+  CC_TP, /* cell temperature before the last grid substep = T_p of its SIPs */
+  CC_N = 16
 };
 
 #define LMD __device__ static inline
