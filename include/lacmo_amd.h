@@ -32,6 +32,7 @@ extern "C" {
 #define LM_ERR_LOST_SIP 1003   /* a SIP moved more than one cell between reindexes */
 #define LM_ERR_REM_CAP 1004    /* too many SIPs reached the ground in one substep */
 #define LM_ERR_OUT_OF_DOMAIN 1005 /* a SIP landed on x == x_max (cell index nx) */
+#define LM_ERR_RND_SHORT 1006  /* the injected uniform stream ran out */
 
 typedef struct LmEngine {
   /* geometry */
@@ -87,7 +88,7 @@ typedef struct LmEngine {
   /* injected uniform stream (parity mode; NULL = Philox) */
   const double *rnd_stream;
   int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */
-  int64_t rnd_pos;  /* stream position consumed so far */
+  int64_t rnd_len;  /* number of values in rnd_stream */
   /* optional kernel timing (lm_prof_*); NULL = off */
   void *prof;
 } LmEngine;
@@ -143,6 +144,27 @@ int lm_advection_increments(LmEngine *e, int32_t second, void *stream);
  */
 int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream);
 
+/*
+ * The particle half of lm_substep alone (integration.py:944-1089,
+ * propagate_particles_subloop_step_np): leaves the per-cell Delta m_l of the
+ * substep in dml for a following lm_grid_substep.
+ */
+int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream);
+
+/*
+ * The grid half of lm_substep alone (integration.py:402-483,
+ * propagate_grid_subloop_step_np): applies dTh, dRv (advection increments)
+ * and dml (condensation) to every cell, then the material properties.
+ */
+int lm_grid_substep(LmEngine *e, void *stream);
+
+/*
+ * grid.py:370-412 update_grid_r_l_np: liquid water mixing ratio of every
+ * cell, r_l = 1e-18 * m_dry^-1 * sum(xi * m_w) over the cell's active SIPs in
+ * ascending id, written into grid_scalar_fields[5] (scal).
+ */
+int lm_update_r_l(LmEngine *e, void *stream);
+
 /* n_steps advection steps (integration.py:1298-1611, integrate_adv_step_np). */
 int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream);
 
@@ -171,21 +193,67 @@ int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, do
 /*
  * Batched box model: one collision step in each of n_boxes independent boxes
  * (collision/all_or_nothing.py:109-185 collision_step_Golovin_np).  Box b
- * holds SIPs [off[b], off[b+1]) of xis/masses; counters no_cols[2*b..].
+ * holds SIPs [off[b], off[b+1]) of xis/masses (at most 1024); counters
+ * no_cols[2*b..].  Draws: Philox keyed (seed, run = b, step, cell = 0, i, j),
+ * or, if rnd_stream is not NULL, its values in row-major pair order, box
+ * after box; the caller guarantees it holds sum_b n_b (n_b - 1) / 2 values.
+ * A box over capacity is left unchanged and reported by lm_box_check.
  */
 int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream);
 
 /*
+ * Batched box model with a kernel K(m1, m2) tabulated on a logarithmic mass
+ * grid (collision/all_or_nothing.py:392-491 collision_step_kernel_grid_m_np;
+ * bin index as :35-65 compute_kernel_index of the mass).  Boxes, draws,
+ * stream length and capacity as for lm_golovin_step.
+ */
+int lm_kernel_grid_m_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
+                          const double *kernel_grid, int32_t nb, double m_kernel_low_log, double bin_factor_m_log,
+                          int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream);
+
+/*
  * Batched box model with the E_col(R1,R2) table and 1-D Beard velocities
  * (collision/all_or_nothing.py:269-390 collision_step_Ecol_grid_R_np followed
  * by collision/kernel.py:193-210 update_velocity_Beard when update_beard != 0,
- * as collision/box_model.py:164-176 does after every step).
+ * as collision/box_model.py:164-176 does after every step).  Draws, stream
+ * length and capacity as for lm_golovin_step.
  */
 int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double *radii, double *vel,
                      const double *rho, double dt_over_dV, const double *E, int32_t nb, double lnR0, double lnf,
                      int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, int32_t update_beard,
                      void *stream);
+
+/*
+ * One collision step in each of n_boxes boxes of two-component SIPs with
+ * caller-supplied radii, densities and 2-D velocities:
+ * collision/all_or_nothing.py:498-635 collision_step_Ecol_grid_R_2D_multicomp_np
+ * (table E, nb > 0) and :637-751 collision_step_Ecol_const_2D_multicomp_np
+ * (E == NULL, constant E_const).  Box b holds SIPs off[b] .. off[b+1]-1 (at
+ * most 1024).  xis, m_w, m_s, radii are updated in place; no_cols is
+ * (n_boxes, 2).  Draws, stream length and capacity as for lm_golovin_step.
+ */
+int lm_box2d_step(int32_t n_boxes, const int32_t *off, double *xis, double *m_w, double *m_s, double *radii,
+                  const double *vel_x, const double *vel_z, const double *rho, double dt_over_dV, const double *E,
+                  double E_const, int32_t nb, double lnR0, double lnf, int64_t *no_cols, uint64_t seed, uint32_t step,
+                  const double *rnd_stream, void *stream);
+
+/*
+ * Synchronises the stream and reports a box that exceeded the 1024-SIP
+ * capacity of the box kernels since the last check (LM_ERR_CELL_CAP; such a
+ * box is left unchanged).  No counterpart in the reference, which has no
+ * capacity limit.
+ */
+int lm_box_check(void *stream);
+
+/*
+ * integration.py:150-286 compute_divergence_upwind_np for n_fields stacked
+ * (nx, nz) fields with one (2, nx+1, nz+1) flux field: limited third-order
+ * upwind flux divergence, boundary type 0 = periodic, 1 = fixed per axis.
+ * out is (n_fields, nx, nz).
+ */
+int lm_divergence_upwind(const double *fields, const double *flux, double *out, int32_t n_fields, int32_t nx,
+                         int32_t nz, double dx, double dz, int32_t bc_x, int32_t bc_z, void *stream);
 
 #ifdef __cplusplus
 }

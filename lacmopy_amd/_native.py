@@ -16,6 +16,7 @@ LM_ERR_NAMES = {
     1003: "LM_ERR_LOST_SIP",
     1004: "LM_ERR_REM_CAP",
     1005: "LM_ERR_OUT_OF_DOMAIN",
+    1006: "LM_ERR_RND_SHORT",
 }
 
 _p = C.c_void_p
@@ -50,20 +51,22 @@ class LmEngine(C.Structure):
         ("rem_cnt", _p), ("rem_slot", _p),
         ("rem_cap", _i32), ("pad0", _i32),
         ("scan_buf", _p), ("scan_part", _p), ("err", _p), ("ovf_list", _p), ("ovf_cnt", _p),
-        ("rnd_stream", _p), ("rnd_off", _p), ("rnd_pos", _i64),
+        ("rnd_stream", _p), ("rnd_off", _p), ("rnd_len", _i64),
         ("prof", _p),
     ]
 
 
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
-                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_pos"]
+                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",
     "lm_get_constants", "lm_num_constants", "lm_engine_init_cells", "lm_collision_step",
     "lm_position_step", "lm_reindex", "lm_advection_increments", "lm_substep",
     "lm_adv_steps", "lm_engine_check", "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe",
-    "lm_prof_enable", "lm_prof_collect", "lm_prof_disable",
+    "lm_prof_enable", "lm_prof_collect", "lm_prof_disable", "lm_particle_substep", "lm_grid_substep",
+    "lm_box2d_step", "lm_box_check", "lm_divergence_upwind", "lm_update_r_l",
+    "lm_kernel_grid_m_step",
 ]
 
 _lib = None
@@ -90,12 +93,19 @@ def lib():
     L.lm_engine_offsets.argtypes = [C.POINTER(_i64), _i32]
     L.lm_get_constants.argtypes = [C.POINTER(_d), _i32]
     L.lm_num_constants.restype = _i32
-    for name in ["lm_engine_init_cells", "lm_collision_step", "lm_reindex", "lm_engine_check"]:
+    for name in ["lm_engine_init_cells", "lm_collision_step", "lm_reindex", "lm_engine_check", "lm_grid_substep",
+                 "lm_update_r_l"]:
         getattr(L, name).argtypes = [E, _p]
         getattr(L, name).restype = C.c_int
     L.lm_position_step.argtypes = [E, _d, _p]
     L.lm_advection_increments.argtypes = [E, _i32, _p]
     L.lm_substep.argtypes = [E, _d, _d, _p]
+    L.lm_particle_substep.argtypes = [E, _d, _d, _p]
+    L.lm_box2d_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _d, _i32, _d, _d, _p,
+                                C.c_uint64, C.c_uint32, _p, _p]
+    L.lm_box_check.argtypes = [_p]
+    L.lm_kernel_grid_m_step.argtypes = [_i32, _p, _p, _p, _d, _p, _i32, _d, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
+    L.lm_divergence_upwind.argtypes = [_p, _p, _p, _i32, _i32, _i32, _d, _d, _i32, _i32, _p]
     L.lm_adv_steps.argtypes = [E, _i32, _p]
     L.lm_golovin_step.argtypes = [_i32, _p, _p, _p, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
     L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
@@ -106,7 +116,8 @@ def lib():
     L.lm_prof_disable.argtypes = [E]
     for name in ["lm_position_step", "lm_advection_increments", "lm_substep", "lm_adv_steps",
                  "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe", "lm_prof_enable", "lm_prof_collect",
-                 "lm_prof_disable"]:
+                 "lm_prof_disable", "lm_particle_substep", "lm_grid_substep", "lm_box2d_step", "lm_box_check",
+                 "lm_divergence_upwind", "lm_kernel_grid_m_step"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -326,6 +326,7 @@ struct KView {
   int32_t *scan_buf, *scan_part, *err, *ovf_list, *ovf_cnt;
   const double *rnd_stream;
   long long *rnd_off;
+  long long rnd_len;
 };
 
 static KView make_view(const LmEngine *e) {
@@ -404,6 +405,7 @@ static KView make_view(const LmEngine *e) {
   v.ovf_cnt = e->ovf_cnt;
   v.rnd_stream = e->rnd_stream;
   v.rnd_off = (long long *)e->rnd_off;
+  v.rnd_len = e->rnd_len;
   return v;
 }
 
@@ -467,6 +469,13 @@ struct CollCell {
   int *sIND, *sSL;
   int n;
 };
+
+/* injected uniform k; past the end of the stream: error, and a value no pair fires on */
+__device__ static inline double rnd_at(const KView &v, long long k) {
+  if (k < v.rnd_len) return v.rnd_stream[k];
+  raise_err(v.err, LM_ERR_RND_SHORT, 0);
+  return 2.0;
+}
 
 /* p_crit of pair (i, j) with the current state (all_or_nothing.py:569-575) */
 __device__ static inline double pair_p(const LmPhys &P, const KView &v, const CollCell &cc, int i, int j, double Ec) {
@@ -681,7 +690,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         if (p_crit > 1.0)
           fires = true;
         else if (v.rnd_stream)
-          fires = p_crit > v.rnd_stream[roff + cnt];
+          fires = p_crit > rnd_at(v, roff + cnt);
         else
           fires = pair_fires(p_crit, lm_word(blk, (uint32_t)t), v.seed, key_run, key_step, key_cell, (uint32_t)i,
                              (uint32_t)d);
@@ -745,7 +754,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         uint32_t aw = 0;
         double rnd_inj = 0.0;
         if (v.rnd_stream) {
-          rnd_inj = v.rnd_stream[roff + cnt];
+          rnd_inj = rnd_at(v, roff + cnt);
         } else {
           const int tag = (i << 16) | (d >> 2);
           const bool cls = (i >> 6) & 1;
@@ -838,6 +847,27 @@ __global__ __launch_bounds__(1024) void k_scan_pairs(long long *a, int L) {
 /* ------------------------------------------------------------------------ */
 /* particle substep: one thread per SIP slot                                */
 /* ------------------------------------------------------------------------ */
+/*
+ * Between two reindexes the members of a cell are looked up in the 3x3 home
+ * segments around it, so a SIP may drift at most one cell (periodic in x) from
+ * the home cell of its slot k; the home cell is the segment that holds k.
+ */
+__device__ static inline bool near_home(const KView &v, int r, long long k, int ii, int jj) {
+  const int32_t *s = v.seg + (size_t)r * (v.nc + 1);
+  int lo = 0, hi = v.nc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s[mid] <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int hx = lo / v.nz, hz = lo % v.nz;
+  const int ddx = ((ii - hx) % v.nx + v.nx) % v.nx;
+  const int ddz = jj - hz;
+  return (ddx <= 1 || ddx == v.nx - 1) && ddz >= -1 && ddz <= 1;
+}
+
 __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, double dt_sub,
                                                    double dt_pos, int pos_only) {
   const LmPhys &P = c_P;
@@ -912,6 +942,7 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
       gc_new = gc;
     } else {
       gc_new = r * nc + (int)ii * v.nz + (int)jj;
+      if (gc_new != gc && !near_home(v, r, k, (int)ii, (int)jj)) raise_err(v.err, LM_ERR_LOST_SIP, (int)k);
     }
   }
   v.x[k] = x;
@@ -982,6 +1013,33 @@ __global__ __launch_bounds__(WAVE) void k_dml(KView v) {
     double s = 0.0;
     for (int p = 0; p < n; ++p) s += sVal[p];
     v.dml[gc] = s * 1.0E-18;
+  }
+}
+
+/* grid.py:370-412 update_grid_r_l_np: r_l = 1e-18 m_dry^-1 sum(xi m_w), summed over the cell in ascending id */
+__global__ __launch_bounds__(WAVE) void k_rl(KView v) {
+  const int nc = v.nc;
+  const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
+  __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
+  __shared__ unsigned char sSub[DML_CAP];
+  __shared__ double sVal[DML_CAP];
+  __shared__ NbrTab tb;
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
+  if (n > DML_CAP) {
+    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+    return;
+  }
+  __syncthreads();
+  rank_members(n, sId, sSub, &tb, sRank);
+  __syncthreads();
+  for (int m = threadIdx.x; m < n; m += WAVE) sVal[sRank[m]] = v.m_w[sSlot[m]] * v.xi[sSlot[m]];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) acc += sVal[p];
+    double *S = v.scal + (size_t)r * 10 * nc;
+    S[5 * nc + c] = acc * (1.0E-18 * S[8 * nc + c]);
   }
 }
 
@@ -1060,33 +1118,58 @@ __device__ static inline double face_flux(double a0, double a1, double a2, doubl
   return u >= 0.0 ? u * f_pos : u * f_neg;
 }
 
-/* integration.py:150-286, periodic x / solid z, at cell (i,j) */
+/* ghost-cell index of the extended field (integration.py:202-212): periodic (bc 0) wraps, fixed (bc 1) clamps */
+__device__ static inline int bc_index(int k, int n, int bc) {
+  if (bc == 0) return ((k % n) + n) % n;
+  return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+}
+
+/*
+ * integration.py:150-286 at cell (i, j): upwind flux through the faces
+ * i, i+1 (x) and j, j+1 (z); a fixed boundary (bc 1) has zero flux through
+ * the outer faces (:239-241, :283-285).  The default of the time step is
+ * periodic x, fixed z.
+ */
 __device__ static double divergence_at(const double *f, const double *flux, int i, int j, int nx, int nz, double dx,
-                                       double dz, double ot) {
+                                       double dz, double ot, int bcx = 0, int bcz = 1) {
   const int nfz = nz + 1;
   const double *FU = flux, *FW = flux + (size_t)(nx + 1) * nfz;
   double Fx[2];
   for (int s = 0; s < 2; ++s) {
     const int fi = i + s;
-    const int i0 = fi % nx, i1 = (fi - 1 + nx) % nx, i2 = (fi - 2 + 2 * nx) % nx, i3 = (fi + 1) % nx;
+    if (bcx == 1 && (fi == 0 || fi == nx)) {
+      Fx[s] = 0.0;
+      continue;
+    }
+    const int i0 = bc_index(fi, nx, bcx), i1 = bc_index(fi - 1, nx, bcx), i2 = bc_index(fi - 2, nx, bcx),
+              i3 = bc_index(fi + 1, nx, bcx);
     Fx[s] = face_flux(f[i0 * nz + j], f[i1 * nz + j], f[i2 * nz + j], f[i3 * nz + j], FU[fi * nfz + j], ot);
   }
   double Fz[2];
   for (int s = 0; s < 2; ++s) {
     const int fj = j + s;
-    if (fj == 0 || fj == nz) {
+    if (bcz == 1 && (fj == 0 || fj == nz)) {
       Fz[s] = 0.0;
       continue;
     }
-    const int j0 = fj > nz - 1 ? nz - 1 : fj;
-    const int j1 = fj - 1;
-    const int j2 = fj - 2 < 0 ? 0 : fj - 2;
-    const int j3 = fj + 1 > nz - 1 ? nz - 1 : fj + 1;
+    const int j0 = bc_index(fj, nz, bcz), j1 = bc_index(fj - 1, nz, bcz), j2 = bc_index(fj - 2, nz, bcz),
+              j3 = bc_index(fj + 1, nz, bcz);
     Fz[s] = face_flux(f[i * nz + j0], f[i * nz + j1], f[i * nz + j2], f[i * nz + j3], FW[i * nfz + fj], ot);
   }
   double out = (Fx[1] - Fx[0]) / dx;
   out = out + (Fz[1] - Fz[0]) / dz;
   return out;
+}
+
+/* divergence of n_fields stacked (nx, nz) fields with one flux field (integration.py:150-286) */
+__global__ void k_divergence(const double *fields, const double *flux, double *out, int n_fields, int nx, int nz,
+                             double dx, double dz, int bcx, int bcz) {
+  const LmPhys &P = c_P;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = nx * nz;
+  if (t >= n_fields * nc) return;
+  const int q = t / nc, c = t % nc;
+  out[t] = divergence_at(fields + (size_t)q * nc, flux, c / nz, c % nz, nx, nz, dx, dz, P.one_third, bcx, bcz);
 }
 
 __global__ void k_adv_incr(KView v, double dt_sub, int second, int act_relax) {
@@ -1292,9 +1375,14 @@ __device__ static long long box_stream_offset(const int32_t *off, int b) {
   return s;
 }
 
-/* all_or_nothing.py:109-185 */
+/*
+ * Single-component box model: all_or_nothing.py:109-185 (Golovin kernel,
+ * K == NULL) and :392-491 (kernel K(m1, m2) tabulated on a log-mass grid,
+ * index of kernel_index(m) as :35-65).
+ */
 __global__ __launch_bounds__(WAVE) void k_golovin(int nbox, const int32_t *off, double *xis, double *masses,
-                                                  double dt_over_dV, long long *no_cols, unsigned long long seed,
+                                                  double dt_over_dV, const double *K, int nb, double lnm0,
+                                                  double lnf, long long *no_cols, unsigned long long seed,
                                                   unsigned step, const double *rnd_stream, int32_t *err) {
   const int b = blockIdx.x, lane = threadIdx.x;
   const int s0 = off[b], n = off[b + 1] - off[b];
@@ -1303,9 +1391,11 @@ __global__ __launch_bounds__(WAVE) void k_golovin(int nbox, const int32_t *off, 
     return;
   }
   __shared__ double sX[BOX_CAP], sM[BOX_CAP];
+  __shared__ int sI[BOX_CAP];
   for (int m = lane; m < n; m += WAVE) {
     sX[m] = xis[s0 + m];
     sM[m] = masses[s0 + m];
+    sI[m] = K ? lmp_kernel_index(sM[m], lnm0, lnf, nb) : 0;
   }
   __syncthreads();
   const long long roff = rnd_stream ? box_stream_offset(off, b) : 0;
@@ -1325,12 +1415,14 @@ __global__ __launch_bounds__(WAVE) void k_golovin(int nbox, const int32_t *off, 
       }
       const double xi_min = sX[imin], xi_max = sX[imax];
       const double m_min = sM[imin], m_max = sM[imax];
-      const double p_crit = xi_max * ((m_min + m_max) * 1.5E-18) * dt_over_dV; /* kernel.py:461-480 */
+      /* kernel.py:461-480 or the table */
+      const double p_crit = xi_max * (K ? K[sI[i] * nb + sI[j]] : (m_min + m_max) * 1.5E-18) * dt_over_dV;
       if (p_crit > 1.0) {
         ++cm;
         const double xi_col = p_crit * xi_min;
         sM[imin] = (xi_min * m_min + xi_col * m_max) / xi_min;
         sX[imax] = xi_max - xi_col;
+        if (K) sI[imin] = lmp_kernel_index(sM[imin], lnm0, lnf, nb);
       } else if (p_crit > 0.0) {
         const long long cnt = (long long)i * n - (long long)i * (i + 1) / 2 + (j - i - 1);
         const double rnd = rnd_stream ? rnd_stream[roff + cnt]
@@ -1346,9 +1438,15 @@ __global__ __launch_bounds__(WAVE) void k_golovin(int nbox, const int32_t *off, 
             const double xmax_new = 0.5 * 0.7 * xi_ges;
             sX[imax] = xmax_new;
             sX[imin] = 0.5 * xi_ges - xmax_new;
+            if (K) {
+              const int ind = lmp_kernel_index(mn, lnm0, lnf, nb);
+              sI[imin] = ind;
+              sI[imax] = ind;
+            }
           } else {
             sM[imin] = m_min + m_max;
             sX[imax] = xi_max - xi_min;
+            if (K) sI[imin] = lmp_kernel_index(sM[imin], lnm0, lnf, nb);
           }
         }
       }
@@ -1500,6 +1598,123 @@ __global__ __launch_bounds__(WAVE) void k_ecol_box(int nbox, const int32_t *off,
 }
 
 /* elementwise FP64 math of the kernels, for the host/device parity test */
+/*
+ * One collision step in each of nbox independent well-mixed boxes of
+ * two-component SIPs with caller-supplied radii, 2-D velocities and densities:
+ * all_or_nothing.py:498-635 (E table; E == NULL selects the constant E_const of
+ * :637-751).  Pairs are visited by anti-diagonals d = i + j, which preserves the
+ * row-major sequential result; pair draws are keyed (seed, box, step, 0, i, j)
+ * or read from the injected stream in row-major order, box after box.
+ */
+__global__ __launch_bounds__(WAVE) void k_box2d(int nbox, const int32_t *off, double *xis, double *m_w,
+                                                double *m_s, double *radii, const double *vx, const double *vz,
+                                                const double *rho, double dt_over_dV, const double *E,
+                                                double E_const, int nb, double lnR0, double lnf, long long *no_cols,
+                                                unsigned long long seed, unsigned step, const double *rnd_stream,
+                                                int32_t *err) {
+  const LmPhys &P = c_P;
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int s0 = off[b], n = off[b + 1] - off[b];
+  if (n > BOX_CAP) {
+    if (lane == 0) raise_err(err, LM_ERR_CELL_CAP, b);
+    return;
+  }
+  __shared__ double sX[BOX_CAP], sMw[BOX_CAP], sMs[BOX_CAP], sR[BOX_CAP], sU[BOX_CAP], sW[BOX_CAP], sRho[BOX_CAP];
+  __shared__ int sI[BOX_CAP];
+  for (int m = lane; m < n; m += WAVE) {
+    sX[m] = xis[s0 + m];
+    sMw[m] = m_w[s0 + m];
+    sMs[m] = m_s[s0 + m];
+    sR[m] = radii[s0 + m];
+    sU[m] = vx[s0 + m];
+    sW[m] = vz[s0 + m];
+    sRho[m] = rho[s0 + m];
+    sI[m] = E ? lmp_kernel_index(sR[m], lnR0, lnf, nb) : 0;
+  }
+  __syncthreads();
+  const long long roff = rnd_stream ? box_stream_offset(off, b) : 0;
+  long long cs = 0, cm = 0;
+  for (int d = 1; d <= 2 * n - 3; ++d) {
+    const int i_lo = d - (n - 1) > 0 ? d - (n - 1) : 0;
+    const int i_hi = (d - 1) >> 1;
+    for (int i = i_lo + lane; i <= i_hi; i += WAVE) {
+      const int j = d - i;
+      int imin, imax;
+      if (sX[i] <= sX[j]) {
+        imin = i;
+        imax = j;
+      } else {
+        imin = j;
+        imax = i;
+      }
+      const double xi_min = sX[imin], xi_max = sX[imax];
+      const double mw_min = sMw[imin], mw_max = sMw[imax];
+      const double ms_min = sMs[imin], ms_max = sMs[imax];
+      const double du = sU[i] - sU[j], dw = sW[i] - sW[j];
+      const double Ec = E ? E[sI[i] * nb + sI[j]] : E_const;
+      const double p_crit = xi_max * dt_over_dV * lmp_kernel_hydro(P, sR[i], sR[j], Ec, sqrt(du * du + dw * dw));
+      if (p_crit > 1.0) {
+        ++cm;
+        const double xi_col = p_crit * xi_min;
+        sMw[imin] = mw_min + p_crit * mw_max;
+        sMs[imin] = ms_min + p_crit * ms_max;
+        sR[imin] = lmp_radius(P, (sMw[imin] + sMs[imin]), sRho[imin]);
+        sX[imax] = xi_max - xi_col;
+        if (E) sI[imin] = lmp_kernel_index(sR[imin], lnR0, lnf, nb);
+      } else {
+        const long long cnt = (long long)i * n - (long long)i * (i + 1) / 2 + (j - i - 1);
+        const double rnd = rnd_stream ? rnd_stream[roff + cnt]
+                                      : lm_pair_uniform(seed, (uint32_t)b, step, 0u, (uint32_t)i, (uint32_t)j);
+        if (p_crit > rnd) {
+          ++cs;
+          const double xi_rel_dev = (xi_max - xi_min) / xi_max;
+          if (xi_rel_dev < 1.0E-5) {
+            const double xi_ges = xi_min + xi_max;
+            const double mw = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
+            const double ms = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
+            sMw[imin] = mw;
+            sMs[imin] = ms;
+            sMw[imax] = mw;
+            sMs[imax] = ms;
+            const double Rn = lmp_radius(P, (mw + ms), sRho[imin]);
+            sR[imin] = Rn;
+            sR[imax] = Rn;
+            const double xmax_new = 0.5 * 0.7 * xi_ges;
+            sX[imax] = xmax_new;
+            sX[imin] = 0.5 * xi_ges - xmax_new;
+            if (E) {
+              const int ind = lmp_kernel_index(Rn, lnR0, lnf, nb);
+              sI[imin] = ind;
+              sI[imax] = ind;
+            }
+          } else {
+            sMw[imin] = mw_min + mw_max;
+            sMs[imin] = ms_min + ms_max;
+            sR[imin] = lmp_radius(P, (sMw[imin] + sMs[imin]), sRho[imin]);
+            sX[imax] = xi_max - xi_min;
+            if (E) sI[imin] = lmp_kernel_index(sR[imin], lnR0, lnf, nb);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int m = lane; m < n; m += WAVE) {
+    xis[s0 + m] = sX[m];
+    m_w[s0 + m] = sMw[m];
+    m_s[s0 + m] = sMs[m];
+    radii[s0 + m] = sR[m];
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    cs += __shfl_down(cs, o);
+    cm += __shfl_down(cm, o);
+  }
+  if (lane == 0) {
+    no_cols[2 * b] += cs;
+    no_cols[2 * b + 1] += cm;
+  }
+}
+
 __global__ void k_math_probe(int which, long long n, const double *x, const double *y, double *out) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -1575,7 +1790,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, scal),      (int64_t)offsetof(LmEngine, grid_vel),
       (int64_t)offsetof(LmEngine, no_cols),   (int64_t)offsetof(LmEngine, rem_cap),
       (int64_t)offsetof(LmEngine, scan_buf),  (int64_t)offsetof(LmEngine, rnd_stream),
-      (int64_t)offsetof(LmEngine, rnd_pos),
+      (int64_t)offsetof(LmEngine, rnd_len),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -1670,7 +1885,7 @@ int lm_advection_increments(LmEngine *e, int32_t second, void *stream) {
   return 0;
 }
 
-int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
+int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
   int rc = check_engine(e);
   if (rc) return rc;
   KView v = make_view(e);
@@ -1686,10 +1901,34 @@ int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
     hipLaunchKernelGGL(k_dml, dim3(ncell), dim3(WAVE), 0, S(stream), v);
   }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
-  hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
   LAUNCH_CHECK();
   e->ccur = 1 - e->ccur;
   return 0;
+}
+
+int lm_grid_substep(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  KView v = make_view(e);
+  const int ncell = e->n_runs * e->nc;
+  hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_update_r_l(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  KView v = make_view(e);
+  hipLaunchKernelGGL(k_rl, dim3(e->n_runs * e->nc), dim3(WAVE), 0, S(stream), v);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
+  int rc = lm_particle_substep(e, dt_sub, dt_sub_pos, stream);
+  if (rc) return rc;
+  return lm_grid_substep(e, stream);
 }
 
 static int subloop(LmEngine *e, int no_cond_steps, int no_col_steps, void *stream) {
@@ -1785,6 +2024,7 @@ int lm_engine_check(LmEngine *e, void *stream) {
                        : h[0] == LM_ERR_LOST_SIP ? "SIP moved more than one cell between reindexes"
                        : h[0] == LM_ERR_REM_CAP  ? "too many SIPs removed in one substep"
                        : h[0] == LM_ERR_OUT_OF_DOMAIN ? "SIP position outside the grid"
+                       : h[0] == LM_ERR_RND_SHORT ? "injected uniform stream shorter than the pairs it must cover"
                                                        : "device error";
     snprintf(g_errmsg, sizeof(g_errmsg), "%s (code %d, detail %d)", what, h[0], h[1]);
     return h[0];
@@ -1800,17 +2040,39 @@ int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, do
   return 0;
 }
 
+static int32_t *g_box_err = nullptr;
+
+static int box_prepare() {
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
+  if (!g_box_err) {
+    HIP_TRY(hipMalloc(&g_box_err, 16));
+    HIP_TRY(hipMemset(g_box_err, 0, 16));
+  }
+  return 0;
+}
+
 int lm_golovin_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
                     int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
   if (n_boxes <= 0 || !off || !xis || !masses || !no_cols) return set_err(LM_ERR_ARG, "bad box arguments");
-  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
-  static int32_t *err_dev = nullptr;
-  if (!err_dev) {
-    HIP_TRY(hipMalloc(&err_dev, 16));
-    HIP_TRY(hipMemset(err_dev, 0, 16));
-  }
+  int rc = box_prepare();
+  if (rc) return rc;
   hipLaunchKernelGGL(k_golovin, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, dt_over_dV,
-                     (long long *)no_cols, (unsigned long long)seed, step, rnd_stream, err_dev);
+                     (const double *)nullptr, 0, 0.0, 1.0, (long long *)no_cols, (unsigned long long)seed, step,
+                     rnd_stream, g_box_err);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_kernel_grid_m_step(int32_t n_boxes, const int32_t *off, double *xis, double *masses, double dt_over_dV,
+                          const double *kernel_grid, int32_t nb, double m_kernel_low_log, double bin_factor_m_log,
+                          int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, void *stream) {
+  if (n_boxes <= 0 || !off || !xis || !masses || !kernel_grid || nb <= 0 || !no_cols)
+    return set_err(LM_ERR_ARG, "bad box arguments");
+  int rc = box_prepare();
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_golovin, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, dt_over_dV,
+                     kernel_grid, nb, m_kernel_low_log, bin_factor_m_log, (long long *)no_cols,
+                     (unsigned long long)seed, step, rnd_stream, g_box_err);
   LAUNCH_CHECK();
   return 0;
 }
@@ -1819,17 +2081,57 @@ int lm_ecol_box_step(int32_t n_boxes, const int32_t *off, double *xis, double *m
                      const double *rho, double dt_over_dV, const double *E, int32_t nb, double lnR0, double lnf,
                      int64_t *no_cols, uint64_t seed, uint32_t step, const double *rnd_stream, int32_t update_beard,
                      void *stream) {
-  if (n_boxes <= 0 || !off || !xis || !masses || !radii || !vel || !rho || !E || !no_cols)
+  if (n_boxes <= 0 || !off || !xis || !masses || !radii || !vel || !rho || !E || nb <= 0 || !no_cols)
     return set_err(LM_ERR_ARG, "bad box arguments");
-  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
-  static int32_t *err_dev = nullptr;
-  if (!err_dev) {
-    HIP_TRY(hipMalloc(&err_dev, 16));
-    HIP_TRY(hipMemset(err_dev, 0, 16));
-  }
+  int rc = box_prepare();
+  if (rc) return rc;
   hipLaunchKernelGGL(k_ecol_box, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, masses, radii, vel,
                      rho, dt_over_dV, E, nb, lnR0, lnf, (long long *)no_cols, (unsigned long long)seed, step,
-                     rnd_stream, err_dev, update_beard);
+                     rnd_stream, g_box_err, update_beard);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_box2d_step(int32_t n_boxes, const int32_t *off, double *xis, double *m_w, double *m_s, double *radii,
+                  const double *vel_x, const double *vel_z, const double *rho, double dt_over_dV, const double *E,
+                  double E_const, int32_t nb, double lnR0, double lnf, int64_t *no_cols, uint64_t seed, uint32_t step,
+                  const double *rnd_stream, void *stream) {
+  if (n_boxes <= 0 || !off || !xis || !m_w || !m_s || !radii || !vel_x || !vel_z || !rho || !no_cols ||
+      (E && nb <= 0))
+    return set_err(LM_ERR_ARG, "bad box arguments");
+  int rc = box_prepare();
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_box2d, dim3(n_boxes), dim3(WAVE), 0, S(stream), n_boxes, off, xis, m_w, m_s, radii, vel_x,
+                     vel_z, rho, dt_over_dV, E, E_const, nb, lnR0, lnf, (long long *)no_cols,
+                     (unsigned long long)seed, step, rnd_stream, g_box_err);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_box_check(void *stream) {
+  if (!g_box_err) return 0;
+  int32_t h[4];
+  HIP_TRY(hipMemcpyAsync(h, g_box_err, sizeof(h), hipMemcpyDeviceToHost, S(stream)));
+  HIP_TRY(hipStreamSynchronize(S(stream)));
+  if (h[0]) {
+    HIP_TRY(hipMemsetAsync(g_box_err, 0, 16, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    char msg[160];
+    snprintf(msg, sizeof msg, "box %d holds more than %d SIPs (box kernels keep a box in LDS)", h[1], BOX_CAP);
+    return set_err(h[0], msg);
+  }
+  return 0;
+}
+
+int lm_divergence_upwind(const double *fields, const double *flux, double *out, int32_t n_fields, int32_t nx,
+                         int32_t nz, double dx, double dz, int32_t bc_x, int32_t bc_z, void *stream) {
+  if (!fields || !flux || !out || n_fields <= 0 || nx < 2 || nz < 2 || bc_x < 0 || bc_x > 1 || bc_z < 0 ||
+      bc_z > 1)
+    return set_err(LM_ERR_ARG, "bad divergence arguments (boundary types are 0 = periodic, 1 = fixed)");
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
+  const long long n = (long long)n_fields * nx * nz;
+  hipLaunchKernelGGL(k_divergence, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), fields, flux, out,
+                     n_fields, nx, nz, dx, dz, bc_x, bc_z);
   LAUNCH_CHECK();
   return 0;
 }

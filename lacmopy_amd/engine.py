@@ -16,6 +16,7 @@ import torch
 from ._native import LmEngine, check, lib
 
 SOLUTES = {"AS": 0, "NaCl": 1}
+CC_N, CC_TP = 16, 15  # per-cell constant table: width, slot of the pre-update cell temperature (lm_physics.h)
 
 
 def _ptr(t):
@@ -60,7 +61,7 @@ class Engine:
         mk("dmxi", (n,), f64)
         mk("scal", (nr, 10, nc), f64)
         mk("mat", (nr, 7, nc), f64)
-        mk("cc", (nr * nc, 16), f64)
+        mk("cc", (nr * nc, CC_N), f64)
         mk("dTh", (nr * nc,), f64)
         mk("dRv", (nr * nc,), f64)
         mk("dml", (nr * nc,), f64)
@@ -186,9 +187,12 @@ class Engine:
         if stream_values is None:
             self.rnd_stream = None
             self.e.rnd_stream = None
+            self.e.rnd_len = 0
             return
-        self.rnd_stream = torch.as_tensor(np.asarray(stream_values, dtype=np.float64), device=self.device)
+        self.rnd_stream = torch.as_tensor(np.asarray(stream_values, dtype=np.float64).reshape(-1),
+                                          device=self.device)
         self.e.rnd_stream = self.rnd_stream.data_ptr()
+        self.e.rnd_len = int(self.rnd_stream.numel())
         self.buf["rnd_off"].zero_()
 
     # ------------------------------------------------------------- stepping
@@ -209,6 +213,36 @@ class Engine:
 
     def substep(self, dt_sub, dt_sub_pos):
         check(self.L.lm_substep(C.byref(self.e), float(dt_sub), float(dt_sub_pos), self.stream))
+
+    def particle_substep(self, dt_sub, dt_sub_pos):
+        check(self.L.lm_particle_substep(C.byref(self.e), float(dt_sub), float(dt_sub_pos), self.stream))
+
+    def grid_substep(self):
+        check(self.L.lm_grid_substep(C.byref(self.e), self.stream))
+
+    def update_r_l(self):
+        check(self.L.lm_update_r_l(C.byref(self.e), self.stream))
+
+    def set_increments(self, run, delta_Theta_ad=None, delta_r_v_ad=None, delta_m_l=None):
+        """Per-cell advection increments / condensed mass of one run ((nx, nz) arrays)."""
+        sl = slice(run * self.nc, (run + 1) * self.nc)
+        for name, a in (("dTh", delta_Theta_ad), ("dRv", delta_r_v_ad), ("dml", delta_m_l)):
+            if a is not None:
+                self.buf[name][sl].copy_(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)))
+
+    def export_increments(self, run):
+        """(delta_Theta_ad, delta_r_v_ad, delta_m_l) of one run as (nx, nz) arrays."""
+        self.check()
+        sl = slice(run * self.nc, (run + 1) * self.nc)
+        return tuple(self.buf[n][sl].cpu().numpy().reshape(self.nx, self.nz).copy() for n in ("dTh", "dRv", "dml"))
+
+    @property
+    def col_step(self):
+        return int(self.e.col_step)
+
+    @col_step.setter
+    def col_step(self, step):
+        self.e.col_step = int(step) & 0xFFFFFFFF
 
     def check(self):
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
@@ -237,6 +271,25 @@ class Engine:
         ids = self.buf[f"id{cur}"][b:b + N].cpu().numpy()
         cell = self.buf[f"cell{ccur}"][b:b + N].cpu().numpy()
         return host_unlayout(g, ids, cell, run * self.nc, self.nz)
+
+    def export_T_p(self, run):
+        """SIP temperatures in ID order: the cell temperature each SIP saw in the last particle substep.
+
+        integration.py:489-507 sets T_p = T[cells] at the start of every particle
+        substep, before positions move; inactive SIPs (cells[1] = -1) read the
+        top row of their column.  The engine keeps the cell of that substep in
+        the other cell buffer and its temperature in the per-cell table.
+        """
+        self.check()
+        cur, ccur = self.e.cur, self.e.ccur
+        b, N = int(self.base[run]), self.n_per_run[run]
+        ids = self.buf[f"id{cur}"][b:b + N].cpu().numpy().astype(np.int64)
+        prev = self.buf[f"cell{1 - ccur}"][b:b + N].cpu().numpy().astype(np.int64)
+        tp = self.buf["cc"][run * self.nc:(run + 1) * self.nc, CC_TP].cpu().numpy()
+        loc = np.where(prev >= 0, prev - run * self.nc, (-prev - 2) * self.nz + (self.nz - 1))
+        out = np.empty(N)
+        out[ids] = tp[loc]
+        return out
 
     def export_grid(self, run):
         self.check()
@@ -297,6 +350,9 @@ def host_layout(runs, nx, nz, n_per_run):
         ca = cells[:, active]
         if np.any((ca[0] < 0) | (ca[0] >= nx) | (ca[1] < 0) | (ca[1] >= nz)):
             raise ValueError("active SIP outside the grid")
+        cd = cells[0, ~active]
+        if np.any((cd < 0) | (cd >= nx)):
+            raise ValueError("inactive SIP with a horizontal cell index outside the grid")
         act = ids[active]
         order = act[np.lexsort((act, key[active]))]
         dead = ids[~active]

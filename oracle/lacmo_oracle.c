@@ -521,6 +521,59 @@ EXPORT void or_collision_golovin(int64_t n, double *xis, double *masses, double 
   }
 }
 
+/* all_or_nothing.py:392-491 (box model, kernel K(m1, m2) tabulated on a log-mass grid) */
+EXPORT void or_collision_kernel_grid_m(int64_t n, double *xis, double *masses, double dt_over_dV,
+                                       const double *kernel_grid, int64_t nb, double lnm0, double lnf,
+                                       int64_t *no_cols, const double *rnd, uint64_t seed, uint32_t run,
+                                       uint32_t step) {
+  or_init_consts();
+  OrRng g = {rnd, 0, seed, run, step, 0};
+  int64_t *ind = (int64_t *)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; ++i) ind[i] = or_kernel_index(masses[i], lnm0, lnf, nb);
+  int64_t cnt = 0;
+  for (int64_t i = 0; i < n - 1; ++i) {
+    for (int64_t j = i + 1; j < n; ++j) {
+      const int64_t imin = xis[i] <= xis[j] ? i : j, imax = xis[i] <= xis[j] ? j : i;
+      const double xi_min = xis[imin], xi_max = xis[imax];
+      const double m_min = masses[imin], m_max = masses[imax];
+      const double p_crit = xi_max * kernel_grid[ind[i] * nb + ind[j]] * dt_over_dV;
+      if (p_crit > 1.0) {
+        const double xi_col = p_crit * xi_min;
+        masses[imin] = (xi_min * m_min + xi_col * m_max) / xi_min;
+        xis[imax] -= xi_col;
+        ind[imin] = or_kernel_index(masses[imin], lnm0, lnf, nb);
+        no_cols[1] += 1;
+      } else if (p_crit > or_draw(&g, cnt, i, j)) {
+        no_cols[0] += 1;
+        if ((xi_max - xi_min) / xi_max < 1.0E-5) {
+          const double xi_ges = xi_min + xi_max;
+          masses[imin] = 2.0 * (xi_min * m_min + xi_max * m_max) / xi_ges;
+          masses[imax] = masses[imin];
+          xis[imax] = 0.5 * 0.7 * xi_ges;
+          xis[imin] = 0.5 * xi_ges - xis[imax];
+          ind[imin] = or_kernel_index(masses[imin], lnm0, lnf, nb);
+          ind[imax] = ind[imin];
+        } else {
+          masses[imin] += m_max;
+          xis[imax] -= xi_min;
+          ind[imin] = or_kernel_index(masses[imin], lnm0, lnf, nb);
+        }
+      }
+      cnt += 1;
+    }
+  }
+  free(ind);
+}
+
+/* grid.py:370-412: r_l = 1e-18 m_dry^-1 sum over active ids (ascending) of xi m_w */
+EXPORT void or_update_grid_r_l(int64_t N, const double *m_w, const double *xi, const int64_t *cells,
+                               const uint8_t *active, int64_t nx, int64_t nz, double *r_l, const double *m_dry_inv) {
+  for (int64_t c = 0; c < nx * nz; ++c) r_l[c] = 0.0;
+  for (int64_t id = 0; id < N; ++id)
+    if (active[id]) r_l[cells[id] * nz + cells[N + id]] += m_w[id] * xi[id];
+  for (int64_t c = 0; c < nx * nz; ++c) r_l[c] *= 1.0E-18 * m_dry_inv[c];
+}
+
 /* all_or_nothing.py:269-390 (box model, E table, 1D velocity) */
 EXPORT void or_collision_ecol_box(int64_t n, double *xis, double *masses, double *radii, const double *vel,
                                   const double *rho, double dt_over_dV, const double *E, int64_t nb, double lnR0,

@@ -58,6 +58,8 @@ def lib(libm=False):
     L.or_collision_golovin.argtypes = [_i64, _p, _p, _d, _p, _p, _u64, _u32, _u32]
     L.or_collision_ecol_box.argtypes = [_i64, _p, _p, _p, _p, _p, _d, _p, _i64, _d, _d, _p, _p, _u64, _u32, _u32]
     L.or_update_velocity_Beard.argtypes = [_i64, _p, _p]
+    L.or_collision_kernel_grid_m.argtypes = [_i64, _p, _p, _d, _p, _i64, _d, _d, _p, _p, _u64, _u32, _u32]
+    L.or_update_grid_r_l.argtypes = [_i64, _p, _p, _p, _p, _i64, _i64, _p, _p]
     L.or_divergence_upwind.argtypes = [_p, _p, _i64, _i64, _d, _d, C.c_int, C.c_int, _p]
     L.or_relaxation_term.argtypes = [_p, _p, _p, _d, _i64, _i64, _p]
     L.or_propagate_grid_subloop_step.argtypes = [_p, _p, _d, _d, _p, _p, _p, _d, _i64]

@@ -424,3 +424,74 @@ def test_libm_and_lm_oracles_agree_closely():
     np.testing.assert_allclose(a[1], b[1], rtol=1e-12)
     np.testing.assert_allclose(a[0], b[0], rtol=1e-12)
     np.testing.assert_allclose(a[5][:8], b[5][:8], rtol=1e-12)
+
+
+@pytest.mark.parametrize("bc", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_divergence_boundary_types_conserve_and_constant_field(bc):
+    """integration.py:202-212, 239-241, 283-285: periodic axes telescope and fixed axes have zero boundary
+    flux, so the divergence sums to zero for any field; a constant field gives c * div(flux) with the
+    fixed-boundary faces zeroed."""
+    nx, nz, dx, dz = 7, 9, 20.0, 15.0
+    rng = np.random.default_rng(bc[0] * 2 + bc[1])
+    flux = rng.uniform(-1, 1, (2, nx + 1, nz + 1))
+    if bc[0] == 0:
+        flux[0, nx] = flux[0, 0]
+    if bc[1] == 0:
+        flux[1, :, nz] = flux[1, :, 0]
+    field = rng.uniform(0.5, 2.0, (nx, nz))
+    out = np.zeros((nx, nz))
+    L.or_divergence_upwind(orc.P(field), orc.P(flux), nx, nz, dx, dz, bc[0], bc[1], orc.P(out))
+    assert abs(out.sum()) < 1e-12 * np.abs(out).sum()
+    c = np.full((nx, nz), 1.3)
+    L.or_divergence_upwind(orc.P(c), orc.P(flux), nx, nz, dx, dz, bc[0], bc[1], orc.P(out))
+    Fx = flux[0][:, :-1].copy()
+    Fz = flux[1][:-1, :].copy()
+    if bc[0] == 1:
+        Fx[0], Fx[-1] = 0.0, 0.0
+    if bc[1] == 1:
+        Fz[:, 0], Fz[:, -1] = 0.0, 0.0
+    div = (Fx[1:] - Fx[:-1]) / dx + (Fz[:, 1:] - Fz[:, :-1]) / dz
+    np.testing.assert_allclose(out, 1.3 * div, rtol=1e-13, atol=1e-14)
+
+
+def test_kernel_grid_m_reduces_to_golovin_on_grid_masses():
+    """all_or_nothing.py:392-491 with the Golovin kernel tabulated at grid masses equals the analytic
+    Golovin step (:109-185) when the masses sit on grid points and only the last pair may fire."""
+    nbm, lnm0, lnfm = 200, np.log(10.0), np.log(1.1)
+    mg = np.exp(lnm0 + lnfm * np.arange(nbm))
+    Kg = (mg[:, None] + mg[None, :]) * 1.5E-18
+    rng = np.random.default_rng(6)
+    n = 40
+    m = mg[rng.integers(20, 180, n)]
+    xi = rng.uniform(1e6, 1e9, n)
+    rnd = np.ones(n * (n - 1) // 2)
+    rnd[-1] = 0.0
+    for dt_over_dV in (1.0, 3.0):
+        a_x, a_m, a_nc = xi.copy(), m.copy(), np.zeros(2, dtype=np.int64)
+        b_x, b_m, b_nc = xi.copy(), m.copy(), np.zeros(2, dtype=np.int64)
+        L.or_collision_kernel_grid_m(n, orc.P(a_x), orc.P(a_m), dt_over_dV, orc.P(Kg), nbm, lnm0, lnfm, orc.P(a_nc),
+                                     orc.P(rnd), 0, 0, 0)
+        L.or_collision_golovin(n, orc.P(b_x), orc.P(b_m), dt_over_dV, orc.P(b_nc), orc.P(rnd), 0, 0, 0)
+        if dt_over_dV == 1.0:
+            assert a_nc.tolist() == [1, 0]
+            np.testing.assert_array_equal(a_x, b_x)
+            np.testing.assert_array_equal(a_m, b_m)
+        # any kernel: total mass conserved, multiplicity accounting
+        np.testing.assert_allclose((a_x * a_m).sum(), (xi * m).sum(), rtol=1e-12)
+        assert a_nc.sum() >= 1
+
+
+def test_update_grid_r_l_sums_liquid_water():
+    """grid.py:370-412: sum over cells of r_l / (1e-18 m_dry^-1) is the total xi m_w of the active SIPs."""
+    rng = np.random.default_rng(7)
+    N, nx, nz = 3000, 6, 5
+    m_w, xi = rng.lognormal(3, 2, N), rng.uniform(1e3, 1e6, N)
+    cells = np.array([rng.integers(0, nx, N), rng.integers(0, nz, N)], dtype=np.int64)
+    act = rng.random(N) > 0.1
+    mdi = rng.uniform(1e-3, 2e-3, (nx, nz))
+    r_l = np.zeros((nx, nz))
+    L.or_update_grid_r_l(N, orc.P(m_w), orc.P(xi), orc.P(cells), orc.P(act.view(np.uint8)), nx, nz, orc.P(r_l),
+                         orc.P(mdi))
+    np.testing.assert_allclose((r_l / (1e-18 * mdi)).sum(), (m_w * xi)[act].sum(), rtol=1e-12)
+    c0 = (cells[0] == 2) & (cells[1] == 3) & act
+    np.testing.assert_allclose(r_l[2, 3], (m_w * xi)[c0].sum() * 1e-18 * mdi[2, 3], rtol=1e-12)
