@@ -12,8 +12,11 @@ Differences a caller can see, all outside the reference's numerics:
   * collision draws come from the counter-based stream of lacmopy_amd.rng
     (the reference's np.random draws inside jitted code are not reproducible
     from any seed);
-  * rel_pos of inactive SIPs is left as passed: the reference keeps rescaling
-    a stale value there (integration.py:548-549) that nothing reads;
+  * rel_pos is an output only: the kernels derive each SIP's relative
+    position from its position and cell, which is the value the reference
+    holds in rel_pos after every position update (integration.py:545-549);
+    rel_pos of inactive SIPs is left as passed, where the reference keeps
+    rescaling a stale value (:548-549) that nothing reads;
   * error conditions the reference would run through silently (a cell with
     more SIPs than the kernels hold, a SIP that left the domain sideways)
     raise lacmopy_amd._native.LacmoError.

@@ -24,6 +24,14 @@ def _grid_arrays(grid):
     return g.scalar_fields(), g.mat_prop()
 
 
+def _rel_pos(grid, pos, cells, active):
+    """integration.py:545-549: the relative positions a consistent caller holds."""
+    rel = np.zeros(pos.shape)
+    for d in (0, 1):
+        rel[d][active] = (pos[d][active] - grid.ranges[d][0]) / grid.steps[d] - cells[d][active]
+    return rel
+
+
 def _oracle_params(grid, N, cfg, E, seed, step):
     return orc.adv_params(grid, N, cfg, seed=seed, col_step=step, E_col_grid=E)
 
@@ -88,7 +96,7 @@ def test_propagate_particles_and_grid_substep(gpu):
     cfg = _config(0.5, 0, 1, 0)
     # GPU
     pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
-    rel = np.zeros((2, N))
+    rel = _rel_pos(grid, pos, cells, active)
     T_p = np.ones(N)
     wr = np.zeros(1)
     dml = np.zeros((8, 8))
@@ -97,7 +105,7 @@ def test_propagate_particles_and_grid_substep(gpu):
                                         0.1, 3, 9.80665)
     # oracle
     o = [np.array(a, copy=True) for a in parts]
-    o_rel, o_Tp, o_wr, o_dml = np.zeros((2, N)), np.ones(N), np.zeros(1), np.zeros((8, 8))
+    o_rel, o_Tp, o_wr, o_dml = _rel_pos(grid, o[0], o[2], o[6]), np.ones(N), np.zeros(1), np.zeros((8, 8))
     p = _oracle_params(grid, N, cfg, 0.5, 0, 0)
     s0, m0 = scal.copy(), mat.copy()
     orc.lib().or_propagate_particles_substep(orc.C.byref(p), 0.1, 0.1, *[orc.P(a) for a in (
@@ -135,7 +143,8 @@ def test_integrate_subloop_n_steps_np(gpu):
     dRv = rs.normal(0, 1e-6, (8, 8))
     dt_sub, n_cond, n_col = 0.1, 3, 3
     pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
-    rel, T_p, wr, nc, dml = np.zeros((2, N)), np.ones(N), np.zeros(1), np.zeros(2, dtype=np.int64), np.zeros((8, 8))
+    rel, T_p, wr, nc, dml = (_rel_pos(grid, pos, cells, active), np.ones(N), np.zeros(1), np.zeros(2, dtype=np.int64),
+                             np.zeros((8, 8)))
     g_s, g_m = scal.copy(), mat.copy()
     rng.seed(31, 10)
     integrate_subloop_n_steps_np(g_s, g_m, grid.velocity, grid.no_cells, grid.ranges, grid.steps, grid.volume_cell,
@@ -145,7 +154,8 @@ def test_integrate_subloop_n_steps_np(gpu):
     assert rng.get_step() == 13
     # oracle: integration.py:1247-1292 composed from its pieces
     o = [np.array(a, copy=True) for a in parts]
-    o_rel, o_Tp, o_wr, o_nc, o_dml = np.zeros((2, N)), np.ones(N), np.zeros(1), np.zeros(2, np.int64), np.zeros((8, 8))
+    o_rel, o_Tp, o_wr, o_nc, o_dml = (_rel_pos(grid, o[0], o[2], o[6]), np.ones(N), np.zeros(1),
+                                      np.zeros(2, np.int64), np.zeros((8, 8)))
     o_s, o_m = scal.copy(), mat.copy()
     p = _oracle_params(grid, N, cfg, E, 31, 0)
     for k in range(n_cond):
@@ -178,6 +188,8 @@ def test_collision_all_cells_np(gpu, kernel):
     N = len(xi)
     cells = cells.copy()
     cells[1, ::17] = -1  # inactive SIPs take no part
+    vel = vel.copy()
+    vel[1] -= np.random.default_rng(1).uniform(0.0, 3.0, N)  # fall speeds, so that pairs collide
     T = grid.temperature.copy()
     o_xi, o_mw, o_ms, o_nc = xi.copy(), m_w.copy(), m_s.copy(), np.zeros(2, dtype=np.int64)
     orc.collision_all_cells(o_xi, o_mw, o_ms, vel, T, cells, grid.no_cells, 0.5 / grid.volume_cell, E, nb or 0,
