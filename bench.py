@@ -33,6 +33,23 @@ B_REINDEX = 140.0    # per SIP per reindex
 B_STEP = 10 * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX   # 1308 B per SIP-step
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_icmw75_64seeds.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the 64-seed workload, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy); None if absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -129,6 +146,20 @@ def cpu_baseline(seed0, spc):
             "sample": f"{steps} advection steps of the plain 75x75 case ({N} SIPs), oracle C restatement, 1 thread"}
 
 
+def copy_probe(mib=1024, reps=3):
+    """A plain device-to-device copy of known size: the calibration kernel for the HBM byte counters."""
+    import torch
+    n = mib * 1024 * 1024 // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "reps": reps}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +172,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--kernel", default="Long_Bott", choices=["Long_Bott", "Hall_Bott", "const"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): independent seeds shard "
@@ -148,6 +181,9 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "needs a GPU"
+    if args.copy_probe:
+        copy_probe()
+        return
 
     log(f"building batch of {args.runs} runs ...")
     t = time.perf_counter()
@@ -178,9 +214,11 @@ def main():
         # dominant per-SIP kernel: the particle substep kernel (one thread per SIP)
         ms = ktimes["particles"][0]
         achieved = n_batch_sips * B_SUBSTEP / (ms * 1e-3) / 1e9
+        default_workload = args.runs == 64 and args.spc == 128.0 and args.seed == 1001 and args.kernel == "Long_Bott"
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_particles",
-                "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic("k_particles") if default_workload else None,
+                "kernel": "k_particles", "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
 
     cpu = None if args.no_cpu else cpu_baseline(args.seed, args.spc)
     if cpu:
