@@ -574,7 +574,7 @@ __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * 
  * (diagonals 4b..4b+3), i.e. four consecutive pairs of the row sharing one
  * block of A words.
  */
-template <int CAP>
+template <int CAP, bool TAB, bool INJ>
 __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   const LmPhys &P = c_P;
   int gc;
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
     double R, ws, rho;
     lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
     sR[p] = R;
-    sIND[p] = v.e_const_mode ? 0 : lmp_kernel_index(R, v.lnR0, v.lnf, v.nb);
+    sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
   }
   /* quads per row: row i holds diagonals d = 2i+1 .. i+n-1, blocks (2i+1)>>2 .. (i+n-1)>>2 */
   if (lane == 0) {
@@ -643,8 +643,9 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   __syncthreads();
 
   const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n};
-  const bool use_tab = !v.e_const_mode;
-  const long long roff = v.rnd_stream ? v.rnd_off[gc] : 0;
+  const bool use_tab = TAB;
+  const long long roff = INJ ? v.rnd_off[gc] : 0;
+  const double dtdV = v.dt_col_over_dV, pi = P.pi;
   const uint32_t key_run = (uint32_t)r, key_step = v.col_step, key_cell = (uint32_t)c;
   double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
   const int nQ = sQ[n - 1];
@@ -657,47 +658,75 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   while (n_events < SPEC_MAX_EVENTS) {
     /* parallel pass: first firing pair with cnt >= k_next under the current state */
     unsigned long long best = ~0ull;
+    int i = 0; /* row of quad q: rows only grow with q, so each lane walks forward */
     for (int q = lane; q < nQ; q += WAVE) {
-      int lo = 0, hi = n - 2; /* row: largest i with sQ[i] <= q */
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sQ[mid] <= q)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      const int i = lo;
+      while (sQ[i + 1] <= q) ++i;
       const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
       const int dlast = (4 * b + 3 < i + n - 1) ? 4 * b + 3 : i + n - 1;
       if (pair_cnt(i, dlast - i, n) < k_next) continue;
       lm_u32x4 blk = {{0u, 0u, 0u, 0u}};
-      if (!v.rnd_stream) blk = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)(4 * b), 0);
+      if (!INJ) blk = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)(4 * b), 0);
       const double xi_i = sX[i], R_i = sR[i], u_i = sU[i], w_i = sW[i];
-      const int ind_i = sIND[i];
+      const double *Erow = TAB ? v.E + (size_t)sIND[i] * v.nb : nullptr;
+      /*
+       * Filter.  p = (xi_max dt/dV) * ((((pi s) s) E) sqrt(d2)) 1e-12 with s = R_i + R_j.  Without the
+       * square root, Pl^2 d2 (Pl = (xi_max dt/dV) (((pi s) s) E) 1e-12) equals p^2 within 10 roundings,
+       * so Pl^2 d2 < thr^2 (1 - 2^-40) proves p < thr, thr = the 27-bit lower bound of the pair's
+       * uniform (or the injected uniform): the pair does not fire and is no multiple collection.  For
+       * Pl <= 1e-140 (and d2 < 1e100) p < 1e-89 < thr whenever thr > 0.  Everything else -- thr = 0,
+       * overflow, NaN -- takes the exact path below.
+       */
+      unsigned maybe = 0;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int d = 4 * b + t;
-        const int j = d - i;
-        if (j <= i || j >= n) continue;
-        const int cnt = pair_cnt(i, j, n);
-        if (cnt < k_next) continue;
-        const double Ec = use_tab ? v.E[ind_i * v.nb + sIND[j]] : v.E_const;
+        const int jr = d - i;
+        const bool valid = (jr > i) & (jr < n) & (pair_cnt(i, jr, n) >= k_next);
+        /* clamped into [0, n-1] (jr is -1 at the start of some rows, >= n at the end): every lane runs the
+           same straight-line code on in-range LDS entries, and invalid pairs are masked out below */
+        const int j = jr < 0 ? 0 : (jr < n ? jr : n - 1);
         const double xi_j = sX[j];
         const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
         const double du = u_i - sU[j], dw = w_i - sW[j];
-        const double p_crit = xi_max * v.dt_col_over_dV * lmp_kernel_hydro(P, R_i, sR[j], Ec, sqrt(du * du + dw * dw));
-        bool fires;
-        if (p_crit > 1.0)
-          fires = true;
-        else if (v.rnd_stream)
-          fires = p_crit > rnd_at(v, roff + cnt);
+        const double d2 = du * du + dw * dw;
+        const double sR2 = R_i + sR[j];
+        const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
+        const double Pl = ((xi_max * dtdV) * (((pi * sR2) * sR2) * Ec)) * 1.0E-12;
+        double thr;
+        if (INJ)
+          thr = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 0.0;
         else
-          fires = pair_fires(p_crit, lm_word(blk, (uint32_t)t), v.seed, key_run, key_step, key_cell, (uint32_t)i,
-                             (uint32_t)d);
-        if (fires) {
-          const unsigned long long key = ((unsigned long long)cnt << 32) | ((unsigned)i << 16) | (unsigned)j;
-          best = key < best ? key : best;
-          break;
+          thr = (double)(lm_word(blk, (uint32_t)t) >> 5) * (1.0 / 134217728.0);
+        const double thr2 = (thr * thr) * (1.0 - 0x1p-40);
+        const bool big = Pl > 1.0E-140;
+        const bool no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 0.0));
+        maybe |= (unsigned)(valid & !no) << t;
+      }
+      if (maybe) {
+        /* rare: the reference's exact p_crit and decision, first firing pair of the quad */
+        for (int t = 0; t < 4; ++t) {
+          if (!((maybe >> t) & 1u)) continue;
+          const int d = 4 * b + t;
+          const int j = d - i;
+          const int cnt = pair_cnt(i, j, n);
+          const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
+          const double xi_j = sX[j];
+          const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
+          const double du = u_i - sU[j], dw = w_i - sW[j];
+          const double p_crit = xi_max * dtdV * lmp_kernel_hydro(P, R_i, sR[j], Ec, sqrt(du * du + dw * dw));
+          bool fires;
+          if (p_crit > 1.0)
+            fires = true;
+          else if (INJ)
+            fires = p_crit > rnd_at(v, roff + cnt);
+          else
+            fires = pair_fires(p_crit, lm_word(blk, (uint32_t)t), v.seed, key_run, key_step, key_cell, (uint32_t)i,
+                               (uint32_t)d);
+          if (fires) {
+            const unsigned long long key = ((unsigned long long)cnt << 32) | ((unsigned)i << 16) | (unsigned)j;
+            best = key < best ? key : best;
+            break;
+          }
         }
       }
     }
@@ -753,7 +782,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         if (cnt < k_next) continue;
         uint32_t aw = 0;
         double rnd_inj = 0.0;
-        if (v.rnd_stream) {
+        if (INJ) {
           rnd_inj = rnd_at(v, roff + cnt);
         } else {
           const int tag = (i << 16) | (d >> 2);
@@ -775,7 +804,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
         bool fires;
         if (p_crit > 1.0)
           fires = true;
-        else if (v.rnd_stream)
+        else if (INJ)
           fires = p_crit > rnd_inj;
         else
           fires = pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d);
@@ -1773,6 +1802,19 @@ static int check_engine(const LmEngine *e) {
 
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
+template <int CAP>
+static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int from_ovf) {
+  const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
+  if (tab && inj)
+    hipLaunchKernelGGL((k_collision<CAP, true, true>), grid, dim3(WAVE), 0, st, v, from_ovf);
+  else if (tab)
+    hipLaunchKernelGGL((k_collision<CAP, true, false>), grid, dim3(WAVE), 0, st, v, from_ovf);
+  else if (inj)
+    hipLaunchKernelGGL((k_collision<CAP, false, true>), grid, dim3(WAVE), 0, st, v, from_ovf);
+  else
+    hipLaunchKernelGGL((k_collision<CAP, false, false>), grid, dim3(WAVE), 0, st, v, from_ovf);
+}
+
 extern "C" {
 
 const char *lm_version(void) { return "lacmopy_amd 0.1 gfx950"; }
@@ -1829,9 +1871,9 @@ int lm_collision_step(LmEngine *e, void *stream) {
   KView v = make_view(e);
   {
     ProfScope ps(e, 1, S(stream));
-    hipLaunchKernelGGL(k_collision<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
+    launch_collision<256>(v, e, dim3(ncell), S(stream), 0);
   }
-  hipLaunchKernelGGL(k_collision<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, 1);
+  launch_collision<1024>(v, e, dim3(ncell < 4096 ? ncell : 4096), S(stream), 1);
   LAUNCH_CHECK();
   e->col_step += 1;
   return 0;
