@@ -1756,6 +1756,7 @@ __global__ void k_math_probe(int which, long long n, const double *x, const doub
     case 3: r = lm_pymod(a, b); break;
     case 4: r = sqrt(a); break;
     case 5: r = a / b; break;
+    case 7: r = lm_pow_third(a); break;
     default: r = floor(a / b); break;
   }
   out[k] = r;

@@ -227,6 +227,70 @@ LM_FN double lm_pow(double x, double y) {
   return sgn * lm_exp_dd(zh, zl);
 }
 
+/*
+ * x^y for y = RN(1/3) = 1/3 - 2^-54/3, the exponent of the radius from the mass
+ * (microphysics.py:83-100), without the log/exp pair of lm_pow:
+ *   c ~ x^(1/3): reciprocal cube root y ~ x^(-1/3) from the exponent bits
+ *     (Kahan's bit trick, a few percent), four Newton steps
+ *     y <- y (4 - x y^3) / 3 (quadratic, no division), c = x y^2;
+ *   c_lo: the residual x - c^3, computed exactly with two TwoProds, over 3c^2;
+ *   x^y = (c + c_lo) x^(-2^-54/3) = (c + c_lo)(1 - eps), eps = ln(x) 2^-54/3,
+ *     where ln x = e ln2 + ln(1+t) only needs ~1e-7 absolute accuracy
+ *     (a degree-7 polynomial, own least-squares fit on [0, 1)).
+ * The value before the final rounding is within ~1e-7 ulp of x^y, so the
+ * result is the correctly rounded power except within that distance of a
+ * rounding boundary.  Valid for 2^-900 <= x < 2^900; everything else
+ * (x <= 0, tiny or huge x, inf, NaN) takes lm_pow.
+ */
+#define LM_CBRT_MAGIC 0x553EF0FF289DD796ull
+#define LM_L1P_1 0x1.fffd2f11ff7f5p-1
+#define LM_L1P_2 -0x1.ff68cfba18ccap-2
+#define LM_L1P_3 0x1.4fceb28c71a45p-2
+#define LM_L1P_4 -0x1.cccfff50f2358p-3
+#define LM_L1P_5 0x1.1119b2a0def2cp-3
+#define LM_L1P_6 -0x1.bbcb388b20f22p-5
+#define LM_L1P_7 0x1.5797072ef79d9p-7
+#define LM_LN2 0x1.62e42fefa39efp-1
+
+#if defined(__HIPCC__)
+__device__ __noinline__ static double lm_pow_third_rare(double x) { return lm_pow(x, 1.0 / 3.0); }
+#else
+static double lm_pow_third_rare(double x) { return lm_pow(x, 1.0 / 3.0); }
+#endif
+
+LM_FN double lm_pow_third(double x) {
+  const uint64_t u = lm_as_u64(x);
+  if (u - 0x07B0000000000000ull >= 0x7080000000000000ull) return lm_pow_third_rare(x);
+  const double third = 1.0 / 3.0;
+  double y = lm_as_f64(LM_CBRT_MAGIC - u / 3u);
+  for (int it = 0; it < 4; ++it) {
+    double t = x * ((y * y) * y);
+    t = 4.0 - t;
+    y = y * t;
+    y = y * third;
+  }
+  const double y2 = y * y;
+  const double c = x * y2;
+  double p, pe, q, qe;
+  lm_two_prod(c, c, &p, &pe);
+  lm_two_prod(p, c, &q, &qe);
+  const double r = (x - q) - (qe + pe * c);
+  const double c_lo = (r * y2) * third;
+  const int e = (int)(u >> 52) - 1023;
+  const double t = lm_as_f64((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull) - 1.0;
+  double P = LM_L1P_7;
+  P = P * t + LM_L1P_6;
+  P = P * t + LM_L1P_5;
+  P = P * t + LM_L1P_4;
+  P = P * t + LM_L1P_3;
+  P = P * t + LM_L1P_2;
+  P = P * t + LM_L1P_1;
+  P = P * t;
+  const double L = (double)e * LM_LN2 + P;
+  const double eps = L * (0x1p-54 / 3.0);
+  return c + (c_lo - c * eps);
+}
+
 /* Python / Numba float floor-modulo a % b (fmod, then shift into the sign of b). */
 LM_FN double lm_pymod(double a, double b) {
   double m = LM_FMOD(a, b);

@@ -58,9 +58,9 @@ LMD double lmp_sigma_water(double T) {
   const double tau = 1 - T / 647.096;
   return 0.2358 * lm_pow(tau, 1.256) * (1 - 0.625 * tau);
 }
-/* microphysics.py:83-100 */
+/* microphysics.py:83-100 (the exponent is one_third = RN(1/3), lm_pow_third's) */
 LMD double lmp_radius(const LmPhys &P, double m, double rho) {
-  return lm_pow(P.four_pi_over_three_inv * m / rho, P.one_third);
+  return lm_pow_third(P.four_pi_over_three_inv * m / rho);
 }
 /* microphysics.py:989-1006 */
 LMD double lmp_l_alpha(const LmPhys &P, double T, double p, double K) {

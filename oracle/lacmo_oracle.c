@@ -32,12 +32,14 @@
 #define OR_EXP exp
 #define OR_LOG log
 #define OR_POW pow
+#define OR_POW_THIRD(x) pow((x), 1.0 / 3.0)
 #include "../lacmopy_amd/csrc/lm_math.h" /* for lm_pymod etc. only */
 #else
 #include "../lacmopy_amd/csrc/lm_math.h"
 #define OR_EXP lm_exp
 #define OR_LOG lm_log
 #define OR_POW lm_pow
+#define OR_POW_THIRD lm_pow_third
 #endif
 #include "../lacmopy_amd/csrc/lm_philox.h"
 
@@ -167,7 +169,7 @@ static double saturation_pressure_vapor_liquid(double T) { return 2.53E11 * OR_E
 /* microphysics.py                                                          */
 /* ------------------------------------------------------------------------ */
 static double radius_from_mass(double m, double rho) { /* :83-100 */
-  return OR_POW(K.four_pi_over_three_inv * m / rho, K.one_third);
+  return OR_POW_THIRD(K.four_pi_over_three_inv * m / rho);
 }
 /* :179-234, dispatch :237-266.  solute 0 = AS, 1 = NaCl */
 static void R_p_w_s_rho_p(double m_w, double m_s, double T, int solute, double *R, double *w_s, double *rho) {
@@ -1196,5 +1198,7 @@ EXPORT void or_lm_vec(int which, int64_t n, const double *x, const double *y, do
       out[k] = lm_pow(x[k], y[k]);
     else if (which == 3)
       out[k] = lm_pymod(x[k], y[k]);
+    else if (which == 7)
+      out[k] = lm_pow_third(x[k]);
   }
 }

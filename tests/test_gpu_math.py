@@ -30,7 +30,7 @@ def _inputs(seed=1, n=400_000):
     return x, y
 
 
-@pytest.mark.parametrize("which", [0, 1, 2])
+@pytest.mark.parametrize("which", [0, 1, 2, 7])
 def test_lm_math_device_equals_host(gpu, which):
     x, y = _inputs(which + 1)
     if which == 0:
