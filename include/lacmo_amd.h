@@ -61,6 +61,8 @@ typedef struct LmEngine {
   int32_t *cell[2]; /* cell[ccur] = current cell; cell[1-ccur] = next */
   int32_t *seg[2];  /* n_runs*(nc+1)+1 segment starts */
   double *dmxi;     /* n_tot scratch: dm * xi per SIP */
+  int32_t *away;    /* n_runs*nc: per home cell, 9-bit mask of the neighbour directions its SIPs moved to
+                       since the last reindex (zero after a reindex or a load) */
   /* grid, per run */
   double *scal; /* n_runs*10*nc: T, p, Theta, rho_d, r_v, r_l, S, e_s, m_dry_inv, rho_d_inv */
   double *mat;  /* n_runs*7*nc: K, D_v, L_v, sigma_w, c_p, mu, rho_f */

@@ -43,7 +43,7 @@ class LmEngine(C.Structure):
         ("x", _p * 2), ("z", _p * 2), ("u", _p * 2), ("w", _p * 2),
         ("m_w", _p * 2), ("m_s", _p * 2), ("xi", _p * 2),
         ("id", _p * 2), ("cell", _p * 2), ("seg", _p * 2),
-        ("dmxi", _p),
+        ("dmxi", _p), ("away", _p),
         ("scal", _p), ("mat", _p), ("cc", _p), ("dTh", _p), ("dRv", _p), ("dml", _p),
         ("grid_vel", _p), ("flux", _p), ("prof_rv", _p), ("prof_Th", _p), ("t_relax", _p),
         ("colsum", _p),
@@ -57,7 +57,7 @@ class LmEngine(C.Structure):
 
 
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
-                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len"]
+                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",

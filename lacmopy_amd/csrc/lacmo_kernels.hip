@@ -144,7 +144,21 @@ struct NbrTab {
 };
 
 /* Lanes 0..8 fill the table; the caller synchronises afterwards. */
-__device__ static inline void nbr_table(int gc, int nc, int nx, int nz, const int32_t *seg_run, NbrTab *tb) {
+/* bit of the move from home cell (hx, hz) to the adjacent cell (cx, cz), x periodic */
+__device__ static inline int dir_bit(int hx, int hz, int cx, int cz, int nx) {
+  const int dxm = ((cx - hx) % nx + nx) % nx;
+  const int ddx = dxm == 0 ? 0 : (dxm == 1 ? 1 : -1);
+  return 1 << ((ddx + 1) * 3 + (cz - hz + 1));
+}
+
+/*
+ * Segments that can hold members of cell gc: its own home segment, and each
+ * neighbouring home segment whose SIPs have moved towards gc since the last
+ * reindex (away mask; a superset is harmless, so bits are never cleared
+ * between reindexes).  Segments that cannot hold members get length 0.
+ */
+__device__ static inline void nbr_table(int gc, int nc, int nx, int nz, const int32_t *seg_run,
+                                        const int32_t *away_run, NbrTab *tb) {
   const int lane = threadIdx.x;
   const int c = gc % nc;
   const int i = c / nz, j = c % nz;
@@ -157,9 +171,10 @@ __device__ static inline void nbr_table(int gc, int nc, int nx, int nz, const in
     /* duplicates of the periodic wrap when nx < 3 are dropped */
     const bool valid = jj >= 0 && jj < nz && !(nx == 1 && di != 0) && !(nx == 2 && di == 1);
     const int cc = ii * nz + jj;
-    const int s0 = valid ? seg_run[cc] : 0;
+    const bool scan = valid && (lane == 4 || (away_run[valid ? cc : 0] & dir_bit(ii, jj, i, j, nx)) != 0);
+    const int s0 = scan ? seg_run[cc] : 0;
     tb->start[lane] = s0;
-    tb->pre[lane + 1] = valid ? seg_run[cc + 1] - s0 : 0;
+    tb->pre[lane + 1] = scan ? seg_run[cc + 1] - s0 : 0;
   }
   __syncthreads();
   if (lane == 0) {
@@ -316,6 +331,7 @@ struct KView {
   double *x2, *z2, *u2, *w2, *m_w2, *m_s2, *xi2;
   int32_t *id2, *cell2, *seg2;
   double *dmxi, *scal, *mat, *cc, *dTh, *dRv, *dml;
+  int32_t *away;
   const double *grid_vel, *flux, *prof_rv, *prof_Th, *t_relax;
   double *colsum;
   long long *no_cols;
@@ -380,6 +396,7 @@ static KView make_view(const LmEngine *e) {
   v.cell_next = e->cell[1 - e->ccur];
   v.cell2 = e->cell[1 - e->ccur];
   v.dmxi = e->dmxi;
+  v.away = e->away;
   v.scal = e->scal;
   v.mat = e->mat;
   v.cc = e->cc;
@@ -599,7 +616,7 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   int *sRank = (int *)sW;
   unsigned char *sSub = (unsigned char *)(sRank + CAP);
 
-  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
   if (n > CAP) {
     if (lane == 0) {
@@ -836,11 +853,16 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   }
 }
 
+__global__ void k_fill_i32(int32_t *a, int32_t value, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = value;
+}
+
 /* member counts -> pair counts for the injected stream offsets */
 __global__ __launch_bounds__(WAVE) void k_count_pairs(KView v) {
   const int gc = blockIdx.x, nc = v.nc, r = gc / nc;
   __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = count_members(gc, v.cell, &tb);
   if (threadIdx.x == 0) v.rnd_off[gc] = (long long)n * (n - 1) / 2;
 }
@@ -881,7 +903,7 @@ __global__ __launch_bounds__(1024) void k_scan_pairs(long long *a, int L) {
  * segments around it, so a SIP may drift at most one cell (periodic in x) from
  * the home cell of its slot k; the home cell is the segment that holds k.
  */
-__device__ static inline bool near_home(const KView &v, int r, long long k, int ii, int jj) {
+__device__ static inline bool near_home(const KView &v, int r, long long k, int ii, int jj, int *home) {
   const int32_t *s = v.seg + (size_t)r * (v.nc + 1);
   int lo = 0, hi = v.nc - 1;
   while (lo < hi) {
@@ -891,6 +913,7 @@ __device__ static inline bool near_home(const KView &v, int r, long long k, int 
     else
       hi = mid - 1;
   }
+  *home = lo;
   const int hx = lo / v.nz, hz = lo % v.nz;
   const int ddx = ((ii - hx) % v.nx + v.nx) % v.nx;
   const int ddz = jj - hz;
@@ -971,7 +994,13 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
       gc_new = gc;
     } else {
       gc_new = r * nc + (int)ii * v.nz + (int)jj;
-      if (gc_new != gc && !near_home(v, r, k, (int)ii, (int)jj)) raise_err(v.err, LM_ERR_LOST_SIP, (int)k);
+      if (gc_new != gc) { /* rare: the SIP changes cell */
+        int home;
+        if (!near_home(v, r, k, (int)ii, (int)jj, &home))
+          raise_err(v.err, LM_ERR_LOST_SIP, (int)k);
+        else if (home != (int)ii * v.nz + (int)jj)
+          atomicOr(&v.away[(size_t)r * nc + home], dir_bit(home / v.nz, home % v.nz, (int)ii, (int)jj, v.nx));
+      }
     }
   }
   v.x[k] = x;
@@ -1027,7 +1056,7 @@ __global__ __launch_bounds__(WAVE) void k_dml(KView v) {
   __shared__ unsigned char sSub[DML_CAP];
   __shared__ double sVal[DML_CAP];
   __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   if (n > DML_CAP) {
     if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
@@ -1053,7 +1082,7 @@ __global__ __launch_bounds__(WAVE) void k_rl(KView v) {
   __shared__ unsigned char sSub[DML_CAP];
   __shared__ double sVal[DML_CAP];
   __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), &tb);
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   if (n > DML_CAP) {
     if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
@@ -1250,7 +1279,7 @@ __global__ __launch_bounds__(WAVE) void k_ri_count(KView v) {
   const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
   const int32_t *sr = v.seg + (size_t)r * (nc + 1);
   __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, sr, &tb);
+  nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
   const int n = count_members(gc, v.cell, &tb);
   int dead = 0;
   for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
@@ -1366,7 +1395,7 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
   __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
   __shared__ unsigned char sSub[DML_CAP];
   __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, sr, &tb);
+  nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   if (n > DML_CAP) {
     if (lane == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
@@ -1794,7 +1823,8 @@ static int check_engine(const LmEngine *e) {
   if (e->n_runs <= 0 || e->nx <= 0 || e->nz <= 0 || e->nc != e->nx * e->nz)
     return set_err(LM_ERR_ARG, "bad grid shape (need nc == nx*nz > 0, n_runs > 0)");
   if (e->cur < 0 || e->cur > 1 || e->ccur < 0 || e->ccur > 1) return set_err(LM_ERR_ARG, "bad buffer index");
-  if (!e->x[0] || !e->x[1] || !e->cell[0] || !e->cell[1] || !e->seg[0] || !e->seg[1] || !e->cc || !e->err)
+  if (!e->x[0] || !e->x[1] || !e->cell[0] || !e->cell[1] || !e->seg[0] || !e->seg[1] || !e->cc || !e->err ||
+      !e->away)
     return set_err(LM_ERR_ARG, "engine buffer pointer is NULL");
   if ((int64_t)e->n_runs * e->nc > 0x7fffffff / 2) return set_err(LM_ERR_ARG, "too many cells");
   if (e->n_tot > 0x7ffffff0) return set_err(LM_ERR_ARG, "too many SIPs for 32-bit slots");
@@ -1833,7 +1863,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, scal),      (int64_t)offsetof(LmEngine, grid_vel),
       (int64_t)offsetof(LmEngine, no_cols),   (int64_t)offsetof(LmEngine, rem_cap),
       (int64_t)offsetof(LmEngine, scan_buf),  (int64_t)offsetof(LmEngine, rnd_stream),
-      (int64_t)offsetof(LmEngine, rnd_len),
+      (int64_t)offsetof(LmEngine, rnd_len),     (int64_t)offsetof(LmEngine, away),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -1907,6 +1937,8 @@ int lm_reindex(LmEngine *e, void *stream) {
     ProfScope ps(e, 3, S(stream));
     hipLaunchKernelGGL(k_ri_scatter, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
   }
+  /* every SIP is in its home cell again (a fill kernel, not a memset: the step stays graph-capturable) */
+  hipLaunchKernelGGL(k_fill_i32, dim3((R * nc + 255) / 256), dim3(256), 0, S(stream), e->away, 0, R * nc);
   LAUNCH_CHECK();
   e->cur = 1 - e->cur;
   e->ccur = 1 - e->ccur;

@@ -59,6 +59,7 @@ class Engine:
             mk(f"cell{b}", (n,), i32)
             mk(f"seg{b}", (nr * (nc + 1) + 1,), i32)
         mk("dmxi", (n,), f64)
+        mk("away", (nr * nc,), i32)
         mk("scal", (nr, 10, nc), f64)
         mk("mat", (nr, 7, nc), f64)
         mk("cc", (nr * nc, CC_N), f64)
@@ -97,7 +98,7 @@ class Engine:
         for b in (0, 1):
             for f in ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "cell", "seg"):
                 getattr(e, f)[b] = self.buf[f"{f}{b}"].data_ptr()
-        for f in ("dmxi", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
+        for f in ("dmxi", "away", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
                   "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
                   "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off"):
             setattr(e, f, self.buf[f].data_ptr())
@@ -175,6 +176,7 @@ class Engine:
         self.buf[f"id{cur}"].copy_(torch.from_numpy(hid))
         self.buf[f"cell{self.e.ccur}"].copy_(torch.from_numpy(hcell))
         self.buf[f"seg{cur}"].copy_(torch.from_numpy(seg))
+        self.buf["away"].zero_()  # every SIP sits in its home cell
 
     def set_counters(self, run, no_cols=None, water_removed=None):
         if no_cols is not None:
