@@ -86,7 +86,7 @@ typedef struct LmEngine {
   int32_t *scan_part; /* scan partials */
   int32_t *err;       /* 4 error words on the device */
   int32_t *ovf_list;  /* cells over the fast-path LDS capacity */
-  int32_t *ovf_cnt;   /* 1 */
+  int32_t *ovf_cnt;   /* 2: cells queued for the large-capacity collision [0] and Delta m_l [1] passes */
   /* injected uniform stream (parity mode; NULL = Philox) */
   const double *rnd_stream;
   int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */

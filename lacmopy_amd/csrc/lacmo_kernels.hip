@@ -313,6 +313,11 @@ __device__ static void rank_members(int n, const int *s_id, const unsigned char 
   }
 }
 
+__global__ void k_fill_i32(int32_t *a, int32_t value, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = value;
+}
+
 /* ------------------------------------------------------------------------ */
 /* kernel view: what the device needs from LmEngine                         */
 /* ------------------------------------------------------------------------ */
@@ -853,11 +858,6 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   }
 }
 
-__global__ void k_fill_i32(int32_t *a, int32_t value, int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) a[t] = value;
-}
-
 /* member counts -> pair counts for the injected stream offsets */
 __global__ __launch_bounds__(WAVE) void k_count_pairs(KView v) {
   const int gc = blockIdx.x, nc = v.nc, r = gc / nc;
@@ -1047,25 +1047,56 @@ __global__ __launch_bounds__(WAVE) void k_water(KView v) {
 
 /*
  * Delta m_l per cell: sum of dm*xi over the cell's members in ascending id
- * (integration.py:915-938), then *1e-18.
+ * (integration.py:915-938), then *1e-18.  CAP bounds the members held in LDS;
+ * the CAP = 256 pass (small LDS, high occupancy) queues larger cells in the
+ * overflow list (counter ovf_cnt[1]) for the CAP = 1024 pass.
  */
-__global__ __launch_bounds__(WAVE) void k_dml(KView v) {
+template <int CAP>
+__global__ __launch_bounds__(WAVE) void k_dml(KView v, int from_ovf) {
   const int nc = v.nc;
-  const int gc = blockIdx.x, r = gc / nc;
-  __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
-  __shared__ unsigned char sSub[DML_CAP];
-  __shared__ double sVal[DML_CAP];
+  int gc;
+  if (from_ovf) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && v.ovf_cnt[1] > (int)gridDim.x) raise_err(v.err, LM_ERR_CELL_CAP, -1);
+    if ((int)blockIdx.x >= v.ovf_cnt[1]) return;
+    gc = v.ovf_list[blockIdx.x];
+  } else {
+    gc = blockIdx.x;
+  }
+  const int r = gc / nc;
+  __shared__ int sSlot[CAP], sIdRank[2 * CAP];
+  __shared__ unsigned char sSub[CAP];
   __shared__ NbrTab tb;
+  int *sId = sIdRank, *sRank = sIdRank + CAP;
+  double *sVal = (double *)sIdRank; /* reuses the id/rank storage once the ranks are in registers */
   nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
-  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
-  if (n > DML_CAP) {
-    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
+  if (n > CAP) {
+    if (threadIdx.x == 0) {
+      if (from_ovf) {
+        raise_err(v.err, LM_ERR_CELL_CAP, gc);
+      } else {
+        const int q = atomicAdd(&v.ovf_cnt[1], 1);
+        v.ovf_list[q] = gc;
+      }
+    }
     return;
   }
   __syncthreads();
   rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
-  for (int m = threadIdx.x; m < n; m += WAVE) sVal[sRank[m]] = v.dmxi[sSlot[m]];
+  constexpr int PER = CAP / WAVE;
+  int rk[PER];
+  double val[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int m = threadIdx.x + q * WAVE;
+    rk[q] = m < n ? sRank[m] : 0;
+    val[q] = m < n ? v.dmxi[sSlot[m]] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q)
+    if (threadIdx.x + q * WAVE < n) sVal[rk[q]] = val[q];
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
@@ -1898,7 +1929,7 @@ int lm_collision_step(LmEngine *e, void *stream) {
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, S(stream), (long long *)e->rnd_off, ncell);
     LAUNCH_CHECK();
   }
-  HIP_TRY(hipMemsetAsync(e->ovf_cnt, 0, sizeof(int32_t), S(stream)));
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt, 0, 1);
   KView v = make_view(e);
   {
     ProfScope ps(e, 1, S(stream));
@@ -1973,7 +2004,9 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
   }
   {
     ProfScope ps(e, 2, S(stream));
-    hipLaunchKernelGGL(k_dml, dim3(ncell), dim3(WAVE), 0, S(stream), v);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 1, 0, 1);
+    hipLaunchKernelGGL(k_dml<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
+    hipLaunchKernelGGL(k_dml<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, 1);
   }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();

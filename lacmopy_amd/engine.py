@@ -82,7 +82,7 @@ class Engine:
         mk("scan_part", ((L + 4095) // 4096 + 1,), i32)
         mk("err", (4,), i32)
         mk("ovf_list", (nr * nc,), i32)
-        mk("ovf_cnt", (1,), i32)
+        mk("ovf_cnt", (2,), i32)
         mk("rnd_off", (nr * nc + 2,), i64)
         self.E_tab = None
         self.rnd_stream = None

@@ -183,3 +183,22 @@ def test_collision_many_events(gpu, kernel):
     _assert_same(eng.counters(0)[0], no_cols, "no_cols")
     assert no_cols[0] > 36 * 3 * 20, f"expected many events per cell, got {no_cols}"
     assert np.isfinite(xi).all()
+
+
+def test_dense_cells_take_the_large_capacity_passes(gpu):
+    """Cells of ~300 SIPs exceed the 256-member fast passes of the collision and Delta m_l kernels and go
+    through their 1024-member overflow passes; results stay bit-identical to the oracle."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(nx=4, nz=4, spc=300, seed=13)
+    parts = _prep(grid, parts)
+    counts = np.bincount(parts[2][0] * 4 + parts[2][1], minlength=16)
+    assert counts.min() > 256
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 5, 1)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 5)
+    eng.adv_steps(1)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, nm)
+    s, m = eng.export_grid(0)
+    _assert_same(s[:8], scal[:8], "grid_scalar_fields")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
