@@ -28,6 +28,7 @@
 
 #define WAVE 64
 #define DML_CAP 512 /* member capacity of the gather kernels (Delta m_l, reindex) */
+#define OVF_GRID 1024 /* blocks of the grid-strided overflow passes */
 
 /* ------------------------------------------------------------------------ */
 /* host-side constants (same expressions as the reference modules)          */
@@ -597,16 +598,9 @@ __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * 
  * block of A words.
  */
 template <int CAP, bool TAB, bool INJ>
-__global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
+__device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   const LmPhys &P = c_P;
-  int gc;
-  if (from_ovf) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && *v.ovf_cnt > (int)gridDim.x) raise_err(v.err, LM_ERR_CELL_CAP, -1);
-    if ((int)blockIdx.x >= *v.ovf_cnt) return;
-    gc = v.ovf_list[blockIdx.x];
-  } else {
-    gc = blockIdx.x;
-  }
+  __syncthreads(); /* the LDS of the previous cell of this block is free */
   const int nc = v.nc;
   const int r = gc / nc, c = gc % nc;
   const int lane = threadIdx.x;
@@ -858,6 +852,17 @@ __global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
   }
 }
 
+/* main pass: one block per cell; overflow pass: the queued cells, grid-strided */
+template <int CAP, bool TAB, bool INJ>
+__global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
+  if (!from_ovf) {
+    collide_cell<CAP, TAB, INJ>(v, blockIdx.x, 0);
+    return;
+  }
+  const int cnt = *v.ovf_cnt;
+  for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) collide_cell<CAP, TAB, INJ>(v, v.ovf_list[idx], 1);
+}
+
 /* member counts -> pair counts for the injected stream offsets */
 __global__ __launch_bounds__(WAVE) void k_count_pairs(KView v) {
   const int gc = blockIdx.x, nc = v.nc, r = gc / nc;
@@ -1052,16 +1057,9 @@ __global__ __launch_bounds__(WAVE) void k_water(KView v) {
  * overflow list (counter ovf_cnt[1]) for the CAP = 1024 pass.
  */
 template <int CAP>
-__global__ __launch_bounds__(WAVE) void k_dml(KView v, int from_ovf) {
+__device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
   const int nc = v.nc;
-  int gc;
-  if (from_ovf) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && v.ovf_cnt[1] > (int)gridDim.x) raise_err(v.err, LM_ERR_CELL_CAP, -1);
-    if ((int)blockIdx.x >= v.ovf_cnt[1]) return;
-    gc = v.ovf_list[blockIdx.x];
-  } else {
-    gc = blockIdx.x;
-  }
+  __syncthreads(); /* the LDS of the previous cell of this block is free */
   const int r = gc / nc;
   __shared__ int sSlot[CAP], sIdRank[2 * CAP];
   __shared__ unsigned char sSub[CAP];
@@ -1103,6 +1101,16 @@ __global__ __launch_bounds__(WAVE) void k_dml(KView v, int from_ovf) {
     for (int p = 0; p < n; ++p) s += sVal[p];
     v.dml[gc] = s * 1.0E-18;
   }
+}
+
+template <int CAP>
+__global__ __launch_bounds__(WAVE) void k_dml(KView v, int from_ovf) {
+  if (!from_ovf) {
+    dml_cell<CAP>(v, blockIdx.x, 0);
+    return;
+  }
+  const int cnt = v.ovf_cnt[1];
+  for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) dml_cell<CAP>(v, v.ovf_list[idx], 1);
 }
 
 /* grid.py:370-412 update_grid_r_l_np: r_l = 1e-18 m_dry^-1 sum(xi m_w), summed over the cell in ascending id */
@@ -1864,6 +1872,12 @@ static int check_engine(const LmEngine *e) {
 
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
+/* mean + 2.5 sqrt(mean) SIPs per cell: the member capacity the main per-cell passes are sized for */
+static int cell_capacity(const LmEngine *e) {
+  const double mean = (double)e->n_tot / ((double)e->n_runs * (double)e->nc);
+  return (int)(mean + 2.5 * sqrt(mean)) + 1;
+}
+
 template <int CAP>
 static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int from_ovf) {
   const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
@@ -1933,9 +1947,17 @@ int lm_collision_step(LmEngine *e, void *stream) {
   KView v = make_view(e);
   {
     ProfScope ps(e, 1, S(stream));
-    launch_collision<256>(v, e, dim3(ncell), S(stream), 0);
+    /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
+       small capacity keeps the LDS footprint, and so the occupancy, low; larger cells go to the overflow pass */
+    const int cap = cell_capacity(e);
+    if (cap <= 160)
+      launch_collision<160>(v, e, dim3(ncell), S(stream), 0);
+    else if (cap <= 256)
+      launch_collision<256>(v, e, dim3(ncell), S(stream), 0);
+    else
+      launch_collision<512>(v, e, dim3(ncell), S(stream), 0);
   }
-  launch_collision<1024>(v, e, dim3(ncell < 4096 ? ncell : 4096), S(stream), 1);
+  launch_collision<1024>(v, e, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), 1);
   LAUNCH_CHECK();
   e->col_step += 1;
   return 0;
@@ -2005,8 +2027,11 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
   {
     ProfScope ps(e, 2, S(stream));
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 1, 0, 1);
-    hipLaunchKernelGGL(k_dml<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
-    hipLaunchKernelGGL(k_dml<1024>, dim3(ncell < 4096 ? ncell : 4096), dim3(WAVE), 0, S(stream), v, 1);
+    if (cell_capacity(e) <= 256)
+      hipLaunchKernelGGL(k_dml<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
+    else
+      hipLaunchKernelGGL(k_dml<512>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
+    hipLaunchKernelGGL(k_dml<1024>, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), dim3(WAVE), 0, S(stream), v, 1);
   }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();

@@ -202,3 +202,29 @@ def test_dense_cells_take_the_large_capacity_passes(gpu):
     s, m = eng.export_grid(0)
     _assert_same(s[:8], scal[:8], "grid_scalar_fields")
     _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+
+
+def test_one_crowded_cell_takes_the_overflow_passes(gpu):
+    """A cell with ~400 SIPs among cells of ~30: the main collision and Delta m_l passes (sized for the mean)
+    queue it for their 1024-member overflow passes; results stay bit-identical to the oracle."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, pos, vel, cells, m_w, m_s, xi, act = _case(nx=4, nz=4, spc=30, seed=19)
+    rs = np.random.default_rng(4)
+    k = 380
+    src = rs.integers(0, len(xi), k)
+    extra_pos = np.array([rs.uniform(20.0, 40.0, k), rs.uniform(20.0, 40.0, k)])
+    parts = (np.concatenate([pos, extra_pos], 1), np.concatenate([vel, vel[:, src]], 1),
+             np.concatenate([cells, np.ones((2, k), dtype=np.int64)], 1), np.concatenate([m_w, m_w[src]]),
+             np.concatenate([m_s, m_s[src]]), np.concatenate([xi, xi[src]]), np.concatenate([act, np.ones(k, bool)]))
+    parts = _prep(grid, parts)
+    counts = np.bincount(parts[2][0] * 4 + parts[2][1], minlength=16)
+    assert counts[5] > 400 and np.median(counts) < 60
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 8, 1)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 8)
+    eng.adv_steps(1)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, nm)
+    s, m = eng.export_grid(0)
+    _assert_same(s[:8], scal[:8], "grid_scalar_fields")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
