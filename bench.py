@@ -1,19 +1,19 @@
 #!/usr/bin/env python3
-"""Benchmark: SIP-steps/s of the 2-D ICMW 75x75 case on one MI355X.
+"""Benchmark: SIP-steps/s of the super-droplet time step on one MI355X.
 
-Workload (synthetic inputs, lacmopy_amd.setup_case): 75x75 cells, 20 m,
-Poisson(128) SIPs per cell, bimodal AS aerosol, Long E_col table, dt_adv = 1 s,
-10 condensation substeps, 2 collision steps per advection step, relaxation on.
-  * batch: 64 independent seeds in one engine (the headline `value`)
+Default workload (--case icmw75; synthetic inputs from lacmopy_amd.setup_case): the 2-D ICMW
+kinematic stratocumulus, 75x75 cells of 20 m, Poisson(128) SIPs per cell, bimodal AS aerosol,
+Long E_col table, dt_adv = 1 s, 10 condensation substeps, 2 collision steps per advection step,
+relaxation on.  --case stress150: 150x150 cells of 10 m, Poisson(256), Hall E_col, dt 0.5 s,
+20 substeps, 16 seeds.
+  * batch: 64 (16) independent seeds in one engine (the headline `value`)
   * plain: one run (reported next to it; ratio = batch / plain)
 A "step" is one advection step of every SIP of every run.
-SIP-steps/s = sum over runs of active SIPs x timed steps / wall seconds, with
-all inputs resident in HBM; the timed region is bracketed by
-torch.cuda.synchronize().
+SIP-steps/s = sum over runs of SIPs x timed steps / wall seconds, with all inputs resident in HBM;
+the timed region is bracketed by torch.cuda.synchronize().
 
-Also measured: the dominant kernel's duration with HIP events (roofline vs
-HBM 8 TB/s, algorithmic bytes per SIP), and the CPU oracle on a bounded sample
-of the plain case (cpu_baseline).
+Also measured: the dominant kernel's duration with HIP events (roofline vs HBM 8 TB/s, algorithmic
+bytes per SIP), and the CPU oracle on a bounded sample of the plain 75x75 case (cpu_baseline).
 """
 import argparse
 import json
@@ -30,7 +30,17 @@ HBM_PEAK_GBS = 8000.0
 B_SUBSTEP = 104.0    # algorithmic bytes per SIP per particle substep (60 read + 44 write)
 B_COLLISION = 64.0   # per SIP per collision step (40 read + 24 write)
 B_REINDEX = 140.0    # per SIP per reindex
-B_STEP = 10 * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX   # 1308 B per SIP-step
+
+# Workloads.  icmw75: the 2-D ICMW kinematic stratocumulus on 75x75 cells of 20 m, Poisson(128) SIPs per
+# cell, Long E_col, dt 1 s, 10 substeps, 64 seeds.  stress150: the same 1500 m domain on 150x150 cells of
+# 10 m, Poisson(256) SIPs per cell, Hall E_col, mass-flux amplitude 1.0 kg m^-2 s^-1, dt 0.5 s,
+# 20 substeps, 16 seeds.  Both: 2 collision steps per advection step, relaxation on.
+CASES = {
+    "icmw75": dict(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, runs=64, kernel="Long_Bott", dt=1.0, sdc=5,
+                   j_max=0.6),
+    "stress150": dict(nx=150, nz=150, dx=10.0, dz=10.0, spc=256.0, runs=16, kernel="Hall_Bott", dt=0.5, sdc=10,
+                      j_max=1.0),
+}
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_icmw75_64seeds.md")
@@ -54,32 +64,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_engine(n_runs, spc, seed0, nx=75, nz=75, kernel="Long_Bott"):
-    """kernel: 'Long_Bott' / 'Hall_Bott' table, or 'const' (Hydro_E_const, E = 0.5)."""
+def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
+    """kernel: 'Long_Bott' / 'Hall_Bott' table, or 'const' (Hydro_E_const, E = 0.5); default: the case's."""
     import torch  # noqa: F401
     from lacmopy_amd import load_kernel_data_Ecol
     from lacmopy_amd.engine import Engine
     from lacmopy_amd.setup_case import init_profiles, make_case
+    c = CASES[case]
+    kernel = kernel or c["kernel"]
     if kernel == "const":
         E, lnR0, lnf = 0.5, None, None
     else:
         E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
     cases = []
     for r in range(n_runs):
-        grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=nx, nz=nz, spc=spc, seed=seed0 + 2 * r)
+        grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=c["nx"], nz=c["nz"], dx=c["dx"], dz=c["dz"], spc=spc,
+                                                             seed=seed0 + 2 * r, j_max_ref=c["j_max"])
         cases.append((grid, (pos, vel, cells, m_w, m_s, xi, act)))
     grid0 = cases[0][0]
-    eng = Engine(nx, nz, [len(c[1][5]) for c in cases], grid0.ranges, grid0.steps, grid0.volume_cell)
-    eng.set_params(solute="AS", no_iter_impl_mass=3, scale_dt_cond=5, no_col_per_adv=2, act_collisions=True,
-                   act_relaxation=True, dt=1.0, dt_col=0.5, g_set=9.80665, seed=seed0)
+    eng = Engine(c["nx"], c["nz"], [len(k[1][5]) for k in cases], grid0.ranges, grid0.steps, grid0.volume_cell)
+    eng.set_params(solute="AS", no_iter_impl_mass=3, scale_dt_cond=c["sdc"], no_col_per_adv=2, act_collisions=True,
+                   act_relaxation=True, dt=c["dt"], dt_col=c["dt"] / 2, g_set=9.80665, seed=seed0)
     eng.set_collision_kernel(E, lnR0, lnf)
     eng.set_flow(grid0.velocity, grid0.mass_flux_air_dry)
     for r, (g, _) in enumerate(cases):
         eng.set_relaxation(r, *init_profiles(g))
         eng.set_grid(r, g.scalar_fields(), g.mat_prop())
     eng.init_cells()
-    eng.load_particles([c[1] for c in cases])
-    n_sips = sum(len(c[1][5]) for c in cases)
+    eng.load_particles([k[1] for k in cases])
+    n_sips = sum(len(k[1][5]) for k in cases)
     return eng, n_sips, cases
 
 
@@ -165,16 +178,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--runs", type=int, default=64)
-    ap.add_argument("--spc", type=float, default=128.0)
+    ap.add_argument("--case", default="icmw75", choices=sorted(CASES))
+    ap.add_argument("--runs", type=int, default=None, help="seeds in the batch (default: the case's)")
+    ap.add_argument("--spc", type=float, default=None, help="mean SIPs per cell (default: the case's)")
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
-    ap.add_argument("--kernel", default="Long_Bott", choices=["Long_Bott", "Hall_Bott", "const"])
+    ap.add_argument("--kernel", default=None, choices=["Long_Bott", "Hall_Bott", "const"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
+    case = CASES[args.case]
+    args.runs = args.runs or case["runs"]
+    args.spc = args.spc or case["spc"]
+    args.kernel = args.kernel or case["kernel"]
+    n_sub = 2 * case["sdc"]
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): independent seeds shard "
               "trivially across GPUs and no multi-GPU path is built", file=sys.stderr)
@@ -187,7 +206,7 @@ def main():
 
     log(f"building batch of {args.runs} runs ...")
     t = time.perf_counter()
-    eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, kernel=args.kernel)
+    eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, case=args.case, kernel=args.kernel)
     log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
     secs = time_steps(eng, args.steps, args.warmup)
     batch_rate = n_sips * args.steps / secs
@@ -202,7 +221,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        eng1, n1, _ = build_engine(1, args.spc, args.seed, kernel=args.kernel)
+        eng1, n1, _ = build_engine(1, args.spc, args.seed, case=args.case, kernel=args.kernel)
         s1 = time_steps(eng1, max(args.steps, 10), args.warmup)
         plain = {"sip_steps_per_s": n1 * max(args.steps, 10) / s1, "ms_per_step": s1 / max(args.steps, 10) * 1e3,
                  "sips": n1}
@@ -214,17 +233,20 @@ def main():
         # dominant per-SIP kernel: the particle substep kernel (one thread per SIP)
         ms = ktimes["particles"][0]
         achieved = n_batch_sips * B_SUBSTEP / (ms * 1e-3) / 1e9
-        default_workload = args.runs == 64 and args.spc == 128.0 and args.seed == 1001 and args.kernel == "Long_Bott"
+        default_workload = (args.case == "icmw75" and args.runs == 64 and args.spc == 128.0 and args.seed == 1001
+                            and args.kernel == "Long_Bott")
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": measured_traffic("k_particles") if default_workload else None,
                 "kernel": "k_particles", "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
 
-    cpu = None if args.no_cpu else cpu_baseline(args.seed, args.spc)
+    # the CPU baseline is the oracle on the plain 75x75 case (the metric's workload)
+    cpu = None if (args.no_cpu or args.case != "icmw75") else cpu_baseline(args.seed, args.spc)
     if cpu:
         log(f"cpu baseline: {cpu}")
     res = {
-        "metric": "SIP-steps/s, 2D ICMW 75x75: 64-seed batch vs plain GPU",
+        "metric": "SIP-steps/s, 2D ICMW 75x75: 64-seed batch vs plain GPU" if args.case == "icmw75"
+        else f"SIP-steps/s, {args.case}",
         "value": batch_rate,
         "unit": "SIP-steps/s",
         "n_gpus": 1,
@@ -236,9 +258,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"ICMW 2D kinematic Sc, 75x75 cells, Poisson({args.spc:g}) SIPs/cell, "
-                               f"{args.runs} seeds batched, Long E_col, dt=1 s, 10 substeps, 2 collision steps",
-                   "runs": args.runs, "sips_total": n_batch_sips, "cells": 75 * 75,
+        "config": {"workload": f"ICMW 2D kinematic Sc ({args.case}), {case['nx']}x{case['nz']} cells of "
+                               f"{case['dx']:g} m, Poisson({args.spc:g}) SIPs/cell, {args.runs} seeds batched, "
+                               f"{args.kernel} E_col, dt={case['dt']:g} s, {n_sub} substeps, 2 collision steps",
+                   "runs": args.runs, "sips_total": n_batch_sips, "cells": case["nx"] * case["nz"],
+                   "algorithmic_bytes_per_sip_step": n_sub * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX,
                    "plain_single_run": plain,
                    "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
                    "kernel_ms": ktimes, "collision_events": ncols},
