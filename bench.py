@@ -96,13 +96,18 @@ def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
     return eng, n_sips, cases
 
 
-def time_steps(eng, steps, warmup):
+def time_steps(eng, steps, warmup, graph=False):
+    """Wall time of `steps` advection steps after `warmup` untimed ones (graph: replays of a captured
+    two-step HIP graph, captured during the warm-up)."""
     import torch
-    eng.adv_steps(warmup)
+    run = eng.adv_steps_graph if graph else eng.adv_steps
+    run(warmup)
+    if graph and warmup < 2:
+        run(2)  # capture outside the timed region
     torch.cuda.synchronize()
     eng.check()
     t0 = time.perf_counter()
-    eng.adv_steps(steps)
+    run(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     eng.check()
@@ -186,6 +191,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--kernel", default=None, choices=["Long_Bott", "Hall_Bott", "const"])
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the step captured in a two-step HIP graph instead of launching kernel by kernel")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
@@ -208,7 +215,7 @@ def main():
     t = time.perf_counter()
     eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, case=args.case, kernel=args.kernel)
     log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
-    secs = time_steps(eng, args.steps, args.warmup)
+    secs = time_steps(eng, args.steps, args.warmup, graph=args.graph)
     batch_rate = n_sips * args.steps / secs
     log(f"batch: {secs / args.steps * 1e3:.2f} ms/step, {batch_rate:.4g} SIP-steps/s")
     ncols = eng.buf["no_cols"].sum(0).cpu().tolist()
@@ -222,7 +229,7 @@ def main():
     plain = None
     if not args.no_plain:
         eng1, n1, _ = build_engine(1, args.spc, args.seed, case=args.case, kernel=args.kernel)
-        s1 = time_steps(eng1, max(args.steps, 10), args.warmup)
+        s1 = time_steps(eng1, max(args.steps, 10), args.warmup, graph=args.graph)
         plain = {"sip_steps_per_s": n1 * max(args.steps, 10) / s1, "ms_per_step": s1 / max(args.steps, 10) * 1e3,
                  "sips": n1}
         log(f"plain: {plain}")
@@ -263,6 +270,7 @@ def main():
                                f"{args.kernel} E_col, dt={case['dt']:g} s, {n_sub} substeps, 2 collision steps",
                    "runs": args.runs, "sips_total": n_batch_sips, "cells": case["nx"] * case["nz"],
                    "algorithmic_bytes_per_sip_step": n_sub * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX,
+                   "launch": "two-step HIP graph replays" if args.graph else "eager",
                    "plain_single_run": plain,
                    "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
                    "kernel_ms": ktimes, "collision_events": ncols},

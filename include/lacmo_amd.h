@@ -51,7 +51,7 @@ typedef struct LmEngine {
   int32_t nb, e_const_mode;
   double E_const, lnR0, lnf;
   uint64_t seed;
-  uint32_t col_step; /* collision-step counter (Philox counter word) */
+  uint32_t col_step; /* host mirror of *col_step_dev, the collision-step counter (Philox counter word) */
   int32_t cur;       /* current ping-pong buffer of the SIP state and seg */
   int32_t ccur;      /* current ping-pong buffer of cell[] */
   int32_t pad1;
@@ -61,6 +61,7 @@ typedef struct LmEngine {
   int32_t *cell[2]; /* cell[ccur] = current cell; cell[1-ccur] = next */
   int32_t *seg[2];  /* n_runs*(nc+1)+1 segment starts */
   double *dmxi;     /* n_tot scratch: dm * xi per SIP */
+  uint32_t *col_step_dev; /* 1: the collision-step counter the kernels read (set with lm_set_col_step) */
   int32_t *away;    /* n_runs*nc: per home cell, 9-bit mask of the neighbour directions its SIPs moved to
                        since the last reindex (zero after a reindex or a load) */
   /* grid, per run */
@@ -169,6 +170,22 @@ int lm_update_r_l(LmEngine *e, void *stream);
 
 /* n_steps advection steps (integration.py:1298-1611, integrate_adv_step_np). */
 int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream);
+
+/* Sets the collision-step counter (device value and host mirror). */
+int lm_set_col_step(LmEngine *e, uint32_t step, void *stream);
+
+/*
+ * lm_adv_steps(e, n_steps) captured into an executable HIP graph on a
+ * non-default stream (n_steps even: a step swaps the SIP buffers once, so an
+ * even count leaves the buffer indices where the capture found them).  Each
+ * lm_graph_launch advances the engine by n_steps advection steps, with the
+ * collision-step counter read from device memory.  The graph holds the engine's
+ * buffers and parameters as they were at capture time.  No counterpart in the
+ * reference (its steps are Python calls).
+ */
+int lm_graph_capture(LmEngine *e, int32_t n_steps, void *stream, void **graph);
+int lm_graph_launch(LmEngine *e, void *graph, void *stream);
+int lm_graph_destroy(void *graph);
 
 /*
  * Kernel timing with HIP events on the launch stream (no reference

@@ -43,7 +43,7 @@ class LmEngine(C.Structure):
         ("x", _p * 2), ("z", _p * 2), ("u", _p * 2), ("w", _p * 2),
         ("m_w", _p * 2), ("m_s", _p * 2), ("xi", _p * 2),
         ("id", _p * 2), ("cell", _p * 2), ("seg", _p * 2),
-        ("dmxi", _p), ("away", _p),
+        ("dmxi", _p), ("col_step_dev", _p), ("away", _p),
         ("scal", _p), ("mat", _p), ("cc", _p), ("dTh", _p), ("dRv", _p), ("dml", _p),
         ("grid_vel", _p), ("flux", _p), ("prof_rv", _p), ("prof_Th", _p), ("t_relax", _p),
         ("colsum", _p),
@@ -57,7 +57,7 @@ class LmEngine(C.Structure):
 
 
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
-                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away"]
+                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",
@@ -66,7 +66,7 @@ EXPORTED = [
     "lm_adv_steps", "lm_engine_check", "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe",
     "lm_prof_enable", "lm_prof_collect", "lm_prof_disable", "lm_particle_substep", "lm_grid_substep",
     "lm_box2d_step", "lm_box_check", "lm_divergence_upwind", "lm_update_r_l",
-    "lm_kernel_grid_m_step",
+    "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture", "lm_graph_launch", "lm_graph_destroy",
 ]
 
 _lib = None
@@ -104,6 +104,10 @@ def lib():
     L.lm_box2d_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _p, _p, _d, _p, _d, _i32, _d, _d, _p,
                                 C.c_uint64, C.c_uint32, _p, _p]
     L.lm_box_check.argtypes = [_p]
+    L.lm_set_col_step.argtypes = [E, C.c_uint32, _p]
+    L.lm_graph_capture.argtypes = [E, _i32, _p, C.POINTER(_p)]
+    L.lm_graph_launch.argtypes = [E, _p, _p]
+    L.lm_graph_destroy.argtypes = [_p]
     L.lm_kernel_grid_m_step.argtypes = [_i32, _p, _p, _p, _d, _p, _i32, _d, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
     L.lm_divergence_upwind.argtypes = [_p, _p, _p, _i32, _i32, _i32, _d, _d, _i32, _i32, _p]
     L.lm_adv_steps.argtypes = [E, _i32, _p]
@@ -117,7 +121,8 @@ def lib():
     for name in ["lm_position_step", "lm_advection_increments", "lm_substep", "lm_adv_steps",
                  "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe", "lm_prof_enable", "lm_prof_collect",
                  "lm_prof_disable", "lm_particle_substep", "lm_grid_substep", "lm_box2d_step", "lm_box_check",
-                 "lm_divergence_upwind", "lm_kernel_grid_m_step"]:
+                 "lm_divergence_upwind", "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture",
+                 "lm_graph_launch", "lm_graph_destroy"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -319,6 +319,14 @@ __global__ void k_fill_i32(int32_t *a, int32_t value, int n) {
   if (t < n) a[t] = value;
 }
 
+__global__ void k_add_u32(uint32_t *a, uint32_t inc) {
+  if (threadIdx.x == 0) a[0] += inc;
+}
+
+__global__ void k_set_u32(uint32_t *a, uint32_t value) {
+  if (threadIdx.x == 0) a[0] = value;
+}
+
 /* ------------------------------------------------------------------------ */
 /* kernel view: what the device needs from LmEngine                         */
 /* ------------------------------------------------------------------------ */
@@ -331,7 +339,7 @@ struct KView {
   int nb, e_const_mode;
   double E_const, lnR0, lnf;
   unsigned long long seed;
-  unsigned col_step;
+  const uint32_t *col_step_dev;
   double *x, *z, *u, *w, *m_w, *m_s, *xi;
   int32_t *id, *cell, *cell_next, *seg;
   double *x2, *z2, *u2, *w2, *m_w2, *m_s2, *xi2;
@@ -378,7 +386,7 @@ static KView make_view(const LmEngine *e) {
   v.lnR0 = e->lnR0;
   v.lnf = e->lnf;
   v.seed = e->seed;
-  v.col_step = e->col_step;
+  v.col_step_dev = e->col_step_dev;
   const int c = e->cur, o = 1 - e->cur;
   v.x = e->x[c];
   v.z = e->z[c];
@@ -662,7 +670,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   const bool use_tab = TAB;
   const long long roff = INJ ? v.rnd_off[gc] : 0;
   const double dtdV = v.dt_col_over_dV, pi = P.pi;
-  const uint32_t key_run = (uint32_t)r, key_step = v.col_step, key_cell = (uint32_t)c;
+  const uint32_t key_run = (uint32_t)r, key_step = *v.col_step_dev, key_cell = (uint32_t)c;
   double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
   const int nQ = sQ[n - 1];
   long long c_single = 0, c_multi = 0, c_eq = 0;
@@ -1863,7 +1871,7 @@ static int check_engine(const LmEngine *e) {
     return set_err(LM_ERR_ARG, "bad grid shape (need nc == nx*nz > 0, n_runs > 0)");
   if (e->cur < 0 || e->cur > 1 || e->ccur < 0 || e->ccur > 1) return set_err(LM_ERR_ARG, "bad buffer index");
   if (!e->x[0] || !e->x[1] || !e->cell[0] || !e->cell[1] || !e->seg[0] || !e->seg[1] || !e->cc || !e->err ||
-      !e->away)
+      !e->away || !e->col_step_dev)
     return set_err(LM_ERR_ARG, "engine buffer pointer is NULL");
   if ((int64_t)e->n_runs * e->nc > 0x7fffffff / 2) return set_err(LM_ERR_ARG, "too many cells");
   if (e->n_tot > 0x7ffffff0) return set_err(LM_ERR_ARG, "too many SIPs for 32-bit slots");
@@ -1909,6 +1917,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, no_cols),   (int64_t)offsetof(LmEngine, rem_cap),
       (int64_t)offsetof(LmEngine, scan_buf),  (int64_t)offsetof(LmEngine, rnd_stream),
       (int64_t)offsetof(LmEngine, rnd_len),     (int64_t)offsetof(LmEngine, away),
+      (int64_t)offsetof(LmEngine, col_step_dev),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -1958,6 +1967,8 @@ int lm_collision_step(LmEngine *e, void *stream) {
       launch_collision<512>(v, e, dim3(ncell), S(stream), 0);
   }
   launch_collision<1024>(v, e, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), 1);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, S(stream), e->col_step_dev, 1u);
   LAUNCH_CHECK();
   e->col_step += 1;
   return 0;
@@ -2092,6 +2103,77 @@ int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream) {
     if ((rc = subloop(e, e->scale_dt_cond - 1, ncol, stream))) return rc;
     if ((rc = lm_substep(e, e->dt_sub, e->dt_sub_half, stream))) return rc;
   }
+  return 0;
+}
+
+int lm_set_col_step(LmEngine *e, uint32_t step, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, S(stream), e->col_step_dev, step);
+  LAUNCH_CHECK();
+  e->col_step = step;
+  return 0;
+}
+
+/* A captured sequence of advection steps: the executable graph and what one launch advances. */
+struct LmGraph {
+  hipGraphExec_t exec;
+  uint32_t col_steps;
+  int cur, ccur; /* buffer indices the captured kernels were bound to */
+};
+
+int lm_graph_capture(LmEngine *e, int32_t n_steps, void *stream, void **graph) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (!graph || n_steps <= 0 || (n_steps & 1))
+    return set_err(LM_ERR_ARG, "graph capture needs an even, positive number of advection steps "
+                               "(one step swaps the SIP buffers once)");
+  if (e->prof) return set_err(LM_ERR_ARG, "disable kernel timing (lm_prof_disable) before capturing a graph");
+  if (!stream) return set_err(LM_ERR_ARG, "graph capture needs a non-default stream");
+  const uint32_t step0 = e->col_step;
+  const int cur0 = e->cur, ccur0 = e->ccur;
+  HIP_TRY(hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal));
+  rc = lm_adv_steps(e, n_steps, stream);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(S(stream), &g);
+  const uint32_t col_steps = e->col_step - step0;
+  e->col_step = step0; /* nothing ran yet */
+  if (rc || ec != hipSuccess || e->cur != cur0 || e->ccur != ccur0) {
+    if (g) (void)hipGraphDestroy(g);
+    e->cur = cur0;
+    e->ccur = ccur0;
+    if (rc) return rc;
+    return set_err(LM_ERR_ARG, "stream capture of the advection steps failed");
+  }
+  hipGraphExec_t ex;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_TRY(ei);
+  LmGraph *h = new LmGraph;
+  h->exec = ex;
+  h->col_steps = col_steps;
+  h->cur = cur0;
+  h->ccur = ccur0;
+  *graph = h;
+  return 0;
+}
+
+int lm_graph_launch(LmEngine *e, void *graph, void *stream) {
+  if (!e || !graph) return set_err(LM_ERR_ARG, "engine/graph is NULL");
+  LmGraph *h = (LmGraph *)graph;
+  if (e->cur != h->cur || e->ccur != h->ccur)
+    return set_err(LM_ERR_ARG, "the engine's buffer indices differ from the graph's (an odd number of eager "
+                               "steps ran since the capture)");
+  HIP_TRY(hipGraphLaunch(h->exec, S(stream)));
+  e->col_step += h->col_steps;
+  return 0;
+}
+
+int lm_graph_destroy(void *graph) {
+  if (!graph) return 0;
+  LmGraph *h = (LmGraph *)graph;
+  (void)hipGraphExecDestroy(h->exec);
+  delete h;
   return 0;
 }
 

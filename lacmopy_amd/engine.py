@@ -60,6 +60,7 @@ class Engine:
             mk(f"seg{b}", (nr * (nc + 1) + 1,), i32)
         mk("dmxi", (n,), f64)
         mk("away", (nr * nc,), i32)
+        mk("col_step_dev", (1,), i32)
         mk("scal", (nr, 10, nc), f64)
         mk("mat", (nr, 7, nc), f64)
         mk("cc", (nr * nc, CC_N), f64)
@@ -98,7 +99,7 @@ class Engine:
         for b in (0, 1):
             for f in ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "cell", "seg"):
                 getattr(e, f)[b] = self.buf[f"{f}{b}"].data_ptr()
-        for f in ("dmxi", "away", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
+        for f in ("dmxi", "away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
                   "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
                   "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off"):
             setattr(e, f, self.buf[f].data_ptr())
@@ -115,6 +116,7 @@ class Engine:
     def set_params(self, solute="AS", no_iter_impl_mass=3, scale_dt_cond=5, no_col_per_adv=2,
                    act_collisions=True, act_relaxation=True, dt=1.0, dt_col=None, g_set=9.80665, seed=0):
         """Step parameters as config.py / init.set_config derive them (init.py:181-206, integration.py:1992)."""
+        self.drop_graph()
         e = self.e
         e.solute = SOLUTES[solute]
         e.no_iter = int(no_iter_impl_mass)
@@ -133,6 +135,7 @@ class Engine:
 
     def set_collision_kernel(self, E_col_grid, R_kernel_low_log=None, bin_factor_R_log=None):
         """Table E_col(R_i,R_j) (Ecol_grid_R) or a float (Ecol_const), as passed to the reference."""
+        self.drop_graph()
         e = self.e
         if isinstance(E_col_grid, float):
             e.e_const_mode, e.E_const, e.nb = 1, float(E_col_grid), 0
@@ -186,6 +189,7 @@ class Engine:
 
     def set_rnd_stream(self, stream_values):
         """Inject the uniform stream (1-D float64, consumed in the reference's order) or None for Philox."""
+        self.drop_graph()
         if stream_values is None:
             self.rnd_stream = None
             self.e.rnd_stream = None
@@ -244,7 +248,51 @@ class Engine:
 
     @col_step.setter
     def col_step(self, step):
-        self.e.col_step = int(step) & 0xFFFFFFFF
+        check(self.L.lm_set_col_step(C.byref(self.e), int(step) & 0xFFFFFFFF, self.stream))
+
+    # ------------------------------------------------------------ graphs
+    def _graph_stream(self):
+        if getattr(self, "_gstream", None) is None:
+            self._gstream = torch.cuda.Stream(device=self.device)
+        return self._gstream
+
+    def graph_capture(self, n_steps=2):
+        """Capture n_steps (even) advection steps into a HIP graph; returns its handle (see graph_launch)."""
+        gs = self._graph_stream()
+        gs.wait_stream(torch.cuda.current_stream(self.device))
+        h = C.c_void_p()
+        check(self.L.lm_graph_capture(C.byref(self.e), int(n_steps), C.c_void_p(gs.cuda_stream), C.byref(h)))
+        return h
+
+    def graph_launch(self, h):
+        gs = self._graph_stream()
+        gs.wait_stream(torch.cuda.current_stream(self.device))
+        check(self.L.lm_graph_launch(C.byref(self.e), h, C.c_void_p(gs.cuda_stream)))
+        torch.cuda.current_stream(self.device).wait_stream(gs)
+
+    def graph_destroy(self, h):
+        check(self.L.lm_graph_destroy(h))
+
+    def adv_steps_graph(self, n):
+        """n advection steps as replays of a cached two-step graph (+ one eager step if n is odd).
+
+        One graph is kept per buffer parity (an odd number of eager steps swaps the SIP buffers); the
+        setters that change what the kernels are bound to (parameters, kernel table, injected stream)
+        drop the cached graphs."""
+        n = int(n)
+        if n >= 2:
+            key = (int(self.e.cur), int(self.e.ccur))
+            graphs = self.__dict__.setdefault("_graphs", {})
+            if key not in graphs:
+                graphs[key] = self.graph_capture(2)
+            for _ in range(n // 2):
+                self.graph_launch(graphs[key])
+        if n % 2:
+            self.adv_steps(1)
+
+    def drop_graph(self):
+        for h in self.__dict__.pop("_graphs", {}).values():
+            self.graph_destroy(h)
 
     def check(self):
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
