@@ -168,6 +168,24 @@ int lm_grid_substep(LmEngine *e, void *stream);
  */
 int lm_update_r_l(LmEngine *e, void *stream);
 
+/*
+ * Derived fields of the current state of every run (evaluation.py:202-509,
+ * generate_field_frame_data_avg, per seed and frame): out[(run*9 + f)*nc + c],
+ * f = r_aero, r_c, r_r (mixing ratios of R_p < 0.5, < 25, >= 25 um),
+ * n_aero, n_c, n_r (numbers per dry-air mass), R_avg, R_2/1, R_eff (radius
+ * moment ratios, the last two of the cloud class).  Sums run over each cell's
+ * active SIPs in ascending id, R_p at the cell temperature.
+ */
+int lm_derived_fields(LmEngine *e, double *out, void *stream);
+
+/*
+ * Ensemble mean and spread over seeds (evaluation.py:443-474): fields is
+ * (n_seeds, n_vals); mean = sum / n (in seed order), std = sqrt(max(0,
+ * sum f^2 - n mean^2) / (n (n-1))) (0 for one seed).
+ */
+int lm_ensemble_stats(const double *fields, int32_t n_seeds, int64_t n_vals, double *mean, double *std,
+                      void *stream);
+
 /* n_steps advection steps (integration.py:1298-1611, integrate_adv_step_np). */
 int lm_adv_steps(LmEngine *e, int32_t n_steps, void *stream);
 

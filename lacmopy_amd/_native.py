@@ -67,6 +67,7 @@ EXPORTED = [
     "lm_prof_enable", "lm_prof_collect", "lm_prof_disable", "lm_particle_substep", "lm_grid_substep",
     "lm_box2d_step", "lm_box_check", "lm_divergence_upwind", "lm_update_r_l",
     "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture", "lm_graph_launch", "lm_graph_destroy",
+    "lm_derived_fields", "lm_ensemble_stats",
 ]
 
 _lib = None
@@ -105,6 +106,8 @@ def lib():
                                 C.c_uint64, C.c_uint32, _p, _p]
     L.lm_box_check.argtypes = [_p]
     L.lm_set_col_step.argtypes = [E, C.c_uint32, _p]
+    L.lm_derived_fields.argtypes = [E, _p, _p]
+    L.lm_ensemble_stats.argtypes = [_p, _i32, _i64, _p, _p, _p]
     L.lm_graph_capture.argtypes = [E, _i32, _p, C.POINTER(_p)]
     L.lm_graph_launch.argtypes = [E, _p, _p]
     L.lm_graph_destroy.argtypes = [_p]
@@ -122,7 +125,7 @@ def lib():
                  "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe", "lm_prof_enable", "lm_prof_collect",
                  "lm_prof_disable", "lm_particle_substep", "lm_grid_substep", "lm_box2d_step", "lm_box_check",
                  "lm_divergence_upwind", "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture",
-                 "lm_graph_launch", "lm_graph_destroy"]:
+                 "lm_graph_launch", "lm_graph_destroy", "lm_derived_fields", "lm_ensemble_stats"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -1148,6 +1148,99 @@ __global__ __launch_bounds__(WAVE) void k_rl(KView v) {
   }
 }
 
+/*
+ * Derived fields of one frame (evaluation.py:363-474) per (run, cell): R_p of every member at the cell
+ * temperature (the solute's law, microphysics.py:179-266), classes by R_p against 0.5 and 25 um
+ * (np.digitize, :319), then, over the members in ascending id, per class sum(xi m_w) and sum(xi), and the
+ * radius moments sum(xi R^n) of all members (n = 0, 1) and of the cloud class (n = 1, 2, 3; R^n by repeated
+ * multiplication).  Eleven lanes run the eleven ordered sums side by side.  out[(run*9 + f)*nc + c], f =
+ * r_aero, r_c, r_r, n_aero, n_c, n_r, R_avg, R_2/1, R_eff.
+ */
+#define DERIV_CAP 1024
+__global__ __launch_bounds__(WAVE) void k_derived(KView v, double *out) {
+  const LmPhys &P = c_P;
+  const int nc = v.nc;
+  const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
+  __shared__ int sSlot[DERIV_CAP], sId[DERIV_CAP], sRank[DERIV_CAP];
+  __shared__ unsigned char sSub[DERIV_CAP], sCls[DERIV_CAP];
+  __shared__ double sMW[DERIV_CAP], sXI[DERIV_CAP], sRp[DERIV_CAP], sSum[11];
+  __shared__ NbrTab tb;
+  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
+  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DERIV_CAP);
+  if (n > DERIV_CAP) {
+    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+    return;
+  }
+  __syncthreads();
+  rank_members(n, sId, sSub, &tb, sRank);
+  __syncthreads();
+  const double *C = v.cc + (size_t)gc * CC_N;
+  for (int m = threadIdx.x; m < n; m += WAVE) {
+    const int k = sSlot[m], p = sRank[m];
+    double R, ws, rho;
+    lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], C[CC_T], C[CC_RHOW], R, ws, rho);
+    sMW[p] = v.m_w[k] * v.xi[k];
+    sXI[p] = v.xi[k];
+    sRp[p] = R;
+    sCls[p] = (unsigned char)(R < 0.5 ? 0 : (R < 25. ? 1 : 2));
+  }
+  __syncthreads();
+  const int q = threadIdx.x;
+  if (q < 11) {
+    double acc = 0.0;
+    for (int p = 0; p < n; ++p) {
+      const int cl = sCls[p];
+      const double x = sXI[p], R = sRp[p];
+      if (q < 3) {
+        if (cl == q) acc += sMW[p];
+      } else if (q < 6) {
+        if (cl == q - 3) acc += x;
+      } else if (q == 6) {
+        acc += x;
+      } else if (q == 7) {
+        acc += x * R;
+      } else if (cl == 1) {
+        acc += q == 8 ? x * R : (q == 9 ? x * (R * R) : x * (R * (R * R)));
+      }
+    }
+    sSum[q] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double mdi = v.scal[((size_t)r * 10 + 8) * nc + c], V0 = v.V_cell;
+    double *o = out + (size_t)r * 9 * nc + c;
+    for (int k = 0; k < 3; ++k) {
+      o[(size_t)k * nc] = sSum[k] * (1.0E-18 * mdi);
+      o[(size_t)(3 + k) * nc] = sSum[3 + k] * mdi;
+    }
+    const double M0 = sSum[6] / V0, M1 = sSum[7] / V0, C1 = sSum[8] / V0, C2 = sSum[9] / V0, C3 = sSum[10] / V0;
+    o[(size_t)6 * nc] = M0 == 0.0 ? 0.0 : M1 / M0;
+    o[(size_t)7 * nc] = C1 == 0.0 ? 0.0 : C2 / C1;
+    o[(size_t)8 * nc] = C2 == 0.0 ? 0.0 : C3 / C2;
+  }
+}
+
+/* evaluation.py:443-474: per value, the mean over seeds (summed in seed order) and the reference's spread */
+__global__ void k_ensemble(const double *f, int n_seeds, long long n_vals, double *mean, double *std) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_vals) return;
+  double s = 0.0, q = 0.0;
+  for (int r = 0; r < n_seeds; ++r) {
+    const double x = f[(size_t)r * n_vals + k];
+    s += x;
+    q += x * x;
+  }
+  const double m = s / (double)n_seeds;
+  mean[k] = m;
+  if (n_seeds <= 1) {
+    std[k] = 0.0;
+  } else {
+    double h = q - (double)n_seeds * m * m;
+    if (h < 0) h = 0.0;
+    std[k] = sqrt(h / (double)((long long)n_seeds * (n_seeds - 1)));
+  }
+}
+
 /* integration.py:402-483 + 359-400 for one cell; refreshes cc */
 __global__ __launch_bounds__(256) void k_grid(KView v) {
   const LmPhys &P = c_P;
@@ -2065,6 +2158,25 @@ int lm_update_r_l(LmEngine *e, void *stream) {
   if (rc) return rc;
   KView v = make_view(e);
   hipLaunchKernelGGL(k_rl, dim3(e->n_runs * e->nc), dim3(WAVE), 0, S(stream), v);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_derived_fields(LmEngine *e, double *out, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (!out) return set_err(LM_ERR_ARG, "output is NULL");
+  KView v = make_view(e);
+  hipLaunchKernelGGL(k_derived, dim3(e->n_runs * e->nc), dim3(WAVE), 0, S(stream), v, out);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_ensemble_stats(const double *fields, int32_t n_seeds, int64_t n_vals, double *mean, double *std,
+                      void *stream) {
+  if (!fields || !mean || !std || n_seeds <= 0 || n_vals <= 0) return set_err(LM_ERR_ARG, "bad ensemble arguments");
+  hipLaunchKernelGGL(k_ensemble, dim3((unsigned)((n_vals + 255) / 256)), dim3(256), 0, S(stream), fields, n_seeds,
+                     (long long)n_vals, mean, std);
   LAUNCH_CHECK();
   return 0;
 }

@@ -576,6 +576,73 @@ EXPORT void or_update_grid_r_l(int64_t N, const double *m_w, const double *xi, c
   for (int64_t c = 0; c < nx * nz; ++c) r_l[c] *= 1.0E-18 * m_dry_inv[c];
 }
 
+/*
+ * evaluation.py:363-474 for one seed and one frame: T_p = T of each SIP's cell (top row for inactive SIPs,
+ * :176-200), R_p at T_p (:317-321), classes by np.digitize(R_p, [0.5, 25]) (:319, :383-385), then per cell,
+ * in ascending id over the active SIPs: the mixing ratio (:93-133) and number per dry mass (:136-175) of
+ * each class, and the radius moments (:1393-1444; R**n as repeated multiplication, R**3 = R*(R*R)).
+ * out[f*nc + c], f = r_aero, r_c, r_r, n_aero, n_c, n_r, R_avg, R_2/1, R_eff (:404-474).
+ */
+EXPORT void or_derived_fields(int64_t N, const double *m_w, const double *m_s, const double *xi,
+                              const int64_t *cells, const uint8_t *active, const double *T_grid,
+                              const double *mass_dry_inv, int64_t nx, int64_t nz, double V0, int solute,
+                              double *out) {
+  or_init_consts();
+  const int64_t nc = nx * nz;
+  double *S = (double *)calloc((size_t)(11 * nc), sizeof(double));
+  double *mass = S, *num = S + 3 * nc, *m0 = S + 6 * nc, *m1 = S + 7 * nc, *c1 = S + 8 * nc, *c2 = S + 9 * nc,
+         *c3 = S + 10 * nc;
+  for (int64_t id = 0; id < N; ++id) {
+    const int64_t cx = cells[id];
+    const int64_t cz = cells[N + id] < 0 ? cells[N + id] + nz : cells[N + id];
+    double R, w, rho;
+    R_p_w_s_rho_p(m_w[id], m_s[id], T_grid[cx * nz + cz], solute, &R, &w, &rho);
+    if (!active[id]) continue;
+    const int cls = R < 0.5 ? 0 : (R < 25. ? 1 : 2);
+    const int64_t c = cx * nz + cz;
+    mass[cls * nc + c] += m_w[id] * xi[id];
+    num[cls * nc + c] += xi[id];
+    m0[c] += xi[id];
+    m1[c] += xi[id] * R;
+    if (cls == 1) {
+      c1[c] += xi[id] * R;
+      c2[c] += xi[id] * (R * R);
+      c3[c] += xi[id] * (R * (R * R));
+    }
+  }
+  for (int64_t c = 0; c < nc; ++c) {
+    for (int k = 0; k < 3; ++k) {
+      out[k * nc + c] = mass[k * nc + c] * (1.0E-18 * mass_dry_inv[c]);
+      out[(3 + k) * nc + c] = num[k * nc + c] * mass_dry_inv[c];
+    }
+    const double M0 = m0[c] / V0, M1 = m1[c] / V0, C1 = c1[c] / V0, C2 = c2[c] / V0, C3 = c3[c] / V0;
+    out[6 * nc + c] = M0 == 0.0 ? 0.0 : M1 / M0;
+    out[7 * nc + c] = C1 == 0.0 ? 0.0 : C2 / C1;
+    out[8 * nc + c] = C2 == 0.0 ? 0.0 : C3 / C2;
+  }
+  free(S);
+}
+
+/* evaluation.py:443-474: mean over seeds (summed in seed order) and sqrt((sum f^2 - n mean^2) / (n (n-1))) */
+EXPORT void or_ensemble_stats(int64_t n_seeds, int64_t n_vals, const double *f, double *mean, double *std) {
+  for (int64_t k = 0; k < n_vals; ++k) {
+    double s = 0.0, q = 0.0;
+    for (int64_t r = 0; r < n_seeds; ++r) {
+      s += f[r * n_vals + k];
+      q += f[r * n_vals + k] * f[r * n_vals + k];
+    }
+    const double m = s / (double)n_seeds;
+    mean[k] = m;
+    if (n_seeds <= 1) {
+      std[k] = 0.0;
+    } else {
+      double h = q - (double)n_seeds * m * m;
+      if (h < 0) h = 0.0;
+      std[k] = sqrt(h / (double)(n_seeds * (n_seeds - 1)));
+    }
+  }
+}
+
 /* all_or_nothing.py:269-390 (box model, E table, 1D velocity) */
 EXPORT void or_collision_ecol_box(int64_t n, double *xis, double *masses, double *radii, const double *vel,
                                   const double *rho, double dt_over_dV, const double *E, int64_t nb, double lnR0,

@@ -60,6 +60,8 @@ def lib(libm=False):
     L.or_update_velocity_Beard.argtypes = [_i64, _p, _p]
     L.or_collision_kernel_grid_m.argtypes = [_i64, _p, _p, _d, _p, _i64, _d, _d, _p, _p, _u64, _u32, _u32]
     L.or_update_grid_r_l.argtypes = [_i64, _p, _p, _p, _p, _i64, _i64, _p, _p]
+    L.or_derived_fields.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _d, C.c_int, _p]
+    L.or_ensemble_stats.argtypes = [_i64, _i64, _p, _p, _p]
     L.or_divergence_upwind.argtypes = [_p, _p, _i64, _i64, _d, _d, C.c_int, C.c_int, _p]
     L.or_relaxation_term.argtypes = [_p, _p, _p, _d, _i64, _i64, _p]
     L.or_propagate_grid_subloop_step.argtypes = [_p, _p, _d, _d, _p, _p, _p, _d, _i64]
@@ -156,3 +158,24 @@ def collision_all_cells(xis, m_w, m_s, vel, grid_T, cells, no_cells, dt_over_dV,
 
 def golovin_step(xis, masses, dt_over_dV, no_cols, rnd=None, seed=0, run=0, step=0):
     lib().or_collision_golovin(len(xis), P(xis), P(masses), dt_over_dV, P(no_cols), P(rnd), seed, run, step)
+
+
+def derived_fields(m_w, m_s, xi, cells, active, T_grid, mass_dry_inv, V0, solute_type="AS"):
+    """evaluation.py:363-474 for one seed and frame: (9, nx, nz) derived fields."""
+    nx, nz = T_grid.shape
+    out = np.zeros((9, nx, nz))
+    lib().or_derived_fields(len(xi), P(np.ascontiguousarray(m_w)), P(np.ascontiguousarray(m_s)),
+                            P(np.ascontiguousarray(xi)), P(np.ascontiguousarray(cells, dtype=np.int64)),
+                            P(np.ascontiguousarray(active, dtype=bool).view(np.uint8)),
+                            P(np.ascontiguousarray(T_grid)), P(np.ascontiguousarray(mass_dry_inv)), nx, nz,
+                            float(V0), 0 if solute_type == "AS" else 1, P(out))
+    return out
+
+
+def ensemble_stats(f):
+    """evaluation.py:443-474 over the first axis (seeds)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    mean = np.zeros(f.shape[1:])
+    std = np.zeros(f.shape[1:])
+    lib().or_ensemble_stats(f.shape[0], int(np.prod(f.shape[1:])), P(f), P(mean), P(std))
+    return mean, std

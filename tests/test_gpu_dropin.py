@@ -363,3 +363,71 @@ def test_update_r_l_after_steps(gpu):
                                  orc.P(ref), orc.P(s[8].copy()))
     _assert_same(s[5], ref, "r_l")
     assert ref.sum() > 0
+
+
+def test_derived_fields_and_ensemble_of_a_batch(gpu):
+    """lm_derived_fields on a 3-run batch after two steps vs the oracle's evaluation.py:363-474 per run,
+    and lm_ensemble_stats vs the oracle's ensemble statistics."""
+    from lacmopy_amd.evaluation import derived_fields, ensemble_stats
+    from test_gpu_parity import _engine
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(nx=8, nz=8, spc=24, seed=33, R_drop=30.0)
+    parts = _prep(grid, parts)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 4, n_runs=3)
+    eng.adv_steps(2)
+    d = derived_fields(eng)
+    ref = []
+    for r in range(3):
+        pos, vel, cells, m_w, m_s, xi, act = eng.export_particles(r)
+        s, _ = eng.export_grid(r)
+        ref.append(orc.derived_fields(m_w, m_s, xi, cells, act, s[0].copy(), s[8].copy(), grid.volume_cell))
+        _assert_same(d[r], ref[r], f"derived fields run {r}")
+    assert (ref[0][2] > 0).any() and (ref[0][1] > 0).any()
+    m, sd = ensemble_stats(d)
+    om, osd = orc.ensemble_stats(np.array(ref))
+    _assert_same(m, om, "ensemble mean")
+    _assert_same(sd, osd, "ensemble spread")
+
+
+def test_generate_field_frame_data_avg_from_files(gpu, tmp_path):
+    """The file-based drop-in over two seeds and two frames vs the oracle applied to the same files."""
+    import os
+    from lacmopy_amd import file_handling as fh
+    from lacmopy_amd.evaluation import generate_field_frame_data_avg
+    from lacmopy_amd.setup_case import make_case
+    paths = []
+    for seed in (3, 4):
+        path = str(tmp_path / f"seed_{seed}") + os.sep
+        os.makedirs(path)
+        grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=6, nz=6, spc=20, seed=seed, cloud_base=30.0,
+                                                             cloud_top=90.0, R_drop=25.0, sigma_drop=1.8)
+        rs = np.random.default_rng(seed)
+        for t in (0, 300):
+            if t:   # a later frame of the same SIPs: grown masses, fewer in number, warmer, some removed
+                m_w = m_w * rs.uniform(0.5, 3.0, m_w.size)
+                xi = xi * rs.uniform(0.8, 1.0, xi.size)
+                grid.temperature = grid.temperature + 1.0
+                act = act.copy()
+                act[::7] = False
+                cells = cells.copy()
+                cells[1, ~act] = -1
+            fh.save_grid_scalar_fields(t, grid.scalar_fields(), path)
+            fh.dump_particle_data_all(t, pos, vel, cells, m_w, m_s, xi, act, path)
+        np.save(path + "grid_save_times.npy", np.array([0, 300]))
+        paths.append(path)
+    mdi = grid.mass_dry_inv
+    got = generate_field_frame_data_avg(paths, np.array([0, 1, 3]), np.array([0, 1]), np.arange(9), mdi,
+                                        grid.volume_cell, grid.no_cells, "AS")
+    per_seed = np.zeros((2, 2, 12, 6, 6))
+    for s_n, path in enumerate(paths):
+        fields = fh.load_grid_scalar_fields(path, [0, 300])
+        vec, cells_t, scal_t, xi_t, act_t = fh.load_particle_data_all(path, [0, 300])
+        for t_n in range(2):
+            per_seed[s_n, t_n, :3] = fields[t_n][[0, 1, 3]]
+            per_seed[s_n, t_n, 3:] = orc.derived_fields(scal_t[t_n, 0], scal_t[t_n, 1], xi_t[t_n], cells_t[t_n],
+                                                        act_t[t_n], fields[t_n, 3].copy(), mdi, grid.volume_cell)
+    om, osd = orc.ensemble_stats(per_seed)
+    _assert_same(got[0], om, "mean")
+    _assert_same(got[1], osd, "spread")
+    assert list(got[2]) == [0, 300] and len(got[3]) == 12

@@ -512,3 +512,53 @@ def test_update_grid_r_l_sums_liquid_water():
     np.testing.assert_allclose((r_l / (1e-18 * mdi)).sum(), (m_w * xi)[act].sum(), rtol=1e-12)
     c0 = (cells[0] == 2) & (cells[1] == 3) & act
     np.testing.assert_allclose(r_l[2, 3], (m_w * xi)[c0].sum() * 1e-18 * mdi[2, 3], rtol=1e-12)
+
+
+def test_derived_fields_conservation_and_special_cells():
+    """evaluation.py:363-474: the three classes partition the active SIPs (mass and number sum to the
+    totals), a lone SIP's moment ratios are its radius, and the R_p class boundaries sit at 0.5 and 25 um."""
+    rng = np.random.default_rng(11)
+    nx, nz, N = 4, 3, 600
+    cells = np.array([rng.integers(0, nx, N), rng.integers(0, nz, N)], dtype=np.int64)
+    cells[:, :3] = [[0, 0, 0], [0, 0, 0]]
+    cells[:, 3:] = np.where(cells[:, 3:] == 0, 1, cells[:, 3:])   # cell (0, 0) holds only ids 0..2
+    m_s = rng.uniform(1e-3, 1.0, N)
+    m_w = np.exp(rng.uniform(np.log(1e-3), np.log(1e9), N))
+    xi = rng.uniform(1e3, 1e6, N)
+    act = rng.random(N) > 0.05
+    act[:3] = [True, False, False]
+    T = rng.uniform(280, 290, (nx, nz))
+    mdi = rng.uniform(1e-3, 2e-3, (nx, nz))
+    out = orc.derived_fields(m_w, m_s, xi, cells, act, T, mdi, 400.0)
+    tot_m = np.zeros((nx, nz))
+    tot_n = np.zeros((nx, nz))
+    np.add.at(tot_m, (cells[0, act], cells[1, act]), m_w[act] * xi[act])
+    np.add.at(tot_n, (cells[0, act], cells[1, act]), xi[act])
+    np.testing.assert_allclose(out[0:3].sum(0), 1e-18 * tot_m * mdi, rtol=1e-12)
+    np.testing.assert_allclose(out[3:6].sum(0), tot_n * mdi, rtol=1e-12)
+    R = np.zeros(N)
+    for k in range(N):
+        rr = np.zeros(1)
+        orc.lib().or_R_p_w_s_rho_p(1, orc.P(m_w[k:k + 1].copy()), orc.P(m_s[k:k + 1].copy()),
+                                   orc.P(np.array([T[cells[0, k], cells[1, k]]])), 0, orc.P(rr), orc.P(np.zeros(1)),
+                                   orc.P(np.zeros(1)))
+        R[k] = rr[0]
+    np.testing.assert_allclose(out[6, 0, 0], R[0], rtol=1e-14)   # the cell's only active SIP
+    cls = np.digitize(R, [0.5, 25.0])
+    for k3 in range(3):
+        sel = act & (cls == k3)
+        ref = np.zeros((nx, nz))
+        np.add.at(ref, (cells[0, sel], cells[1, sel]), xi[sel])
+        np.testing.assert_allclose(out[3 + k3], ref * mdi, rtol=1e-12)
+    assert (cls == 0).any() and (cls == 1).any() and (cls == 2).any()
+
+
+def test_ensemble_stats_two_seed_closed_form():
+    """evaluation.py:443-474 with two seeds a, b: mean (a+b)/2 and spread |a-b|/2; one seed: spread 0."""
+    rng = np.random.default_rng(5)
+    f = rng.normal(0, 1, (2, 50))
+    m, s = orc.ensemble_stats(f)
+    np.testing.assert_allclose(m, f.mean(0), rtol=1e-15)
+    np.testing.assert_allclose(s, np.abs(f[0] - f[1]) / 2, rtol=1e-10, atol=1e-15)
+    m1, s1 = orc.ensemble_stats(f[:1])
+    assert np.array_equal(m1, f[0]) and not s1.any()
