@@ -933,8 +933,8 @@ __device__ static inline bool near_home(const KView &v, int r, long long k, int 
   return (ddx <= 1 || ddx == v.nx - 1) && ddz >= -1 && ddz <= 1;
 }
 
-__global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, double dt_sub,
-                                                   double dt_pos, int pos_only) {
+template <int SOLUTE, bool POS_ONLY>
+__global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, double dt_sub, double dt_pos) {
   const LmPhys &P = c_P;
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_tot) return;
@@ -947,15 +947,15 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
   const int r = gc / nc, c = gc % nc;
   const int i = c / v.nz, j = c % v.nz;
   double x = v.x[k], z = v.z[k], u = v.u[k], w = v.w[k];
-  if (!pos_only) {
+  if (!POS_ONLY) {
     const double *C = v.cc + (size_t)gc * CC_N;
     const double T = C[CC_T];
     double m_w = v.m_w[k];
     const double m_s = v.m_s[k], xi = v.xi[k];
     double R, w_s, rho;
-    lmp_R_ws_rho(P, v.solute, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
-    const double sigma_p = lmp_sigma_solution(P, v.solute, w_s, C[CC_SIGW]);
-    const double dm = lmp_newton_dm(P, v.solute, dt_sub, v.no_iter, m_w, m_s, w_s, R, T, rho, C, sigma_p);
+    lmp_R_ws_rho(P, SOLUTE, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
+    const double sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, C[CC_SIGW]);
+    const double dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, m_w, m_s, w_s, R, T, rho, C, sigma_p);
     m_w += dm;
     v.m_w[k] = m_w;
     v.dmxi[k] = dm * xi;
@@ -2072,7 +2072,9 @@ int lm_position_step(LmEngine *e, double dt, void *stream) {
   if (rc) return rc;
   KView v = make_view(e);
   const long long n = e->n_tot;
-  if (n > 0) hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n, 0.0, dt, 1);
+  if (n > 0)
+    hipLaunchKernelGGL((k_particles<0, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n, 0.0,
+                       dt);
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();
   e->ccur = 1 - e->ccur;
@@ -2125,8 +2127,12 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
   const int ncell = e->n_runs * e->nc;
   if (n > 0) {
     ProfScope ps(e, 0, S(stream));
-    hipLaunchKernelGGL(k_particles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n, dt_sub,
-                       dt_sub_pos, 0);
+    if (e->solute == 0)
+      hipLaunchKernelGGL((k_particles<0, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n,
+                         dt_sub, dt_sub_pos);
+    else
+      hipLaunchKernelGGL((k_particles<1, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n,
+                         dt_sub, dt_sub_pos);
   }
   {
     ProfScope ps(e, 2, S(stream));

@@ -293,6 +293,7 @@ LM_FN double lm_pow_third(double x) {
 
 /* Python / Numba float floor-modulo a % b (fmod, then shift into the sign of b). */
 LM_FN double lm_pymod(double a, double b) {
+  if (a > 0.0 && a < b) return a; /* fmod(a, b) = a, already of b's sign */
   double m = LM_FMOD(a, b);
   if (m != 0.0) {
     if ((b < 0.0) != (m < 0.0)) m += b;
