@@ -33,6 +33,21 @@ extern "C" {
 #define LM_ERR_REM_CAP 1004    /* too many SIPs reached the ground in one substep */
 #define LM_ERR_OUT_OF_DOMAIN 1005 /* a SIP landed on x == x_max (cell index nx) */
 #define LM_ERR_RND_SHORT 1006  /* the injected uniform stream ran out */
+#define LM_ERR_EV_CAP 1007     /* the collision-event log of an adjoint window is full */
+
+/*
+ * One collision event of an adjoint window, recorded by lm_collision_step
+ * (ev_mode 1) and replayed as a fixed event (ev_mode 2): the pair, the branch
+ * of all_or_nothing.py:577-632 it took, p_crit of a multiple collection, and
+ * the state of the pair before the event.
+ */
+typedef struct LmEvent {
+  int32_t gid_min, gid_max; /* global SIP ids of the smaller- and the larger-multiplicity SIP */
+  int32_t cell, seq;        /* global cell, event number within the cell's collision step */
+  int32_t kind, pad;        /* 1 multiple, 2 single, 3 near-equal single collection */
+  double p;                 /* p_crit of a multiple collection (kind 1) */
+  double xi_min, xi_max, mw_min, mw_max, ms_min, ms_max;
+} LmEvent;
 
 typedef struct LmEngine {
   /* geometry */
@@ -94,7 +109,38 @@ typedef struct LmEngine {
   int64_t rnd_len;  /* number of values in rnd_stream */
   /* optional kernel timing (lm_prof_*); NULL = off */
   void *prof;
+  /*
+   * Adjoint window (all NULL/0 outside one).  ev_mode 1: lm_collision_step
+   * records every event into ev; ev_mode 2: it applies the recorded events of
+   * the collision step instead of drawing (fixed events).  The host counters
+   * ev_step, tape_sub and tape_inc advance with each collision step, substep
+   * and increment evaluation.
+   */
+  int32_t ev_mode, ev_cap;
+  int32_t ev_step, ev_nstep;   /* host: collision step within the window (selects ev_off), steps ev_off holds */
+  int32_t tape_sub, tape_nsub; /* host: particle/grid substep within the tape, substeps the tape holds */
+  int32_t tape_inc, tape_ninc; /* host: increment evaluation within the tape, evaluations it holds */
+  LmEvent *ev;              /* ev_cap records */
+  int32_t *ev_cnt;          /* 1: records written so far */
+  int32_t *ev_count;        /* n_runs*nc scratch: events per cell of the current collision step */
+  int32_t *ev_off;          /* per collision step of the window: n_runs*nc+1 offsets into ev_order */
+  int32_t *ev_order;        /* ev_cap: record indices ordered by (collision step, cell, seq) */
+  const int64_t *run_base;  /* n_runs+1: global id of SIP 0 of each run */
+  int32_t *slot_of;         /* n_tot: slot of each global id in the current buffer (ev_mode 2) */
+  /* tape of an adjoint segment (NULL: no tape is written) */
+  double *tape_part;    /* per substep 7*n_tot, slot order: d m_w'/d(m_w, m_s, T, p, S), dm, xi */
+  int32_t *tape_part_i; /* per substep 2*n_tot: global id, cell of the condensation update (-1: none) */
+  double *tape_grid;    /* per substep 30*n_runs*nc: d(Theta', r_v', T', p', S')/d(Theta, r_v, T, dml,
+                           dTheta_adv, dr_v_adv) of the grid substep */
+  double *tape_fields;  /* per increment evaluation 2*n_runs*nc: r_v, Theta */
+  double *ccd;          /* n_runs*nc*12 scratch: d(l_alpha, l_beta, c1, c2, rho_w, sigma_w)/d(T, p) */
 } LmEngine;
+
+/* Adjoint variables of a window (lm_adj_*): per global SIP id and per cell. */
+typedef struct LmAdjoint {
+  double *lam_mw, *lam_ms, *lam_xi; /* n_tot each */
+  double *lam_cell; /* 8*n_runs*nc: Theta, r_v, T, p, S, dml, dTheta_adv, dr_v_adv */
+} LmAdjoint;
 
 /* Library identity: returns a static version string. */
 const char *lm_version(void);
@@ -291,6 +337,53 @@ int lm_box_check(void *stream);
  */
 int lm_divergence_upwind(const double *fields, const double *flux, double *out, int32_t n_fields, int32_t nx,
                          int32_t nz, double dx, double dz, int32_t bc_x, int32_t bc_z, void *stream);
+
+/*
+ * Adjoint of a window of advection steps (the reference has no adjoint; these
+ * entry points differentiate its step, integration.py:1298-1611, as this
+ * library computes it).  Collisions are fixed events: a window is first run
+ * with ev_mode 1 (lm_adv_steps records every event with its pair, branch and
+ * p_crit), then re-run from checkpoints with ev_mode 2 (the recorded events are
+ * applied to the current state instead of drawing) and a tape, and swept
+ * backwards segment by segment with lm_adj_adv_steps.  The gradients are those
+ * of the computed map: the fixed 3-iteration Newton with its clamps, the grid
+ * feedback, advection with the limiter, relaxation and the fixed events.
+ * Positions and velocities carry no sensitivity: with fixed events and cells,
+ * the objectives depend on them only through cell membership (piecewise
+ * constant) and through p_crit of multiple collections, which a fixed event
+ * keeps as recorded.
+ */
+
+/* gid -> slot map of the current buffer (ev_mode 2 builds it after every reindex). */
+int lm_build_slot_of(LmEngine *e, void *stream);
+
+/*
+ * Objective sums per cell, over the active members in ascending id, as
+ * compensated sums (out[(2+q)*n_runs*nc + gc] holds the correction of sum q):
+ * out[gc] = sum xi m_w, out[n_runs*nc + gc] = sum xi I(R) with the smooth
+ * cloud-class indicator I(R) = s((ln R - ln 0.5)/w) s((ln 25 - ln R)/w),
+ * s the logistic function, w = nc_width, R the radius at the cell temperature
+ * (microphysics.py:179-266).  The hard class of evaluation.py:319 has zero
+ * derivative almost everywhere; I(R) is its smooth stand-in.
+ */
+int lm_objective(LmEngine *e, double nc_width, double *out, void *stream);
+
+/*
+ * Seeds the adjoint with d J / d(state) at the end of the window, J = w_lwp
+ * J_lwp + w_nc J_nc, J_lwp = 1e-18 sum xi m_w / (X dy) (domain-mean liquid
+ * water path, kg m^-2), J_nc = 1e-6 sum_cells m_dry_inv sum xi I(R) / (n_runs
+ * nc) (domain-mean cloud droplet number, mg^-1).  Clears every other adjoint
+ * variable.
+ */
+int lm_adj_seed(LmEngine *e, LmAdjoint *a, double w_nc, double w_lwp, double nc_width, void *stream);
+
+/*
+ * Reverse sweep over n_steps advection steps whose tape (substeps, increment
+ * fields) starts at index 0 of e->tape_*, and whose first collision step is
+ * window collision step col_index0 (events in e->ev, e->ev_off).  Moves the
+ * adjoint from the end of the last step to the start of the first.
+ */
+int lm_adj_adv_steps(LmEngine *e, LmAdjoint *a, int32_t n_steps, int32_t col_index0, void *stream);
 
 #ifdef __cplusplus
 }

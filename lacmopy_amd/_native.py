@@ -17,6 +17,7 @@ LM_ERR_NAMES = {
     1004: "LM_ERR_REM_CAP",
     1005: "LM_ERR_OUT_OF_DOMAIN",
     1006: "LM_ERR_RND_SHORT",
+    1007: "LM_ERR_EV_CAP",
 }
 
 _p = C.c_void_p
@@ -53,11 +54,26 @@ class LmEngine(C.Structure):
         ("scan_buf", _p), ("scan_part", _p), ("err", _p), ("ovf_list", _p), ("ovf_cnt", _p),
         ("rnd_stream", _p), ("rnd_off", _p), ("rnd_len", _i64),
         ("prof", _p),
+        ("ev_mode", _i32), ("ev_cap", _i32), ("ev_step", _i32), ("ev_nstep", _i32), ("tape_sub", _i32),
+        ("tape_nsub", _i32), ("tape_inc", _i32), ("tape_ninc", _i32),
+        ("ev", _p), ("ev_cnt", _p), ("ev_count", _p), ("ev_off", _p), ("ev_order", _p), ("run_base", _p),
+        ("slot_of", _p),
+        ("tape_part", _p), ("tape_part_i", _p), ("tape_grid", _p), ("tape_fields", _p), ("ccd", _p),
     ]
 
 
+class LmAdjoint(C.Structure):
+    """Mirror of struct LmAdjoint (include/lacmo_amd.h)."""
+
+    _fields_ = [("lam_mw", _p), ("lam_ms", _p), ("lam_xi", _p), ("lam_cell", _p)]
+
+
+LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
+
+
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
-                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev"]
+                 "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev",
+                 "ev_mode", "ev", "run_base", "tape_part", "ccd"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",
@@ -67,7 +83,8 @@ EXPORTED = [
     "lm_prof_enable", "lm_prof_collect", "lm_prof_disable", "lm_particle_substep", "lm_grid_substep",
     "lm_box2d_step", "lm_box_check", "lm_divergence_upwind", "lm_update_r_l",
     "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture", "lm_graph_launch", "lm_graph_destroy",
-    "lm_derived_fields", "lm_ensemble_stats",
+    "lm_derived_fields", "lm_ensemble_stats", "lm_build_slot_of", "lm_objective", "lm_adj_seed",
+    "lm_adj_adv_steps",
 ]
 
 _lib = None
@@ -114,6 +131,11 @@ def lib():
     L.lm_kernel_grid_m_step.argtypes = [_i32, _p, _p, _p, _d, _p, _i32, _d, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
     L.lm_divergence_upwind.argtypes = [_p, _p, _p, _i32, _i32, _i32, _d, _d, _i32, _i32, _p]
     L.lm_adv_steps.argtypes = [E, _i32, _p]
+    A = C.POINTER(LmAdjoint)
+    L.lm_build_slot_of.argtypes = [E, _p]
+    L.lm_objective.argtypes = [E, _d, _p, _p]
+    L.lm_adj_seed.argtypes = [E, A, _d, _d, _d, _p]
+    L.lm_adj_adv_steps.argtypes = [E, A, _i32, _i32, _p]
     L.lm_golovin_step.argtypes = [_i32, _p, _p, _p, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
     L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
                                    C.c_uint64, C.c_uint32, _p, _i32, _p]
@@ -125,7 +147,8 @@ def lib():
                  "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe", "lm_prof_enable", "lm_prof_collect",
                  "lm_prof_disable", "lm_particle_substep", "lm_grid_substep", "lm_box2d_step", "lm_box_check",
                  "lm_divergence_upwind", "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture",
-                 "lm_graph_launch", "lm_graph_destroy", "lm_derived_fields", "lm_ensemble_stats"]:
+                 "lm_graph_launch", "lm_graph_destroy", "lm_derived_fields", "lm_ensemble_stats",
+                 "lm_build_slot_of", "lm_objective", "lm_adj_seed", "lm_adj_adv_steps"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include "../../include/lacmo_amd.h"
+#include "lm_dual.h"
 #include "lm_philox.h"
 #include "lm_physics.h"
 
@@ -357,6 +358,13 @@ struct KView {
   const double *rnd_stream;
   long long *rnd_off;
   long long rnd_len;
+  /* adjoint window: event log (record / replay) and the tape of the current substep */
+  int ev_mode, ev_cap;
+  LmEvent *ev;
+  int32_t *ev_cnt, *ev_count, *slot_of;
+  const long long *run_base;
+  double *tape_part, *ccd;
+  int32_t *tape_part_i;
 };
 
 static KView make_view(const LmEngine *e) {
@@ -437,6 +445,18 @@ static KView make_view(const LmEngine *e) {
   v.rnd_stream = e->rnd_stream;
   v.rnd_off = (long long *)e->rnd_off;
   v.rnd_len = e->rnd_len;
+  v.ev_mode = e->ev_mode;
+  v.ev_cap = e->ev_cap;
+  v.ev = e->ev;
+  v.ev_cnt = e->ev_cnt;
+  v.ev_count = e->ev_count;
+  v.slot_of = e->slot_of;
+  v.run_base = (const long long *)e->run_base;
+  if (e->tape_part) {
+    v.tape_part = e->tape_part + (size_t)e->tape_sub * 7 * (size_t)e->n_tot;
+    v.tape_part_i = e->tape_part_i + (size_t)e->tape_sub * 2 * (size_t)e->n_tot;
+  }
+  v.ccd = e->ccd;
   return v;
 }
 
@@ -499,7 +519,50 @@ struct CollCell {
   double *sX, *sR, *sU, *sW;
   int *sIND, *sSL;
   int n;
+  int gc;   /* global cell */
+  int *seq; /* LDS: events of this cell so far (event log) */
 };
+
+/* global SIP id of slot k: runs occupy the slot ranges [run_base[r], run_base[r+1]) */
+__device__ static inline int slot_run(const long long *run_base, int n_runs, long long k) {
+  int lo = 0, hi = n_runs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (run_base[mid] <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* appends one event to the log of an adjoint window (ev_mode 1) */
+__device__ static void record_event(const KView &v, const CollCell &cc, int kmin, int kmax, int kind, double p,
+                                   double xi_min, double xi_max, double mw_min, double mw_max, double ms_min,
+                                   double ms_max) {
+  const int seq = atomicAdd(cc.seq, 1);
+  const int e = atomicAdd(v.ev_cnt, 1);
+  if (e >= v.ev_cap) {
+    raise_err(v.err, LM_ERR_EV_CAP, cc.gc);
+    return;
+  }
+  const int r = cc.gc / v.nc;
+  LmEvent E;
+  E.gid_min = (int)(v.run_base[r] + v.id[kmin]);
+  E.gid_max = (int)(v.run_base[r] + v.id[kmax]);
+  E.cell = cc.gc;
+  E.seq = seq;
+  E.kind = kind;
+  E.pad = 0;
+  E.p = p;
+  E.xi_min = xi_min;
+  E.xi_max = xi_max;
+  E.mw_min = mw_min;
+  E.mw_max = mw_max;
+  E.ms_min = ms_min;
+  E.ms_max = ms_max;
+  v.ev[e] = E;
+}
 
 /* injected uniform k; past the end of the stream: error, and a value no pair fires on */
 __device__ static inline double rnd_at(const KView &v, long long k) {
@@ -538,6 +601,10 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
   const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
   const bool table = !v.e_const_mode;
+  if (v.ev_mode == 1) {
+    const int kind = p_crit > 1.0 ? 1 : ((xi_max - xi_min) / xi_max < 1.0E-5 ? 3 : 2);
+    record_event(v, cc, kmin, kmax, kind, p_crit, xi_min, xi_max, mw_min, mw_max, ms_min, ms_max);
+  }
   if (p_crit > 1.0) {
     const double xi_col = p_crit * xi_min;
     const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
@@ -618,6 +685,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
   __shared__ int sIND[CAP], sSL[CAP], sQ[CAP + 1];
   __shared__ NbrTab tb;
+  __shared__ int sSeq;
   int *sSlot = (int *)sU;
   int *sId = sSlot + CAP;
   int *sRank = (int *)sW;
@@ -637,6 +705,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     return;
   }
   if (n < 2) return;
+  if (lane == 0) sSeq = 0;
   __syncthreads();
   rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
@@ -666,7 +735,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   }
   __syncthreads();
 
-  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n};
+  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n, gc, &sSeq};
   const bool use_tab = TAB;
   const long long roff = INJ ? v.rnd_off[gc] : 0;
   const double dtdV = v.dt_col_over_dV, pi = P.pi;
@@ -854,6 +923,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     c_eq += __shfl_down(c_eq, off);
   }
   if (lane == 0) {
+    if (v.ev_mode == 1) v.ev_count[gc] = sSeq;
     if (c_single) atomicAdd((unsigned long long *)&v.no_cols[2 * r], (unsigned long long)c_single);
     if (c_multi) atomicAdd((unsigned long long *)&v.no_cols[2 * r + 1], (unsigned long long)c_multi);
     if (c_eq) atomicAdd((unsigned long long *)&v.n_equal_xi[r], (unsigned long long)c_eq);
@@ -933,7 +1003,47 @@ __device__ static inline bool near_home(const KView &v, int r, long long k, int 
   return (ddx <= 1 || ddx == v.nx - 1) && ddz >= -1 && ddz <= 1;
 }
 
-template <int SOLUTE, bool POS_ONLY>
+/*
+ * Condensation update with its local Jacobian (adjoint tape): the Newton of
+ * lmp_newton_dm on LmDual<5> values seeded in (m_w, m_s, T, p, S); the cell
+ * constants carry their derivatives in T and p from ccd.  The value part is
+ * the double computation itself, so dm is the step's dm bit for bit.
+ */
+template <int SOLUTE>
+__device__ static double newton_dm_tape(const LmPhys &P, const KView &v, long long k, long long n_tot, int gc,
+                                        double dt_sub, double m_w, double m_s, double xi, const double *C) {
+  using D5 = LmDual<5>;
+  const double *G = v.ccd + (size_t)gc * 12;
+  D5 cx[CC_N];
+  cx[CC_T] = D5::seed(C[CC_T], 2);
+  cx[CC_S] = D5::seed(C[CC_S], 4);
+  const int ent[6] = {CC_LALPHA, CC_LBETA, CC_C1, CC_C2, CC_RHOW, CC_SIGW};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    D5 x(C[ent[q]]);
+    x.d[2] = G[2 * q];
+    x.d[3] = G[2 * q + 1];
+    cx[ent[q]] = x;
+  }
+  const D5 mw = D5::seed(m_w, 0), ms = D5::seed(m_s, 1);
+  D5 R, w_s, rho;
+  lmp_R_ws_rho(P, SOLUTE, mw, ms, cx[CC_T], cx[CC_RHOW], R, w_s, rho);
+  const D5 sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, cx[CC_SIGW]);
+  const D5 dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, mw, ms, w_s, R, cx[CC_T], rho, cx, sigma_p);
+  const size_t n = (size_t)n_tot;
+  double *t = v.tape_part + k;
+  t[0] = 1.0 + dm.d[0];
+#pragma unroll
+  for (int q = 1; q < 5; ++q) t[q * n] = dm.d[q];
+  t[5 * n] = dm.v;
+  t[6 * n] = xi;
+  const int r = gc / v.nc;
+  v.tape_part_i[k] = (int)(v.run_base[r] + v.id[k]);
+  v.tape_part_i[n + k] = gc;
+  return dm.v;
+}
+
+template <int SOLUTE, bool POS_ONLY, bool TAPE>
 __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, double dt_sub, double dt_pos) {
   const LmPhys &P = c_P;
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -941,6 +1051,7 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
   const int gc = v.cell[k];
   if (gc < 0) {
     v.cell_next[k] = gc;
+    if (TAPE) v.tape_part_i[n_tot + k] = -1;
     return;
   }
   const int nc = v.nc;
@@ -953,9 +1064,14 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
     double m_w = v.m_w[k];
     const double m_s = v.m_s[k], xi = v.xi[k];
     double R, w_s, rho;
-    lmp_R_ws_rho(P, SOLUTE, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
-    const double sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, C[CC_SIGW]);
-    const double dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, m_w, m_s, w_s, R, T, rho, C, sigma_p);
+    double dm;
+    if (TAPE) {
+      dm = newton_dm_tape<SOLUTE>(P, v, k, n_tot, gc, dt_sub, m_w, m_s, xi, C);
+    } else {
+      lmp_R_ws_rho(P, SOLUTE, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
+      const double sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, C[CC_SIGW]);
+      dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, m_w, m_s, w_s, R, T, rho, C, sigma_p);
+    }
     m_w += dm;
     v.m_w[k] = m_w;
     v.dmxi[k] = dm * xi;
@@ -1298,21 +1414,25 @@ __global__ void k_colsum(KView v, double dt_sub, int act_relax) {
   }
 }
 
-__device__ static inline double lim_fn(double r, double one_third) { /* integration.py:80-102 */
-  const double Kv = (1.0 + 2.0 * r) * one_third;
-  return lm_np_max(0.0, lm_np_fmin(2.0 * r, lm_np_fmin(Kv, 2.0)));
+/* templates over the value type (double for the step, LmDual<4> for the adjoint's face derivatives) */
+template <class R>
+__device__ static inline R lim_fn(R r, double one_third) { /* integration.py:80-102 */
+  const R Kv = (1.0 + 2.0 * r) * one_third;
+  return lm_np_max(R(0.0), lm_np_fmin(2.0 * r, lm_np_fmin(Kv, R(2.0))));
 }
-__device__ static inline double lim_up(double a0, double a1, double a2, double one_third) { /* :113-148 */
-  double r;
-  if (fabs(a1 - a2) > 1.0E-8 * fabs(a0 - a1))
+template <class R>
+__device__ static inline R lim_up(R a0, R a1, R a2, double one_third) { /* :113-148 */
+  R r;
+  if (fabs(lm_val(a1 - a2)) > 1.0E-8 * fabs(lm_val(a0 - a1)))
     r = (a0 - a1) / (a1 - a2);
   else
-    r = (a0 - a1) * 1.0E8 * lm_np_sign(a1 - a2);
+    r = (a0 - a1) * 1.0E8 * lm_np_sign(lm_val(a1 - a2));
   return lim_fn(r, one_third);
 }
-__device__ static inline double face_flux(double a0, double a1, double a2, double a3, double u, double ot) {
-  const double f_pos = a1 + 0.5 * lim_up(a0, a1, a2, ot) * (a1 - a2);
-  const double f_neg = a0 + 0.5 * lim_up(a1, a0, a3, ot) * (a0 - a3);
+template <class R>
+__device__ static inline R face_flux(R a0, R a1, R a2, R a3, double u, double ot) {
+  const R f_pos = a1 + 0.5 * lim_up(a0, a1, a2, ot) * (a1 - a2);
+  const R f_neg = a0 + 0.5 * lim_up(a1, a0, a3, ot) * (a0 - a3);
   return u >= 0.0 ? u * f_pos : u * f_neg;
 }
 
@@ -1968,6 +2088,13 @@ static int check_engine(const LmEngine *e) {
     return set_err(LM_ERR_ARG, "engine buffer pointer is NULL");
   if ((int64_t)e->n_runs * e->nc > 0x7fffffff / 2) return set_err(LM_ERR_ARG, "too many cells");
   if (e->n_tot > 0x7ffffff0) return set_err(LM_ERR_ARG, "too many SIPs for 32-bit slots");
+  if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
+  if (e->ev_mode && (!e->ev || !e->ev_off || !e->ev_order || !e->run_base || e->ev_cap <= 0))
+    return set_err(LM_ERR_ARG, "event log buffers missing");
+  if (e->ev_mode == 1 && (!e->ev_cnt || !e->ev_count)) return set_err(LM_ERR_ARG, "event counters missing");
+  if (e->ev_mode == 2 && !e->slot_of) return set_err(LM_ERR_ARG, "slot_of buffer missing");
+  if (e->tape_part && (!e->tape_part_i || !e->tape_grid || !e->tape_fields || !e->ccd || !e->run_base))
+    return set_err(LM_ERR_ARG, "tape buffers missing");
   return 0;
 }
 
@@ -1978,6 +2105,8 @@ static int cell_capacity(const LmEngine *e) {
   const double mean = (double)e->n_tot / ((double)e->n_runs * (double)e->nc);
   return (int)(mean + 2.5 * sqrt(mean)) + 1;
 }
+
+#include "lm_adjoint.h"
 
 template <int CAP>
 static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int from_ovf) {
@@ -2010,7 +2139,9 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, no_cols),   (int64_t)offsetof(LmEngine, rem_cap),
       (int64_t)offsetof(LmEngine, scan_buf),  (int64_t)offsetof(LmEngine, rnd_stream),
       (int64_t)offsetof(LmEngine, rnd_len),     (int64_t)offsetof(LmEngine, away),
-      (int64_t)offsetof(LmEngine, col_step_dev),
+      (int64_t)offsetof(LmEngine, col_step_dev), (int64_t)offsetof(LmEngine, ev_mode),
+      (int64_t)offsetof(LmEngine, ev),        (int64_t)offsetof(LmEngine, run_base),
+      (int64_t)offsetof(LmEngine, tape_part), (int64_t)offsetof(LmEngine, ccd),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -2036,8 +2167,23 @@ int lm_engine_init_cells(LmEngine *e, void *stream) {
 int lm_collision_step(LmEngine *e, void *stream) {
   int rc = check_engine(e);
   if (rc) return rc;
-  if (!e->e_const_mode && (!e->E || e->nb <= 0)) return set_err(LM_ERR_ARG, "collision table missing");
   const int ncell = e->n_runs * e->nc;
+  if (e->ev_mode && (e->ev_step < 0 || e->ev_step >= e->ev_nstep))
+    return set_err(LM_ERR_ARG, "collision step outside the adjoint window's event log");
+  if (e->ev_mode == 2) {
+    /* fixed events: the recorded events of this collision step, no draws */
+    const int32_t *off = e->ev_off + (size_t)e->ev_step * (ncell + 1);
+    hipLaunchKernelGGL(k_ev_replay, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), make_view(e), off, e->ev_order,
+                       ncell);
+    hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, S(stream), e->col_step_dev, 1u);
+    LAUNCH_CHECK();
+    e->col_step += 1;
+    e->ev_step += 1;
+    return 0;
+  }
+  if (!e->e_const_mode && (!e->E || e->nb <= 0)) return set_err(LM_ERR_ARG, "collision table missing");
+  if (e->ev_mode == 1)
+    hipLaunchKernelGGL(k_fill_i32, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), e->ev_count, 0, ncell);
   if (e->rnd_stream) {
     if (!e->rnd_off) return set_err(LM_ERR_ARG, "rnd_off scratch missing");
     KView v = make_view(e);
@@ -2061,6 +2207,14 @@ int lm_collision_step(LmEngine *e, void *stream) {
   }
   launch_collision<1024>(v, e, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), 1);
   LAUNCH_CHECK();
+  if (e->ev_mode == 1) {
+    /* the step's records in (cell, seq) order for the replay and the reverse sweep */
+    int32_t *off = e->ev_off + (size_t)e->ev_step * (ncell + 1);
+    hipLaunchKernelGGL(k_ev_scan, dim3(1), dim3(EV_SCAN_BLOCK), 0, S(stream), e->ev_count, ncell, e->ev_cnt, off);
+    hipLaunchKernelGGL(k_ev_order, dim3(256), dim3(256), 0, S(stream), e->ev, e->ev_cap, off, ncell, e->ev_order);
+    LAUNCH_CHECK();
+    e->ev_step += 1;
+  }
   hipLaunchKernelGGL(k_add_u32, dim3(1), dim3(64), 0, S(stream), e->col_step_dev, 1u);
   LAUNCH_CHECK();
   e->col_step += 1;
@@ -2073,8 +2227,8 @@ int lm_position_step(LmEngine *e, double dt, void *stream) {
   KView v = make_view(e);
   const long long n = e->n_tot;
   if (n > 0)
-    hipLaunchKernelGGL((k_particles<0, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n, 0.0,
-                       dt);
+    hipLaunchKernelGGL((k_particles<0, true, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n,
+                       0.0, dt);
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();
   e->ccur = 1 - e->ccur;
@@ -2101,6 +2255,7 @@ int lm_reindex(LmEngine *e, void *stream) {
   LAUNCH_CHECK();
   e->cur = 1 - e->cur;
   e->ccur = 1 - e->ccur;
+  if (e->ev_mode == 2) return lm_build_slot_of(e, stream);
   return 0;
 }
 
@@ -2109,6 +2264,12 @@ int lm_advection_increments(LmEngine *e, int32_t second, void *stream) {
   if (rc) return rc;
   KView v = make_view(e);
   const int ncell = e->n_runs * e->nc;
+  if (e->tape_fields) {
+    if (e->tape_inc < 0 || e->tape_inc >= e->tape_ninc) return set_err(LM_ERR_ARG, "tape full (increments)");
+    hipLaunchKernelGGL(k_tape_fields, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v,
+                       e->tape_fields + (size_t)e->tape_inc * 2 * ncell);
+    e->tape_inc += 1;
+  }
   if (e->act_relaxation) {
     const int nt = e->n_runs * e->nz;
     hipLaunchKernelGGL(k_colsum, dim3((nt + 127) / 128), dim3(128), 0, S(stream), v, e->dt_sub, 1);
@@ -2122,17 +2283,25 @@ int lm_advection_increments(LmEngine *e, int32_t second, void *stream) {
 int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *stream) {
   int rc = check_engine(e);
   if (rc) return rc;
+  if (e->tape_part && (e->tape_sub < 0 || e->tape_sub >= e->tape_nsub))
+    return set_err(LM_ERR_ARG, "tape full (substeps)");
   KView v = make_view(e);
   const long long n = e->n_tot;
   const int ncell = e->n_runs * e->nc;
   if (n > 0) {
     ProfScope ps(e, 0, S(stream));
-    if (e->solute == 0)
-      hipLaunchKernelGGL((k_particles<0, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n,
-                         dt_sub, dt_sub_pos);
-    else
-      hipLaunchKernelGGL((k_particles<1, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n,
-                         dt_sub, dt_sub_pos);
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (e->tape_part) {
+      hipLaunchKernelGGL(k_ccd, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
+      if (e->solute == 0)
+        hipLaunchKernelGGL((k_particles<0, false, true>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+      else
+        hipLaunchKernelGGL((k_particles<1, false, true>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+    } else if (e->solute == 0) {
+      hipLaunchKernelGGL((k_particles<0, false, false>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+    } else {
+      hipLaunchKernelGGL((k_particles<1, false, false>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+    }
   }
   {
     ProfScope ps(e, 2, S(stream));
@@ -2154,8 +2323,14 @@ int lm_grid_substep(LmEngine *e, void *stream) {
   if (rc) return rc;
   KView v = make_view(e);
   const int ncell = e->n_runs * e->nc;
+  if (e->tape_grid && (e->tape_sub < 0 || e->tape_sub >= e->tape_nsub))
+    return set_err(LM_ERR_ARG, "tape full (substeps)");
+  if (e->tape_grid)
+    hipLaunchKernelGGL(k_grid_jac, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v,
+                       e->tape_grid + (size_t)e->tape_sub * 30 * ncell);
   hipLaunchKernelGGL(k_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
   LAUNCH_CHECK();
+  if (e->tape_part || e->tape_grid) e->tape_sub += 1;
   return 0;
 }
 
@@ -2358,6 +2533,7 @@ int lm_engine_check(LmEngine *e, void *stream) {
                        : h[0] == LM_ERR_REM_CAP  ? "too many SIPs removed in one substep"
                        : h[0] == LM_ERR_OUT_OF_DOMAIN ? "SIP position outside the grid"
                        : h[0] == LM_ERR_RND_SHORT ? "injected uniform stream shorter than the pairs it must cover"
+                       : h[0] == LM_ERR_EV_CAP    ? "collision-event log of the adjoint window is full"
                                                        : "device error";
     snprintf(g_errmsg, sizeof(g_errmsg), "%s (code %d, detail %d)", what, h[0], h[1]);
     return h[0];
@@ -2465,6 +2641,133 @@ int lm_divergence_upwind(const double *fields, const double *flux, double *out, 
   const long long n = (long long)n_fields * nx * nz;
   hipLaunchKernelGGL(k_divergence, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), fields, flux, out,
                      n_fields, nx, nz, dx, dz, bc_x, bc_z);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* adjoint window                                                           */
+/* ------------------------------------------------------------------------ */
+int lm_build_slot_of(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (!e->slot_of || !e->run_base) return set_err(LM_ERR_ARG, "slot_of / run_base missing");
+  const long long n = e->n_tot;
+  if (n > 0)
+    hipLaunchKernelGGL(k_slot_of, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), e->id[e->cur],
+                       (const long long *)e->run_base, e->n_runs, n, e->slot_of);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_objective(LmEngine *e, double nc_width, double *out, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (!out || !(nc_width > 0.0)) return set_err(LM_ERR_ARG, "objective: out NULL or width <= 0");
+  hipLaunchKernelGGL(k_objective, dim3(e->n_runs * e->nc), dim3(WAVE), 0, S(stream), make_view(e), nc_width, out);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+static int check_adjoint(const LmEngine *e, const LmAdjoint *a) {
+  if (!a || !a->lam_mw || !a->lam_ms || !a->lam_xi || !a->lam_cell)
+    return set_err(LM_ERR_ARG, "adjoint buffers missing");
+  if (!e->run_base) return set_err(LM_ERR_ARG, "run_base missing");
+  return 0;
+}
+
+int lm_adj_seed(LmEngine *e, LmAdjoint *a, double w_nc, double w_lwp, double nc_width, void *stream) {
+  int rc = check_engine(e);
+  if (rc || (rc = check_adjoint(e, a))) return rc;
+  if (!(nc_width > 0.0)) return set_err(LM_ERR_ARG, "nc_width must be positive");
+  const long long n = e->n_tot;
+  const int ncell = e->n_runs * e->nc;
+  HIP_TRY(hipMemsetAsync(a->lam_mw, 0, sizeof(double) * (size_t)n, S(stream)));
+  HIP_TRY(hipMemsetAsync(a->lam_ms, 0, sizeof(double) * (size_t)n, S(stream)));
+  HIP_TRY(hipMemsetAsync(a->lam_xi, 0, sizeof(double) * (size_t)n, S(stream)));
+  HIP_TRY(hipMemsetAsync(a->lam_cell, 0, sizeof(double) * (size_t)LA_N * ncell, S(stream)));
+  if (n > 0)
+    hipLaunchKernelGGL(k_adj_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), make_view(e), w_nc,
+                       w_lwp, nc_width, *a);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+/* the operations of one advection step in the order lm_adv_steps runs them */
+enum { OP_INC1 = 0, OP_INC2, OP_COL, OP_SUB };
+static int step_ops(const LmEngine *e, int *ops, int cap) {
+  int n = 0;
+  auto push = [&](int op) {
+    if (n < cap) ops[n] = op;
+    ++n;
+  };
+  auto subloop = [&](int no_cond, int no_col) {
+    if (e->act_collisions && no_col == 1) push(OP_COL);
+    for (int q = 0; q < no_cond; ++q) {
+      if (e->act_collisions && no_col > 1) push(OP_COL);
+      push(OP_SUB);
+    }
+    if (e->act_collisions && no_col > no_cond) push(OP_COL);
+  };
+  push(OP_INC1);
+  subloop(e->scale_dt_cond, e->no_col_per_adv == 1 ? 0 : (e->no_col_per_adv == 2 ? 1 : e->scale_dt_cond));
+  push(OP_INC2);
+  subloop(e->scale_dt_cond - 1, (e->no_col_per_adv == 1 || e->no_col_per_adv == 2) ? 1 : e->scale_dt_cond);
+  push(OP_SUB);
+  return n;
+}
+
+int lm_adj_adv_steps(LmEngine *e, LmAdjoint *a, int32_t n_steps, int32_t col_index0, void *stream) {
+  int rc = check_engine(e);
+  if (rc || (rc = check_adjoint(e, a))) return rc;
+  if (!e->tape_part || !e->tape_part_i || !e->tape_grid || !e->tape_fields)
+    return set_err(LM_ERR_ARG, "reverse sweep needs the segment's tape");
+  if (e->act_collisions && (!e->ev || !e->ev_off || !e->ev_order))
+    return set_err(LM_ERR_ARG, "reverse sweep needs the window's event log");
+  int ops[256];
+  const int nops = step_ops(e, ops, 256);
+  if (nops > 256) return set_err(LM_ERR_ARG, "too many operations per advection step");
+  int nsub = 0, ncol = 0;
+  for (int q = 0; q < nops; ++q) {
+    nsub += ops[q] == OP_SUB;
+    ncol += ops[q] == OP_COL;
+  }
+  if (n_steps < 0 || (long long)n_steps * nsub > e->tape_nsub || 2LL * n_steps > e->tape_ninc)
+    return set_err(LM_ERR_ARG, "reverse sweep longer than the tape");
+  if (ncol && (col_index0 < 0 || col_index0 + (long long)n_steps * ncol > e->ev_nstep))
+    return set_err(LM_ERR_ARG, "reverse sweep outside the window's event log");
+  const long long n = e->n_tot;
+  const int ncell = e->n_runs * e->nc;
+  const KView v = make_view(e);
+  const int nfaces = e->n_runs * ((e->nx + 1) * e->nz + e->nx * (e->nz - 1));
+  for (int s = n_steps - 1; s >= 0; --s) {
+    int sub = (s + 1) * nsub, col = col_index0 + (s + 1) * ncol, inc = 2 * (s + 1);
+    for (int q = nops - 1; q >= 0; --q) {
+      const int op = ops[q];
+      if (op == OP_SUB) {
+        --sub;
+        hipLaunchKernelGGL(k_adj_grid, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), ncell,
+                           e->tape_grid + (size_t)sub * 30 * ncell, a->lam_cell);
+        if (n > 0)
+          hipLaunchKernelGGL(k_adj_part, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), n, ncell,
+                             e->tape_part + (size_t)sub * 7 * (size_t)n, e->tape_part_i + (size_t)sub * 2 * (size_t)n,
+                             *a);
+      } else if (op == OP_COL) {
+        --col;
+        hipLaunchKernelGGL(k_adj_events, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), e->ev,
+                           e->ev_off + (size_t)col * (ncell + 1), e->ev_order, ncell, *a);
+      } else {
+        --inc;
+        const int second = op == OP_INC2;
+        const double mult = second ? 2.0 : 1.0;
+        hipLaunchKernelGGL(k_adj_inc_faces, dim3((nfaces + 255) / 256), dim3(256), 0, S(stream), v,
+                           e->tape_fields + (size_t)inc * 2 * ncell, mult, e->dt_sub, a->lam_cell);
+        const int nt = e->n_runs * e->nz;
+        hipLaunchKernelGGL(k_adj_inc_rows, dim3((nt + 127) / 128), dim3(128), 0, S(stream), v, mult, e->dt_sub,
+                           e->act_relaxation, second, a->lam_cell);
+      }
+    }
+  }
   LAUNCH_CHECK();
   return 0;
 }

@@ -35,58 +35,64 @@ enum {
   CC_N = 16
 };
 
+#if defined(__HIPCC__)
 #define LMD __device__ static inline
+#else
+#define LMD static inline
+#endif
+/* templates over the value type: double for the step, LmDual<N> (lm_dual.h) for its local Jacobians */
+#define LMDT template <class R> LMD
 
-LMD double lmp_poly(const double *par, int n, double x) { /* algebra.py:11-34 */
-  double res = par[0] * x + par[1];
+LMDT R lmp_poly(const double *par, int n, R x) { /* algebra.py:11-34 */
+  R res = par[0] * x + par[1];
   for (int k = 2; k < n; ++k) res = res * x + par[k];
   return res;
 }
 
 /* material_properties.py:31-50 */
-LMD double lmp_density_water(const LmPhys &P, double T) {
-  const double d = T - P.par_water_dens[2];
+LMDT R lmp_density_water(const LmPhys &P, R T) {
+  const R d = T - P.par_water_dens[2];
   return P.par_water_dens[0] + P.par_water_dens[1] * (d * d);
 }
 /* material_properties.py:60-84 */
-LMD double lmp_density_NaCl(const LmPhys &P, double w, double T) {
+LMDT R lmp_density_NaCl(const LmPhys &P, R w, R T) {
   const double *p = P.par_sol_dens_NaCl;
   return p[0] + p[1] * w + p[2] * T + p[3] * w * w + p[4] * w * T + p[5] * T * T;
 }
 /* material_properties.py:224-244 */
-LMD double lmp_sigma_water(double T) {
-  const double tau = 1 - T / 647.096;
+LMDT R lmp_sigma_water(R T) {
+  const R tau = 1 - T / 647.096;
   return 0.2358 * lm_pow(tau, 1.256) * (1 - 0.625 * tau);
 }
 /* microphysics.py:83-100 (the exponent is one_third = RN(1/3), lm_pow_third's) */
-LMD double lmp_radius(const LmPhys &P, double m, double rho) {
+LMDT R lmp_radius(const LmPhys &P, R m, R rho) {
   return lm_pow_third(P.four_pi_over_three_inv * m / rho);
 }
 /* microphysics.py:989-1006 */
-LMD double lmp_l_alpha(const LmPhys &P, double T, double p, double K) {
+LMDT R lmp_l_alpha(const LmPhys &P, R T, R p, R K) {
   return (P.c_alpha_1 + P.c_alpha_2 * (T - P.T_alpha_0)) * K / p;
 }
 /* microphysics.py:1026-1051 */
-LMD double lmp_l_beta(const LmPhys &P, double T, double D_v) {
+LMDT R lmp_l_beta(const LmPhys &P, R T, R D_v) {
   return P.c_beta_1 * (1.0 + P.c_beta_2 * (T - P.T_alpha_0)) * D_v;
 }
 
 /* Solution density at the cell temperature: AS (material_properties.py:89-115) or NaCl. */
-LMD double lmp_density_solution(const LmPhys &P, int solute, double w, double T, double rho_w_T) {
+LMDT R lmp_density_solution(const LmPhys &P, int solute, R w, R T, R rho_w_T) {
   return solute == 0 ? rho_w_T * lmp_poly(P.par_rho_AS, 4, w) : lmp_density_NaCl(P, w, T);
 }
 
 /* microphysics.py:179-234 */
-LMD void lmp_R_ws_rho(const LmPhys &P, int solute, double m_w, double m_s, double T, double rho_w_T, double &R,
-                      double &w_s, double &rho) {
-  const double m_p = m_w + m_s;
+template <class V>
+LMD void lmp_R_ws_rho(const LmPhys &P, int solute, V m_w, V m_s, V T, V rho_w_T, V &R, V &w_s, V &rho) {
+  const V m_p = m_w + m_s;
   w_s = m_s / m_p;
   rho = lmp_density_solution(P, solute, w_s, T, rho_w_T);
   R = lmp_radius(P, m_p, rho);
 }
 
 /* Surface tension of the solution: material_properties.py:274-292 (NaCl), 346-366 (AS). */
-LMD double lmp_sigma_solution(const LmPhys &P, int solute, double w, double sigma_w_T) {
+LMDT R lmp_sigma_solution(const LmPhys &P, int solute, R w, R sigma_w_T) {
   return sigma_w_T * (1.0 + (solute == 0 ? P.par_sigma_AS : P.par_sigma_NaCl) * w / (1. - w));
 }
 
@@ -94,36 +100,34 @@ LMD double lmp_sigma_solution(const LmPhys &P, int solute, double w, double sigm
  * Condensation mass rate gamma and d gamma / d m_w (Fukuta-Walter):
  * microphysics.py:1359-1469 (AS), 1237-1353 (NaCl).
  */
-LMD void lmp_mass_rate(const LmPhys &P, int solute, double m_w, double m_s, double w_s, double R_p, double T_p,
-                       double rho_p, double S_amb, const double *cc, double sigma_p, double &gamma,
-                       double &dgamma) {
-  const double l_alpha_plus_R_p = 1.0E-6 * (R_p + cc[CC_LALPHA]);
-  const double l_beta_plus_R_p = 1.0E-6 * (R_p + cc[CC_LBETA]);
-  const double m_p_inv_SI = 1.0E18 / (m_w + m_s);
-  double drho_dm_over_rho;
+LMDT void lmp_mass_rate(const LmPhys &P, int solute, R m_w, R m_s, R w_s, R R_p, R T_p, R rho_p, R S_amb,
+                        const R *cc, R sigma_p, R &gamma, R &dgamma) {
+  const R l_alpha_plus_R_p = 1.0E-6 * (R_p + cc[CC_LALPHA]);
+  const R l_beta_plus_R_p = 1.0E-6 * (R_p + cc[CC_LBETA]);
+  const R m_p_inv_SI = 1.0E18 / (m_w + m_s);
+  R drho_dm_over_rho;
   if (solute == 0) {
     drho_dm_over_rho = -cc[CC_RHOW] * m_p_inv_SI / rho_p * w_s * lmp_poly(P.par_rho_deriv_AS, 3, w_s);
   } else {
     const double *p = P.par_sol_dens_NaCl;
     drho_dm_over_rho = -w_s * m_p_inv_SI / rho_p * (p[1] + 2.0 * p[3] * w_s + p[4] * T_p);
   }
-  const double dR_p_dm_over_R_p = P.one_third * (m_p_inv_SI - drho_dm_over_rho);
-  const double dR_p_dm = 1.0E-6 * dR_p_dm_over_R_p * R_p;
-  const double eps_k = 2.0E6 * sigma_p / (P.R_v * T_p * rho_p * R_p);
-  const double kelvin_term = lm_exp(eps_k);
+  const R dR_p_dm_over_R_p = P.one_third * (m_p_inv_SI - drho_dm_over_rho);
+  const R dR_p_dm = 1.0E-6 * dR_p_dm_over_R_p * R_p;
+  const R eps_k = 2.0E6 * sigma_p / (P.R_v * T_p * rho_p * R_p);
+  const R kelvin_term = lm_exp(eps_k);
   const double *aw = solute == 0 ? P.par_wat_act_AS : P.par_wat_act_NaCl;
   const double *daw = solute == 0 ? P.par_wat_act_deriv_AS : P.par_wat_act_deriv_NaCl;
   const double psig = solute == 0 ? P.par_sigma_AS : P.par_sigma_NaCl;
-  const double a_w = lmp_poly(aw, 5, w_s);
-  const double da_w_dm = -m_p_inv_SI * w_s * lmp_poly(daw, 4, w_s);
-  const double dsigma_dm = -psig * cc[CC_SIGW] * m_p_inv_SI * (w_s / ((1. - w_s) * (1. - w_s)));
-  double S_eq = a_w * kelvin_term;
-  const double dSeq_dm =
-      da_w_dm * kelvin_term + S_eq * eps_k * (dsigma_dm / sigma_p - drho_dm_over_rho - dR_p_dm_over_R_p);
-  const double c1 = cc[CC_C1], c2 = cc[CC_C2];
-  const double f3 = 1.0 / ((l_alpha_plus_R_p)*S_eq * c1 + (l_beta_plus_R_p)*c2);
-  const double f1f3 = 4.0 * P.pi * R_p * R_p * f3;
-  const double dg1_dm = (dSeq_dm * (l_alpha_plus_R_p) + S_eq * dR_p_dm) * c1 + dR_p_dm * c2;
+  const R a_w = lmp_poly(aw, 5, w_s);
+  const R da_w_dm = -m_p_inv_SI * w_s * lmp_poly(daw, 4, w_s);
+  const R dsigma_dm = -psig * cc[CC_SIGW] * m_p_inv_SI * (w_s / ((1. - w_s) * (1. - w_s)));
+  R S_eq = a_w * kelvin_term;
+  const R dSeq_dm = da_w_dm * kelvin_term + S_eq * eps_k * (dsigma_dm / sigma_p - drho_dm_over_rho - dR_p_dm_over_R_p);
+  const R c1 = cc[CC_C1], c2 = cc[CC_C2];
+  const R f3 = 1.0 / ((l_alpha_plus_R_p)*S_eq * c1 + (l_beta_plus_R_p)*c2);
+  const R f1f3 = 4.0 * P.pi * R_p * R_p * f3;
+  const R dg1_dm = (dSeq_dm * (l_alpha_plus_R_p) + S_eq * dR_p_dm) * c1 + dR_p_dm * c2;
   S_eq = S_amb - S_eq;
   gamma = 1.0E6 * f1f3 * S_eq;
   dgamma = 1.0E-12 * f1f3 * (S_eq * (2.0 * dR_p_dm_over_R_p - f3 * dg1_dm) - dSeq_dm);
@@ -135,24 +139,24 @@ LMD void lmp_mass_rate(const LmPhys &P, int solute, double m_w, double m_s, doub
  * and surface tension for both solutes, as the reference does).
  * Returns dm = mass_new - m_w.
  */
-LMD double lmp_newton_dm(const LmPhys &P, int solute, double dt_sub, int no_iter, double m_w, double m_s,
-                         double w_s, double R_p, double T_p, double rho_p, const double *cc, double sigma_p) {
-  const double m_w_effl = m_s * (P.w_s_max_AS_inv - 1.0);
-  const double S_amb = cc[CC_S];
-  double gamma, dgamma;
+LMDT R lmp_newton_dm(const LmPhys &P, int solute, double dt_sub, int no_iter, R m_w, R m_s, R w_s, R R_p, R T_p,
+                      R rho_p, const R *cc, R sigma_p) {
+  const R m_w_effl = m_s * (P.w_s_max_AS_inv - 1.0);
+  const R S_amb = cc[CC_S];
+  R gamma, dgamma;
   lmp_mass_rate(P, solute, m_w, m_s, w_s, R_p, T_p, rho_p, S_amb, cc, sigma_p, gamma, dgamma);
-  double d = dt_sub * dgamma;
-  double denom_inv = d < 0.9 ? 1.0 / (1.0 - d) : 10.0;
-  double mass_new = lm_np_max(m_w_effl, m_w + dt_sub * gamma * denom_inv);
+  R d = dt_sub * dgamma;
+  R denom_inv = d < 0.9 ? R(1.0 / (1.0 - d)) : R(10.0);
+  R mass_new = lm_np_max(m_w_effl, m_w + dt_sub * gamma * denom_inv);
   for (int it = 0; it < no_iter - 1; ++it) {
-    const double m_p = mass_new + m_s;
-    const double ws = m_s / m_p;
-    const double rho = cc[CC_RHOW] * lmp_poly(P.par_rho_AS, 4, ws);
-    const double R = lmp_radius(P, m_p, rho);
-    const double sigma = cc[CC_SIGW] * (1.0 + P.par_sigma_AS * ws / (1. - ws));
-    lmp_mass_rate(P, solute, mass_new, m_s, ws, R, T_p, rho, S_amb, cc, sigma, gamma, dgamma);
+    const R m_p = mass_new + m_s;
+    const R ws = m_s / m_p;
+    const R rho = cc[CC_RHOW] * lmp_poly(P.par_rho_AS, 4, ws);
+    const R Rr = lmp_radius(P, m_p, rho);
+    const R sigma = cc[CC_SIGW] * (1.0 + P.par_sigma_AS * ws / (1. - ws));
+    lmp_mass_rate(P, solute, mass_new, m_s, ws, Rr, T_p, rho, S_amb, cc, sigma, gamma, dgamma);
     d = dt_sub * dgamma;
-    denom_inv = d < 0.9 ? 1.0 / (1.0 - d) : 10.0;
+    denom_inv = d < 0.9 ? R(1.0 / (1.0 - d)) : R(10.0);
     mass_new += (dt_sub * gamma + m_w - mass_new) * denom_inv;
     mass_new = lm_np_max(m_w_effl, mass_new);
   }

@@ -297,6 +297,24 @@ class Engine:
     def check(self):
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
 
+    # ------------------------------------------------------- checkpoints
+    _SNAP_SIP = ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "seg")
+    _SNAP_GRID = ("away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "no_cols", "water_removed",
+                  "n_equal_xi", "rem_cnt", "err")
+
+    def snapshot(self):
+        """Device copy of the evolving state (current SIP buffer and cells, grid, counters)."""
+        cur, ccur = int(self.e.cur), int(self.e.ccur)
+        names = [f"{f}{cur}" for f in self._SNAP_SIP] + [f"cell{ccur}"] + list(self._SNAP_GRID)
+        return {"buf": {n: self.buf[n].clone() for n in names}, "cur": cur, "ccur": ccur,
+                "col_step": int(self.e.col_step)}
+
+    def restore(self, snap):
+        """Returns the engine to a snapshot (buffers are copied, the snapshot stays valid)."""
+        for n, t in snap["buf"].items():
+            self.buf[n].copy_(t)
+        self.e.cur, self.e.ccur, self.e.col_step = snap["cur"], snap["ccur"], snap["col_step"]
+
     KINDS = ("particles", "collision", "dml", "reindex_scatter")
 
     def prof_enable(self, max_events=4096):

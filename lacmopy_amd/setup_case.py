@@ -45,8 +45,12 @@ def _profiles_without_liquid(z, z0, p0, p_ref, Theta_l, r_tot):
 
 def make_case(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, seed=1001, p_0=101500.0, p_ref=1.0E5,
               Theta_l=289.0, r_tot_0=7.5E-3, j_max_ref=0.6, cloud_base=600.0, cloud_top=1000.0,
-              DNC0=(60.0E6, 40.0E6), mu_R=(0.02, 0.075), sigma_R=(1.4, 1.6), R_drop=9.0, sigma_drop=1.35):
-    """Returns (grid, pos, vel, cells, m_w, m_s, xi, active_ids) in the reference's layout."""
+              DNC0=(60.0E6, 40.0E6), mu_R=(0.02, 0.075), sigma_R=(1.4, 1.6), R_drop=9.0, sigma_drop=1.35,
+              return_draws=False):
+    """Returns (grid, pos, vel, cells, m_w, m_s, xi, active_ids) in the reference's layout.
+
+    return_draws: also return the draws that map the aerosol-mode parameters to the SIPs' dry masses
+    and multiplicities (see aerosol_sips)."""
     rng = np.random.default_rng(seed)
     grid = Grid([[0.0, nx * dx], [0.0, nz * dz]], [dx, dz], 1.0)
     nx, nz = int(grid.no_cells[0]), int(grid.no_cells[1])
@@ -70,7 +74,8 @@ def make_case(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, seed=1001, p_0=101500.0
     mode = (rng.random(N) >= frac0).astype(np.int64)
     mu = np.array(mu_R)[mode]
     sig = np.array(sigma_R)[mode]
-    r_dry = mu * np.exp(np.log(sig) * rng.standard_normal(N))
+    z_draw = rng.standard_normal(N)
+    r_dry = mu * np.exp(np.log(sig) * z_draw)
     m_s = 4.0 * math.pi / 3.0 * RHO_AS_DRY * r_dry ** 3
     rho_scale = rho_c[cz] / rho_c[0]
     exp_per_mode = np.where(mode == 0, spc * frac0, spc * (1 - frac0))
@@ -119,7 +124,33 @@ def make_case(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, seed=1001, p_0=101500.0
     # initial SIP velocity: the flow velocity at the SIP (host bilinear interpolation)
     vel = np.array(interpolate_flow(grid, cells, pos))
     active = np.ones(N, dtype=bool)
+    if return_draws:
+        draws = dict(mode=mode, z=z_draw, rho_scale=rho_scale, exp_per_mode=exp_per_mode,
+                     volume_cell=float(grid.volume_cell))
+        return grid, pos, vel, cells, m_w, m_s, xi, active, draws
     return grid, pos, vel, cells, m_w, m_s, xi, active
+
+
+def aerosol_sips(theta, draws):
+    """Dry masses and multiplicities of the SIPs from the six aerosol-mode parameters.
+
+    theta = (N_0 [m^-3], r_med_0 [um], sigma_0, N_1, r_med_1, sigma_1) as a float64 torch tensor (may
+    require grad); draws from make_case(..., return_draws=True).  Same map as make_case:
+    r_dry = r_med exp(ln(sigma) z), m_s = 4/3 pi 1774 r_dry^3 (1e-18 kg), xi = N V (rho_d / rho_d0) /
+    expected SIPs of the mode.  Returns (xi, m_s) in ID order, differentiable in theta.
+    """
+    import torch
+    mode = torch.as_tensor(draws["mode"], device=theta.device)
+    z = torch.as_tensor(draws["z"], dtype=torch.float64, device=theta.device)
+    scale = torch.as_tensor(draws["rho_scale"] / draws["exp_per_mode"] * draws["volume_cell"], dtype=torch.float64,
+                            device=theta.device)
+    N = theta[0::3][mode]
+    r_med = theta[1::3][mode]
+    sig = theta[2::3][mode]
+    r_dry = r_med * torch.exp(torch.log(sig) * z)
+    m_s = 4.0 * math.pi / 3.0 * RHO_AS_DRY * r_dry ** 3
+    xi = N * scale
+    return xi, m_s
 
 
 def interpolate_flow(grid, cells, pos):

@@ -1,0 +1,154 @@
+"""GPU: the adjoint window (lacmopy_amd.adjoint) -- fixed-event replay, taped re-runs, gradients.
+
+The gradient of the domain-mean liquid water path and cloud droplet number at
+the end of a window, with respect to the multiplicities and dry masses of the
+SIPs at its start, is checked against central finite differences of the same
+window (collisions replayed as the recorded fixed events) on 20 random
+droplets (R > 1 um) of two runs and on the six aerosol-mode parameters
+(through the initialisation map of setup_case.aerosol_sips):
+
+  * h = 1e-6 relative: tolerance 1e-4 relative plus the noise floor of the
+    difference quotient, 256 eps |J| / (2 h |x|).  A perturbation changes the
+    rounding of every later state it reaches; over a 6-step window that moves
+    J by up to ~100 ulps whatever h is (measured: the deviation of J(x + h x)
+    from the linear prediction is 1 ulp after one step and stays of the same
+    size from h = 1e-10 to 1e-4), so at this h most quotients sit near it;
+  * h = 1e-3 relative for the multiplicities (J is nearly linear in them: the
+    quotients at h = 1e-2 and 1e-3 agree to 1e-6) and, with the fourth-order
+    difference quotient, for the mode parameters: tolerance 1e-5 relative
+    plus the same floor, which is 1000 times smaller here, and most
+    comparisons must stand ten times above it.
+
+Planted mistakes fail both gradient tests: dropping the grid feedback from
+the reverse sweep moves d J_nc / d N_0 by 17%, dropping the adjoint of the
+advection moves it by 1.1e-3 and the multiplicity derivatives by up to 3.5e-4.
+"""
+import numpy as np
+import pytest
+import torch
+
+from lacmopy_amd import load_kernel_data_Ecol
+from lacmopy_amd.adjoint import AdjointWindow, window_objective
+from lacmopy_amd.setup_case import aerosol_sips
+from test_gpu_parity import _case, _config, _engine
+
+pytestmark = pytest.mark.gpu
+
+THETA0 = (60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6)
+H = 1.0E-6
+H2 = 1.0E-3
+EPS = np.finfo(np.float64).eps
+
+
+def _window(n_runs=1, n_steps=6, K=2, seed=5):
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts, draws = _case(seed=seed, return_draws=True)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11, n_runs=n_runs)
+    # a short spin-up without collisions: the window starts from an evolved state
+    eng.e.act_collisions = 0
+    eng.adv_steps(2)
+    eng.e.act_collisions = 1
+    return eng, AdjointWindow(eng, n_steps, checkpoint_every=K), draws
+
+
+def _state(eng):
+    s = eng.snapshot()
+    return {k: v.cpu().numpy() for k, v in s["buf"].items()}
+
+
+def _close(adj, fd, J, x, h, rtol, what, seen=None):
+    floor = 256.0 * EPS * abs(J) / (2 * h * abs(x))
+    assert abs(adj - fd) <= rtol * max(abs(fd), abs(adj)) + floor, \
+        f"{what}: adjoint {adj!r} vs finite difference {fd!r} (h {h:g}, floor {floor:.2e})"
+    if seen is not None and abs(fd) > 10.0 * floor:
+        seen.append(abs(adj - fd) / abs(fd))
+
+
+def _fd(w, xi0, ms0, var, gid, h):
+    base = xi0 if var == "xi" else ms0
+    x = float(base[gid])
+    J = []
+    for sgn in (1.0, -1.0):
+        p = base.clone()
+        p[gid] = x * (1.0 + sgn * h)
+        J.append(w.run(p if var == "xi" else xi0, p if var == "m_s" else ms0))
+    return x, [(np.sum(J[0][q]) - np.sum(J[1][q])) / (2 * h * x) for q in range(2)]
+
+
+def test_replay_reproduces_recorded_window(gpu):
+    eng, w, _ = _window(n_runs=2)
+    J_rec = w.run(record=True)
+    assert w.n_events > 0, "the window must hold collision events"
+    end_rec = _state(eng)
+    J_rep = w.run()
+    end_rep = _state(eng)
+    for k in end_rec:
+        assert np.array_equal(end_rec[k], end_rep[k]), f"replay differs from the recorded run in {k}"
+    assert np.array_equal(J_rec[0], J_rep[0]) and np.array_equal(J_rec[1], J_rep[1])
+
+
+def test_taped_rerun_reproduces_window(gpu):
+    eng, w, _ = _window(n_runs=2, n_steps=4, K=2)
+    w.run(record=True)
+    w.gradient(w_nc=1.0, w_lwp=1.0, verify=True)
+
+
+def test_gradient_vs_finite_differences_sips(gpu):
+    eng, w, _ = _window(n_runs=2)
+    slots = w.slot_map()
+    cur = int(eng.e.cur)
+    xi0 = eng.buf[f"xi{cur}"][slots].clone()
+    ms0 = eng.buf[f"m_s{cur}"][slots].clone()
+    act = (eng.buf[f"cell{int(eng.e.ccur)}"][slots] >= 0).cpu().numpy()
+    drop = act & (eng.buf[f"m_w{cur}"][slots] > 4.19E3).cpu().numpy()  # R > 1 um (m_w in 1e-18 kg)
+    J_nc, J_lwp = w.run(xi0, ms0, record=True)
+    Jb = (np.sum(J_nc), np.sum(J_lwp))
+    g = {name: w.gradient(w_nc=float(name == "nc"), w_lwp=float(name == "lwp")) for name in ("nc", "lwp")}
+    rng = np.random.default_rng(7)
+    pick = rng.choice(np.flatnonzero(drop), 20, replace=False)
+    seen6, seen4 = [], []
+    for gid in pick:
+        for var, h, rtol, seen in (("xi", H, 1.0E-4, seen6), ("m_s", H, 1.0E-4, seen6), ("xi", H2, 1.0E-5, seen4)):
+            x, fd = _fd(w, xi0, ms0, var, gid, h)
+            for q, name in enumerate(("nc", "lwp")):
+                lam = g[name][0 if var == "xi" else 1]
+                _close(float(lam[gid]), fd[q], Jb[q], x, h, rtol, f"d J_{name} / d {var}[{gid}]", seen)
+    assert len(seen4) >= 30, len(seen4)
+    print(f"SIP gradients: h=1e-6 {len(seen6)} of 80 resolved (max rel. deviation {max(seen6, default=0):.1e}); "
+          f"h=1e-3 (xi) {len(seen4)} of 40 resolved, max rel. deviation {max(seen4):.1e}, median "
+          f"{float(np.median(seen4)):.1e}")
+
+
+def test_gradient_vs_finite_differences_mode_parameters(gpu):
+    eng, w, draws = _window(n_runs=1)
+    theta = torch.tensor(THETA0, dtype=torch.float64, device="cuda", requires_grad=True)
+    xi0, ms0 = aerosol_sips(theta, draws)
+    J = window_objective(xi0, ms0, w)
+    grads = []
+    for q in range(2):
+        (gq,) = torch.autograd.grad(J[q], theta, retain_graph=True)
+        grads.append(gq.cpu().numpy())
+    base = np.array(THETA0)
+    seen = {H: [], H2: []}
+
+    def J_at(th):
+        x, m = aerosol_sips(torch.tensor(th, dtype=torch.float64, device="cuda"), draws)
+        return np.array([np.sum(v) for v in w.run(x, m)])
+
+    for k in range(6):
+        def shifted(f):
+            th = base.copy()
+            th[k] *= 1.0 + f
+            return J_at(th)
+        # h = 1e-6: central difference; h = 1e-3: fourth-order central difference (truncation ~h^4)
+        fd6 = (shifted(H) - shifted(-H)) / (2 * H * base[k])
+        fd3 = (8.0 * (shifted(H2) - shifted(-H2)) - (shifted(2 * H2) - shifted(-2 * H2))) / (12 * H2 * base[k])
+        for h, fd, rtol in ((H, fd6, 1.0E-4), (H2, fd3, 1.0E-5)):
+            for q, name in enumerate(("nc", "lwp")):
+                _close(float(grads[q][k]), fd[q], float(J[q].detach()), base[k], h, rtol,
+                       f"d J_{name} / d theta[{k}] (h {h:g})", seen[h])
+    assert len(seen[H2]) >= 10, len(seen[H2])
+    print(f"mode-parameter gradients: h=1e-6 {len(seen[H])} of 12 resolved (max rel. deviation "
+          f"{max(seen[H], default=0):.1e}); h=1e-3 (fourth order) {len(seen[H2])} of 12, max rel. deviation "
+          f"{max(seen[H2]):.1e}")

@@ -91,3 +91,41 @@ def test_gpu_entry_points_fail_loudly_without_gpu():
     with pytest.raises(RuntimeError):
         propagate_grid_subloop_step_np(np.ones((10, 3, 3)), np.ones((7, 3, 3)), 1e5, 1e-5, np.zeros((3, 3)),
                                        np.zeros((3, 3)), np.zeros((3, 3)), 1.0)
+
+
+def test_dual_newton_jacobian(tmp_path):
+    """The adjoint's local Jacobian (lm_physics.h on lm_dual.h, compiled for the host with the kernels'
+    -ffp-contract=off): the dual value part equals the double Newton bit for bit, and its derivatives in
+    (m_w, m_s, T, p, S) match central finite differences of the double Newton."""
+    import ctypes as C
+    import shutil
+    import subprocess
+
+    from lacmopy_amd._native import lib
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    csrc = os.path.join(here, "..", "lacmopy_amd", "csrc")
+    exe = str(tmp_path / "dual_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", csrc, os.path.join(here, "dual_check.cpp"),
+                    "-o", exe], check=True)
+    L = lib()
+    n_const = L.lm_num_constants()
+    consts = (C.c_double * n_const)()
+    L.lm_get_constants(consts, n_const)
+    rng = np.random.default_rng(3)
+    n = 400
+    m_s = 10.0 ** rng.uniform(-3, 2, n)
+    m_w = m_s * 10.0 ** rng.uniform(0.3, 4, n)
+    T = rng.uniform(272.0, 292.0, n)
+    p = rng.uniform(8.5E4, 1.0E5, n)
+    S = rng.uniform(0.95, 1.005, n)
+    lines = [" ".join(repr(float(c)) for c in consts), str(n)]
+    lines += [" ".join(repr(float(v)) for v in row) for row in zip(m_w, m_s, T, p, S)]
+    out = subprocess.run([exe], input="\n".join(lines) + "\n", capture_output=True, text=True, check=True).stdout
+    rows = [ln.split() for ln in out.strip().splitlines()]
+    assert len(rows) == n
+    same = np.array([int(r[0]) for r in rows])
+    err = np.array([float(r[1]) for r in rows])
+    assert same.all(), f"{(same == 0).sum()} dual values differ from the double Newton"
+    assert np.median(err) < 1.0E-8 and err.max() < 1.0E-4, (np.median(err), err.max())
