@@ -78,16 +78,17 @@ def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
         E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
     cases = []
     for r in range(n_runs):
-        grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=c["nx"], nz=c["nz"], dx=c["dx"], dz=c["dz"], spc=spc,
-                                                             seed=seed0 + 2 * r, j_max_ref=c["j_max"])
-        cases.append((grid, (pos, vel, cells, m_w, m_s, xi, act)))
+        grid, pos, vel, cells, m_w, m_s, xi, act, draws = make_case(nx=c["nx"], nz=c["nz"], dx=c["dx"], dz=c["dz"],
+                                                                    spc=spc, seed=seed0 + 2 * r,
+                                                                    j_max_ref=c["j_max"], return_draws=True)
+        cases.append((grid, (pos, vel, cells, m_w, m_s, xi, act), draws))
     grid0 = cases[0][0]
     eng = Engine(c["nx"], c["nz"], [len(k[1][5]) for k in cases], grid0.ranges, grid0.steps, grid0.volume_cell)
     eng.set_params(solute="AS", no_iter_impl_mass=3, scale_dt_cond=c["sdc"], no_col_per_adv=2, act_collisions=True,
                    act_relaxation=True, dt=c["dt"], dt_col=c["dt"] / 2, g_set=9.80665, seed=seed0)
     eng.set_collision_kernel(E, lnR0, lnf)
     eng.set_flow(grid0.velocity, grid0.mass_flux_air_dry)
-    for r, (g, _) in enumerate(cases):
+    for r, (g, _, _) in enumerate(cases):
         eng.set_relaxation(r, *init_profiles(g))
         eng.set_grid(r, g.scalar_fields(), g.mat_prop())
     eng.init_cells()
@@ -128,6 +129,106 @@ def kernel_times(eng, steps):
     eng.prof_disable()
     eng.check()
     return {k: (ms / max(n, 1), n) for k, (ms, n) in res.items()}
+
+
+def adjoint_cost(seed0, spc, window, every, spinup, check=0):
+    """Adjoint of a `window`-step window of the plain 75x75 run after `spinup` steps without collisions:
+    wall time of the plain forward window, of the recording forward (events + checkpoints every `every`
+    steps) and of one gradient (taped re-runs + reverse sweep), for the cloud-droplet-number objective.
+    check > 0: also compare the gradient with central finite differences on `check` random droplets
+    (multiplicity, h = 1e-3 relative) and on the six aerosol-mode parameters (fourth order, h = 1e-3)."""
+    import torch
+    from lacmopy_amd.adjoint import AdjointWindow
+    eng, n1, cases = build_engine(1, spc, seed0, case="icmw75")
+    e = eng.e
+    e.act_collisions = 0
+    t = time.perf_counter()
+    eng.adv_steps(spinup)
+    torch.cuda.synchronize()
+    log(f"adjoint: spin-up {spinup} steps in {time.perf_counter() - t:.1f} s")
+    e.act_collisions = 1
+    eng.check()
+    start = eng.snapshot()
+    eng.adv_steps(2)  # warm-up of the collision path
+    eng.restore(start)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.adv_steps(window)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    eng.check()
+    eng.restore(start)
+    w = AdjointWindow(eng, window, checkpoint_every=every)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = w.run(record=True)
+    torch.cuda.synchronize()
+    t_rec = time.perf_counter() - t0
+    w.gradient(w_nc=1.0, w_lwp=0.0)  # allocates the tape; the timed call below reuses it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lam_xi, lam_ms = w.gradient(w_nc=1.0, w_lwp=0.0)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    res = {"workload": f"plain 75x75 run ({n1} SIPs), {window}-step window (dt 1 s, 10 substeps, 2 collision "
+                       f"steps) after a {spinup}-step spin-up without collisions; fixed collision events; "
+                       f"checkpoints every {every} steps",
+           "sips": n1, "window_steps": window, "checkpoint_every": every, "events": w.n_events,
+           "J_nc": float(J[0][0]), "J_lwp": float(J[1][0]),
+           "forward_s": t_fwd, "record_s": t_rec, "gradient_s": t_grad,
+           "cost_ratio": (t_rec + t_grad) / t_fwd, "gradient_over_forward": t_grad / t_fwd}
+    log(f"adjoint: {res}")
+    if check:
+        res["fd_check"] = adjoint_fd_check(w, eng, cases, lam_xi, check)
+        log(f"adjoint fd check: {res['fd_check']}")
+    del w, eng
+    torch.cuda.empty_cache()
+    return res
+
+
+def adjoint_fd_check(w, eng, cases, lam_xi, n_check):
+    import torch
+    from lacmopy_amd.setup_case import aerosol_sips
+    eps = np.finfo(np.float64).eps
+    slots = w.slot_map()
+    cur = int(eng.e.cur)
+    xi0 = eng.buf[f"xi{cur}"][slots].clone()
+    ms0 = eng.buf[f"m_s{cur}"][slots].clone()
+    drop = ((eng.buf[f"cell{int(eng.e.ccur)}"][slots] >= 0) & (eng.buf[f"m_w{cur}"][slots] > 4.19E3)).cpu().numpy()
+    J0 = float(np.sum(w.run(xi0, ms0)[0]))
+    out = {"sips": [], "modes": []}
+    rng = np.random.default_rng(11)
+    for gid in rng.choice(np.flatnonzero(drop), n_check, replace=False):
+        x, h = float(xi0[gid]), 1.0E-3
+        Jpm = []
+        for sgn in (1.0, -1.0):
+            p = xi0.clone()
+            p[gid] = x * (1.0 + sgn * h)
+            Jpm.append(float(np.sum(w.run(p, ms0)[0])))
+        fd = (Jpm[0] - Jpm[1]) / (2 * h * x)
+        a = float(lam_xi[gid])
+        out["sips"].append({"gid": int(gid), "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300),
+                            "floor_rel": 256 * eps * abs(J0) / (2 * h * x) / max(abs(fd), 1e-300)})
+    # mode parameters through the initialisation map (fourth-order quotient, h = 1e-3)
+    draws = cases[0][2] if len(cases[0]) > 2 else None
+    if draws is not None:
+        theta0 = np.array([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6])
+        theta = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
+        xi_m, ms_m = aerosol_sips(theta, draws)
+        w.run(xi_m.detach(), ms_m.detach())
+        g_xi, g_ms = w.gradient(w_nc=1.0, w_lwp=0.0)
+        (grad,) = torch.autograd.grad((xi_m * g_xi).sum() + (ms_m * g_ms).sum(), theta)
+        for k in range(6):
+            def J_at(f):
+                th = theta0.copy()
+                th[k] *= 1.0 + f
+                xx, mm = aerosol_sips(torch.tensor(th, dtype=torch.float64, device="cuda"), draws)
+                return float(np.sum(w.run(xx, mm)[0]))
+            h = 1.0E-3
+            fd = (8.0 * (J_at(h) - J_at(-h)) - (J_at(2 * h) - J_at(-2 * h))) / (12 * h * theta0[k])
+            a = float(grad[k])
+            out["modes"].append({"k": k, "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300)})
+    return out
 
 
 def cpu_baseline(seed0, spc):
@@ -193,6 +294,13 @@ def main():
     ap.add_argument("--kernel", default=None, choices=["Long_Bott", "Hall_Bott", "const"])
     ap.add_argument("--graph", action="store_true",
                     help="replay the step captured in a two-step HIP graph instead of launching kernel by kernel")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-window", type=int, default=600, help="advection steps of the adjoint window")
+    ap.add_argument("--adj-every", type=int, default=60, help="checkpoint interval of the adjoint window")
+    ap.add_argument("--adj-spinup", type=int, default=7200, help="spin-up steps before the adjoint window")
+    ap.add_argument("--adj-check", type=int, default=0,
+                    help="also check the adjoint against finite differences on this many droplets and the "
+                         "six aerosol-mode parameters")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
@@ -247,12 +355,17 @@ def main():
                 "traffic": measured_traffic("k_particles") if default_workload else None,
                 "kernel": "k_particles", "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
 
+    adj = None
+    if not args.no_adjoint and args.case == "icmw75":
+        adj = adjoint_cost(args.seed, args.spc, args.adj_window, args.adj_every, args.adj_spinup, args.adj_check)
+
     # the CPU baseline is the oracle on the plain 75x75 case (the metric's workload)
     cpu = None if (args.no_cpu or args.case != "icmw75") else cpu_baseline(args.seed, args.spc)
     if cpu:
         log(f"cpu baseline: {cpu}")
     res = {
-        "metric": "SIP-steps/s, 2D ICMW 75x75: 64-seed batch vs plain GPU" if args.case == "icmw75"
+        "metric": "SIP-steps/s, 2D ICMW 75x75: 64-seed batch vs plain GPU; adjoint/forward cost"
+        if args.case == "icmw75"
         else f"SIP-steps/s, {args.case}",
         "value": batch_rate,
         "unit": "SIP-steps/s",
@@ -273,7 +386,8 @@ def main():
                    "launch": "two-step HIP graph replays" if args.graph else "eager",
                    "plain_single_run": plain,
                    "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
-                   "kernel_ms": ktimes, "collision_events": ncols},
+                   "kernel_ms": ktimes, "collision_events": ncols,
+                   "adjoint": adj},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -136,10 +136,16 @@ typedef struct LmEngine {
   double *ccd;          /* n_runs*nc*12 scratch: d(l_alpha, l_beta, c1, c2, rho_w, sigma_w)/d(T, p) */
 } LmEngine;
 
-/* Adjoint variables of a window (lm_adj_*): per global SIP id and per cell. */
+/*
+ * Adjoint variables of a window (lm_adj_*): per global SIP id and per cell,
+ * plus the scratch the reverse sweep keeps them in while it reverses one
+ * advection step (the step's slot order, and the slot of each global id).
+ */
 typedef struct LmAdjoint {
-  double *lam_mw, *lam_ms, *lam_xi; /* n_tot each */
+  double *lam_mw, *lam_ms, *lam_xi; /* n_tot each, by global id */
   double *lam_cell; /* 8*n_runs*nc: Theta, r_v, T, p, S, dml, dTheta_adv, dr_v_adv */
+  double *sl_mw, *sl_ms, *sl_xi;    /* n_tot each, by slot of the step being reversed */
+  int32_t *sl_of;                   /* n_tot: slot of each global id in that step */
 } LmAdjoint;
 
 /* Library identity: returns a static version string. */

@@ -65,7 +65,8 @@ class LmEngine(C.Structure):
 class LmAdjoint(C.Structure):
     """Mirror of struct LmAdjoint (include/lacmo_amd.h)."""
 
-    _fields_ = [("lam_mw", _p), ("lam_ms", _p), ("lam_xi", _p), ("lam_cell", _p)]
+    _fields_ = [("lam_mw", _p), ("lam_ms", _p), ("lam_xi", _p), ("lam_cell", _p),
+                ("sl_mw", _p), ("sl_ms", _p), ("sl_xi", _p), ("sl_of", _p)]
 
 
 LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double

@@ -80,6 +80,8 @@ class AdjointWindow:
         self.slot_of = torch.zeros(n, **i32)
         self.ccd = torch.zeros(self.ncell * 12, **f64)
         self.lam_mw, self.lam_ms, self.lam_xi = (torch.zeros(n, **f64) for _ in range(3))
+        self.sl_mw, self.sl_ms, self.sl_xi = (torch.zeros(n, **f64) for _ in range(3))
+        self.sl_of = torch.zeros(n, **i32)
         self.lam_cell = torch.zeros(8 * self.ncell, **f64)
         self.tape = None
         self.start = None
@@ -206,7 +208,8 @@ class AdjointWindow:
         eng, e = self.eng, self.eng.e
         self._alloc_tape()
         A = LmAdjoint(self.lam_mw.data_ptr(), self.lam_ms.data_ptr(), self.lam_xi.data_ptr(),
-                      self.lam_cell.data_ptr())
+                      self.lam_cell.data_ptr(), self.sl_mw.data_ptr(), self.sl_ms.data_ptr(), self.sl_xi.data_ptr(),
+                      self.sl_of.data_ptr())
         eng.restore(self.end)
         self._bind(0)
         check(eng.L.lm_adj_seed(C.byref(e), C.byref(A), float(w_nc), float(w_lwp), self.width, eng.stream))

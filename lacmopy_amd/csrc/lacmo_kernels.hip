@@ -1037,8 +1037,7 @@ __device__ static double newton_dm_tape(const LmPhys &P, const KView &v, long lo
   for (int q = 1; q < 5; ++q) t[q * n] = dm.d[q];
   t[5 * n] = dm.v;
   t[6 * n] = xi;
-  const int r = gc / v.nc;
-  v.tape_part_i[k] = (int)(v.run_base[r] + v.id[k]);
+  v.tape_part_i[k] = (int)(v.run_base[gc / v.nc] + v.id[k]);
   v.tape_part_i[n + k] = gc;
   return dm.v;
 }
@@ -1051,7 +1050,10 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
   const int gc = v.cell[k];
   if (gc < 0) {
     v.cell_next[k] = gc;
-    if (TAPE) v.tape_part_i[n_tot + k] = -1;
+    if (TAPE) {
+      v.tape_part_i[k] = (int)(v.run_base[slot_run(v.run_base, v.n_runs, k)] + v.id[k]);
+      v.tape_part_i[n_tot + k] = -1;
+    }
     return;
   }
   const int nc = v.nc;
@@ -2670,7 +2672,8 @@ int lm_objective(LmEngine *e, double nc_width, double *out, void *stream) {
 }
 
 static int check_adjoint(const LmEngine *e, const LmAdjoint *a) {
-  if (!a || !a->lam_mw || !a->lam_ms || !a->lam_xi || !a->lam_cell)
+  if (!a || !a->lam_mw || !a->lam_ms || !a->lam_xi || !a->lam_cell || !a->sl_mw || !a->sl_ms || !a->sl_xi ||
+      !a->sl_of)
     return set_err(LM_ERR_ARG, "adjoint buffers missing");
   if (!e->run_base) return set_err(LM_ERR_ARG, "run_base missing");
   return 0;
@@ -2740,8 +2743,12 @@ int lm_adj_adv_steps(LmEngine *e, LmAdjoint *a, int32_t n_steps, int32_t col_ind
   const int ncell = e->n_runs * e->nc;
   const KView v = make_view(e);
   const int nfaces = e->n_runs * ((e->nx + 1) * e->nz + e->nx * (e->nz - 1));
+  const unsigned gn = (unsigned)((n + 255) / 256);
   for (int s = n_steps - 1; s >= 0; --s) {
     int sub = (s + 1) * nsub, col = col_index0 + (s + 1) * ncol, inc = 2 * (s + 1);
+    /* the step's slot order (global id of each slot, the same in all its substeps) */
+    const int32_t *gid_of_slot = e->tape_part_i + (size_t)(s * nsub) * 2 * (size_t)n;
+    if (n > 0) hipLaunchKernelGGL(k_adj_to_slots, dim3(gn), dim3(256), 0, S(stream), n, gid_of_slot, *a);
     for (int q = nops - 1; q >= 0; --q) {
       const int op = ops[q];
       if (op == OP_SUB) {
@@ -2763,10 +2770,11 @@ int lm_adj_adv_steps(LmEngine *e, LmAdjoint *a, int32_t n_steps, int32_t col_ind
         hipLaunchKernelGGL(k_adj_inc_faces, dim3((nfaces + 255) / 256), dim3(256), 0, S(stream), v,
                            e->tape_fields + (size_t)inc * 2 * ncell, mult, e->dt_sub, a->lam_cell);
         const int nt = e->n_runs * e->nz;
-        hipLaunchKernelGGL(k_adj_inc_rows, dim3((nt + 127) / 128), dim3(128), 0, S(stream), v, mult, e->dt_sub,
+        hipLaunchKernelGGL(k_adj_inc_rows, dim3(nt), dim3(WAVE), 0, S(stream), v, mult, e->dt_sub,
                            e->act_relaxation, second, a->lam_cell);
       }
     }
+    if (n > 0) hipLaunchKernelGGL(k_adj_to_gids, dim3(gn), dim3(256), 0, S(stream), n, gid_of_slot, *a);
   }
   LAUNCH_CHECK();
   return 0;

@@ -239,6 +239,27 @@ __global__ void k_adj_grid(int ncell, const double *J, double *lam) {
   lam[(size_t)LA_DRV * ncell + gc] += g[5];
 }
 
+/* into the slot order of the step being reversed (and the slot of each global id) */
+__global__ void k_adj_to_slots(long long n_tot, const int32_t *gid_of_slot, LmAdjoint a) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_tot) return;
+  const int g = gid_of_slot[k];
+  a.sl_mw[k] = a.lam_mw[g];
+  a.sl_ms[k] = a.lam_ms[g];
+  a.sl_xi[k] = a.lam_xi[g];
+  a.sl_of[g] = (int32_t)k;
+}
+
+/* back to global-id order once the step is reversed */
+__global__ void k_adj_to_gids(long long n_tot, const int32_t *gid_of_slot, LmAdjoint a) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_tot) return;
+  const int g = gid_of_slot[k];
+  a.lam_mw[g] = a.sl_mw[k];
+  a.lam_ms[g] = a.sl_ms[k];
+  a.lam_xi[g] = a.sl_xi[k];
+}
+
 /*
  * Particle substep of the condensation update (integration.py:840-940):
  * m_w' = m_w + dm(m_w, m_s, T, p, S of the cell), dml = 1e-18 sum xi dm.
@@ -249,35 +270,33 @@ __global__ __launch_bounds__(256) void k_adj_part(long long n_tot, int ncell, co
   const int gc = k < n_tot ? ti[n_tot + k] : -1;
   double yT = 0.0, yp = 0.0, yS = 0.0;
   if (gc >= 0) {
-    const int gid = ti[k];
     const double J0 = tp[k], J1 = tp[n_tot + k], J2 = tp[2 * n_tot + k], J3 = tp[3 * n_tot + k],
                  J4 = tp[4 * n_tot + k], dm = tp[5 * n_tot + k], xi = tp[6 * n_tot + k];
     const double ld = a.lam_cell[(size_t)LA_DML * ncell + gc] * 1.0E-18;
-    const double y = a.lam_mw[gid] + xi * ld;
-    a.lam_mw[gid] = y * J0 - xi * ld;
-    a.lam_ms[gid] += y * J1;
-    a.lam_xi[gid] += ld * dm;
+    const double y = a.sl_mw[k] + xi * ld;
+    a.sl_mw[k] = y * J0 - xi * ld;
+    a.sl_ms[k] += y * J1;
+    a.sl_xi[k] += ld * dm;
     yT = y * J2;
     yp = y * J3;
     yS = y * J4;
   }
-  /* cell sums: one atomic per wave when all its SIPs with a cell share it (slots are sorted by cell) */
-  const unsigned long long act = __ballot(gc >= 0);
-  if (act == 0) return;
-  const int lead = __builtin_ctzll(act);
-  const int g_lead = __shfl(gc, lead);
-  if (__all(gc < 0 || gc == g_lead)) {
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-      yT += __shfl_xor(yT, off);
-      yp += __shfl_xor(yp, off);
-      yS += __shfl_xor(yS, off);
+  /* cell sums: a segmented reduction over runs of equal cells in the wave (slots are sorted by cell), then one
+     atomic per run */
+  const int lane = (int)(threadIdx.x & (WAVE - 1));
+  const int nxt = __shfl_down(gc, 1);
+  const unsigned long long bnd = __ballot(lane == WAVE - 1 || nxt != gc);
+  const int end = lane + __builtin_ctzll(bnd >> lane);
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const double t0 = __shfl_down(yT, off), t1 = __shfl_down(yp, off), t2 = __shfl_down(yS, off);
+    if (lane + off <= end) {
+      yT += t0;
+      yp += t1;
+      yS += t2;
     }
-    if ((int)(threadIdx.x & (WAVE - 1)) == lead) {
-      atomicAdd(&a.lam_cell[(size_t)LA_T * ncell + g_lead], yT);
-      atomicAdd(&a.lam_cell[(size_t)LA_P * ncell + g_lead], yp);
-      atomicAdd(&a.lam_cell[(size_t)LA_S * ncell + g_lead], yS);
-    }
-  } else if (gc >= 0) {
+  }
+  const int prv = __shfl_up(gc, 1);
+  if ((lane == 0 || prv != gc) && gc >= 0) {
     atomicAdd(&a.lam_cell[(size_t)LA_T * ncell + gc], yT);
     atomicAdd(&a.lam_cell[(size_t)LA_P * ncell + gc], yp);
     atomicAdd(&a.lam_cell[(size_t)LA_S * ncell + gc], yS);
@@ -357,10 +376,11 @@ __global__ void k_adj_inc_faces(KView v, const double *F, double mult, double dt
  * Relaxation part of the increments (integration.py:325-354 column means;
  * mult 1 or 2), then the increments' adjoint moves on: after the second
  * evaluation (d2 = ... - d1) it becomes -lam_d for the first subloop, after
- * the first it is cleared.  One thread per (run, level).
+ * the first it is cleared.  One wavefront per (run, level).
  */
-__global__ void k_adj_inc_rows(KView v, double mult, double dt_sub, int act_relax, int second, double *lam) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(WAVE) void k_adj_inc_rows(KView v, double mult, double dt_sub, int act_relax,
+                                                      int second, double *lam) {
+  const int t = blockIdx.x;
   if (t >= v.n_runs * v.nz) return;
   const int r = t / v.nz, j = t % v.nz, nc = v.nc, nz = v.nz, nx = v.nx;
   const int ncell = v.n_runs * nc;
@@ -368,17 +388,21 @@ __global__ void k_adj_inc_rows(KView v, double mult, double dt_sub, int act_rela
   double *drv = lam + (size_t)LA_DRV * ncell + r * nc, *dth = lam + (size_t)LA_DTH * ncell + r * nc;
   if (act_relax) {
     double srv = 0.0, sth = 0.0;
-    for (int i = 0; i < nx; ++i) {
+    for (int i = threadIdx.x; i < nx; i += WAVE) {
       srv += drv[i * nz + j];
       sth += dth[i * nz + j];
     }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+      srv += __shfl_xor(srv, off);
+      sth += __shfl_xor(sth, off);
+    }
     const double cf = mult * (-dt_sub / (double)nx / v.t_relax[j]);
-    for (int i = 0; i < nx; ++i) {
+    for (int i = threadIdx.x; i < nx; i += WAVE) {
       lrv[i * nz + j] += cf * srv;
       lth[i * nz + j] += cf * sth;
     }
   }
-  for (int i = 0; i < nx; ++i) {
+  for (int i = threadIdx.x; i < nx; i += WAVE) {
     drv[i * nz + j] = second ? -drv[i * nz + j] : 0.0;
     dth[i * nz + j] = second ? -dth[i * nz + j] : 0.0;
   }
@@ -391,28 +415,28 @@ __global__ void k_adj_events(const LmEvent *ev, const int32_t *off, const int32_
   const int q0 = off[gc], q1 = off[gc + 1];
   for (int q = q1 - 1; q >= q0; --q) {
     const LmEvent E = ev[order[q]];
-    const int i = E.gid_min, j = E.gid_max;
+    const int i = a.sl_of[E.gid_min], j = a.sl_of[E.gid_max];
     if (E.kind == 1) {
-      a.lam_xi[i] -= E.p * a.lam_xi[j];
-      a.lam_mw[j] += E.p * a.lam_mw[i];
-      a.lam_ms[j] += E.p * a.lam_ms[i];
+      a.sl_xi[i] -= E.p * a.sl_xi[j];
+      a.sl_mw[j] += E.p * a.sl_mw[i];
+      a.sl_ms[j] += E.p * a.sl_ms[i];
     } else if (E.kind == 2) {
-      a.lam_xi[i] -= a.lam_xi[j];
-      a.lam_mw[j] += a.lam_mw[i];
-      a.lam_ms[j] += a.lam_ms[i];
+      a.sl_xi[i] -= a.sl_xi[j];
+      a.sl_mw[j] += a.sl_mw[i];
+      a.sl_ms[j] += a.sl_ms[i];
     } else {
       /* m' = 2 (xi_i m_i + xi_j m_j) / G for both, xi_j' = 0.35 G, xi_i' = 0.5 G - xi_j', G = xi_i + xi_j */
       const double G = E.xi_min + E.xi_max;
       const double mw = 2.0 * (E.xi_min * E.mw_min + E.xi_max * E.mw_max) / G;
       const double ms = 2.0 * (E.xi_min * E.ms_min + E.xi_max * E.ms_max) / G;
-      const double A = a.lam_mw[i] + a.lam_mw[j], B = a.lam_ms[i] + a.lam_ms[j];
-      const double LG = 0.5 * 0.7 * a.lam_xi[j] + (0.5 - 0.5 * 0.7) * a.lam_xi[i];
-      a.lam_xi[i] = A * (2.0 * E.mw_min - mw) / G + B * (2.0 * E.ms_min - ms) / G + LG;
-      a.lam_xi[j] = A * (2.0 * E.mw_max - mw) / G + B * (2.0 * E.ms_max - ms) / G + LG;
-      a.lam_mw[i] = A * 2.0 * E.xi_min / G;
-      a.lam_mw[j] = A * 2.0 * E.xi_max / G;
-      a.lam_ms[i] = B * 2.0 * E.xi_min / G;
-      a.lam_ms[j] = B * 2.0 * E.xi_max / G;
+      const double A = a.sl_mw[i] + a.sl_mw[j], B = a.sl_ms[i] + a.sl_ms[j];
+      const double LG = 0.5 * 0.7 * a.sl_xi[j] + (0.5 - 0.5 * 0.7) * a.sl_xi[i];
+      a.sl_xi[i] = A * (2.0 * E.mw_min - mw) / G + B * (2.0 * E.ms_min - ms) / G + LG;
+      a.sl_xi[j] = A * (2.0 * E.mw_max - mw) / G + B * (2.0 * E.ms_max - ms) / G + LG;
+      a.sl_mw[i] = A * 2.0 * E.xi_min / G;
+      a.sl_mw[j] = A * 2.0 * E.xi_max / G;
+      a.sl_ms[i] = B * 2.0 * E.xi_min / G;
+      a.sl_ms[j] = B * 2.0 * E.xi_max / G;
     }
   }
 }
