@@ -294,6 +294,7 @@ def main():
     ap.add_argument("--kernel", default=None, choices=["Long_Bott", "Hall_Bott", "const"])
     ap.add_argument("--graph", action="store_true",
                     help="replay the step captured in a two-step HIP graph instead of launching kernel by kernel")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-window", type=int, default=600, help="advection steps of the adjoint window")
     ap.add_argument("--adj-every", type=int, default=60, help="checkpoint interval of the adjoint window")
@@ -323,6 +324,17 @@ def main():
     t = time.perf_counter()
     eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, case=args.case, kernel=args.kernel)
     log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
+    tune = None
+    if not args.no_tune:
+        t = time.perf_counter()
+        eng.adv_steps(1)  # tune on an evolved state
+        tr = eng.autotune()
+        tune = {"particle_block": tr["chosen"][0], "sips_per_thread": tr["chosen"][1],
+                "collision_capacity": tr["chosen"][2],
+                "particle_ms": {f"{b}x{sp}": round(ms, 4) for (b, sp), ms in tr["particles"].items()},
+                "collision_ms": {str(c): round(ms, 4) for c, ms in tr["collision"].items()},
+                "seconds": time.perf_counter() - t}
+        log(f"autotune: {tune}")
     secs = time_steps(eng, args.steps, args.warmup, graph=args.graph)
     batch_rate = n_sips * args.steps / secs
     log(f"batch: {secs / args.steps * 1e3:.2f} ms/step, {batch_rate:.4g} SIP-steps/s")
@@ -337,9 +349,13 @@ def main():
     plain = None
     if not args.no_plain:
         eng1, n1, _ = build_engine(1, args.spc, args.seed, case=args.case, kernel=args.kernel)
+        shape1 = None
+        if not args.no_tune:
+            eng1.adv_steps(1)
+            shape1 = eng1.autotune()["chosen"]
         s1 = time_steps(eng1, max(args.steps, 10), args.warmup, graph=args.graph)
         plain = {"sip_steps_per_s": n1 * max(args.steps, 10) / s1, "ms_per_step": s1 / max(args.steps, 10) * 1e3,
-                 "sips": n1}
+                 "sips": n1, "launch_shape": shape1}
         log(f"plain: {plain}")
         del eng1
 
@@ -384,6 +400,7 @@ def main():
                    "runs": args.runs, "sips_total": n_batch_sips, "cells": case["nx"] * case["nz"],
                    "algorithmic_bytes_per_sip_step": n_sub * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX,
                    "launch": "two-step HIP graph replays" if args.graph else "eager",
+                   "autotune": tune,
                    "plain_single_run": plain,
                    "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
                    "kernel_ms": ktimes, "collision_events": ncols,

@@ -134,6 +134,12 @@ typedef struct LmEngine {
                            dTheta_adv, dr_v_adv) of the grid substep */
   double *tape_fields;  /* per increment evaluation 2*n_runs*nc: r_v, Theta */
   double *ccd;          /* n_runs*nc*12 scratch: d(l_alpha, l_beta, c1, c2, rho_w, sigma_w)/d(T, p) */
+  /*
+   * Launch shape (0 = default; set by an autotuner, e.g. lacmopy_amd.Engine.autotune): threads per
+   * block (64..512, a multiple of 64) and SIPs per thread (1 or 2) of the particle kernel, and the
+   * member capacity of the main collision pass (160, 256 or 512; larger cells take the overflow pass).
+   */
+  int32_t part_block, part_spt, col_cap, pad3;
 } LmEngine;
 
 /*

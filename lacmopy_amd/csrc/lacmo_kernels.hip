@@ -30,6 +30,8 @@
 #define WAVE 64
 #define DML_CAP 512 /* member capacity of the gather kernels (Delta m_l, reindex) */
 #define OVF_GRID 1024 /* blocks of the grid-strided overflow passes */
+#define PART_BLOCK_MAX 512 /* largest block of the particle kernel */
+#define TAPE_BLOCK 256     /* block of the taped particle kernel (its dual Newton needs the registers of one wave per SIMD) */
 
 /* ------------------------------------------------------------------------ */
 /* host-side constants (same expressions as the reference modules)          */
@@ -1042,11 +1044,11 @@ __device__ static double newton_dm_tape(const LmPhys &P, const KView &v, long lo
   return dm.v;
 }
 
+/* one SIP slot of the particle substep */
 template <int SOLUTE, bool POS_ONLY, bool TAPE>
-__global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, double dt_sub, double dt_pos) {
+__device__ __forceinline__ static void particle_update(const KView &v, long long k, long long n_tot, double dt_sub,
+                                                       double dt_pos) {
   const LmPhys &P = c_P;
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_tot) return;
   const int gc = v.cell[k];
   if (gc < 0) {
     v.cell_next[k] = gc;
@@ -1139,6 +1141,22 @@ __global__ __launch_bounds__(256) void k_particles(KView v, long long n_tot, dou
   v.u[k] = u;
   v.w[k] = w;
   v.cell_next[k] = gc_new;
+}
+
+/*
+ * A block covers blockDim.x * SPT consecutive slots, SPT SIPs per thread (the block size and SPT are launch
+ * parameters an autotuner picks; SPT is a template parameter because a run-time loop costs the kernel a
+ * quarter of its occupancy).
+ */
+template <int SOLUTE, bool POS_ONLY, bool TAPE, int SPT>
+__global__ __launch_bounds__(TAPE ? TAPE_BLOCK : PART_BLOCK_MAX) void k_particles(KView v, long long n_tot,
+                                                                                  double dt_sub, double dt_pos) {
+  const long long base = (long long)blockIdx.x * blockDim.x * SPT + threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < SPT; ++s) {
+    const long long k = base + (long long)s * blockDim.x;
+    if (k < n_tot) particle_update<SOLUTE, POS_ONLY, TAPE>(v, k, n_tot, dt_sub, dt_pos);
+  }
 }
 
 /* water_removed[r] += xi*m_w of this substep's removed SIPs, ascending id (integration.py:620-628) */
@@ -2091,6 +2109,10 @@ static int check_engine(const LmEngine *e) {
   if ((int64_t)e->n_runs * e->nc > 0x7fffffff / 2) return set_err(LM_ERR_ARG, "too many cells");
   if (e->n_tot > 0x7ffffff0) return set_err(LM_ERR_ARG, "too many SIPs for 32-bit slots");
   if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
+  if (e->part_block < 0 || e->part_block > PART_BLOCK_MAX || (e->part_block % WAVE) || e->part_spt < 0 ||
+      e->part_spt > 2 || (e->col_cap && e->col_cap != 160 && e->col_cap != 256 && e->col_cap != 512))
+    return set_err(LM_ERR_ARG, "bad launch shape (block 64..512 in steps of 64, 1 or 2 SIPs per thread, "
+                               "collision capacity 160, 256 or 512)");
   if (e->ev_mode && (!e->ev || !e->ev_off || !e->ev_order || !e->run_base || e->ev_cap <= 0))
     return set_err(LM_ERR_ARG, "event log buffers missing");
   if (e->ev_mode == 1 && (!e->ev_cnt || !e->ev_count)) return set_err(LM_ERR_ARG, "event counters missing");
@@ -2103,6 +2125,14 @@ static int check_engine(const LmEngine *e) {
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
 /* mean + 2.5 sqrt(mean) SIPs per cell: the member capacity the main per-cell passes are sized for */
+/* launch shape of the particle kernel: tuned values or 256 threads, one SIP per thread */
+static int part_block(const LmEngine *e) { return e->part_block > 0 ? e->part_block : 256; }
+static int part_spt(const LmEngine *e) { return e->part_spt > 0 ? e->part_spt : 1; }
+static dim3 part_grid(const LmEngine *e, int spt) {
+  const long long per = (long long)part_block(e) * spt;
+  return dim3((unsigned)((e->n_tot + per - 1) / per));
+}
+
 static int cell_capacity(const LmEngine *e) {
   const double mean = (double)e->n_tot / ((double)e->n_runs * (double)e->nc);
   return (int)(mean + 2.5 * sqrt(mean)) + 1;
@@ -2144,6 +2174,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, col_step_dev), (int64_t)offsetof(LmEngine, ev_mode),
       (int64_t)offsetof(LmEngine, ev),        (int64_t)offsetof(LmEngine, run_base),
       (int64_t)offsetof(LmEngine, tape_part), (int64_t)offsetof(LmEngine, ccd),
+      (int64_t)offsetof(LmEngine, part_block),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -2199,7 +2230,7 @@ int lm_collision_step(LmEngine *e, void *stream) {
     ProfScope ps(e, 1, S(stream));
     /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
        small capacity keeps the LDS footprint, and so the occupancy, low; larger cells go to the overflow pass */
-    const int cap = cell_capacity(e);
+    const int cap = e->col_cap > 0 ? e->col_cap : cell_capacity(e);
     if (cap <= 160)
       launch_collision<160>(v, e, dim3(ncell), S(stream), 0);
     else if (cap <= 256)
@@ -2229,8 +2260,8 @@ int lm_position_step(LmEngine *e, double dt, void *stream) {
   KView v = make_view(e);
   const long long n = e->n_tot;
   if (n > 0)
-    hipLaunchKernelGGL((k_particles<0, true, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, n,
-                       0.0, dt);
+    hipLaunchKernelGGL((k_particles<0, true, false, 1>), part_grid(e, 1), dim3(part_block(e)), 0, S(stream), v, n, 0.0,
+                       dt);
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();
   e->ccur = 1 - e->ccur;
@@ -2292,17 +2323,28 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
   const int ncell = e->n_runs * e->nc;
   if (n > 0) {
     ProfScope ps(e, 0, S(stream));
-    const dim3 g((unsigned)((n + 255) / 256));
+    const dim3 b(part_block(e));
+    const int spt = part_spt(e);
     if (e->tape_part) {
       hipLaunchKernelGGL(k_ccd, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
+      const dim3 gt((unsigned)((n + TAPE_BLOCK - 1) / TAPE_BLOCK));
       if (e->solute == 0)
-        hipLaunchKernelGGL((k_particles<0, false, true>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+        hipLaunchKernelGGL((k_particles<0, false, true, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
+                           dt_sub_pos);
       else
-        hipLaunchKernelGGL((k_particles<1, false, true>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+        hipLaunchKernelGGL((k_particles<1, false, true, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
+                           dt_sub_pos);
+    } else if (spt == 2) {
+      if (e->solute == 0)
+        hipLaunchKernelGGL((k_particles<0, false, false, 2>), part_grid(e, 2), b, 0, S(stream), v, n, dt_sub,
+                           dt_sub_pos);
+      else
+        hipLaunchKernelGGL((k_particles<1, false, false, 2>), part_grid(e, 2), b, 0, S(stream), v, n, dt_sub,
+                           dt_sub_pos);
     } else if (e->solute == 0) {
-      hipLaunchKernelGGL((k_particles<0, false, false>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+      hipLaunchKernelGGL((k_particles<0, false, false, 1>), part_grid(e, 1), b, 0, S(stream), v, n, dt_sub, dt_sub_pos);
     } else {
-      hipLaunchKernelGGL((k_particles<1, false, false>), g, dim3(256), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+      hipLaunchKernelGGL((k_particles<1, false, false, 1>), part_grid(e, 1), b, 0, S(stream), v, n, dt_sub, dt_sub_pos);
     }
   }
   {

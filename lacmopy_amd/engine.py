@@ -297,6 +297,54 @@ class Engine:
     def check(self):
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
 
+    # ------------------------------------------------------------ autotune
+    def set_launch_shape(self, part_block=0, part_spt=0, col_cap=0):
+        """Launch shape of the particle kernel (threads per block, SIPs per thread) and the member capacity
+        of the main collision pass; 0 restores the defaults (256, 1, from the mean SIPs per cell)."""
+        self.drop_graph()
+        self.e.part_block, self.e.part_spt, self.e.col_cap = int(part_block), int(part_spt), int(col_cap)
+
+    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), col_caps=(160, 256, 512), reps=3):
+        """Times each candidate launch shape on the engine's current state with HIP events on its stream
+        (the particle substep for every (block, SIPs per thread), the collision step for every main-pass
+        capacity), keeps the fastest and restores the state.  Results do not depend on the launch shape.
+        Returns {"particles": {(block, spt): ms}, "collision": {cap: ms}, "chosen": (block, spt, cap)}."""
+        snap = self.snapshot()
+        e = self.e
+        stream = torch.cuda.current_stream(self.device)
+
+        def timed(run):
+            self.restore(snap)
+            run()  # first launch of the candidate, untimed
+            self.restore(snap)
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record(stream)
+            for _ in range(reps):
+                run()
+            t1.record(stream)
+            t1.synchronize()
+            self.check()
+            return t0.elapsed_time(t1) / reps
+
+        res = {"particles": {}, "collision": {}}
+        cap0 = int(e.col_cap)
+        for b in blocks:
+            for spt in spts:
+                self.set_launch_shape(b, spt, cap0)
+                res["particles"][(b, spt)] = timed(lambda: self.particle_substep(e.dt_sub, e.dt_sub))
+        best_b, best_s = min(res["particles"], key=res["particles"].get)
+        if e.act_collisions:
+            for cap in col_caps:
+                self.set_launch_shape(best_b, best_s, cap)
+                res["collision"][cap] = timed(self.collision_step)
+            best_c = min(res["collision"], key=res["collision"].get)
+        else:
+            best_c = 0
+        self.set_launch_shape(best_b, best_s, best_c)
+        self.restore(snap)
+        res["chosen"] = (best_b, best_s, best_c)
+        return res
+
     # ------------------------------------------------------- checkpoints
     _SNAP_SIP = ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "seg")
     _SNAP_GRID = ("away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "no_cols", "water_removed",

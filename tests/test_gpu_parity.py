@@ -228,3 +228,32 @@ def test_one_crowded_cell_takes_the_overflow_passes(gpu):
     s, m = eng.export_grid(0)
     _assert_same(s[:8], scal[:8], "grid_scalar_fields")
     _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+
+
+def test_launch_shapes_and_autotune_leave_results_unchanged(gpu):
+    """The particle kernel's block size and SIPs per thread and the collision main-pass capacity change
+    only how the work is launched: the state after two advection steps is the same bit for bit, and
+    Engine.autotune restores the state it measured on."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case()
+    parts = _prep(grid, parts)
+    ref = None
+    for shape in [(0, 0, 0), (64, 2, 512), (512, 1, 160), (192, 2, 256)]:
+        eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
+        eng.set_launch_shape(*shape)
+        eng.adv_steps(2)
+        got = eng.export_particles(0) + tuple(eng.export_grid(0))
+        if ref is None:
+            ref = got
+            continue
+        for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active", "scal", "mat"], got, ref):
+            _assert_same(a, b, f"{nm} with launch shape {shape}")
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
+    eng.adv_steps(1)
+    before = {k: v.cpu().numpy() for k, v in eng.snapshot()["buf"].items()}
+    res = eng.autotune(reps=1)
+    after = {k: v.cpu().numpy() for k, v in eng.snapshot()["buf"].items()}
+    for k in before:
+        _assert_same(after[k], before[k], f"{k} after autotune")
+    assert res["chosen"][0] in (64, 128, 256, 512) and res["chosen"][1] in (1, 2)
