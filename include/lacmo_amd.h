@@ -140,6 +140,9 @@ typedef struct LmEngine {
    * member capacity of the main collision pass (160, 256 or 512; larger cells take the overflow pass).
    */
   int32_t part_block, part_spt, col_cap, pad3;
+  /* largest entry of the E_col table (0: unknown); lets the collision kernel reject most pairs before it
+     reads their table entry */
+  double E_max;
 } LmEngine;
 
 /*

@@ -340,7 +340,7 @@ struct KView {
   double g_set, dt_col_over_dV;
   const double *E;
   int nb, e_const_mode;
-  double E_const, lnR0, lnf;
+  double E_const, lnR0, lnf, E_max;
   unsigned long long seed;
   const uint32_t *col_step_dev;
   double *x, *z, *u, *w, *m_w, *m_s, *xi;
@@ -392,6 +392,7 @@ static KView make_view(const LmEngine *e) {
   v.E = e->E;
   v.nb = e->nb;
   v.e_const_mode = e->e_const_mode;
+  v.E_max = e->E_max;
   v.E_const = e->E_const;
   v.lnR0 = e->lnR0;
   v.lnf = e->lnf;
@@ -785,16 +786,26 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
         const double du = u_i - sU[j], dw = w_i - sW[j];
         const double d2 = du * du + dw * dw;
         const double sR2 = R_i + sR[j];
-        const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
-        const double Pl = ((xi_max * dtdV) * (((pi * sR2) * sR2) * Ec)) * 1.0E-12;
         double thr;
         if (INJ)
           thr = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 0.0;
         else
           thr = (double)(lm_word(blk, (uint32_t)t) >> 5) * (1.0 / 134217728.0);
         const double thr2 = (thr * thr) * (1.0 - 0x1p-40);
-        const bool big = Pl > 1.0E-140;
-        const bool no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 0.0));
+        const double geo = (xi_max * dtdV) * ((pi * sR2) * sR2);
+        /* first with the table's largest entry: every rounding step is monotone in E, so a pair rejected
+           with E_max is rejected with its own entry; only the others read the table */
+        bool no = false;
+        if (TAB && v.E_max > 0.0) {
+          const double Pm = (geo * v.E_max) * 1.0E-12;
+          no = (Pm > 1.0E-140) & ((Pm * Pm) * d2 < thr2);
+        }
+        if (!no) {
+          const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
+          const double Pl = (geo * Ec) * 1.0E-12;
+          const bool big = Pl > 1.0E-140;
+          no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 0.0));
+        }
         maybe |= (unsigned)(valid & !no) << t;
       }
       if (maybe) {
@@ -2174,7 +2185,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, col_step_dev), (int64_t)offsetof(LmEngine, ev_mode),
       (int64_t)offsetof(LmEngine, ev),        (int64_t)offsetof(LmEngine, run_base),
       (int64_t)offsetof(LmEngine, tape_part), (int64_t)offsetof(LmEngine, ccd),
-      (int64_t)offsetof(LmEngine, part_block),
+      (int64_t)offsetof(LmEngine, part_block), (int64_t)offsetof(LmEngine, E_max),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];

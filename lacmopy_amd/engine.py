@@ -141,10 +141,13 @@ class Engine:
             e.e_const_mode, e.E_const, e.nb = 1, float(E_col_grid), 0
             self.E_tab = None
             e.E = None
+            e.E_max = 0.0
         else:
             E = np.ascontiguousarray(E_col_grid, dtype=np.float64)
             self.E_tab = torch.from_numpy(E).to(self.device)
             e.E = self.E_tab.data_ptr()
+            # bound for the collision kernel's first filter (disabled for non-finite or negative tables)
+            e.E_max = float(E.max()) if np.isfinite(E).all() and E.min() >= 0.0 else 0.0
             e.nb = E.shape[0]
             e.e_const_mode = 0
             e.lnR0 = float(R_kernel_low_log)
