@@ -146,6 +146,68 @@ typedef struct LmEngine {
 } LmEngine;
 
 /*
+ * Parameters of the initial SIP ensemble and saturation adjustment (config.py:84-132 as
+ * init.py:43-206 derives them).  Up to LM_INIT_MAX_MODES lognormal dry-mass modes.
+ */
+#define LM_INIT_MAX_MODES 4
+typedef struct LmInitPar {
+  int32_t n_modes, solute, weak_threshold, newton_iterations;
+  int32_t iter_cnt_limit, nx, n_runs, pad;
+  int32_t l_max[LM_INIT_MAX_MODES];     /* bins per mode: ceil(kappa log10(m_high_over_m_low)) */
+  double mu_m_log[LM_INIT_MAX_MODES];   /* ln of the geometric mean dry mass (1e-18 kg) */
+  double sigma_m_log[LM_INIT_MAX_MODES];
+  double kappa[LM_INIT_MAX_MODES];      /* SinSIP bins per decade */
+  double r_critmin[LM_INIT_MAX_MODES];  /* smallest dry radius (um) */
+  double no_rpcm_0[LM_INIT_MAX_MODES];  /* ceil(V_cell DNC0) */
+  double eta, mass_density_dry, S_init_max, dt_init;
+  double p_ref, Theta_l, r_tot_0, rho_dry_0;
+  double X, dx, dy, dz;
+} LmInitPar;
+
+/* sizeof(LmInitPar), for bindings that mirror the struct */
+int64_t lm_init_par_sizeof(void);
+
+/*
+ * Per-run level state of the initialisation, LM_INIT_LV_N doubles: p_bot, p_top, T_bot, T_top,
+ * T_avg, p_avg, rho_dry_avg, S_avg, r_v_avg, r_l_avg, dry-air mass, vapour mass, liquid mass,
+ * particle mass (kg, level), rho_dry scale of the particle counts, adjustment iterations, last
+ * relative liquid change, then the real particles per cell of each mode.
+ */
+#define LM_INIT_LV_N 21
+
+/*
+ * One level of the initial state (init.py:697-740, generation_SIP_ensemble.py:330-404): every run's
+ * dry profile from its bottom pressure pbot[run] (device), the level state L (device, n_runs x
+ * LM_INIT_LV_N) and the number of SinSIPs of each (run, cell, mode) into count (device,
+ * n_runs*nx*n_modes).  Uniform draws: Philox-4x32-10 keyed by each run's seed seeds[run] (device,
+ * the reference's seed_SIP_gen of that run) instead of NumPy's generator.
+ */
+int lm_init_level_profile(const LmInitPar *q, const uint64_t *seeds, int32_t level, double z_bot, double z_top,
+                          const double *pbot, double *L, int32_t *count, void *stream);
+
+/*
+ * The level's SIPs and its saturation adjustment (init.py:741-1010): SinSIPs written at off[(run, cell,
+ * mode)] (device int64) into m_s, xi, mode, cell, run; the initial water mass in equilibrium with the
+ * level's S and T (microphysics.py:686-966) into m_w (m_p: scratch); then one workgroup per run repeats
+ * implicit Newton condensation steps (dt_init) of the level's SIPs with the level thermodynamics
+ * updated after each until the liquid water changes by less than 1e-5 relative or iter_cnt_limit
+ * steps.  cell_off (device int64, n_runs*nx+1): first SIP of each (run, cell).  Writes L, the next
+ * level's bottom pressures pbot_next and per (run, cell) r_v, r_l, e_s, S into fields (n_runs x 4 x nx).
+ */
+int lm_init_level_sips(const LmInitPar *q, const uint64_t *seeds, int32_t level, double *L, const int64_t *off,
+                       const int64_t *cell_off, int64_t n_level, double *m_s, double *xi, int32_t *mode,
+                       int32_t *cell, int32_t *run, double *m_w, double *m_p, double *dmxi, double *pbot_next,
+                       double *fields, void *stream);
+
+/*
+ * Positions uniform in each SIP's cell (init.py:322-381): x = corner_x[ci] + dx u, z = corner_z[cj] + dz u',
+ * (u, u') from seeds[run[k]] and idx[k] (the SIP's index within its run) into rel (2 x n, device).
+ */
+int lm_init_positions(int64_t n, const uint64_t *seeds, const int32_t *run, const int32_t *idx, const int32_t *ci,
+                      const int32_t *cj, const double *corner_x, const double *corner_z, double dx, double dz,
+                      double *x, double *z, double *rel, void *stream);
+
+/*
  * Adjoint variables of a window (lm_adj_*): per global SIP id and per cell,
  * plus the scratch the reverse sweep keeps them in while it reverses one
  * advection step (the step's slot order, and the slot of each global id).

@@ -71,6 +71,20 @@ class LmAdjoint(C.Structure):
                 ("sl_mw", _p), ("sl_ms", _p), ("sl_xi", _p), ("sl_of", _p)]
 
 
+class LmInitPar(C.Structure):
+    """Mirror of struct LmInitPar (include/lacmo_amd.h)."""
+
+    _fields_ = [
+        ("n_modes", _i32), ("solute", _i32), ("weak_threshold", _i32), ("newton_iterations", _i32),
+        ("iter_cnt_limit", _i32), ("nx", _i32), ("n_runs", _i32), ("pad", _i32),
+        ("l_max", _i32 * 4), ("mu_m_log", _d * 4), ("sigma_m_log", _d * 4), ("kappa", _d * 4),
+        ("r_critmin", _d * 4), ("no_rpcm_0", _d * 4),
+        ("eta", _d), ("mass_density_dry", _d), ("S_init_max", _d), ("dt_init", _d),
+        ("p_ref", _d), ("Theta_l", _d), ("r_tot_0", _d), ("rho_dry_0", _d),
+        ("X", _d), ("dx", _d), ("dy", _d), ("dz", _d),
+    ]
+
+
 LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
 
 
@@ -87,7 +101,7 @@ EXPORTED = [
     "lm_box2d_step", "lm_box_check", "lm_divergence_upwind", "lm_update_r_l",
     "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture", "lm_graph_launch", "lm_graph_destroy",
     "lm_derived_fields", "lm_ensemble_stats", "lm_build_slot_of", "lm_objective", "lm_adj_seed",
-    "lm_adj_adv_steps",
+    "lm_adj_adv_steps", "lm_init_par_sizeof", "lm_init_level_profile", "lm_init_level_sips", "lm_init_positions",
 ]
 
 _lib = None
@@ -139,6 +153,11 @@ def lib():
     L.lm_objective.argtypes = [E, _d, _p, _p]
     L.lm_adj_seed.argtypes = [E, A, _d, _d, _d, _p]
     L.lm_adj_adv_steps.argtypes = [E, A, _i32, _i32, _p]
+    Q = C.POINTER(LmInitPar)
+    L.lm_init_par_sizeof.restype = _i64
+    L.lm_init_level_profile.argtypes = [Q, _p, _i32, _d, _d, _p, _p, _p, _p]
+    L.lm_init_level_sips.argtypes = [Q, _p, _i32, _p, _p, _p, _i64] + [_p] * 10 + [_p]
+    L.lm_init_positions.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _p, _d, _d, _p, _p, _p, _p]
     L.lm_golovin_step.argtypes = [_i32, _p, _p, _p, _d, _p, C.c_uint64, C.c_uint32, _p, _p]
     L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
                                    C.c_uint64, C.c_uint32, _p, _i32, _p]
@@ -151,7 +170,8 @@ def lib():
                  "lm_prof_disable", "lm_particle_substep", "lm_grid_substep", "lm_box2d_step", "lm_box_check",
                  "lm_divergence_upwind", "lm_kernel_grid_m_step", "lm_set_col_step", "lm_graph_capture",
                  "lm_graph_launch", "lm_graph_destroy", "lm_derived_fields", "lm_ensemble_stats",
-                 "lm_build_slot_of", "lm_objective", "lm_adj_seed", "lm_adj_adv_steps"]:
+                 "lm_build_slot_of", "lm_objective", "lm_adj_seed", "lm_adj_adv_steps", "lm_init_level_profile",
+                 "lm_init_level_sips", "lm_init_positions"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

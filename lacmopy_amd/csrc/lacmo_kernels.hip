@@ -2150,6 +2150,7 @@ static int cell_capacity(const LmEngine *e) {
 }
 
 #include "lm_adjoint.h"
+#include "lm_init.h"
 
 template <int CAP>
 static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int from_ovf) {
@@ -2829,6 +2830,79 @@ int lm_adj_adv_steps(LmEngine *e, LmAdjoint *a, int32_t n_steps, int32_t col_ind
     }
     if (n > 0) hipLaunchKernelGGL(k_adj_to_gids, dim3(gn), dim3(256), 0, S(stream), n, gid_of_slot, *a);
   }
+  LAUNCH_CHECK();
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* initial state                                                            */
+/* ------------------------------------------------------------------------ */
+int64_t lm_init_par_sizeof(void) { return (int64_t)sizeof(LmInitPar); }
+
+static int check_init(const LmInitPar *q) {
+  if (!q || q->n_modes < 1 || q->n_modes > LM_INIT_MAX_MODES || q->n_runs < 1 || q->nx < 1 ||
+      q->nx > SATADJ_MAX_NX || q->solute < 0 || q->solute > 1 || q->iter_cnt_limit < 0 ||
+      q->iter_cnt_limit > 10000000 || q->newton_iterations < 1 || q->newton_iterations > 100)
+    return set_err(LM_ERR_ARG, "bad initialisation parameters");
+  for (int m = 0; m < q->n_modes; ++m)
+    if (q->l_max[m] < 1 || q->l_max[m] > 4096 || !(q->kappa[m] > 0.0))
+      return set_err(LM_ERR_ARG, "bad SinSIP bins");
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
+  return 0;
+}
+
+int lm_init_level_profile(const LmInitPar *q, const uint64_t *seeds, int32_t level, double z_bot, double z_top,
+                          const double *pbot, double *L, int32_t *count, void *stream) {
+  int rc = check_init(q);
+  if (rc) return rc;
+  if (!seeds || !pbot || !L || !count) return set_err(LM_ERR_ARG, "initialisation buffers missing");
+  if (level < 0) return set_err(LM_ERR_ARG, "negative level");
+  hipLaunchKernelGGL(k_init_profile, dim3((q->n_runs + 63) / 64), dim3(64), 0, S(stream), *q, z_bot, z_top, pbot, L);
+  const int nt = q->n_runs * q->nx * q->n_modes;
+  hipLaunchKernelGGL(k_sinsip, dim3((nt + 127) / 128), dim3(128), 0, S(stream), *q, seeds, level, L,
+                     (const long long *)nullptr, count, nullptr, nullptr, nullptr, nullptr, nullptr);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_init_level_sips(const LmInitPar *q, const uint64_t *seeds, int32_t level, double *L, const int64_t *off,
+                       const int64_t *cell_off, int64_t n_level, double *m_s, double *xi, int32_t *mode,
+                       int32_t *cell, int32_t *run, double *m_w, double *m_p, double *dmxi, double *pbot_next,
+                       double *fields, void *stream) {
+  int rc = check_init(q);
+  if (rc) return rc;
+  if (!seeds || !L || !off || !cell_off || !pbot_next || !fields ||
+      (n_level > 0 && (!m_s || !xi || !mode || !cell || !run || !m_w || !m_p || !dmxi)))
+    return set_err(LM_ERR_ARG, "initialisation buffers missing");
+  if (level < 0 || n_level < 0) return set_err(LM_ERR_ARG, "negative level or SIP count");
+  const int nt = q->n_runs * q->nx * q->n_modes;
+  hipLaunchKernelGGL(k_sinsip, dim3((nt + 127) / 128), dim3(128), 0, S(stream), *q, seeds, level, L,
+                     (const long long *)off, nullptr, m_s, xi, mode, cell, run);
+  if (n_level > 0) {
+    const dim3 g((unsigned)((n_level + 127) / 128));
+    if (q->solute == 0)
+      hipLaunchKernelGGL(k_init_mass<0>, g, dim3(128), 0, S(stream), (long long)n_level, run, L, m_s, m_w, m_p);
+    else
+      hipLaunchKernelGGL(k_init_mass<1>, g, dim3(128), 0, S(stream), (long long)n_level, run, L, m_s, m_w, m_p);
+  }
+  if (q->solute == 0)
+    hipLaunchKernelGGL(k_satadj<0>, dim3(q->n_runs), dim3(SATADJ_BLOCK), 0, S(stream), *q,
+                       (const long long *)cell_off, cell, m_s, xi, m_p, m_w, dmxi, L, pbot_next, fields);
+  else
+    hipLaunchKernelGGL(k_satadj<1>, dim3(q->n_runs), dim3(SATADJ_BLOCK), 0, S(stream), *q,
+                       (const long long *)cell_off, cell, m_s, xi, m_p, m_w, dmxi, L, pbot_next, fields);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_init_positions(int64_t n, const uint64_t *seeds, const int32_t *run, const int32_t *idx, const int32_t *ci,
+                      const int32_t *cj, const double *corner_x, const double *corner_z, double dx, double dz,
+                      double *x, double *z, double *rel, void *stream) {
+  if (n <= 0) return 0;
+  if (!seeds || !run || !idx || !ci || !cj || !corner_x || !corner_z || !x || !z || !rel)
+    return set_err(LM_ERR_ARG, "position buffers missing");
+  hipLaunchKernelGGL(k_init_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), (long long)n,
+                     seeds, run, idx, ci, cj, corner_x, corner_z, dx, dz, x, z, rel);
   LAUNCH_CHECK();
   return 0;
 }

@@ -73,6 +73,21 @@ LMP_FN lm_u32x4 lm_pair_block(uint64_t seed, uint32_t run, uint32_t step, uint32
   return lm_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
 }
 
+/*
+ * Uniform [0, 1) with 53 bits for the initial SIP ensemble, keyed by the run's own seed: counter (a, b, level,
+ * 0x40000000 | stream << 24), so these draws never meet the collision draws (whose last counter word is below
+ * 2^31 with bit 30 clear for runs below 2^30) even under the same key.
+ */
+LMP_FN double lm_init_uniform(uint64_t seed, uint32_t level, uint32_t a, uint32_t b, uint32_t stream) {
+  lm_u32x4 c;
+  c.v[0] = a;
+  c.v[1] = b;
+  c.v[2] = level;
+  c.v[3] = 0x40000000u | ((stream & 0x3Fu) << 24);
+  const lm_u32x4 o = lm_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return lm_res53(o.v[0], o.v[1]);
+}
+
 LMP_FN uint32_t lm_word(lm_u32x4 o, uint32_t k) {
   return k == 0 ? o.v[0] : (k == 1 ? o.v[1] : (k == 2 ? o.v[2] : o.v[3]));
 }

@@ -1,0 +1,235 @@
+"""Initial state of the 2-D kinematic case on the GPU (init.py:439-1236 initialize_grid_and_particles_SinSIP).
+
+The reference builds the state level by level in Python: the dry hydrostatic profile of the level,
+SinSIP dry masses and multiplicities in every cell (generation_SIP_ensemble.py:147-404), each
+particle's initial water from the Köhler curve (microphysics.py:686-966, with scipy's fminbound and
+brentq), then a saturation adjustment that repeats implicit Newton condensation steps of all the
+level's particles with the level thermodynamics updated after each, until the liquid water stops
+changing; the adjusted pressure at the top of a level is the bottom pressure of the next.  Here
+every step of a level is a HIP kernel (lm_init_level_profile, lm_init_level_sips) over all runs of a
+batch at once, one workgroup per run for the adjustment, and the levels follow each other on the
+stream.  Differences from the reference:
+
+  * the uniform draws are Philox-4x32-10 keyed by each run's seed_SIP_gen (counter: level, bin,
+    cell and mode; positions: the SIP's index) instead of NumPy's MT19937 stream;
+  * the maximum of the Köhler curve comes from 100 golden-section steps and its root from bisection
+    to the last bit, where the reference calls fminbound (xtol 1e-12) and brentq (xtol 1e-15);
+  * the particle and cell sums of a level run in particle order instead of NumPy's pairwise order.
+"""
+import collections
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import torch
+
+from . import file_handling as fh
+from ._native import LmInitPar, check, lib
+from .grid import Grid
+from .setup_case import C_PD, R_D, interpolate_flow, kinematic_flow
+
+LV_N = 21  # LM_INIT_LV_N
+(LV_PBOT, LV_PTOP, LV_TBOT, LV_TTOP, LV_TAVG, LV_PAVG, LV_RHO, LV_SAVG, LV_RV, LV_RL, LV_MAIR, LV_MVAP, LV_MLIQ,
+ LV_MPART, LV_SCALE, LV_ITER, LV_DMLREL, LV_NRPCM) = range(18)
+MASS_DENSITY_DRY = {"AS": 1774.0, "NaCl": 2163.0}  # constants.py:51-53
+
+InitialState = collections.namedtuple(
+    "InitialState", "grid pos cells vel m_w m_s xi active_ids modes no_rpcm_scale_factors iterations rel_pos")
+
+
+def _p(t):
+    """Device address of a tensor that the caller keeps alive until the work using it is enqueued (a freed
+    temporary's block goes back to torch's allocator at once and may back the next tensor)."""
+    assert t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def init_parameters(config, grid):
+    """LmInitPar from the reference's config keys (config.py:84-132), derived as init.py:92-135 and
+    :596-680 derive them: the modes with no_spcm > 0; mass parameters from the radius parameters of those
+    modes (mu_R converted with the NaCl dry density whatever the solute, init.py:130-131); SinSIP bins per
+    decade kappa = max(ceil(no_spcm / 20 * c) / 10, 0.1) with c = (33, 25) for two modes and 28 otherwise;
+    real particles per cell ceil(V DNC0).  With one mode every per-mode value is that mode's; with more,
+    kappa, r_critmin and DNC0 are indexed by the mode's position in the full config arrays, as the
+    reference indexes them (init.py:612-619, 660, 779-786)."""
+    no_spcm = np.atleast_1d(np.asarray(config["no_spcm"]))
+    idx = np.nonzero(no_spcm)[0]
+    n_modes = len(idx)
+    if not 1 <= n_modes <= 4:
+        raise ValueError("one to four aerosol modes with no_spcm > 0")
+    if config.get("dist", "lognormal") != "lognormal":
+        raise ValueError("only the lognormal dry distribution is defined (as in the reference)")
+    sigma_R = np.atleast_1d(np.asarray(config["sigma_R"], dtype=np.float64))[idx]
+    mu_R = np.atleast_1d(np.asarray(config["mu_R"], dtype=np.float64))[idx]
+    sigma_m_log = 3.0 * np.log(sigma_R)
+    mu_m_log = np.log(4.0 * math.pi / 3.0 * 2163.0 * mu_R * mu_R * mu_R)
+    r_critmin = np.atleast_1d(np.asarray(config["r_critmin"], dtype=np.float64))
+    DNC0 = np.atleast_1d(np.asarray(config["DNC0"], dtype=np.float64))
+    spcm = no_spcm.astype(np.float64)
+    if n_modes == 1:
+        kappa = np.array([max(math.ceil(spcm[idx[0]] / 20 * 28) * 0.1, 0.1)])
+        r_critmin, DNC0 = r_critmin[idx], DNC0[idx]
+    elif n_modes == 2:
+        kappa = np.maximum(np.ceil(spcm / 20 * np.array([33, 25])) * 0.1, 0.1)
+    else:
+        kappa = np.maximum(np.ceil(spcm / 20 * 28) * 0.1, 0.1)
+    l_max = [int(math.ceil(kappa[m] * np.log10(config["m_high_over_m_low"]))) for m in range(n_modes)]
+    no_rpcm_0 = np.ceil(grid.volume_cell * DNC0).astype(np.int64)
+    q = LmInitPar()
+    q.n_modes = n_modes
+    q.solute = {"AS": 0, "NaCl": 1}[config["solute_type"]]
+    q.weak_threshold = int(config.get("eta_threshold", "fix") == "weak")
+    q.newton_iterations = int(config["Newton_iterations"])
+    q.iter_cnt_limit = int(config["iter_cnt_limit"])
+    q.nx = int(grid.no_cells[0])
+    for m in range(n_modes):
+        q.l_max[m] = l_max[m]
+        q.mu_m_log[m] = float(mu_m_log[m])
+        q.sigma_m_log[m] = float(sigma_m_log[m])
+        q.kappa[m] = float(kappa[m])
+        q.r_critmin[m] = float(r_critmin[m])
+        q.no_rpcm_0[m] = float(no_rpcm_0[m])
+    q.eta = float(config["eta"])
+    q.mass_density_dry = MASS_DENSITY_DRY[config["solute_type"]]
+    q.S_init_max = float(config["S_init_max"])
+    q.dt_init = float(config["dt_init"])
+    q.p_ref = float(config["p_ref"])
+    q.Theta_l = float(config["Theta_l"])
+    q.r_tot_0 = float(config["r_tot_0"])
+    q.rho_dry_0 = float(config["p_0"]) / (R_D * 293.0)
+    q.X = float(grid.sizes[0])
+    q.dx, q.dz = float(grid.steps[0]), float(grid.steps[1])
+    q.dy = float(grid.step_y)
+    return q
+
+
+def initialize_batch(config, seeds, device="cuda"):
+    """Initial states of len(seeds) runs of the case, one per SIP-generation seed, computed together on the GPU.
+
+    Returns one InitialState per seed: the reference's (grid, pos, cells, vel, m_w, m_s, xi, active_ids) of
+    init.py:1236, the SIP modes and per-level particle-count scale factors it saves (init.py:1194-1196), the
+    adjustment iterations of each level and the positions relative to the cell (2, N)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("lacmopy_amd.initialize needs a GPU (HIP device); there is no CPU fallback")
+    L_ = lib()
+    dev = torch.device(device)
+    grid0 = Grid(config["grid_ranges"], config["grid_steps"], config["dy"])
+    nx, nz = int(grid0.no_cells[0]), int(grid0.no_cells[1])
+    q = init_parameters(config, grid0)
+    R = len(seeds)
+    if R < 1:
+        raise ValueError("at least one seed")
+    q.n_runs = R
+    nm = q.n_modes
+    levels_z = grid0.corners[1][0]
+    f64 = dict(dtype=torch.float64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    i64 = dict(dtype=torch.int64, device=dev)
+    stream = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    seeds_h = np.array([int(s) & 0xFFFFFFFFFFFFFFFF for s in seeds], dtype=np.uint64)
+    seeds_d = torch.as_tensor(seeds_h.view(np.int64), **i64)
+    pbot = torch.full((R,), float(config["p_0"]), **f64)
+    pnext = torch.empty(R, **f64)
+    L = torch.zeros((R, LV_N), **f64)
+    count = torch.zeros(R * nx * nm, **i32)
+    fields = torch.zeros((R, 4, nx), **f64)
+    lv_state, lv_fields, lv_run_off, lv_bufs = [], [], [], []
+    for j in range(nz):
+        check(L_.lm_init_level_profile(C.byref(q), _p(seeds_d), j, float(levels_z[j]), float(levels_z[j + 1]),
+                                       _p(pbot), _p(L), _p(count), stream))
+        cnt = count.cpu().numpy().astype(np.int64)  # (run, cell, mode)
+        if cnt.shape != (R * nx * nm,) or cnt.min() < 0 or cnt.max() > max(q.l_max[m] for m in range(nm)):
+            raise RuntimeError(f"level {j}: SIP counts out of range")
+        off = np.concatenate([[0], np.cumsum(cnt)])
+        n_level = int(off[-1])
+        cell_off = np.ascontiguousarray(off[0::nm])  # first SIP of each (run, cell), and the level's end
+        n = max(n_level, 1)
+        b = {k: torch.empty(n, **f64) for k in ("m_s", "xi", "m_w", "m_p", "dmxi")}
+        b.update({k: torch.empty(n, **i32) for k in ("mode", "cell", "run")})
+        off_d = torch.as_tensor(np.ascontiguousarray(off[:-1]), **i64)
+        cell_off_d = torch.as_tensor(cell_off, **i64)
+        assert off_d.numel() == R * nx * nm and cell_off_d.numel() == R * nx + 1 and int(cell_off[-1]) == n_level
+        check(L_.lm_init_level_sips(C.byref(q), _p(seeds_d), j, _p(L), _p(off_d), _p(cell_off_d), n_level,
+                                    _p(b["m_s"]), _p(b["xi"]), _p(b["mode"]), _p(b["cell"]), _p(b["run"]),
+                                    _p(b["m_w"]), _p(b["m_p"]), _p(b["dmxi"]), _p(pnext), _p(fields), stream))
+        lv_run_off.append(cell_off[0::nx])
+        lv_bufs.append({k: b[k][:n_level] for k in ("m_s", "xi", "m_w", "mode", "cell")})
+        lv_state.append(L.clone())
+        lv_fields.append(fields.clone())
+        pbot, pnext = pnext, pbot
+    # level-major buffers -> run-major (level, cell, mode, bin within each run), as init.py:1040-1045 concatenates
+    base = np.concatenate([[0], np.cumsum([r_off[-1] for r_off in lv_run_off])])
+    order, run_n = [], []
+    for r in range(R):
+        parts = [np.arange(base[j] + lv_run_off[j][r], base[j] + lv_run_off[j][r + 1]) for j in range(nz)]
+        order.extend(parts)
+        run_n.append(sum(len(p) for p in parts))
+    order = torch.as_tensor(np.concatenate(order), **i64)
+    cat = {k: torch.cat([bf[k] for bf in lv_bufs])[order] for k in lv_bufs[0]}
+    cj = torch.cat([torch.full((bf["m_s"].numel(),), j, **i32) for j, bf in enumerate(lv_bufs)])[order]
+    N = int(order.numel())
+    run = torch.as_tensor(np.repeat(np.arange(R), run_n), **i32)
+    run_start = np.concatenate([[0], np.cumsum(run_n)])
+    idx = torch.arange(N, **i32) - torch.as_tensor(run_start[:-1], **i32)[run.long()]
+    x, z = torch.empty(N, **f64), torch.empty(N, **f64)
+    rel = torch.empty(2 * max(N, 1), **f64)
+    cx = torch.as_tensor(grid0.corners[0][:, 0].copy(), **f64)
+    cz = torch.as_tensor(levels_z.copy(), **f64)
+    assert all(t.numel() == N for t in (run, idx, cat["cell"], cj, x, z)) and cx.numel() == nx + 1
+    check(L_.lm_init_positions(N, _p(seeds_d), _p(run), _p(idx), _p(cat["cell"]), _p(cj), _p(cx), _p(cz),
+                               float(grid0.steps[0]), float(grid0.steps[1]), _p(x), _p(z), _p(rel), stream))
+    torch.cuda.synchronize(dev)
+    host = {k: v.cpu().numpy() for k, v in cat.items()}
+    xh, zh, cjh = x.cpu().numpy(), z.cpu().numpy(), cj.cpu().numpy()
+    relh = rel[:2 * N].cpu().numpy().reshape(2, N)
+    Ls = np.stack([s.cpu().numpy() for s in lv_state], axis=1)      # (run, level, LV_N)
+    Fs = np.stack([f.cpu().numpy() for f in lv_fields], axis=-1)    # (run, 4, nx, level)
+    out = []
+    for r in range(R):
+        a, e = run_start[r], run_start[r + 1]
+        cells = np.array([host["cell"][a:e], cjh[a:e]], dtype=np.int64)
+        pos = np.array([xh[a:e], zh[a:e]])
+        grid = _grid_of_run(config, Ls[r], Fs[r])
+        vel = np.array(interpolate_flow(grid, cells, pos, relh[:, a:e]))
+        out.append(InitialState(grid, pos, cells, vel, host["m_w"][a:e].copy(), host["m_s"][a:e].copy(),
+                                host["xi"][a:e].copy(), np.full(e - a, True), host["mode"][a:e].astype(np.int64),
+                                Ls[r][:, LV_SCALE].copy(), Ls[r][:, LV_ITER].astype(np.int64),
+                                relh[:, a:e].copy()))
+    return out
+
+
+def _grid_of_run(config, Lr, F):
+    """The grid of one run after the adjustment of all levels (init.py:1068-1182): level-centre p, T and rho_dry
+    in every column, Theta from them, e_s(T), the last adjustment iteration's r_v, r_l and S per cell, the
+    kinematic flow with the velocity from the level densities, material properties."""
+    grid = Grid(config["grid_ranges"], config["grid_steps"], config["dy"])
+    nx = int(grid.no_cells[0])
+    rho_c = Lr[:, LV_RHO].copy()
+    grid.pressure = np.repeat(Lr[None, :, LV_PAVG], nx, axis=0)
+    grid.temperature = np.repeat(Lr[None, :, LV_TAVG], nx, axis=0)
+    grid.mass_density_air_dry = np.repeat(rho_c[None, :], nx, axis=0)
+    p_dry = grid.mass_density_air_dry * R_D * grid.temperature
+    grid.potential_temperature = grid.temperature * (1.0E5 / p_dry) ** (R_D / C_PD)
+    grid.saturation_pressure = 2.53E11 * np.exp(-5420.0 / grid.temperature)
+    grid.mixing_ratio_water_vapor = F[0].copy()
+    grid.mixing_ratio_water_liquid = F[1].copy()
+    grid.saturation = F[3].copy()
+    kinematic_flow(grid, rho_c, 0.6)
+    grid.update_material_properties()
+    grid.rho_dry_inv = np.ones_like(grid.mass_density_air_dry) / grid.mass_density_air_dry
+    grid.mass_dry_inv = (1.0 / grid.volume_cell) * grid.rho_dry_inv
+    return grid
+
+
+def initialize_grid_and_particles_SinSIP(config):
+    """init.py:439-1236: the initial grid and SIPs of seed config['seed_SIP_gen'], computed on the GPU, and the
+    reference's files of t = 0 (grid and particles, modes_0.npy, no_rpcm_scale_factors_lvl_wise.npy) in
+    config['paths']['grid'].  Returns (grid, pos, cells, vel, m_w, m_s, xi, active_ids)."""
+    s = initialize_batch(config, [config["seed_SIP_gen"]])[0]
+    path = config["paths"]["grid"]
+    os.makedirs(path, exist_ok=True)
+    fh.save_grid_and_particles_full(0, s.grid, s.pos, s.cells, s.vel, s.m_w, s.m_s, s.xi, s.active_ids, path)
+    np.save(path + "modes_0", s.modes)
+    np.save(path + "no_rpcm_scale_factors_lvl_wise", s.no_rpcm_scale_factors)
+    return s.grid, s.pos, s.cells, s.vel, s.m_w, s.m_s, s.xi, s.active_ids

@@ -3,8 +3,9 @@
 Host-side set-up (not part of the step).  The thermodynamic profile,
 the dry-air mass-flux field and the velocity staggering follow the
 reference's initialisation (init.py:384-437, 239-273, 1083-1182); the SIP
-ensemble is a seeded Poisson sampler instead of the reference's SinSIP +
-saturation adjustment (init.py:439-1236, out of scope here):
+ensemble is a seeded Poisson sampler, a quick stand-in for the reference's
+SinSIP + saturation adjustment (init.py:439-1236, which
+lacmopy_amd.initialize computes on the GPU) with collisions firing at once:
 
   * per-cell SIP count ~ Poisson(spc), positions uniform in the cell;
   * dry radii from the bimodal lognormal of config.py:106-114
@@ -105,19 +106,7 @@ def make_case(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, seed=1001, p_0=101500.0
     grid.rho_dry_inv = np.ones_like(rho_d) / rho_d
     grid.mass_dry_inv = (1.0 / grid.volume_cell) * grid.rho_dry_inv
 
-    # kinematic mass flux and velocity (init.py:239-273, 1146-1182)
-    X, Z = grid.sizes
-    j_max = j_max_ref * math.sqrt(X * X + Z * Z) / math.sqrt(2.0 * 1500.0 * 1500.0)
-    k_x, k_z = 2.0 * np.pi / X, np.pi / Z
-    ux, uz = grid.corners[0], grid.corners[1] + 0.5 * dz
-    wx, wz = grid.corners[0] + 0.5 * dx, grid.corners[1]
-    j_x = j_max * (X / Z) * np.cos(k_x * ux) * np.cos(k_z * uz)
-    j_z = 2 * j_max * np.sin(k_x * wx) * np.sin(k_z * wz)
-    grid.mass_flux_air_dry = np.array([j_x, j_z])
-    rho_bot = 0.5 * (rho_c[0:-1] + rho_c[1:])
-    rho_bot = np.insert(rho_bot, 0, 0.5 * (3.0 * rho_c[0] - rho_c[1]))
-    rho_bot = np.append(rho_bot, 0.5 * (3.0 * rho_c[-1] - rho_c[-2]))
-    grid.velocity = np.array([j_x / rho_bot, j_z / np.append(rho_c, rho_c[-1])])
+    kinematic_flow(grid, rho_c, j_max_ref)
 
     pos = np.array([x, z])
     cells = np.array([cx, cz], dtype=np.int64)
@@ -153,13 +142,35 @@ def aerosol_sips(theta, draws):
     return xi, m_s
 
 
-def interpolate_flow(grid, cells, pos):
-    """Host interpolation of the face velocities at SIP positions (grid.py:257-323)."""
+def kinematic_flow(grid, rho_c, j_max_ref=0.6):
+    """Dry-air mass flux of the ICMW 2012 case 1 stream function and the velocity from it (init.py:239-273,
+    1146-1182; the velocity divides by the dry density at the faces built from the level centres rho_c)."""
+    dx, dz = grid.steps
+    X, Z = grid.sizes
+    j_max = j_max_ref * math.sqrt(X * X + Z * Z) / math.sqrt(2.0 * 1500.0 * 1500.0)
+    k_x, k_z = 2.0 * np.pi / X, np.pi / Z
+    ux, uz = grid.corners[0], grid.corners[1] + 0.5 * dz
+    wx, wz = grid.corners[0] + 0.5 * dx, grid.corners[1]
+    j_x = j_max * (X / Z) * np.cos(k_x * ux) * np.cos(k_z * uz)
+    j_z = 2 * j_max * np.sin(k_x * wx) * np.sin(k_z * wz)
+    grid.mass_flux_air_dry = np.array([j_x, j_z])
+    rho_bot = 0.5 * (rho_c[0:-1] + rho_c[1:])
+    rho_bot = np.insert(rho_bot, 0, 0.5 * (3.0 * rho_c[0] - rho_c[1]))
+    rho_bot = np.append(rho_bot, 0.5 * (3.0 * rho_c[-1] - rho_c[-2]))
+    grid.velocity = np.array([j_x / rho_bot, j_z / np.append(rho_c, rho_c[-1])])
+
+
+def interpolate_flow(grid, cells, pos, rel=None):
+    """Host interpolation of the face velocities at SIP positions (grid.py:257-323); rel: the positions relative
+    to the cell (2, N) where known (init.py:1228-1230 passes the uniform draws), else from pos."""
     nz = int(grid.no_cells[1])
     U, V = grid.velocity
     i, j = cells
-    a = pos[0] / grid.steps[0] - i
-    b = pos[1] / grid.steps[1] - j
+    if rel is None:
+        a = pos[0] / grid.steps[0] - i
+        b = pos[1] / grid.steps[1] - j
+    else:
+        a, b = rel
 
     def bil(ii, jj, aa, bb, f):
         return aa * (bb * f[ii + 1, jj + 1] + (1 - bb) * f[ii + 1, jj]) + \

@@ -1269,3 +1269,328 @@ EXPORT void or_lm_vec(int which, int64_t n, const double *x, const double *y, do
       out[k] = lm_pow_third(x[k]);
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* initial state: init.py:439-1236 (one run), generation_SIP_ensemble.py:   */
+/* 147-404, microphysics.py:531-610, 686-966, atmosphere.py:77-135           */
+/* ------------------------------------------------------------------------ */
+/*
+ * Same layout as LmInitPar of the product's C ABI (the parameters init.py derives from the config).
+ * Substitutions, shared with the product and documented there: Philox-4x32-10 uniforms keyed by the
+ * run's seed (lm_init_uniform) for np.random.rand; 100 golden-section steps for fminbound and bisection
+ * to the last bit for brentq; plain left-to-right sums for np.sum.
+ */
+typedef struct {
+  int32_t n_modes, solute, weak_threshold, newton_iterations;
+  int32_t iter_cnt_limit, nx, n_runs, pad;
+  int32_t l_max[4];
+  double mu_m_log[4], sigma_m_log[4], kappa[4], r_critmin[4], no_rpcm_0[4];
+  double eta, mass_density_dry, S_init_max, dt_init;
+  double p_ref, Theta_l, r_tot_0, rho_dry_0;
+  double X, dx, dy, dz;
+} OrInitPar;
+
+#define OR_G 9.80665
+
+/* init.py:384-437 with atmosphere.py:77-97 (kappa_moist) and :103-135 (beta) */
+static void or_profile_dry(double z, double z_0, double p_0, double p_ref, double Theta_l, double r_tot, double *T,
+                           double *p, double *rho_dry, double *S) {
+  const double p_0_over_p_ref = p_0 / p_ref;
+  const double kappa_tot = K.kappa_dry * (1 - 0.289 * r_tot);
+  const double kappa_tot_inv = 1.0 / kappa_tot;
+  const double p_0_over_p_ref_to_kappa_tot = OR_POW(p_0_over_p_ref, kappa_tot);
+  const double beta_tot =
+      OR_G * (K.kappa_dry * (1 - 0.289 * r_tot)) * (1 + r_tot) / (K.R_d * Theta_l * (1 + r_tot / K.eps_gc));
+  const double T_over_Theta_l = p_0_over_p_ref_to_kappa_tot - beta_tot * (z - z_0);
+  *T = T_over_Theta_l * Theta_l;
+  *p = OR_POW(T_over_Theta_l, kappa_tot_inv) * p_ref;
+  *rho_dry = *p * beta_tot / (T_over_Theta_l * OR_G * kappa_tot);
+  /* atmosphere.py:314-335: e = rho_v T R_v; material_properties.py:546-563 */
+  *S = *rho_dry * r_tot * *T * K.R_v / saturation_pressure_vapor_liquid(*T);
+}
+
+/* microphysics.py:400-446 (Kelvin term from the mass fraction), :570-610 (S_eq of AS / NaCl) */
+static double or_S_eq_mf(int solute, double w_s, double T_p, double m_s) {
+  const double rho_p = solute == 0 ? density_AS_solution(w_s, T_p) : density_NaCl_solution(w_s, T_p);
+  const double sigma_p = solute == 0 ? surface_tension_AS(w_s, T_p) : surface_tension_NaCl(w_s, T_p);
+  const double a_w = solute == 0 ? water_activity_AS(w_s) : water_activity_NaCl(w_s);
+  const double volume_to_radius = OR_POW(K.four_pi_over_three_inv, K.one_third); /* constants.py:32 */
+  const double kelvin = OR_EXP(2.0 * sigma_p * 1.0E6 * OR_POW(w_s / m_s, K.one_third) /
+                               (K.R_v * T_p * OR_POW(rho_p, 0.66666667) * volume_to_radius));
+  return a_w * kelvin;
+}
+
+/*
+ * microphysics.py:866-966 (AS, w_s_max 0.78) and :686-785 (NaCl, w_s_max 0.45): the initial solute mass
+ * fraction.  The maximum of S_eq on [1e-8, w_s_max] by golden-section search (100 steps, the interval
+ * shrinks below one ulp long before), w_act = 1.000001 x its midpoint; the root of S_eq - S_amb on
+ * [w_act, w_s_max] by bisection until the midpoint is an end point.
+ */
+EXPORT double or_initial_mass_fraction(int solute, double m_s, double S_amb, double T_amb) {
+  or_init_consts();
+  const double w_s_max = solute == 0 ? 0.78 : 0.45;
+  if (S_amb <= or_S_eq_mf(solute, w_s_max, T_amb, m_s)) return w_s_max;
+  const double inv_phi = 0.6180339887498949; /* (sqrt(5) - 1) / 2 */
+  double lo = 1.0E-8, hi = w_s_max;
+  double x1 = hi - inv_phi * (hi - lo), x2 = lo + inv_phi * (hi - lo);
+  double f1 = or_S_eq_mf(solute, x1, T_amb, m_s), f2 = or_S_eq_mf(solute, x2, T_amb, m_s);
+  for (int step = 0; step < 100; ++step) {
+    if (f1 >= f2) {
+      hi = x2;
+      x2 = x1;
+      f2 = f1;
+      x1 = hi - inv_phi * (hi - lo);
+      f1 = or_S_eq_mf(solute, x1, T_amb, m_s);
+    } else {
+      lo = x1;
+      x1 = x2;
+      f1 = f2;
+      x2 = lo + inv_phi * (hi - lo);
+      f2 = or_S_eq_mf(solute, x2, T_amb, m_s);
+    }
+  }
+  const double w_act = 0.5 * (lo + hi) * 1.000001;
+  if (S_amb > or_S_eq_mf(solute, w_act, T_amb, m_s)) return w_act;
+  double a = w_act, b = w_s_max;
+  for (int step = 0; step < 200; ++step) {
+    const double c = 0.5 * (a + b);
+    if (c <= a || c >= b) break;
+    if (or_S_eq_mf(solute, c, T_amb, m_s) - S_amb > 0.0)
+      a = c;
+    else
+      b = c;
+  }
+  const double c = 0.5 * (a + b);
+  return c <= a ? a : c;
+}
+
+EXPORT double or_S_eq_mass_fraction(int solute, double w_s, double T_p, double m_s) {
+  or_init_consts();
+  return or_S_eq_mf(solute, w_s, T_p, m_s);
+}
+
+/*
+ * generation_SIP_ensemble.py:147-244 (gen_mass_ensemble_weights_SinSIP_lognormal) for one cell and mode,
+ * the draw of bin b being lm_init_uniform(seed, level, b, cell << 4 | mode, 0) (1 for the weak threshold's
+ * second draw).  Writes the kept masses and weights, returns their number.
+ */
+EXPORT int or_sinsip_cell(const OrInitPar *q, uint64_t seed, uint32_t level, uint32_t cell, int mode, double *masses,
+                          double *weights) {
+  or_init_consts();
+  const double r = q->r_critmin[mode];
+  const double m_low = K.four_pi_over_three * q->mass_density_dry * r * r * r; /* microphysics.py:61-80 */
+  const double bin_factor = OR_POW(10.0, 1.0 / q->kappa[mode]);
+  const int l_max = q->l_max[mode];
+  const double two_pi_sqrt = 2.5066282746310002; /* distributions.py:69 */
+  double wbuf[4096], mbuf[4096];
+  double m_left = m_low;
+  for (int bin = 0; bin < l_max; ++bin) {
+    const double m_right = m_left * bin_factor;
+    const double bin_width = m_right - m_left;
+    const double mu = m_left + lm_init_uniform(seed, level, (uint32_t)bin, (cell << 4) | (uint32_t)mode, 0) * bin_width;
+    /* distributions.py:75-96 */
+    const double t = (OR_LOG(mu) - q->mu_m_log[mode]) / q->sigma_m_log[mode];
+    wbuf[bin] = OR_EXP(-0.5 * (t * t)) / (mu * two_pi_sqrt * q->sigma_m_log[mode]) * bin_width;
+    mbuf[bin] = mu;
+    m_left = m_right;
+  }
+  double weight_max = wbuf[0];
+  for (int bin = 1; bin < l_max; ++bin)
+    if (wbuf[bin] > weight_max) weight_max = wbuf[bin];
+  const double weight_critmin = weight_max * q->eta;
+  int n = 0;
+  for (int bin = 0; bin < l_max; ++bin) {
+    double w = wbuf[bin];
+    int valid = 1;
+    if (w < weight_critmin) {
+      valid = 0;
+      if (q->weak_threshold &&
+          lm_init_uniform(seed, level, (uint32_t)bin, (cell << 4) | (uint32_t)mode, 1) < w / weight_critmin) {
+        w = weight_critmin;
+        valid = 1;
+      }
+    }
+    if (valid) {
+      masses[n] = mbuf[bin];
+      weights[n] = w;
+      ++n;
+    }
+  }
+  return n;
+}
+
+/*
+ * init.py:439-1236 for one run: the SIP arrays in the reference's order (level, cell, mode, bin) and
+ * positions (init.py:322-381, draws lm_init_uniform(seed, 0, n, 0, 2 / 3) for the run's n-th SIP), the
+ * level state after the adjustment (per level 21 doubles in the product's order: p_bot, p_top, T_bot,
+ * T_top, T_avg, p_avg, rho_dry_avg, S_avg, r_v_avg, r_l_avg, m_air, m_vap, m_liq, m_part, scale factor,
+ * iterations, last relative change, real particles per cell of each of 4 modes), and per cell r_v, r_l,
+ * e_s, S (4 x nx x nz).  Returns the number of SIPs, or -1 if cap is too small.
+ */
+EXPORT int64_t or_init_SinSIP(const OrInitPar *q, uint64_t seed, int32_t nz, const double *levels_z, double p_0,
+                              const double *corner_x, int64_t cap, double *m_s, double *xi, double *m_w, int32_t *mode,
+                              int32_t *cell_x, int32_t *cell_z, double *x, double *z, double *rel, double *lv,
+                              double *fields) {
+  or_init_consts();
+  const int nx = q->nx, no_modes = q->n_modes;
+  const double r_tot = q->r_tot_0, dy = q->dy, dz = q->dz;
+  const double kappa_tot_inv = 1.0 / (K.kappa_dry * (1 - 0.289 * r_tot));
+  double *r_l_cell = fields + (size_t)nx * nz, *r_v_cell = fields, *e_s_cell = fields + (size_t)2 * nx * nz,
+         *S_cell = fields + (size_t)3 * nx * nz;
+  double p_bot = p_0;
+  int64_t n = 0;
+  double mb[4096], wb[4096];
+  int64_t *cell_start = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nx + 1));
+  if (!cell_start) return -1;
+  for (int j = 0; j < nz; ++j) {
+    const double z_bot = levels_z[j], z_top = levels_z[j + 1];
+    double T_bot, rho_dry_bot, S_bot, T_top, p_top, rho_dry_top, S_top;
+    or_profile_dry(z_bot, z_bot, p_bot, q->p_ref, q->Theta_l, r_tot, &T_bot, &p_bot, &rho_dry_bot, &S_bot);
+    or_profile_dry(z_top, z_bot, p_bot, q->p_ref, q->Theta_l, r_tot, &T_top, &p_top, &rho_dry_top, &S_top);
+    double p_avg = (p_bot + p_top) * 0.5, T_avg = (T_bot + T_top) * 0.5;
+    double rho_dry_avg = (rho_dry_bot + rho_dry_top) * 0.5, S_avg = (S_bot + S_top) * 0.5, r_v_avg = r_tot;
+    double mass_air_dry_level = q->X * dy * (p_bot - p_top) / (OR_G * (1 + r_tot));
+    double mass_water_vapor_level = r_v_avg * mass_air_dry_level;
+    double mass_water_liquid_level = 0.0, mass_particles_level = 0.0, dm_l_level = 0.0, dm_p_level = 0.0;
+    const double no_rpcm_scale_factor = rho_dry_avg / q->rho_dry_0;
+    double no_rpcm[4] = {0, 0, 0, 0};
+    for (int m = 0; m < no_modes; ++m) no_rpcm[m] = rint(q->no_rpcm_0[m] * no_rpcm_scale_factor);
+    /* generation_SIP_ensemble.py:330-404: cells of the level, modes of the cell */
+    const int64_t lvl0 = n;
+    for (int i = 0; i < nx; ++i) {
+      cell_start[i] = n;
+      for (int m = 0; m < no_modes; ++m) {
+        const int k = or_sinsip_cell(q, seed, (uint32_t)j, (uint32_t)i, m, mb, wb);
+        if (n + k > cap) {
+          free(cell_start);
+          return -1;
+        }
+        for (int b = 0; b < k; ++b) {
+          m_s[n] = mb[b];
+          xi[n] = wb[b] * no_rpcm[m];
+          mode[n] = m;
+          cell_x[n] = i;
+          cell_z[n] = j;
+          ++n;
+        }
+      }
+    }
+    cell_start[nx] = n;
+    /* init.py:787-800: initial water from the Köhler curve */
+    for (int64_t k = lvl0; k < n; ++k) {
+      const double w_s = or_initial_mass_fraction(q->solute, m_s[k], S_avg, T_avg);
+      const double m_p = m_s[k] / w_s;
+      m_w[k] = m_p - m_s[k];
+      dm_l_level += m_w[k] * xi[k];
+      dm_p_level += m_p * xi[k];
+    }
+    /* init.py:810-862 */
+    dm_l_level *= 1.0E-18;
+    dm_p_level *= 1.0E-18;
+    mass_water_liquid_level += dm_l_level;
+    mass_particles_level += dm_p_level;
+    mass_water_vapor_level -= dm_l_level;
+    double heat_capacity_level = mass_air_dry_level * K.c_pd + mass_water_vapor_level * 1906.0 +
+                                 mass_particles_level * 4183.8; /* constants.py:101-111 */
+    double dT_avg = dm_l_level * heat_of_vaporization(T_avg) / heat_capacity_level;
+    T_avg += dT_avg;
+    T_bot += dT_avg;
+    T_top += dT_avg;
+    r_v_avg = mass_water_vapor_level / mass_air_dry_level;
+    p_top = p_bot * OR_POW(T_top / T_bot, ((1 + r_tot) * OR_G * dz) /
+                                              ((1 + r_v_avg / K.eps_gc) * K.R_d * (T_bot - T_top)));
+    p_avg = 0.5 * (p_bot + p_top);
+    rho_dry_avg = (p_bot - p_top) / (dz * (1 + r_tot) * OR_G);
+    mass_air_dry_level = q->X * dy * dz * rho_dry_avg;
+    double r_l_avg = mass_water_liquid_level / mass_air_dry_level;
+    r_v_avg = r_tot - r_l_avg;
+    mass_water_vapor_level = r_v_avg * mass_air_dry_level;
+    double e_s_avg = saturation_pressure_vapor_liquid(T_avg);
+    S_avg = rho_dry_avg * r_v_avg * T_avg * K.R_v / e_s_avg;
+    for (int i = 0; i < nx; ++i) {
+      const size_t c = (size_t)i * nz + j;
+      r_v_cell[c] = r_v_avg;
+      r_l_cell[c] = r_l_avg;
+      e_s_cell[c] = e_s_avg;
+      S_cell[c] = S_avg;
+    }
+    /* init.py:893-1008: saturation adjustment */
+    dm_l_level = mass_water_liquid_level;
+    int iter_cnt = 0;
+    while (fabs(dm_l_level / mass_water_liquid_level) > 1e-5 && iter_cnt < q->iter_cnt_limit) {
+      dm_l_level = 0.0;
+      dm_p_level = 0.0;
+      const double D_v = diffusion_constant(T_avg, p_avg);
+      const double Kc = thermal_conductivity_air(T_avg);
+      const double L_v = heat_of_vaporization(T_avg);
+      for (int i = 0; i < nx; ++i) {
+        const size_t c = (size_t)i * nz + j;
+        const double e_s = e_s_cell[c];
+        const double S2 = S_cell[c] < q->S_init_max ? S_cell[c] : q->S_init_max;
+        const double sigma_p_cell = surface_tension_water(T_avg);
+        double sum = 0.0;
+        for (int64_t k = cell_start[i]; k < cell_start[i + 1]; ++k) {
+          double R_p, w_s, rho_p;
+          R_p_w_s_rho_p(m_w[k], m_s[k], T_avg, q->solute, &R_p, &w_s, &rho_p);
+          const double dm = or_dml_impl_Newton(q->solute, q->dt_init, q->newton_iterations, m_w[k], m_s[k], w_s, R_p,
+                                               T_avg, rho_p, T_avg, p_avg, S2, e_s, L_v, Kc, D_v, sigma_p_cell,
+                                               NULL);
+          m_w[k] += dm;
+          sum += dm * xi[k];
+        }
+        dm_l_level += sum;
+        dm_p_level += sum;
+      }
+      dm_l_level *= 1.0E-18;
+      dm_p_level *= 1.0E-18;
+      mass_water_liquid_level += dm_l_level;
+      mass_particles_level += dm_p_level;
+      mass_water_vapor_level -= dm_l_level;
+      heat_capacity_level =
+          mass_air_dry_level * K.c_pd + mass_water_vapor_level * 1906.0 + mass_particles_level * 4183.8;
+      dT_avg = dm_l_level * heat_of_vaporization(T_avg) / heat_capacity_level;
+      T_avg += dT_avg;
+      T_bot += dT_avg;
+      T_top += dT_avg;
+      r_v_avg = mass_water_vapor_level / mass_air_dry_level;
+      p_top = p_bot * OR_POW(T_top / T_bot, kappa_tot_inv * (K.eps_gc + r_tot) / (K.eps_gc + r_v_avg));
+      p_avg = 0.5 * (p_bot + p_top);
+      rho_dry_avg = (p_bot - p_top) / (dz * (1 + r_tot) * OR_G);
+      mass_air_dry_level = q->X * dy * dz * rho_dry_avg;
+      const double mass_air_dry_cell = q->dx * dy * dz * rho_dry_avg;
+      r_l_avg = mass_water_liquid_level / mass_air_dry_level;
+      r_v_avg = r_tot - r_l_avg;
+      const double e_s_T = saturation_pressure_vapor_liquid(T_avg);
+      for (int i = 0; i < nx; ++i) {
+        const size_t c = (size_t)i * nz + j;
+        double s = 0.0;
+        for (int64_t k = cell_start[i]; k < cell_start[i + 1]; ++k) s += m_w[k] * xi[k];
+        r_l_cell[c] = s * (1.0E-18 / mass_air_dry_cell);
+        r_v_cell[c] = r_tot - r_l_cell[c];
+        e_s_cell[c] = e_s_T;
+        S_cell[c] = rho_dry_avg * r_v_cell[c] * T_avg * K.R_v / e_s_cell[c];
+      }
+      mass_water_vapor_level = r_v_avg * mass_air_dry_level;
+      e_s_avg = saturation_pressure_vapor_liquid(T_avg);
+      S_avg = rho_dry_avg * r_v_avg * T_avg * K.R_v / e_s_avg;
+      ++iter_cnt;
+    }
+    double *L = lv + (size_t)j * 21;
+    const double st[17] = {p_bot, p_top, T_bot, T_top, T_avg, p_avg, rho_dry_avg, S_avg, r_v_avg, r_l_avg,
+                           mass_air_dry_level, mass_water_vapor_level, mass_water_liquid_level, mass_particles_level,
+                           no_rpcm_scale_factor, (double)iter_cnt, dm_l_level / mass_water_liquid_level};
+    for (int e = 0; e < 17; ++e) L[e] = st[e];
+    for (int m = 0; m < 4; ++m) L[17 + m] = no_rpcm[m];
+    p_bot = p_top; /* init.py:1025 */
+  }
+  free(cell_start);
+  /* init.py:322-381, 1224-1226 */
+  for (int64_t k = 0; k < n; ++k) {
+    const double u = lm_init_uniform(seed, 0, (uint32_t)k, 0, 2), w = lm_init_uniform(seed, 0, (uint32_t)k, 0, 3);
+    x[k] = corner_x[cell_x[k]] + q->dx * u;
+    z[k] = levels_z[cell_z[k]] + dz * w;
+    rel[k] = u;
+    rel[cap + k] = w;
+  }
+  return n;
+}

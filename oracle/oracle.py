@@ -28,6 +28,20 @@ class OrAdvParams(C.Structure):
     ]
 
 
+class OrInitPar(C.Structure):
+    """Parameters of the initial state (same layout as the product's LmInitPar)."""
+
+    _fields_ = [
+        ("n_modes", _i32), ("solute", _i32), ("weak_threshold", _i32), ("newton_iterations", _i32),
+        ("iter_cnt_limit", _i32), ("nx", _i32), ("n_runs", _i32), ("pad", _i32),
+        ("l_max", _i32 * 4), ("mu_m_log", _d * 4), ("sigma_m_log", _d * 4), ("kappa", _d * 4),
+        ("r_critmin", _d * 4), ("no_rpcm_0", _d * 4),
+        ("eta", _d), ("mass_density_dry", _d), ("S_init_max", _d), ("dt_init", _d),
+        ("p_ref", _d), ("Theta_l", _d), ("r_tot_0", _d), ("rho_dry_0", _d),
+        ("X", _d), ("dx", _d), ("dy", _d), ("dz", _d),
+    ]
+
+
 def lib(libm=False):
     name = "liblacmo_oracle_libm.so" if libm else "liblacmo_oracle.so"
     if name in _libs:
@@ -72,6 +86,15 @@ def lib(libm=False):
     L.or_lm_vec.argtypes = [C.c_int, _i64, _p, _p, _p]
     L.or_philox.argtypes = [_p, _p, _p]
     L.or_R_p_w_s_rho_p.argtypes = [_i64, _p, _p, _p, C.c_int, _p, _p, _p]
+    Q = C.POINTER(OrInitPar)
+    L.or_initial_mass_fraction.restype = _d
+    L.or_initial_mass_fraction.argtypes = [C.c_int, _d, _d, _d]
+    L.or_S_eq_mass_fraction.restype = _d
+    L.or_S_eq_mass_fraction.argtypes = [C.c_int, _d, _d, _d]
+    L.or_sinsip_cell.restype = C.c_int
+    L.or_sinsip_cell.argtypes = [Q, _u64, _u32, _u32, C.c_int, _p, _p]
+    L.or_init_SinSIP.restype = _i64
+    L.or_init_SinSIP.argtypes = [Q, _u64, _i32, _p, _d, _p, _i64] + [_p] * 11
     _libs[name] = L
     return L
 
@@ -179,3 +202,35 @@ def ensemble_stats(f):
     std = np.zeros(f.shape[1:])
     lib().or_ensemble_stats(f.shape[0], int(np.prod(f.shape[1:])), P(f), P(mean), P(std))
     return mean, std
+
+
+def init_SinSIP(q, seed, levels_z, p_0, corner_x, cap=None, libm=False):
+    """init.py:439-1236 for one run (seed): dict of the SIP arrays in the reference's order, positions, relative
+    positions (2, N), the level state (nz, 21) and the per-cell fields r_v, r_l, e_s, S (4, nx, nz)."""
+    levels_z = np.ascontiguousarray(levels_z, dtype=np.float64)
+    corner_x = np.ascontiguousarray(corner_x, dtype=np.float64)
+    nz = len(levels_z) - 1
+    nx = q.nx
+    if cap is None:
+        cap = nx * nz * sum(q.l_max[m] for m in range(q.n_modes))
+    a = {k: np.zeros(cap) for k in ("m_s", "xi", "m_w", "x", "z")}
+    a.update({k: np.zeros(cap, dtype=np.int32) for k in ("mode", "cell_x", "cell_z")})
+    rel = np.zeros((2, cap))
+    lv = np.zeros((nz, 21))
+    fields = np.zeros((4, nx, nz))
+    n = lib(libm).or_init_SinSIP(C.byref(q), int(seed) & 0xFFFFFFFFFFFFFFFF, nz, P(levels_z), float(p_0),
+                                 P(corner_x), cap, P(a["m_s"]), P(a["xi"]), P(a["m_w"]), P(a["mode"]),
+                                 P(a["cell_x"]), P(a["cell_z"]), P(a["x"]), P(a["z"]), P(rel), P(lv), P(fields))
+    if n < 0:
+        raise RuntimeError("or_init_SinSIP: capacity too small")
+    out = {k: v[:n].copy() for k, v in a.items()}
+    out.update(rel=rel[:, :n].copy(), lv=lv, fields=fields, n=int(n))
+    return out
+
+
+def sinsip_cell(q, seed, level, cell, mode, libm=False):
+    """generation_SIP_ensemble.py:147-244 for one cell and mode: (masses, weights)."""
+    m = np.zeros(4096)
+    w = np.zeros(4096)
+    k = lib(libm).or_sinsip_cell(C.byref(q), int(seed) & 0xFFFFFFFFFFFFFFFF, level, cell, mode, P(m), P(w))
+    return m[:k].copy(), w[:k].copy()
