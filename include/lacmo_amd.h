@@ -85,7 +85,7 @@ typedef struct LmEngine {
   double *cc;   /* n_runs*nc*16 per-cell constants for the particle kernel */
   double *dTh, *dRv; /* n_runs*nc advective increments of Theta and r_v */
   double *dml;       /* n_runs*nc Delta m_l of the last substep (kg) */
-  const double *grid_vel; /* 2*(nx+1)*(nz+1), shared by all runs */
+  const double *grid_vel; /* 2*(nx+1)*(nz+1) face velocities, per run at vel_run_stride */
   const double *flux;     /* 2*(nx+1)*(nz+1) dry-air mass flux */
   const double *prof_rv, *prof_Th; /* n_runs*nz initial profiles (relaxation) */
   const double *t_relax;           /* nz */
@@ -143,6 +143,10 @@ typedef struct LmEngine {
   /* largest entry of the E_col table (0: unknown); lets the collision kernel reject most pairs before it
      reads their table entry */
   double E_max;
+  /* elements between the face-velocity fields of consecutive runs in grid_vel: 0 (one field for all
+     runs) or 2*(nx+1)*(nz+1) (one per run: runs initialised apart have their own dry-density profile,
+     and the velocity is the mass flux over it) */
+  int64_t vel_run_stride;
 } LmEngine;
 
 /*

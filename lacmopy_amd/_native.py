@@ -60,7 +60,7 @@ class LmEngine(C.Structure):
         ("slot_of", _p),
         ("tape_part", _p), ("tape_part_i", _p), ("tape_grid", _p), ("tape_fields", _p), ("ccd", _p),
         ("part_block", _i32), ("part_spt", _i32), ("col_cap", _i32), ("pad3", _i32),
-        ("E_max", _d),
+        ("E_max", _d), ("vel_run_stride", _i64),
     ]
 
 
@@ -90,7 +90,7 @@ LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
 
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
                  "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev",
-                 "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max"]
+                 "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max", "vel_run_stride"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",

@@ -341,6 +341,7 @@ struct KView {
   const double *E;
   int nb, e_const_mode;
   double E_const, lnR0, lnf, E_max;
+  long long vel_stride;
   unsigned long long seed;
   const uint32_t *col_step_dev;
   double *x, *z, *u, *w, *m_w, *m_s, *xi;
@@ -393,6 +394,7 @@ static KView make_view(const LmEngine *e) {
   v.nb = e->nb;
   v.e_const_mode = e->e_const_mode;
   v.E_max = e->E_max;
+  v.vel_stride = (long long)e->vel_run_stride;
   v.E_const = e->E_const;
   v.lnR0 = e->lnR0;
   v.lnf = e->lnf;
@@ -1095,7 +1097,7 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
     /* grid.py:257-323 with rel_pos of integration.py:545-549 */
     const double wx = (x - v.x_min) / v.dx - (double)i;
     const double wy = (z - v.z_min) / v.dz - (double)j;
-    const double *U = v.grid_vel, *V = v.grid_vel + (size_t)(v.nx + 1) * (v.nz + 1);
+    const double *U = v.grid_vel + (size_t)r * v.vel_stride, *V = U + (size_t)(v.nx + 1) * (v.nz + 1);
     double uf, vf;
     lmp_interp_vel(i, j, wx, wy, U, V, v.nx, v.nz, uf, vf);
     /* integration.py:699-718 */
@@ -2119,6 +2121,8 @@ static int check_engine(const LmEngine *e) {
     return set_err(LM_ERR_ARG, "engine buffer pointer is NULL");
   if ((int64_t)e->n_runs * e->nc > 0x7fffffff / 2) return set_err(LM_ERR_ARG, "too many cells");
   if (e->n_tot > 0x7ffffff0) return set_err(LM_ERR_ARG, "too many SIPs for 32-bit slots");
+  if (e->vel_run_stride != 0 && e->vel_run_stride != (int64_t)2 * (e->nx + 1) * (e->nz + 1))
+    return set_err(LM_ERR_ARG, "vel_run_stride must be 0 or 2*(nx+1)*(nz+1)");
   if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
   if (e->part_block < 0 || e->part_block > PART_BLOCK_MAX || (e->part_block % WAVE) || e->part_spt < 0 ||
       e->part_spt > 2 || (e->col_cap && e->col_cap != 160 && e->col_cap != 256 && e->col_cap != 512))
@@ -2187,6 +2191,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, ev),        (int64_t)offsetof(LmEngine, run_base),
       (int64_t)offsetof(LmEngine, tape_part), (int64_t)offsetof(LmEngine, ccd),
       (int64_t)offsetof(LmEngine, part_block), (int64_t)offsetof(LmEngine, E_max),
+      (int64_t)offsetof(LmEngine, vel_run_stride),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];

@@ -67,7 +67,7 @@ class Engine:
         mk("dTh", (nr * nc,), f64)
         mk("dRv", (nr * nc,), f64)
         mk("dml", (nr * nc,), f64)
-        mk("grid_vel", (2, self.nx + 1, self.nz + 1), f64)
+        mk("grid_vel", (nr, 2, self.nx + 1, self.nz + 1), f64)  # face velocities of each run
         mk("flux", (2, self.nx + 1, self.nz + 1), f64)
         mk("prof_rv", (nr, self.nz), f64)
         mk("prof_Th", (nr, self.nz), f64)
@@ -104,9 +104,49 @@ class Engine:
                   "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off"):
             setattr(e, f, self.buf[f].data_ptr())
         e.rem_cap = rem_cap
+        e.vel_run_stride = 2 * (self.nx + 1) * (self.nz + 1)
         e.e_const_mode = 1
         e.E_const = 0.5
         self.set_params()
+
+    @classmethod
+    def from_initial_states(cls, states, config, E_col_grid, device="cuda"):
+        """One run per initial state (lacmopy_amd.initialize.initialize_batch: grid and SIPs of one
+        seed_SIP_gen each), all with the step parameters of config as simulate reads them
+        (integration.py:1908-1923): solute_type, dt_adv, dt_col, scale_dt_cond, no_col_per_adv,
+        no_iter_impl_mass, g_set, act_collisions, act_relaxation, seed_sim and, for a table E_col_grid,
+        R_kernel_low_log and bin_factor_R_log.  Each run keeps its own grid fields, relaxation profiles
+        (integration.py:1948-1955, from its own initial grid) and face velocities."""
+        from .integration import update_cells_and_rel_pos
+        from .setup_case import init_profiles
+        g0 = states[0].grid
+        eng = cls(int(g0.no_cells[0]), int(g0.no_cells[1]), [len(s.xi) for s in states], g0.ranges, g0.steps,
+                  g0.volume_cell, device=device)
+        eng.set_params(solute=config["solute_type"], no_iter_impl_mass=config["no_iter_impl_mass"],
+                       scale_dt_cond=config["scale_dt_cond"], no_col_per_adv=config["no_col_per_adv"],
+                       act_collisions=config["act_collisions"], act_relaxation=config["act_relaxation"],
+                       dt=config["dt_adv"], dt_col=config["dt_col"], g_set=config["g_set"], seed=config["seed_sim"])
+        if isinstance(E_col_grid, float):
+            eng.set_collision_kernel(E_col_grid)
+        else:
+            eng.set_collision_kernel(E_col_grid, config["R_kernel_low_log"], config["bin_factor_R_log"])
+        eng.set_flow(g0.velocity, g0.mass_flux_air_dry)
+        for r, s in enumerate(states):
+            if s.grid.velocity.shape != g0.velocity.shape or not np.array_equal(s.grid.ranges, g0.ranges):
+                raise ValueError("the runs of one engine share the grid geometry")
+            eng.set_flow(s.grid.velocity, g0.mass_flux_air_dry, run=r)
+            eng.set_relaxation(r, *init_profiles(s.grid))
+            eng.set_grid(r, s.grid.scalar_fields(), s.grid.mat_prop())
+        eng.init_cells()
+        runs = []
+        for s in states:
+            # simulate() starts by deriving the cells from the positions (integration.py:1932-1933); a position
+            # drawn at the far edge of its cell can round onto the next cell's corner
+            cells = np.array(s.cells, copy=True)
+            update_cells_and_rel_pos(s.pos, cells, np.zeros_like(s.pos), s.active_ids, s.grid.ranges, s.grid.steps)
+            runs.append((s.pos, s.vel, cells, s.m_w, s.m_s, s.xi, s.active_ids))
+        eng.load_particles(runs)
+        return eng
 
     # ------------------------------------------------------------------ setup
     @property
@@ -153,8 +193,15 @@ class Engine:
             e.lnR0 = float(R_kernel_low_log)
             e.lnf = float(bin_factor_R_log)
 
-    def set_flow(self, velocity, mass_flux_air_dry):
-        self.buf["grid_vel"].copy_(torch.from_numpy(np.ascontiguousarray(velocity, dtype=np.float64)))
+    def set_flow(self, velocity, mass_flux_air_dry, run=None):
+        """The dry-air mass flux (one field, the same for all runs) and the face velocities derived from it:
+        velocity (2, nx+1, nz+1) for every run, or for run `run` only (runs initialised apart divide the flux
+        by their own dry-density profile, init.py:1179-1182)."""
+        v = torch.from_numpy(np.ascontiguousarray(velocity, dtype=np.float64))
+        if run is None:
+            self.buf["grid_vel"].copy_(v.expand_as(self.buf["grid_vel"]))
+        else:
+            self.buf["grid_vel"][run].copy_(v)
         self.buf["flux"].copy_(torch.from_numpy(np.ascontiguousarray(mass_flux_air_dry, dtype=np.float64)))
 
     def set_relaxation(self, run, init_profile_r_v, init_profile_Theta, relaxation_time_profile):

@@ -80,3 +80,42 @@ def test_dropin_writes_reference_files(gpu, tmp_path):
     assert os.path.exists(os.path.join(tmp_path, "modes_0.npy"))
     scale = np.load(os.path.join(tmp_path, "no_rpcm_scale_factors_lvl_wise.npy"))
     assert scale.shape == (30,) and np.all(np.diff(scale) < 0), "dry density falls with height"
+
+
+def test_step_from_gpu_initial_states_matches_oracle(gpu):
+    """Initial states of two seeds -> one engine (each run with its own grid, relaxation profiles and face
+    velocities) -> two advection steps, each run bit-exact against the oracle's step of that run."""
+    from lacmopy_amd import load_kernel_data_Ecol
+    from lacmopy_amd.engine import Engine
+    from lacmopy_amd.setup_case import init_profiles
+    from test_gpu_parity import _assert_same, _config
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = dict(_config(E, lnR0, lnf, nb), seed_sim=41)
+    states = initialize_batch(init_config(nx=4, nz=75), [101, 103])
+    assert not np.array_equal(states[0].grid.velocity, states[1].grid.velocity), "velocities differ per seed"
+    eng = Engine.from_initial_states(states, cfg, E)
+    eng.adv_steps(2)
+    for r, s in enumerate(states):
+        g = s.grid
+        pos, vel, cells, m_w, m_s, xi, act = [np.array(a, copy=True) for a in
+                                              (s.pos, s.vel, s.cells, s.m_w, s.m_s, s.xi, s.active_ids)]
+        N = len(xi)
+        # simulate() derives the cells from the positions first (integration.py:1932-1933)
+        p0 = orc.adv_params(g, N, cfg, E_col_grid=E)
+        orc.lib().or_update_pos_and_cells(orc.C.byref(p0), 0.0, orc.P(pos), orc.P(vel), orc.P(cells),
+                                          orc.P(np.zeros((2, N))), orc.P(m_w), orc.P(xi), orc.P(act.view(np.uint8)),
+                                          orc.P(np.zeros(1)))
+        scal, mat = g.scalar_fields(), g.mat_prop()
+        prof_rv, prof_Th, t_relax = init_profiles(g)
+        wr, no_cols = np.zeros(1), np.zeros(2, dtype=np.int64)
+        p = orc.adv_params(g, N, cfg, seed=41, run=r, E_col_grid=E)
+        orc.integrate_adv_steps(p, 2, pos, vel, cells, np.zeros((2, N)), m_w, m_s, xi, act, np.ones(N), wr, no_cols,
+                                scal, mat, g.velocity, g.mass_flux_air_dry, E, prof_rv, prof_Th, t_relax)
+        got = eng.export_particles(r)
+        for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], got,
+                            (pos, vel, cells, m_w, m_s, xi, act)):
+            _assert_same(a, b, f"run {r} {nm}")
+        sg, mg = eng.export_grid(r)
+        _assert_same(sg[:8], scal[:8], f"run {r} grid_scalar_fields")
+        _assert_same(mg, mat, f"run {r} grid_mat_prop")
+        _assert_same(eng.counters(r)[0], no_cols, f"run {r} no_cols")
