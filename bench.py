@@ -279,6 +279,43 @@ def copy_probe(mib=1024, reps=3):
           flush=True)
 
 
+def init_cost(n_seeds, cpu=True):
+    """GPU initial states (SinSIP + saturation adjustment, lacmopy_amd.initialize) of n_seeds seeds of the
+    reference's configuration on 75x75 cells, wall time after a warm-up; the oracle's time for the same
+    initialisation measured on 4 of the 75 columns (all 75 levels) and scaled to one full seed."""
+    import torch
+    from lacmopy_amd.grid import Grid
+    from lacmopy_amd.initialize import init_parameters, initialize_batch, reference_config
+    cfg = reference_config()
+    initialize_batch(reference_config(nx=4, nz=6), [1])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    states = initialize_batch(cfg, list(range(3711, 3711 + 2 * n_seeds, 2)))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    out = {"seeds": n_seeds, "cells": "75x75", "wall_s": wall,
+           "sips_per_seed": float(np.mean([len(s.xi) for s in states])),
+           "adjustment_iterations_per_seed": float(np.mean([s.iterations.sum() for s in states]))}
+    del states
+    if cpu:
+        import ctypes
+        from oracle import oracle as orc
+        c4 = reference_config(nx=4)
+        g = Grid(c4["grid_ranges"], c4["grid_steps"], c4["dy"])
+        q = init_parameters(c4, g)
+        q.n_runs = 1
+        oq = orc.OrInitPar()
+        ctypes.memmove(ctypes.addressof(oq), ctypes.addressof(q), ctypes.sizeof(q))
+        t = time.perf_counter()
+        orc.init_SinSIP(oq, 3711, g.corners[1][0], c4["p_0"], g.corners[0][:, 0])
+        s4 = time.perf_counter() - t
+        out["cpu_oracle_s_per_seed"] = s4 * 75 / 4
+        out["cpu_sample"] = "oracle (1 thread) on 4 columns x 75 levels of one seed, scaled by 75/4"
+        out["gpu_over_cpu"] = out["cpu_oracle_s_per_seed"] * n_seeds / wall
+    log(f"init: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,6 +339,9 @@ def main():
     ap.add_argument("--adj-check", type=int, default=0,
                     help="also check the adjoint against finite differences on this many droplets and the "
                          "six aerosol-mode parameters")
+    ap.add_argument("--no-init", action="store_true", help="skip timing the GPU initial states")
+    ap.add_argument("--init-seeds", type=int, default=64, help="seeds of the timed GPU initialisation")
+    ap.add_argument("--init-only", action="store_true", help="only time the GPU initialisation (no CPU leg)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
@@ -318,6 +358,9 @@ def main():
     assert torch.cuda.is_available(), "needs a GPU"
     if args.copy_probe:
         copy_probe()
+        return
+    if args.init_only:
+        print(json.dumps({"init": init_cost(args.init_seeds, cpu=False)}), flush=True)
         return
 
     log(f"building batch of {args.runs} runs ...")
@@ -375,6 +418,8 @@ def main():
     if not args.no_adjoint and args.case == "icmw75":
         adj = adjoint_cost(args.seed, args.spc, args.adj_window, args.adj_every, args.adj_spinup, args.adj_check)
 
+    init = None if (args.no_init or args.case != "icmw75") else init_cost(args.init_seeds, cpu=not args.no_cpu)
+
     # the CPU baseline is the oracle on the plain 75x75 case (the metric's workload)
     cpu = None if (args.no_cpu or args.case != "icmw75") else cpu_baseline(args.seed, args.spc)
     if cpu:
@@ -404,7 +449,8 @@ def main():
                    "plain_single_run": plain,
                    "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
                    "kernel_ms": ktimes, "collision_events": ncols,
-                   "adjoint": adj},
+                   "adjoint": adj,
+                   "initial_states": init},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

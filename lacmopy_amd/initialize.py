@@ -38,6 +38,20 @@ InitialState = collections.namedtuple(
     "InitialState", "grid pos cells vel m_w m_s xi active_ids modes no_rpcm_scale_factors iterations rel_pos")
 
 
+def reference_config(nx=75, nz=75, **overrides):
+    """The initialisation keys of the reference's config.py:76-132 (ICMW 2012 case 1: 20 m cells, p_0 1015 hPa,
+    Theta_l 289 K, r_tot 7.5 g/kg, ammonium sulfate, two lognormal modes of 60 and 40 cm^-3 with SinSIP at
+    no_spcm = (26, 38)), on nx x nz cells; overrides replace keys."""
+    c = dict(grid_ranges=[[0.0, 20.0 * nx], [0.0, 20.0 * nz]], grid_steps=[20.0, 20.0], dy=1.0, p_0=101500.0,
+             p_ref=1.0E5, r_tot_0=7.5E-3, Theta_l=289.0, solute_type="AS", no_spcm=np.array([26, 38]),
+             dist="lognormal", DNC0=np.array([60.0E6, 40.0E6]), mu_R=0.5 * np.array([0.04, 0.15]),
+             sigma_R=np.array([1.4, 1.6]), eta=1E-10, eta_threshold="fix", r_critmin=np.array([1., 3.]) * 1E-3,
+             m_high_over_m_low=1.0E8, S_init_max=1.04, dt_init=0.1, Newton_iterations=2, iter_cnt_limit=4000,
+             seed_SIP_gen=3711)
+    c.update(overrides)
+    return c
+
+
 def _p(t):
     """Device address of a tensor that the caller keeps alive until the work using it is enqueued (a freed
     temporary's block goes back to torch's allocator at once and may back the next tensor)."""
