@@ -44,6 +44,22 @@ CASES = {
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_icmw75_64seeds.md")
+VALU_PROFILE = os.path.join(ROOT, "profiles", "valu_fp64_icmw75_64seeds.md")
+VALU_ISSUE_PEAK = 256 * 2.4e9  # 64-lane VALU wave instructions per second: one per CU per cycle at 2.4 GHz
+FP64_PEAK_TFLOPS = 78.6
+
+
+def profile_value(path, tag, kernel):
+    """A `TAG kernel value` line of a committed rocprofv3 summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == tag and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(kernel):
@@ -413,6 +429,15 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": measured_traffic("k_particles") if default_workload else None,
                 "kernel": "k_particles", "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
+        valu = profile_value(VALU_PROFILE, "VALU", "k_particles") if default_workload else None
+        flops = profile_value(VALU_PROFILE, "FP64FLOPS", "k_particles") if default_workload else None
+        if valu and flops:
+            # the co-bound of this kernel: VALU instruction issue (counts per launch from the committed --pmc
+            # summary of the same workload, over the launch time measured here)
+            roof["valu"] = {"achieved": valu / (ms * 1e-3), "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
+                            "frac": valu / (ms * 1e-3) / VALU_ISSUE_PEAK,
+                            "fp64_tflops": flops / (ms * 1e-3) / 1e12,
+                            "fp64_frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
 
     adj = None
     if not args.no_adjoint and args.case == "icmw75":
