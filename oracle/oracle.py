@@ -234,3 +234,33 @@ def sinsip_cell(q, seed, level, cell, mode, libm=False):
     w = np.zeros(4096)
     k = lib(libm).or_sinsip_cell(C.byref(q), int(seed) & 0xFFFFFFFFFFFFFFFF, level, cell, mode, P(m), P(w))
     return m[:k].copy(), w[:k].copy()
+
+
+_tlm = None
+
+
+def tlm_window(p, n_steps, width, pos, vel, cells, m_w, m_s, xi, active, scal, mat, grid_vel, flux, prof_rv, prof_Th,
+               t_relax, dir_xi, dir_ms):
+    """Tangent-linear reference of an advection window without collisions (tlm_window.cpp): returns
+    (J, dJ) with J = (J_nc, J_lwp) at the end and dJ (2, nd) their derivatives along the nd <= 8 directions
+    dir_xi, dir_ms (nd, N) of the initial multiplicities and dry masses.  Inputs are not modified."""
+    global _tlm
+    if _tlm is None:
+        path = os.path.join(HERE, "liblacmo_tlm.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        _tlm = C.CDLL(path)
+        _tlm.or_tlm_window.restype = C.c_int
+        _tlm.or_tlm_window.argtypes = [C.POINTER(OrAdvParams), _i64, _d] + [_p] * 14 + [C.c_int, _p, _p, _p, _p]
+    c = lambda a, dt=np.float64: np.ascontiguousarray(a, dtype=dt)  # noqa: E731
+    dir_xi, dir_ms = c(np.atleast_2d(dir_xi)), c(np.atleast_2d(dir_ms))
+    nd = dir_xi.shape[0]
+    J = np.zeros(2)
+    dJ = np.zeros((2, nd))
+    args = [c(pos), c(vel), c(cells, np.int64), c(m_w), c(m_s), c(xi), c(active, np.uint8), c(scal), c(mat),
+            c(grid_vel), c(flux), c(prof_rv), c(prof_Th), c(t_relax)]
+    rc = _tlm.or_tlm_window(C.byref(p), int(n_steps), float(width), *[P(a) for a in args], nd, P(dir_xi), P(dir_ms),
+                            P(J), P(dJ))
+    if rc != 0:
+        raise RuntimeError("or_tlm_window: collisions must be off and at most 8 directions")
+    return J, dJ

@@ -1,0 +1,604 @@
+/*
+ * tlm_window.cpp -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Tangent-linear (forward-mode) FP64 reference for the adjoint window: the oracle's restatement of
+ * the advection step (integration.py:1298-1611 without collisions; the pieces restated in
+ * lacmo_oracle.c function by function) evaluated on dual numbers carrying up to TLM_ND
+ * directional derivatives, seeded in the multiplicities and dry masses of the SIPs at the window
+ * start, and the window objectives (domain-mean liquid water path and smooth cloud-droplet number)
+ * at its end.  The value part of every operation is the oracle's double operation in the oracle's
+ * order, so the value trajectory is the oracle's bit for bit; the derivative parts follow the
+ * differentiation rules of each operation (own small dual type below, independent of the
+ * product's).  Branches (clamps, limiter cases, cell assignment, inactivation) follow the values.
+ *
+ * Built by oracle/Makefile into liblacmo_tlm.so; called only by tests/ as the CPU reference
+ * gradient of lacmopy_amd.adjoint.
+ */
+#include "lacmo_oracle.c"
+
+#include <vector>
+
+#define TLM_ND 8
+
+namespace tlm {
+
+struct D {
+  double v;
+  double d[TLM_ND];
+  D() : v(0.0) {
+    for (int i = 0; i < TLM_ND; ++i) d[i] = 0.0;
+  }
+  D(double x) : v(x) {
+    for (int i = 0; i < TLM_ND; ++i) d[i] = 0.0;
+  }
+};
+
+static inline D operator+(const D &a, const D &b) {
+  D r(a.v + b.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+static inline D operator-(const D &a, const D &b) {
+  D r(a.v - b.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+static inline D operator-(const D &a) {
+  D r(-a.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+static inline D operator*(const D &a, const D &b) {
+  D r(a.v * b.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+static inline D operator/(const D &a, const D &b) {
+  D r(a.v / b.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+  return r;
+}
+static inline D operator+(const D &a, double b) { return a + D(b); }
+static inline D operator+(double a, const D &b) { return D(a) + b; }
+static inline D operator-(const D &a, double b) { return a - D(b); }
+static inline D operator-(double a, const D &b) { return D(a) - b; }
+static inline D operator*(const D &a, double b) {
+  D r(a.v * b);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = a.d[i] * b;
+  return r;
+}
+static inline D operator*(double a, const D &b) {
+  D r(a * b.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = a * b.d[i];
+  return r;
+}
+static inline D operator/(const D &a, double b) {
+  D r(a.v / b);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = a.d[i] / b;
+  return r;
+}
+static inline D operator/(double a, const D &b) { return D(a) / b; }
+static inline D &operator+=(D &a, const D &b) { return a = a + b; }
+static inline D &operator-=(D &a, const D &b) { return a = a - b; }
+static inline D &operator*=(D &a, const D &b) { return a = a * b; }
+
+/* d exp x = exp x dx; d log x = dx / x; d x^c = c x^c / x dx (c constant); d sqrt x = dx / (2 sqrt x) */
+static inline D dexp(const D &x) {
+  D r(OR_EXP(x.v));
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = r.v * x.d[i];
+  return r;
+}
+static inline D dlog(const D &x) {
+  D r(OR_LOG(x.v));
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = x.d[i] / x.v;
+  return r;
+}
+static inline D dpow(const D &x, double c) {
+  D r(OR_POW(x.v, c));
+  const double g = c * r.v / x.v;
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = g * x.d[i];
+  return r;
+}
+static inline D dpow_third(const D &x) {
+  D r(OR_POW_THIRD(x.v));
+  const double g = r.v / (3.0 * x.v);
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = g * x.d[i];
+  return r;
+}
+static inline D dsqrt(const D &x) {
+  D r(sqrt(x.v));
+  const double g = r.v > 0.0 ? 0.5 / r.v : 0.0;
+  for (int i = 0; i < TLM_ND; ++i) r.d[i] = g * x.d[i];
+  return r;
+}
+/* numpy semantics on values (NaN cases as lm_np_*), derivative of the chosen operand */
+static inline D np_max(const D &a, const D &b) {
+  if (a.v != a.v) return a;
+  if (b.v != b.v) return b;
+  return a.v >= b.v ? a : b;
+}
+static inline D np_min(const D &a, const D &b) {
+  if (a.v != a.v) return a;
+  if (b.v != b.v) return b;
+  return a.v <= b.v ? a : b;
+}
+static inline D np_fmin(const D &a, const D &b) {
+  if (a.v != a.v) return b;
+  if (b.v != b.v) return a;
+  return a.v <= b.v ? a : b;
+}
+
+/* ---------------------------------------------------------------- material laws (material_properties.py) */
+static inline D poly(const double *par, int n, const D &x) { /* algebra.py:11-34 */
+  D res = par[0] * x + par[1];
+  for (int k = 2; k < n; ++k) res = res * x + par[k];
+  return res;
+}
+static inline D density_water(const D &T) {
+  const D d = T - K.par_water_dens[2];
+  return K.par_water_dens[0] + K.par_water_dens[1] * (d * d);
+}
+static inline D density_NaCl_solution(const D &w, const D &T) {
+  const double *p = K.par_sol_dens_NaCl;
+  return p[0] + p[1] * w + p[2] * T + p[3] * w * w + p[4] * w * T + p[5] * T * T;
+}
+static inline D density_AS_solution(const D &w, const D &T) { return density_water(T) * poly(K.par_rho_AS, 4, w); }
+static inline D surface_tension_water(const D &T) {
+  const D tau = 1 - T / 647.096;
+  return 0.2358 * dpow(tau, 1.256) * (1 - 0.625 * tau);
+}
+static inline D surface_tension_NaCl(const D &w, const D &T) {
+  return surface_tension_water(T) * (1.0 + K.par_sigma_NaCl * w / (1. - w));
+}
+static inline D surface_tension_AS(const D &w, const D &T) {
+  return surface_tension_water(T) * (1.0 + K.par_sigma_AS * w / (1. - w));
+}
+static inline D thermal_conductivity_air(const D &T) { return 4.1868E-3 * (5.69 + 0.017 * (T - 273.15)); }
+static inline D viscosity_air(const D &T) { return 1.458E-6 * dpow(T, 1.5) / (T + 110.4); }
+static inline D diffusion_constant(const D &T, const D &p) { return 4.01218E-5 * dpow(T, 1.94) / p; }
+static inline D heat_of_vaporization(const D &T) { return 1.0E3 * (2500.82 - 2.358 * (T - 273.0)); }
+static inline D saturation_pressure_vapor_liquid(const D &T) { return 2.53E11 * dexp(-5420.0 / T); }
+
+/* ---------------------------------------------------------------- microphysics.py */
+static inline D radius_from_mass(const D &m, const D &rho) { return dpow_third(K.four_pi_over_three_inv * m / rho); }
+static inline void R_p_w_s_rho_p(const D &m_w, const D &m_s, const D &T, int solute, D &R, D &w_s, D &rho) {
+  const D m_p = m_w + m_s;
+  const D w = m_s / m_p;
+  const D r = solute == 0 ? density_AS_solution(w, T) : density_NaCl_solution(w, T);
+  R = radius_from_mass(m_p, r);
+  w_s = w;
+  rho = r;
+}
+static inline D particle_reynolds_number(const D &R, const D &dv, const D &rho_f, const D &mu) {
+  return 2.0E-6 * rho_f * R * dv / mu;
+}
+static inline D kelvin_argument(const D &R, const D &T, const D &rho, const D &sigma) {
+  return 2.0E6 * sigma / (K.R_v * T * rho * R);
+}
+static inline D l_alpha_lin(const D &T, const D &p, const D &Kc) {
+  return (K.c_alpha_1 + K.c_alpha_2 * (T - K.T_alpha_0)) * Kc / p;
+}
+static inline D l_beta_lin(const D &T, const D &D_v) { return K.c_beta_1 * (1.0 + K.c_beta_2 * (T - K.T_alpha_0)) * D_v; }
+
+/* microphysics.py:1359-1469 (AS), :1237-1353 (NaCl) */
+static void mass_rate_and_derivative(int solute, const D &m_w, const D &m_s, const D &w_s, const D &R_p, const D &T_p,
+                                     const D &rho_p, const D &T_amb, const D &p_amb, const D &S_amb, const D &e_s_amb,
+                                     const D &L_v, const D &Kc, const D &D_v, const D &sigma_p, D &gamma, D &dgamma) {
+  const D l_alpha_plus_R_p = 1.0E-6 * (R_p + l_alpha_lin(T_amb, p_amb, Kc));
+  const D l_beta_plus_R_p = 1.0E-6 * (R_p + l_beta_lin(T_amb, D_v));
+  const D m_p_inv_SI = 1.0E18 / (m_w + m_s);
+  D drho_dm_over_rho, a_w, da_w_dm, dsigma_dm;
+  if (solute == 0) {
+    drho_dm_over_rho = -density_water(T_p) * m_p_inv_SI / rho_p * w_s * poly(K.par_rho_deriv_AS, 3, w_s);
+  } else {
+    const double *p = K.par_sol_dens_NaCl;
+    drho_dm_over_rho = -w_s * m_p_inv_SI / rho_p * (p[1] + 2.0 * p[3] * w_s + p[4] * T_p);
+  }
+  const D dR_p_dm_over_R_p = K.one_third * (m_p_inv_SI - drho_dm_over_rho);
+  const D dR_p_dm = 1.0E-6 * dR_p_dm_over_R_p * R_p;
+  const D eps_k = kelvin_argument(R_p, T_p, rho_p, sigma_p);
+  const D kelvin_term = dexp(eps_k);
+  if (solute == 0) {
+    a_w = poly(K.par_wat_act_AS, 5, w_s);
+    da_w_dm = -m_p_inv_SI * w_s * poly(K.par_wat_act_deriv_AS, 4, w_s);
+    dsigma_dm = -K.par_sigma_AS * surface_tension_water(T_p) * m_p_inv_SI * (w_s / ((1. - w_s) * (1. - w_s)));
+  } else {
+    a_w = poly(K.par_wat_act_NaCl, 5, w_s);
+    da_w_dm = -m_p_inv_SI * w_s * poly(K.par_wat_act_deriv_NaCl, 4, w_s);
+    dsigma_dm = -K.par_sigma_NaCl * surface_tension_water(T_p) * m_p_inv_SI * (w_s / ((1. - w_s) * (1. - w_s)));
+  }
+  D S_eq = a_w * kelvin_term;
+  const D dSeq_dm = da_w_dm * kelvin_term + S_eq * eps_k * (dsigma_dm / sigma_p - drho_dm_over_rho - dR_p_dm_over_R_p);
+  const D c1 = L_v * L_v / (K.R_v * Kc * T_amb * T_amb);
+  const D c2 = K.R_v * T_amb / (D_v * e_s_amb);
+  const D f3 = 1.0 / ((l_alpha_plus_R_p)*S_eq * c1 + (l_beta_plus_R_p)*c2);
+  const D f1f3 = 4.0 * K.pi * R_p * R_p * f3;
+  const D dg1_dm = (dSeq_dm * (l_alpha_plus_R_p) + S_eq * dR_p_dm) * c1 + dR_p_dm * c2;
+  S_eq = S_amb - S_eq;
+  gamma = 1.0E6 * f1f3 * S_eq;
+  dgamma = 1.0E-12 * f1f3 * (S_eq * (2.0 * dR_p_dm_over_R_p - f3 * dg1_dm) - dSeq_dm);
+}
+
+/* integration.py:728-836 */
+static D dml_impl_Newton(int solute, double dt_sub, int no_iter, const D &m_w, const D &m_s, const D &w_s,
+                         const D &R_p, const D &T_p, const D &rho_p, const D &T_amb, const D &p_amb, const D &S_amb,
+                         const D &e_s_amb, const D &L_v, const D &Kc, const D &D_v, const D &sigma_p) {
+  const D m_w_effl = m_s * (K.w_s_max_AS_inv - 1.0);
+  D gamma0, dgamma;
+  mass_rate_and_derivative(solute, m_w, m_s, w_s, R_p, T_p, rho_p, T_amb, p_amb, S_amb, e_s_amb, L_v, Kc, D_v,
+                           sigma_p, gamma0, dgamma);
+  D d = dt_sub * dgamma;
+  D denom_inv = d.v < 0.9 ? 1.0 / (1.0 - d) : D(10.0);
+  D mass_new = np_max(m_w_effl, m_w + dt_sub * gamma0 * denom_inv);
+  for (int cnt = 0; cnt < no_iter - 1; ++cnt) {
+    const D m_p = mass_new + m_s;
+    const D ws = m_s / m_p;
+    const D rho = density_AS_solution(ws, T_p);
+    const D R = radius_from_mass(m_p, rho);
+    const D sigma = surface_tension_AS(ws, T_p);
+    D gamma;
+    mass_rate_and_derivative(solute, mass_new, m_s, ws, R, T_p, rho, T_amb, p_amb, S_amb, e_s_amb, L_v, Kc, D_v,
+                             sigma, gamma, dgamma);
+    d = dt_sub * dgamma;
+    denom_inv = d.v < 0.9 ? 1.0 / (1.0 - d) : D(10.0);
+    mass_new += (dt_sub * gamma + m_w - mass_new) * denom_inv;
+    mass_new = np_max(m_w_effl, mass_new);
+  }
+  return mass_new - m_w;
+}
+
+/* ---------------------------------------------------------------- grid (integration.py:359-483) */
+static void update_material_properties_cell(D *scal, D *mat, int64_t c, int64_t nc) {
+  const D T = scal[0 * nc + c], p = scal[1 * nc + c];
+  mat[0 * nc + c] = thermal_conductivity_air(T);
+  mat[1 * nc + c] = diffusion_constant(T, p);
+  mat[2 * nc + c] = heat_of_vaporization(T);
+  mat[3 * nc + c] = surface_tension_water(T);
+  mat[4 * nc + c] = K.c_pd * (1 + 0.897 * scal[4 * nc + c]);
+  mat[5 * nc + c] = viscosity_air(T);
+  mat[6 * nc + c] = scal[3 * nc + c] * (1.0 + scal[4 * nc + c]);
+}
+
+static void propagate_grid_subloop_step(D *scal, D *mat, double p_ref, double p_ref_inv, const D *dTheta_ad,
+                                        const D *dr_v_ad, const D *delta_m_l, double V_cell, int64_t nc) {
+  for (int64_t c = 0; c < nc; ++c) {
+    D &T = scal[0 * nc + c], &p = scal[1 * nc + c], &Th = scal[2 * nc + c];
+    const D rho_d = scal[3 * nc + c];
+    D &r_v = scal[4 * nc + c], &S = scal[6 * nc + c], &e_s = scal[7 * nc + c];
+    const D m_dry_inv = scal[8 * nc + c];
+    const D dml = delta_m_l[c];
+    Th += dTheta_ad[c];
+    r_v += dr_v_ad[c] - dml * m_dry_inv;
+    const D Theta_over_T = dpow(p_ref_inv * K.R_d * rho_d * Th, K.kappa_factor2);
+    Th += Theta_over_T * (mat[2 * nc + c] * dml) / (K.c_pd * rho_d * V_cell * (1.0 + r_v * K.heat_factor_r_v));
+    const D pdr = dpow(rho_d * Th * K.R_d * p_ref_inv, K.kappa_factor);
+    T = Th * dpow(pdr, K.kappa_dry);
+    p = pdr * p_ref * (1 + r_v / K.eps_gc);
+    e_s = saturation_pressure_vapor_liquid(T);
+    S = (rho_d * r_v) * T * K.R_v / e_s;
+    update_material_properties_cell(scal, mat, c, nc);
+  }
+}
+
+/* integration.py:80-148 */
+static D limiter(const D &r) {
+  const D Kv = (1.0 + 2.0 * r) * K.one_third;
+  return np_max(D(0.0), np_fmin(2.0 * r, np_fmin(Kv, D(2.0))));
+}
+static D limiter_upwind(const D &a0, const D &a1, const D &a2) {
+  D r;
+  if (fabs(a1.v - a2.v) > 1.0E-8 * fabs(a0.v - a1.v))
+    r = (a0 - a1) / (a1 - a2);
+  else
+    r = (a0 - a1) * 1.0E8 * lm_np_sign(a1.v - a2.v);
+  return limiter(r);
+}
+
+/* integration.py:150-286, x periodic, z solid */
+static void divergence_upwind(const D *field, const double *flux, int64_t nx, int64_t nz, double dx, double dz,
+                              D *out) {
+  const int64_t fz = nz + 1;
+  std::vector<D> F((size_t)((nx + 1) * (nz + 1)));
+  for (int64_t i = 0; i <= nx; ++i) {
+    for (int64_t j = 0; j < nz; ++j) {
+      const int64_t ia0 = wrap(i, nx), ia1 = wrap(i - 1, nx), ia2 = wrap(i - 2, nx), ia3 = wrap(i + 1, nx);
+      const D a0 = field[ia0 * nz + j], a1 = field[ia1 * nz + j], a2 = field[ia2 * nz + j], a3 = field[ia3 * nz + j];
+      const D f_pos = a1 + 0.5 * limiter_upwind(a0, a1, a2) * (a1 - a2);
+      const D f_neg = a0 + 0.5 * limiter_upwind(a1, a0, a3) * (a0 - a3);
+      const double u = flux[0 * (nx + 1) * fz + i * fz + j];
+      F[i * nz + j] = u >= 0.0 ? u * f_pos : u * f_neg;
+    }
+  }
+  for (int64_t i = 0; i < nx; ++i)
+    for (int64_t j = 0; j < nz; ++j) out[i * nz + j] = (F[(i + 1) * nz + j] - F[i * nz + j]) / dx;
+  for (int64_t j = 0; j <= nz; ++j) {
+    for (int64_t i = 0; i < nx; ++i) {
+      int64_t ja0 = j, ja1 = j - 1, ja2 = j - 2, ja3 = j + 1;
+      ja0 = ja0 < 0 ? 0 : (ja0 > nz - 1 ? nz - 1 : ja0);
+      ja1 = ja1 < 0 ? 0 : (ja1 > nz - 1 ? nz - 1 : ja1);
+      ja2 = ja2 < 0 ? 0 : (ja2 > nz - 1 ? nz - 1 : ja2);
+      ja3 = ja3 < 0 ? 0 : (ja3 > nz - 1 ? nz - 1 : ja3);
+      const D a0 = field[i * nz + ja0], a1 = field[i * nz + ja1], a2 = field[i * nz + ja2], a3 = field[i * nz + ja3];
+      const D f_pos = a1 + 0.5 * limiter_upwind(a0, a1, a2) * (a1 - a2);
+      const D f_neg = a0 + 0.5 * limiter_upwind(a1, a0, a3) * (a0 - a3);
+      const double w = flux[1 * (nx + 1) * fz + i * fz + j];
+      F[j * nx + i] = w >= 0.0 ? w * f_pos : w * f_neg;
+    }
+  }
+  for (int64_t i = 0; i < nx; ++i) F[0 * nx + i] = F[nz * nx + i] = D(0.0);
+  for (int64_t i = 0; i < nx; ++i)
+    for (int64_t j = 0; j < nz; ++j) out[i * nz + j] = out[i * nz + j] + (F[(j + 1) * nx + i] - F[j * nx + i]) / dz;
+}
+
+/* integration.py:325-354 */
+static void relaxation_term(const D *field, const double *profile0, const double *t_relax, double dt, int64_t nx,
+                            int64_t nz, D *out) {
+  for (int64_t j = 0; j < nz; ++j) {
+    D s(0.0);
+    for (int64_t i = 0; i < nx; ++i) s += field[i * nz + j];
+    out[j] = dt * (profile0[j] - s / (double)nx) / t_relax[j];
+  }
+}
+
+/* ---------------------------------------------------------------- particles (integration.py:489-1089) */
+struct Sim {
+  int64_t N, nx, nz;
+  double x_min, x_max, z_min, z_max, dx, dz, V_cell, p_ref, p_ref_inv, g_set;
+  int solute, no_iter;
+  std::vector<D> pos, vel, rel_pos, m_w, m_s, xi, T_p, scal, mat, delta_m_l;
+  std::vector<int64_t> cells;
+  std::vector<uint8_t> active;
+  const double *grid_vel, *flux;
+  D water_removed;
+};
+
+static void update_cells_and_rel_pos(Sim &s) {
+  const int64_t N = s.N;
+  for (int64_t id = 0; id < N; ++id) {
+    if (s.active[id]) {
+      const D x = s.pos[id], z = s.pos[N + id];
+      s.rel_pos[id] = x - s.x_min;
+      s.rel_pos[N + id] = z - s.z_min;
+      s.cells[id] = (int64_t)floor(x.v / s.dx);
+      s.cells[N + id] = (int64_t)floor(z.v / s.dz);
+    }
+  }
+  for (int64_t id = 0; id < N; ++id) {
+    s.rel_pos[id] = s.rel_pos[id] / s.dx - (double)s.cells[id];
+    s.rel_pos[N + id] = s.rel_pos[N + id] / s.dz - (double)s.cells[N + id];
+  }
+}
+
+static void update_pos_from_vel_BC_PS(Sim &s, double dt) {
+  const int64_t N = s.N;
+  const double zcap = s.z_max * 0.999999;
+  for (int64_t id = 0; id < N; ++id) {
+    s.pos[id] += dt * s.vel[id];
+    s.pos[N + id] += dt * s.vel[N + id];
+  }
+  for (int64_t id = 0; id < N; ++id) {
+    D x = s.pos[id];
+    x.v = lm_pymod(x.v, s.x_max); /* x - k x_max: the derivative passes */
+    s.pos[id] = x;
+    s.pos[N + id] = np_min(s.pos[N + id], D(zcap));
+  }
+  for (int64_t id = 0; id < N; ++id) {
+    if (!s.active[id]) continue;
+    if (s.pos[N + id].v <= s.z_min) {
+      s.pos[N + id] = D(s.z_min - 0.01 * s.dz);
+      s.vel[id] = D(0.0);
+      s.vel[N + id] = D(0.0);
+      s.active[id] = 0;
+      s.water_removed += s.xi[id] * s.m_w[id];
+      s.cells[N + id] = -1;
+    }
+  }
+}
+
+static D bilinear_weight(int64_t i, int64_t j, const D &a, const D &b, const double *f, int64_t nfx, int64_t nfz) {
+  const int64_t i0 = i < 0 ? i + nfx : i, i1 = (i + 1) < 0 ? i + 1 + nfx : i + 1;
+  const int64_t j0 = j < 0 ? j + nfz : j, j1 = (j + 1) < 0 ? j + 1 + nfz : j + 1;
+  return a * (b * f[i1 * nfz + j1] + (1 - b) * f[i1 * nfz + j0]) +
+         (1 - a) * (b * f[i0 * nfz + j1] + (1 - b) * f[i0 * nfz + j0]);
+}
+
+static void interpolate_velocity(int64_t i, int64_t j, const D &wx, const D &wy, const double *grid_vel, int64_t nx,
+                                 int64_t nz, D &u, D &v) {
+  const int64_t nfx = nx + 1, nfz = nz + 1;
+  const double *U = grid_vel, *V = grid_vel + nfx * nfz;
+  if ((j == 0 && wy.v <= 0.5) || (j == nz - 1 && wy.v >= 0.5))
+    u = wx * U[(i + 1) * nfz + j] + (1 - wx) * U[i * nfz + j];
+  else if (wy.v > 0.5)
+    u = bilinear_weight(i, j, wx, wy - 0.5, U, nfx, nfz);
+  else
+    u = bilinear_weight(i, j - 1, wx, wy + 0.5, U, nfx, nfz);
+  if (wx.v > 0.5)
+    v = bilinear_weight(i, j, wx - 0.5, wy, V, nfx, nfz);
+  else
+    v = bilinear_weight(i - 1, j, wx + 0.5, wy, V, nfx, nfz);
+}
+
+static void update_vel_impl(Sim &s, const std::vector<D> &R_p, const std::vector<D> &rho_p, double dt) {
+  const int64_t N = s.N, nc = s.nx * s.nz;
+  for (int64_t id = 0; id < N; ++id) {
+    if (!s.active[id]) continue;
+    const int64_t i = s.cells[id], j = s.cells[N + id];
+    D uf, vf;
+    interpolate_velocity(i, j, s.rel_pos[id], s.rel_pos[N + id], s.grid_vel, s.nx, s.nz, uf, vf);
+    const D R = R_p[id];
+    const D mu = s.mat[5 * nc + i * s.nz + j];
+    const D rho_f = s.mat[6 * nc + i * s.nz + j];
+    const D dv0 = uf - s.vel[id], dv1 = vf - s.vel[N + id];
+    const D vel_dev = dsqrt(dv0 * dv0 + dv1 * dv1);
+    const D Re = particle_reynolds_number(R, vel_dev, rho_f, mu);
+    D k_dt = 4.5E12 * dt * mu / (rho_p[id] * R * R);
+    if (Re.v > 0.5) {
+      if (Re.v < 1000.0)
+        k_dt *= (1 + 0.15 * dpow(Re, 0.687));
+      else
+        k_dt *= 0.0183333333333 * Re;
+    }
+    s.vel[id] = (s.vel[id] + k_dt * uf) / (1.0 + k_dt);
+    s.vel[N + id] = (s.vel[N + id] + k_dt * vf - dt * s.g_set) / (1.0 + k_dt);
+  }
+}
+
+static void propagate_particles_subloop_step(Sim &s, double dt_sub, double dt_sub_pos, std::vector<D> &R_p,
+                                             std::vector<D> &w_s, std::vector<D> &rho_p) {
+  const int64_t N = s.N, nz = s.nz, nc = s.nx * s.nz;
+  const D *T_g = s.scal.data(), *p_g = s.scal.data() + nc, *S_g = s.scal.data() + 6 * nc,
+          *es_g = s.scal.data() + 7 * nc;
+  const D *K_g = s.mat.data(), *Dv_g = s.mat.data() + nc, *Lv_g = s.mat.data() + 2 * nc;
+  for (int64_t id = 0; id < N; ++id) {
+    int64_t j = s.cells[N + id];
+    if (j < 0) j += nz;
+    s.T_p[id] = T_g[s.cells[id] * nz + j];
+  }
+  for (int64_t id = 0; id < N; ++id) R_p_w_s_rho_p(s.m_w[id], s.m_s[id], s.T_p[id], s.solute, R_p[id], w_s[id], rho_p[id]);
+  for (int64_t c = 0; c < nc; ++c) s.delta_m_l[c] = D(0.0);
+  for (int64_t id = 0; id < N; ++id) {
+    if (!s.active[id]) continue;
+    const int64_t c = s.cells[id] * nz + s.cells[N + id];
+    const D sigma_p = s.solute == 0 ? surface_tension_AS(w_s[id], s.T_p[id]) : surface_tension_NaCl(w_s[id], s.T_p[id]);
+    const D dm = dml_impl_Newton(s.solute, dt_sub, s.no_iter, s.m_w[id], s.m_s[id], w_s[id], R_p[id], s.T_p[id],
+                                 rho_p[id], T_g[c], p_g[c], S_g[c], es_g[c], Lv_g[c], K_g[c], Dv_g[c], sigma_p);
+    s.m_w[id] += dm;
+    s.delta_m_l[c] += dm * s.xi[id];
+  }
+  for (int64_t c = 0; c < nc; ++c) s.delta_m_l[c] = s.delta_m_l[c] * 1.0E-18;
+  for (int64_t id = 0; id < N; ++id) R_p_w_s_rho_p(s.m_w[id], s.m_s[id], s.T_p[id], 0, R_p[id], w_s[id], rho_p[id]);
+  update_vel_impl(s, R_p, rho_p, dt_sub);
+  update_pos_from_vel_BC_PS(s, dt_sub_pos);
+  update_cells_and_rel_pos(s);
+}
+
+static void subloop(Sim &s, const std::vector<D> &dTh, const std::vector<D> &dRv, double dt_sub, int no_cond_steps,
+                    std::vector<D> &R_p, std::vector<D> &w_s, std::vector<D> &rho_p) {
+  for (int n = 0; n < no_cond_steps; ++n) {
+    propagate_particles_subloop_step(s, dt_sub, dt_sub, R_p, w_s, rho_p);
+    propagate_grid_subloop_step(s.scal.data(), s.mat.data(), s.p_ref, s.p_ref_inv, dTh.data(), dRv.data(),
+                                s.delta_m_l.data(), s.V_cell, s.nx * s.nz);
+  }
+}
+
+/* smooth cloud-class indicator of the adjoint's droplet-number objective */
+static D cloud_indicator(const D &R, double w) {
+  const double ln_half = -0x1.62e42fefa39efp-1, ln_25 = 3.2188758248682006;
+  const D lr = dlog(R);
+  const D a = (lr - ln_half) / w;
+  const D b = (ln_25 - lr) / w;
+  return 1.0 / (1.0 + dexp(-a)) * (1.0 / (1.0 + dexp(-b)));
+}
+
+} /* namespace tlm */
+
+/*
+ * n_steps advection steps without collisions from the given state (reference layout), with the derivative
+ * parts of xi and m_s seeded by nd <= 8 directions (dir_xi, dir_ms: nd x N); returns the objectives at the
+ * end, J[0] = J_nc (mg^-1) and J[1] = J_lwp (kg m^-2), and their directional derivatives dJ[q * nd + k].
+ * J_nc = mean over cells of 1e-6 m_dry_inv sum_cell xi I(R), R at the cell temperature with the solute's
+ * law; J_lwp = 1e-18 sum xi m_w / (X dy).  Returns 0, or -1 for collisions switched on or nd out of range.
+ */
+extern "C" EXPORT int or_tlm_window(const OrAdvParams *P, int64_t n_steps, double nc_width, const double *pos, const double *vel,
+                         const int64_t *cells, const double *m_w, const double *m_s, const double *xi,
+                         const uint8_t *active, const double *scal, const double *mat, const double *grid_vel,
+                         const double *flux, const double *prof_rv, const double *prof_Th, const double *t_relax,
+                         int nd, const double *dir_xi, const double *dir_ms, double *J, double *dJ) {
+  using namespace tlm;
+  or_init_consts();
+  if (P->act_collisions || nd < 0 || nd > TLM_ND) return -1;
+  const int64_t N = P->N, nx = P->nx, nz = P->nz, nc = nx * nz;
+  Sim s;
+  s.N = N;
+  s.nx = nx;
+  s.nz = nz;
+  s.x_min = P->x_min;
+  s.x_max = P->x_max;
+  s.z_min = P->z_min;
+  s.z_max = P->z_max;
+  s.dx = P->dx;
+  s.dz = P->dz;
+  s.V_cell = P->V_cell;
+  s.p_ref = P->p_ref;
+  s.p_ref_inv = P->p_ref_inv;
+  s.g_set = P->g_set;
+  s.solute = P->solute;
+  s.no_iter = P->no_iter;
+  s.grid_vel = grid_vel;
+  s.flux = flux;
+  s.water_removed = D(0.0);
+  s.pos.assign(pos, pos + 2 * N);
+  s.vel.assign(vel, vel + 2 * N);
+  s.rel_pos.assign((size_t)(2 * N), D(0.0));
+  s.m_w.assign(m_w, m_w + N);
+  s.m_s.assign(m_s, m_s + N);
+  s.xi.assign(xi, xi + N);
+  s.T_p.assign((size_t)N, D(1.0));
+  s.scal.assign(scal, scal + 10 * nc);
+  s.mat.assign(mat, mat + 7 * nc);
+  s.delta_m_l.assign((size_t)nc, D(0.0));
+  s.cells.assign(cells, cells + 2 * N);
+  s.active.assign(active, active + N);
+  for (int k = 0; k < nd; ++k)
+    for (int64_t id = 0; id < N; ++id) {
+      s.xi[id].d[k] = dir_xi[k * N + id];
+      s.m_s[id].d[k] = dir_ms[k * N + id];
+    }
+  std::vector<D> R_p((size_t)N), w_s((size_t)N), rho_p((size_t)N), div((size_t)nc), dRv((size_t)nc),
+      dTh((size_t)nc), rel((size_t)nz);
+  const double dt_sub = P->dt_sub;
+  for (int64_t step = 0; step < n_steps; ++step) {
+    update_pos_from_vel_BC_PS(s, P->dt_sub_half);
+    update_cells_and_rel_pos(s);
+    divergence_upwind(s.scal.data() + 4 * nc, flux, nx, nz, P->dx, P->dz, div.data());
+    for (int64_t c = 0; c < nc; ++c) dRv[c] = -dt_sub * div[c] * s.scal[9 * nc + c];
+    divergence_upwind(s.scal.data() + 2 * nc, flux, nx, nz, P->dx, P->dz, div.data());
+    for (int64_t c = 0; c < nc; ++c) dTh[c] = -dt_sub * div[c] * s.scal[9 * nc + c];
+    if (P->act_relaxation) {
+      relaxation_term(s.scal.data() + 4 * nc, prof_rv, t_relax, dt_sub, nx, nz, rel.data());
+      for (int64_t c = 0; c < nc; ++c) dRv[c] += rel[c % nz];
+      relaxation_term(s.scal.data() + 2 * nc, prof_Th, t_relax, dt_sub, nx, nz, rel.data());
+      for (int64_t c = 0; c < nc; ++c) dTh[c] += rel[c % nz];
+    }
+    subloop(s, dTh, dRv, dt_sub, P->scale_dt_cond, R_p, w_s, rho_p);
+    divergence_upwind(s.scal.data() + 4 * nc, flux, nx, nz, P->dx, P->dz, div.data());
+    for (int64_t c = 0; c < nc; ++c) dRv[c] = -2.0 * dt_sub * div[c] * s.scal[9 * nc + c] - dRv[c];
+    divergence_upwind(s.scal.data() + 2 * nc, flux, nx, nz, P->dx, P->dz, div.data());
+    for (int64_t c = 0; c < nc; ++c) dTh[c] = -2.0 * dt_sub * div[c] * s.scal[9 * nc + c] - dTh[c];
+    if (P->act_relaxation) {
+      relaxation_term(s.scal.data() + 4 * nc, prof_rv, t_relax, dt_sub, nx, nz, rel.data());
+      for (int64_t c = 0; c < nc; ++c) dRv[c] += 2. * rel[c % nz];
+      relaxation_term(s.scal.data() + 2 * nc, prof_Th, t_relax, dt_sub, nx, nz, rel.data());
+      for (int64_t c = 0; c < nc; ++c) dTh[c] += 2. * rel[c % nz];
+    }
+    subloop(s, dTh, dRv, dt_sub, P->scale_dt_cond - 1, R_p, w_s, rho_p);
+    propagate_particles_subloop_step(s, dt_sub, P->dt_sub_half, R_p, w_s, rho_p);
+    propagate_grid_subloop_step(s.scal.data(), s.mat.data(), s.p_ref, s.p_ref_inv, dTh.data(), dRv.data(),
+                                s.delta_m_l.data(), s.V_cell, nc);
+  }
+  /* objectives: per cell in ascending id, then over the cells */
+  std::vector<D> a((size_t)nc, D(0.0)), b((size_t)nc, D(0.0));
+  for (int64_t id = 0; id < N; ++id) {
+    if (!s.active[id]) continue;
+    const int64_t c = s.cells[id] * nz + s.cells[N + id];
+    D R, ws, rho;
+    R_p_w_s_rho_p(s.m_w[id], s.m_s[id], s.scal[c], s.solute, R, ws, rho);
+    a[c] += s.xi[id] * s.m_w[id];
+    b[c] += s.xi[id] * cloud_indicator(R, nc_width);
+  }
+  D J_lwp(0.0), J_nc(0.0);
+  for (int64_t c = 0; c < nc; ++c) {
+    J_lwp += a[c];
+    J_nc += s.scal[8 * nc + c] * b[c];
+  }
+  const double c_lwp = 1.0E-18 / ((P->x_max - P->x_min) * (P->V_cell / (P->dx * P->dz)));
+  J_lwp = c_lwp * J_lwp;
+  J_nc = 1.0E-6 * J_nc / (double)nc;
+  J[0] = J_nc.v;
+  J[1] = J_lwp.v;
+  for (int k = 0; k < nd; ++k) {
+    dJ[k] = J_nc.d[k];
+    dJ[nd + k] = J_lwp.d[k];
+  }
+  return 0;
+}

@@ -152,3 +152,58 @@ def test_gradient_vs_finite_differences_mode_parameters(gpu):
     print(f"mode-parameter gradients: h=1e-6 {len(seen[H])} of 12 resolved (max rel. deviation "
           f"{max(seen[H], default=0):.1e}); h=1e-3 (fourth order) {len(seen[H2])} of 12, max rel. deviation "
           f"{max(seen[H2]):.1e}")
+
+
+def test_gradient_against_cpu_tangent_linear_reference(gpu):
+    """The adjoint of a window without collisions against the FP64 tangent-linear restatement on the CPU
+    (oracle/tlm_window.cpp): the same window from the same exported state, eight directions in the initial
+    multiplicities and dry masses (all SIPs at once, and single droplets); the directional derivatives
+    lambda . v must agree to 1e-6 relative (the reference-gradient tolerance); measured: 2e-15 of sum |lambda_k v_k|."""
+    import math
+    from lacmopy_amd.setup_case import init_profiles
+    from oracle import oracle as orc
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb, act_collisions=False)
+    grid, *parts = _case(seed=5)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
+    eng.e.act_collisions = 0
+    eng.adv_steps(2)
+    n_steps = 6
+    start = eng.export_particles(0)
+    scal, mat = eng.export_grid(0)
+    w = AdjointWindow(eng, n_steps, checkpoint_every=2)
+    J_gpu = [float(np.sum(j)) for j in w.run(record=True)]
+    g = {name: w.gradient(w_nc=float(name == "nc"), w_lwp=float(name == "lwp")) for name in ("nc", "lwp")}
+    pos, vel, cells, m_w, m_s, xi, act = start
+    N = len(xi)
+    rng = np.random.default_rng(3)
+    drops = np.flatnonzero(act & (m_w > 4.19E3))
+    ks = rng.choice(drops, 4, replace=False)
+    dir_xi, dir_ms = np.zeros((8, N)), np.zeros((8, N))
+    dir_xi[0] = rng.uniform(-1, 1, N) * xi
+    dir_ms[1] = rng.uniform(-1, 1, N) * m_s
+    dir_xi[2] = rng.uniform(-1, 1, N) * xi
+    dir_ms[2] = rng.uniform(-1, 1, N) * m_s
+    for q, k in enumerate(ks):
+        (dir_xi if q % 2 == 0 else dir_ms)[3 + q, k] = (xi if q % 2 == 0 else m_s)[k]
+    dir_xi[7, ks[1]] = xi[ks[1]]
+    dir_ms[7, ks[0]] = m_s[ks[0]]
+    p = orc.adv_params(grid, N, cfg, E_col_grid=E)
+    prof_rv, prof_Th, t_relax = init_profiles(grid)
+    J_cpu, dJ = orc.tlm_window(p, n_steps, w.width, pos, vel, cells, m_w, m_s, xi, act, scal, mat, grid.velocity,
+                               grid.mass_flux_air_dry, prof_rv, prof_Th, t_relax, dir_xi, dir_ms)
+    for q in range(2):
+        assert math.isclose(J_gpu[q], J_cpu[q], rel_tol=1e-13), (q, J_gpu[q], J_cpu[q])
+    worst = 0.0
+    for q, name in enumerate(("nc", "lwp")):
+        lam_xi, lam_ms = (np.asarray(t.cpu().numpy() if hasattr(t, "cpu") else t) for t in g[name][:2])
+        for d in range(8):
+            terms = np.concatenate([lam_xi * dir_xi[d], lam_ms * dir_ms[d]])
+            adj = math.fsum(terms)
+            scale = math.fsum(np.abs(terms))
+            assert scale > 0
+            err = abs(adj - dJ[q, d]) / max(abs(dJ[q, d]), 1e-300)
+            assert abs(adj - dJ[q, d]) <= 1e-6 * abs(dJ[q, d]) + 1e-12 * scale, (name, d, adj, dJ[q, d], scale)
+            worst = max(worst, abs(adj - dJ[q, d]) / scale)
+    print(f"adjoint vs CPU tangent-linear: J rel diff {abs(J_gpu[0] - J_cpu[0]) / J_cpu[0]:.1e}; "
+          f"largest |lambda.v - dJ| / sum|lambda_k v_k| over 16 derivatives {worst:.1e}")
