@@ -247,7 +247,24 @@ def adjoint_fd_check(w, eng, cases, lam_xi, n_check):
     return out
 
 
-def cpu_baseline(seed0, spc):
+def _cpu_run(args):
+    """One CPU worker of the all-cores baseline: its own seed of the plain case, SIP-steps/s of its steps."""
+    seed0, spc, max_steps, max_s = args
+    return cpu_baseline(seed0, spc, max_steps=max_steps, max_s=max_s)["value"]
+
+
+def cpu_baseline_all_cores(seed0, spc, workers):
+    """The reference's own pattern for many runs (one single-threaded process per run, run_lacmo.sh:7-16):
+    `workers` processes, each the oracle on its own seed of the plain 75x75 case, started together; the sum
+    of their SIP-steps/s."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        rates = pool.map(_cpu_run, [(seed0 + 2 * k, spc, 2, 8.0) for k in range(workers)])
+    return float(np.sum(rates))
+
+
+def cpu_baseline(seed0, spc, max_steps=3, max_s=10.0):
     """Oracle (C restatement, 1 thread) on the plain 75x75 case: bounded sample of adv steps."""
     from lacmopy_amd import load_kernel_data_Ecol
     from lacmopy_amd.setup_case import init_profiles, make_case
@@ -274,7 +291,7 @@ def cpu_baseline(seed0, spc):
         orc.integrate_adv_steps(p, 1, pos, vel, cells, rel, m_w, m_s, xi, act, T_p, wr, nco, scal, mat,
                                 grid.velocity, grid.mass_flux_air_dry, E, prof[0], prof[1], prof[2])
         steps += 1
-        if time.perf_counter() - t0 > 10.0 or steps >= 3:
+        if time.perf_counter() - t0 > max_s or steps >= max_steps:
             break
     dt = time.perf_counter() - t0
     return {"value": N * steps / dt, "unit": "SIP-steps/s", "cores": 1, "kind": "port",
@@ -342,6 +359,8 @@ def main():
     ap.add_argument("--spc", type=float, default=None, help="mean SIPs per cell (default: the case's)")
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=min(16, len(os.sched_getaffinity(0))),
+                    help="processes of the all-cores CPU baseline (1: single core only)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--kernel", default=None, choices=["Long_Bott", "Hall_Bott", "const"])
@@ -447,6 +466,15 @@ def main():
 
     # the CPU baseline is the oracle on the plain 75x75 case (the metric's workload)
     cpu = None if (args.no_cpu or args.case != "icmw75") else cpu_baseline(args.seed, args.spc)
+    if cpu and args.cpu_workers > 1:
+        t = time.perf_counter()
+        single = dict(cpu)
+        agg = cpu_baseline_all_cores(args.seed, args.spc, args.cpu_workers)
+        cpu = {"value": agg, "unit": "SIP-steps/s", "cores": args.cpu_workers, "kind": "port",
+               "sample": f"{args.cpu_workers} processes started together, each the oracle C restatement (1 thread) "
+                         f"on its own seed of the plain 75x75 case for 2 advection steps (the reference's one "
+                         f"single-threaded process per run); sum of their SIP-steps/s",
+               "single_core": single, "seconds": time.perf_counter() - t}
     if cpu:
         log(f"cpu baseline: {cpu}")
     res = {

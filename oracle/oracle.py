@@ -240,10 +240,12 @@ _tlm = None
 
 
 def tlm_window(p, n_steps, width, pos, vel, cells, m_w, m_s, xi, active, scal, mat, grid_vel, flux, prof_rv, prof_Th,
-               t_relax, dir_xi, dir_ms):
-    """Tangent-linear reference of an advection window without collisions (tlm_window.cpp): returns
-    (J, dJ) with J = (J_nc, J_lwp) at the end and dJ (2, nd) their derivatives along the nd <= 8 directions
-    dir_xi, dir_ms (nd, N) of the initial multiplicities and dry masses.  Inputs are not modified."""
+               t_relax, dir_xi, dir_ms, events=None):
+    """Tangent-linear reference of an advection window (tlm_window.cpp): returns (J, dJ) with J = (J_nc,
+    J_lwp) at the end and dJ (2, nd) their derivatives along the nd <= 8 directions dir_xi, dir_ms (nd, N) of
+    the initial multiplicities and dry masses.  With collisions on (p.act_collisions), events = (step_off,
+    gid_min, gid_max, kind, p_crit) are the fixed events of the window's collision steps.  Inputs are not
+    modified."""
     global _tlm
     if _tlm is None:
         path = os.path.join(HERE, "liblacmo_tlm.so")
@@ -251,7 +253,8 @@ def tlm_window(p, n_steps, width, pos, vel, cells, m_w, m_s, xi, active, scal, m
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
         _tlm = C.CDLL(path)
         _tlm.or_tlm_window.restype = C.c_int
-        _tlm.or_tlm_window.argtypes = [C.POINTER(OrAdvParams), _i64, _d] + [_p] * 14 + [C.c_int, _p, _p, _p, _p]
+        _tlm.or_tlm_window.argtypes = ([C.POINTER(OrAdvParams), _i64, _d] + [_p] * 14 + [C.c_int, _p, _p, _i64] +
+                                       [_p] * 5 + [_p, _p])
     c = lambda a, dt=np.float64: np.ascontiguousarray(a, dtype=dt)  # noqa: E731
     dir_xi, dir_ms = c(np.atleast_2d(dir_xi)), c(np.atleast_2d(dir_ms))
     nd = dir_xi.shape[0]
@@ -259,8 +262,16 @@ def tlm_window(p, n_steps, width, pos, vel, cells, m_w, m_s, xi, active, scal, m
     dJ = np.zeros((2, nd))
     args = [c(pos), c(vel), c(cells, np.int64), c(m_w), c(m_s), c(xi), c(active, np.uint8), c(scal), c(mat),
             c(grid_vel), c(flux), c(prof_rv), c(prof_Th), c(t_relax)]
+    if events is None:
+        ev = [None] * 5
+        n_col = 0
+    else:
+        ev = [c(events[0], np.int64), c(events[1], np.int32), c(events[2], np.int32), c(events[3], np.int32),
+              c(events[4])]
+        n_col = len(ev[0]) - 1
     rc = _tlm.or_tlm_window(C.byref(p), int(n_steps), float(width), *[P(a) for a in args], nd, P(dir_xi), P(dir_ms),
-                            P(J), P(dJ))
+                            n_col, *[P(a) for a in ev], P(J), P(dJ))
     if rc != 0:
-        raise RuntimeError("or_tlm_window: collisions must be off and at most 8 directions")
+        raise RuntimeError(f"or_tlm_window: error {rc} (-1: more than 8 directions; -2: events do not match the "
+                           "window's collision steps)")
     return J, dJ

@@ -472,13 +472,61 @@ static void propagate_particles_subloop_step(Sim &s, double dt_sub, double dt_su
   update_cells_and_rel_pos(s);
 }
 
-static void subloop(Sim &s, const std::vector<D> &dTh, const std::vector<D> &dRv, double dt_sub, int no_cond_steps,
-                    std::vector<D> &R_p, std::vector<D> &w_s, std::vector<D> &rho_p) {
+/* Collision events of one collision step, fixed as recorded (pair, branch, p_crit of a multiple collection),
+   applied with the update expressions of all_or_nothing.py:577-632 in each cell's recorded order. */
+struct Events {
+  int64_t n_steps, next;
+  const int64_t *step_off;
+  const int32_t *gmin, *gmax, *kind;
+  const double *p;
+};
+
+static int replay_collision_step(Sim &s, Events &ev) {
+  if (ev.next >= ev.n_steps) return -1;
+  for (int64_t q = ev.step_off[ev.next]; q < ev.step_off[ev.next + 1]; ++q) {
+    const int64_t kmin = ev.gmin[q], kmax = ev.gmax[q];
+    const D xi_min = s.xi[kmin], xi_max = s.xi[kmax];
+    const D mw_min = s.m_w[kmin], mw_max = s.m_w[kmax];
+    const D ms_min = s.m_s[kmin], ms_max = s.m_s[kmax];
+    if (ev.kind[q] == 1) { /* multiple collection, p_crit > 1 (:577-592) */
+      const double pc = ev.p[q];
+      const D xi_col = pc * xi_min;
+      s.m_w[kmin] = mw_min + pc * mw_max;
+      s.m_s[kmin] = ms_min + pc * ms_max;
+      s.xi[kmax] = xi_max - xi_col;
+    } else if (ev.kind[q] == 3) { /* near-equal multiplicities (:595-620) */
+      const D xi_ges = xi_min + xi_max;
+      const D mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
+      const D msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
+      s.m_w[kmin] = mwn;
+      s.m_s[kmin] = msn;
+      s.m_w[kmax] = mwn;
+      s.m_s[kmax] = msn;
+      const D xmax_new = 0.5 * 0.7 * xi_ges;
+      s.xi[kmax] = xmax_new;
+      s.xi[kmin] = 0.5 * xi_ges - xmax_new;
+    } else { /* single collection (:621-632) */
+      s.m_w[kmin] = mw_min + mw_max;
+      s.m_s[kmin] = ms_min + ms_max;
+      s.xi[kmax] = xi_max - xi_min;
+    }
+  }
+  ++ev.next;
+  return 0;
+}
+
+/* integration.py:1095-1294 with the collision steps replayed (no_col_steps 1: one at the subloop start) */
+static int subloop(Sim &s, const std::vector<D> &dTh, const std::vector<D> &dRv, double dt_sub, int no_cond_steps,
+                   int no_col_steps, Events *ev, std::vector<D> &R_p, std::vector<D> &w_s, std::vector<D> &rho_p) {
+  if (ev && no_col_steps == 1 && replay_collision_step(s, *ev)) return -1;
   for (int n = 0; n < no_cond_steps; ++n) {
+    if (ev && no_col_steps > 1 && replay_collision_step(s, *ev)) return -1;
     propagate_particles_subloop_step(s, dt_sub, dt_sub, R_p, w_s, rho_p);
     propagate_grid_subloop_step(s.scal.data(), s.mat.data(), s.p_ref, s.p_ref_inv, dTh.data(), dRv.data(),
                                 s.delta_m_l.data(), s.V_cell, s.nx * s.nz);
   }
+  if (ev && no_col_steps > no_cond_steps && replay_collision_step(s, *ev)) return -1;
+  return 0;
 }
 
 /* smooth cloud-class indicator of the adjoint's droplet-number objective */
@@ -493,20 +541,27 @@ static D cloud_indicator(const D &R, double w) {
 } /* namespace tlm */
 
 /*
- * n_steps advection steps without collisions from the given state (reference layout), with the derivative
- * parts of xi and m_s seeded by nd <= 8 directions (dir_xi, dir_ms: nd x N); returns the objectives at the
- * end, J[0] = J_nc (mg^-1) and J[1] = J_lwp (kg m^-2), and their directional derivatives dJ[q * nd + k].
- * J_nc = mean over cells of 1e-6 m_dry_inv sum_cell xi I(R), R at the cell temperature with the solute's
- * law; J_lwp = 1e-18 sum xi m_w / (X dy).  Returns 0, or -1 for collisions switched on or nd out of range.
+ * n_steps advection steps from the given state (reference layout), with the derivative parts of xi and m_s
+ * seeded by nd <= 8 directions (dir_xi, dir_ms: nd x N); returns the objectives at the end, J[0] = J_nc
+ * (mg^-1) and J[1] = J_lwp (kg m^-2), and their directional derivatives dJ[q * nd + k].  J_nc = mean over
+ * cells of 1e-6 m_dry_inv sum_cell xi I(R), R at the cell temperature with the solute's law; J_lwp =
+ * 1e-18 sum xi m_w / (X dy).  With collisions on, the window's collision steps replay fixed events: step
+ * k applies events step_off[k] .. step_off[k+1]-1 (SIP ids gmin, gmax in ID order, branch kind 1 multiple
+ * / 2 single / 3 near-equal, p_crit of kind 1), each cell's events in their recorded order.  Returns 0, -1
+ * for nd out of range, -2 if the events do not match the window's collision steps.
  */
 extern "C" EXPORT int or_tlm_window(const OrAdvParams *P, int64_t n_steps, double nc_width, const double *pos, const double *vel,
                          const int64_t *cells, const double *m_w, const double *m_s, const double *xi,
                          const uint8_t *active, const double *scal, const double *mat, const double *grid_vel,
                          const double *flux, const double *prof_rv, const double *prof_Th, const double *t_relax,
-                         int nd, const double *dir_xi, const double *dir_ms, double *J, double *dJ) {
+                         int nd, const double *dir_xi, const double *dir_ms, int64_t n_col_steps,
+                         const int64_t *ev_step_off, const int32_t *ev_gmin, const int32_t *ev_gmax,
+                         const int32_t *ev_kind, const double *ev_p, double *J, double *dJ) {
   using namespace tlm;
   or_init_consts();
-  if (P->act_collisions || nd < 0 || nd > TLM_ND) return -1;
+  if (nd < 0 || nd > TLM_ND) return -1;
+  Events events = {n_col_steps, 0, ev_step_off, ev_gmin, ev_gmax, ev_kind, ev_p};
+  Events *ev = P->act_collisions ? &events : nullptr;
   const int64_t N = P->N, nx = P->nx, nz = P->nz, nc = nx * nz;
   Sim s;
   s.N = N;
@@ -560,7 +615,8 @@ extern "C" EXPORT int or_tlm_window(const OrAdvParams *P, int64_t n_steps, doubl
       relaxation_term(s.scal.data() + 2 * nc, prof_Th, t_relax, dt_sub, nx, nz, rel.data());
       for (int64_t c = 0; c < nc; ++c) dTh[c] += rel[c % nz];
     }
-    subloop(s, dTh, dRv, dt_sub, P->scale_dt_cond, R_p, w_s, rho_p);
+    int no_col_steps = P->no_col_per_adv == 1 ? 0 : (P->no_col_per_adv == 2 ? 1 : P->scale_dt_cond);
+    if (subloop(s, dTh, dRv, dt_sub, P->scale_dt_cond, no_col_steps, ev, R_p, w_s, rho_p)) return -2;
     divergence_upwind(s.scal.data() + 4 * nc, flux, nx, nz, P->dx, P->dz, div.data());
     for (int64_t c = 0; c < nc; ++c) dRv[c] = -2.0 * dt_sub * div[c] * s.scal[9 * nc + c] - dRv[c];
     divergence_upwind(s.scal.data() + 2 * nc, flux, nx, nz, P->dx, P->dz, div.data());
@@ -571,11 +627,13 @@ extern "C" EXPORT int or_tlm_window(const OrAdvParams *P, int64_t n_steps, doubl
       relaxation_term(s.scal.data() + 2 * nc, prof_Th, t_relax, dt_sub, nx, nz, rel.data());
       for (int64_t c = 0; c < nc; ++c) dTh[c] += 2. * rel[c % nz];
     }
-    subloop(s, dTh, dRv, dt_sub, P->scale_dt_cond - 1, R_p, w_s, rho_p);
+    no_col_steps = (P->no_col_per_adv == 1 || P->no_col_per_adv == 2) ? 1 : P->scale_dt_cond;
+    if (subloop(s, dTh, dRv, dt_sub, P->scale_dt_cond - 1, no_col_steps, ev, R_p, w_s, rho_p)) return -2;
     propagate_particles_subloop_step(s, dt_sub, P->dt_sub_half, R_p, w_s, rho_p);
     propagate_grid_subloop_step(s.scal.data(), s.mat.data(), s.p_ref, s.p_ref_inv, dTh.data(), dRv.data(),
                                 s.delta_m_l.data(), s.V_cell, nc);
   }
+  if (ev && ev->next != ev->n_steps) return -2;
   /* objectives: per cell in ascending id, then over the cells */
   std::vector<D> a((size_t)nc, D(0.0)), b((size_t)nc, D(0.0));
   for (int64_t id = 0; id < N; ++id) {

@@ -154,25 +154,55 @@ def test_gradient_vs_finite_differences_mode_parameters(gpu):
           f"{max(seen[H2]):.1e}")
 
 
-def test_gradient_against_cpu_tangent_linear_reference(gpu):
-    """The adjoint of a window without collisions against the FP64 tangent-linear restatement on the CPU
-    (oracle/tlm_window.cpp): the same window from the same exported state, eight directions in the initial
-    multiplicities and dry masses (all SIPs at once, and single droplets); the directional derivatives
-    lambda . v must agree to 1e-6 relative (the reference-gradient tolerance); measured: 2e-15 of sum |lambda_k v_k|."""
+EVENT_DTYPE = np.dtype([("gid_min", "<i4"), ("gid_max", "<i4"), ("cell", "<i4"), ("seq", "<i4"), ("kind", "<i4"),
+                        ("pad", "<i4"), ("p", "<f8"), ("xi_min", "<f8"), ("xi_max", "<f8"), ("mw_min", "<f8"),
+                        ("mw_max", "<f8"), ("ms_min", "<f8"), ("ms_max", "<f8")])  # struct LmEvent, 80 bytes
+
+
+def _window_events(w):
+    """The recorded events of a one-run window per collision step, each cell's in recorded order:
+    (step_off, gid_min, gid_max, kind, p_crit)."""
+    assert EVENT_DTYPE.itemsize == 80
+    n = int(w.n_events)
+    ev = w.ev.cpu().numpy().view(np.uint8)[: n * 80].view(EVENT_DTYPE)
+    S = w.ncol_step * w.n_steps
+    off = w.ev_off.cpu().numpy()[: S * (w.ncell + 1)].reshape(S, w.ncell + 1)
+    order = w.ev_order.cpu().numpy()
+    idx, step_off = [], [0]
+    for st in range(S):
+        for gc in range(w.ncell):
+            idx.extend(order[off[st, gc]:off[st, gc + 1]].tolist())
+        step_off.append(len(idx))
+    e = ev[np.asarray(idx, dtype=np.int64)]
+    assert len(idx) == n
+    return np.asarray(step_off), e["gid_min"], e["gid_max"], e["kind"], e["p"]
+
+
+@pytest.mark.parametrize("collisions", [False, True])
+def test_gradient_against_cpu_tangent_linear_reference(gpu, collisions):
+    """The adjoint of a window against the FP64 tangent-linear restatement on the CPU (oracle/tlm_window.cpp):
+    the same window from the same exported state (with collisions: replaying the GPU's recorded events as
+    fixed events), eight directions in the initial multiplicities and dry masses (all SIPs at once, and single
+    droplets); the directional derivatives lambda . v must agree to 1e-6 relative (the reference-gradient
+    tolerance); measured: about 1e-15 of sum |lambda_k v_k|."""
     import math
     from lacmopy_amd.setup_case import init_profiles
     from oracle import oracle as orc
     E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
-    cfg = _config(E, lnR0, lnf, nb, act_collisions=False)
+    cfg = _config(E, lnR0, lnf, nb, act_collisions=collisions)
     grid, *parts = _case(seed=5)
     eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
     eng.e.act_collisions = 0
     eng.adv_steps(2)
+    eng.e.act_collisions = int(collisions)
     n_steps = 6
     start = eng.export_particles(0)
     scal, mat = eng.export_grid(0)
     w = AdjointWindow(eng, n_steps, checkpoint_every=2)
     J_gpu = [float(np.sum(j)) for j in w.run(record=True)]
+    events = _window_events(w) if collisions else None
+    if collisions:
+        assert w.n_events > 50, "the window must hold collision events"
     g = {name: w.gradient(w_nc=float(name == "nc"), w_lwp=float(name == "lwp")) for name in ("nc", "lwp")}
     pos, vel, cells, m_w, m_s, xi, act = start
     N = len(xi)
@@ -191,19 +221,19 @@ def test_gradient_against_cpu_tangent_linear_reference(gpu):
     p = orc.adv_params(grid, N, cfg, E_col_grid=E)
     prof_rv, prof_Th, t_relax = init_profiles(grid)
     J_cpu, dJ = orc.tlm_window(p, n_steps, w.width, pos, vel, cells, m_w, m_s, xi, act, scal, mat, grid.velocity,
-                               grid.mass_flux_air_dry, prof_rv, prof_Th, t_relax, dir_xi, dir_ms)
+                               grid.mass_flux_air_dry, prof_rv, prof_Th, t_relax, dir_xi, dir_ms, events=events)
     for q in range(2):
         assert math.isclose(J_gpu[q], J_cpu[q], rel_tol=1e-13), (q, J_gpu[q], J_cpu[q])
     worst = 0.0
     for q, name in enumerate(("nc", "lwp")):
-        lam_xi, lam_ms = (np.asarray(t.cpu().numpy() if hasattr(t, "cpu") else t) for t in g[name][:2])
+        lam_xi, lam_ms = (t.cpu().numpy() for t in g[name][:2])
         for d in range(8):
             terms = np.concatenate([lam_xi * dir_xi[d], lam_ms * dir_ms[d]])
             adj = math.fsum(terms)
             scale = math.fsum(np.abs(terms))
             assert scale > 0
-            err = abs(adj - dJ[q, d]) / max(abs(dJ[q, d]), 1e-300)
             assert abs(adj - dJ[q, d]) <= 1e-6 * abs(dJ[q, d]) + 1e-12 * scale, (name, d, adj, dJ[q, d], scale)
             worst = max(worst, abs(adj - dJ[q, d]) / scale)
-    print(f"adjoint vs CPU tangent-linear: J rel diff {abs(J_gpu[0] - J_cpu[0]) / J_cpu[0]:.1e}; "
-          f"largest |lambda.v - dJ| / sum|lambda_k v_k| over 16 derivatives {worst:.1e}")
+    print(f"adjoint vs CPU tangent-linear (collisions {collisions}, {w.n_events} events): J rel diff "
+          f"{abs(J_gpu[0] - J_cpu[0]) / J_cpu[0]:.1e}; largest |lambda.v - dJ| / sum|lambda_k v_k| over 16 "
+          f"derivatives {worst:.1e}")
