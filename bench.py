@@ -147,15 +147,25 @@ def kernel_times(eng, steps):
     return {k: (ms / max(n, 1), n) for k, (ms, n) in res.items()}
 
 
-def adjoint_cost(seed0, spc, window, every, spinup, check=0):
-    """Adjoint of a `window`-step window of the plain 75x75 run after `spinup` steps without collisions:
+# adjoint window per case: (window steps, checkpoint interval, spin-up steps without collisions, event capacity).
+# icmw75: 600 s after a 2 h spin-up, checkpoints every 60 steps; stress150: the last 300 s (600 steps of 0.5 s)
+# of 1800 s, checkpoints every 10 steps (the tape of 10 steps of 5.8M SIPs x 20 substeps is 74 GB).
+ADJ_DEFAULTS = {"icmw75": (600, 60, 7200, 1 << 20), "stress150": (600, 10, 3000, 1 << 23)}
+
+
+def adjoint_cost(seed0, spc, window, every, spinup, check=0, case="icmw75"):
+    """Adjoint of a `window`-step window of one run of `case` after `spinup` steps without collisions:
     wall time of the plain forward window, of the recording forward (events + checkpoints every `every`
-    steps) and of one gradient (taped re-runs + reverse sweep), for the cloud-droplet-number objective.
+    steps) and of one gradient (taped re-runs + reverse sweep), for the cloud-droplet-number objective,
+    and the gradient with respect to the six aerosol-mode parameters through the SIP initialisation map
+    (the window-start multiplicities and dry masses are still the map's: no collision before the window).
     check > 0: also compare the gradient with central finite differences on `check` random droplets
     (multiplicity, h = 1e-3 relative) and on the six aerosol-mode parameters (fourth order, h = 1e-3)."""
     import torch
     from lacmopy_amd.adjoint import AdjointWindow
-    eng, n1, cases = build_engine(1, spc, seed0, case="icmw75")
+    from lacmopy_amd.setup_case import aerosol_sips
+    eng, n1, cases = build_engine(1, spc, seed0, case=case)
+    c = CASES[case]
     e = eng.e
     e.act_collisions = 0
     t = time.perf_counter()
@@ -174,7 +184,7 @@ def adjoint_cost(seed0, spc, window, every, spinup, check=0):
     t_fwd = time.perf_counter() - t0
     eng.check()
     eng.restore(start)
-    w = AdjointWindow(eng, window, checkpoint_every=every)
+    w = AdjointWindow(eng, window, checkpoint_every=every, ev_cap=ADJ_DEFAULTS[case][3])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J = w.run(record=True)
@@ -186,13 +196,25 @@ def adjoint_cost(seed0, spc, window, every, spinup, check=0):
     lam_xi, lam_ms = w.gradient(w_nc=1.0, w_lwp=0.0)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
-    res = {"workload": f"plain 75x75 run ({n1} SIPs), {window}-step window (dt 1 s, 10 substeps, 2 collision "
-                       f"steps) after a {spinup}-step spin-up without collisions; fixed collision events; "
-                       f"checkpoints every {every} steps",
+    # aerosol-mode parameters (N, r_med, sigma of both modes) -> window-start xi, m_s -> J_nc, by the chain rule
+    slots = w.slot_map()
+    cur = int(eng.e.cur)
+    theta = torch.tensor([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6], dtype=torch.float64, device="cuda",
+                         requires_grad=True)
+    xi_m, ms_m = aerosol_sips(theta, cases[0][2])
+    # the map restated in torch rounds differently from make_case's numpy in the last bits
+    map_dev = max(float(((xi_m.detach() - eng.buf[f"xi{cur}"][slots]).abs() / xi_m.detach().abs()).max()),
+                  float(((ms_m.detach() - eng.buf[f"m_s{cur}"][slots]).abs() / ms_m.detach().abs()).max()))
+    (g_theta,) = torch.autograd.grad((xi_m * lam_xi).sum() + (ms_m * lam_ms).sum(), theta)
+    res = {"workload": f"one {case} run ({c['nx']}x{c['nz']} cells, {n1} SIPs), {window}-step window (dt "
+                       f"{c['dt']:g} s, {2 * c['sdc']} substeps, 2 collision steps) after a {spinup}-step spin-up "
+                       f"without collisions; fixed collision events; checkpoints every {every} steps",
            "sips": n1, "window_steps": window, "checkpoint_every": every, "events": w.n_events,
            "J_nc": float(J[0][0]), "J_lwp": float(J[1][0]),
            "forward_s": t_fwd, "record_s": t_rec, "gradient_s": t_grad,
-           "cost_ratio": (t_rec + t_grad) / t_fwd, "gradient_over_forward": t_grad / t_fwd}
+           "cost_ratio": (t_rec + t_grad) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
+           "dJnc_dmodes": g_theta.tolist(), "mode_map_rel_dev_from_window_start": map_dev,
+           "tape_gb": sum(t.numel() * t.element_size() for t in w.tape.values()) / 1e9}
     log(f"adjoint: {res}")
     if check:
         res["fd_check"] = adjoint_fd_check(w, eng, cases, lam_xi, check)
@@ -368,9 +390,12 @@ def main():
                     help="replay the step captured in a two-step HIP graph instead of launching kernel by kernel")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adj-window", type=int, default=600, help="advection steps of the adjoint window")
-    ap.add_argument("--adj-every", type=int, default=60, help="checkpoint interval of the adjoint window")
-    ap.add_argument("--adj-spinup", type=int, default=7200, help="spin-up steps before the adjoint window")
+    ap.add_argument("--adj-window", type=int, default=None,
+                    help="advection steps of the adjoint window (default 600 for both cases)")
+    ap.add_argument("--adj-every", type=int, default=None,
+                    help="checkpoint interval of the adjoint window (default 60; stress150: 10)")
+    ap.add_argument("--adj-spinup", type=int, default=None,
+                    help="spin-up steps before the adjoint window (default 7200; stress150: 3000)")
     ap.add_argument("--adj-check", type=int, default=0,
                     help="also check the adjoint against finite differences on this many droplets and the "
                          "six aerosol-mode parameters")
@@ -385,6 +410,10 @@ def main():
     args.spc = args.spc or case["spc"]
     args.kernel = args.kernel or case["kernel"]
     n_sub = 2 * case["sdc"]
+    adj_w, adj_k, adj_s, _ = ADJ_DEFAULTS[args.case]
+    args.adj_window = args.adj_window or adj_w
+    args.adj_every = args.adj_every or adj_k
+    args.adj_spinup = args.adj_spinup if args.adj_spinup is not None else adj_s
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): independent seeds shard "
               "trivially across GPUs and no multi-GPU path is built", file=sys.stderr)
@@ -407,10 +436,12 @@ def main():
         t = time.perf_counter()
         eng.adv_steps(1)  # tune on an evolved state
         tr = eng.autotune()
-        tune = {"particle_block": tr["chosen"][0], "sips_per_thread": tr["chosen"][1],
-                "collision_capacity": tr["chosen"][2],
+        ch = tr["chosen"]
+        tune = {"chosen": ch,
                 "particle_ms": {f"{b}x{sp}": round(ms, 4) for (b, sp), ms in tr["particles"].items()},
+                "dml_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["dml"].items()},
                 "collision_ms": {str(c): round(ms, 4) for c, ms in tr["collision"].items()},
+                "collision_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["collision_work"].items()},
                 "seconds": time.perf_counter() - t}
         log(f"autotune: {tune}")
     secs = time_steps(eng, args.steps, args.warmup, graph=args.graph)
@@ -459,8 +490,9 @@ def main():
                             "fp64_frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
 
     adj = None
-    if not args.no_adjoint and args.case == "icmw75":
-        adj = adjoint_cost(args.seed, args.spc, args.adj_window, args.adj_every, args.adj_spinup, args.adj_check)
+    if not args.no_adjoint:
+        adj = adjoint_cost(args.seed, args.spc, args.adj_window, args.adj_every, args.adj_spinup, args.adj_check,
+                           case=args.case)
 
     init = None if (args.no_init or args.case != "icmw75") else init_cost(args.init_seeds, cpu=not args.no_cpu)
 

@@ -147,7 +147,20 @@ typedef struct LmEngine {
      runs) or 2*(nx+1)*(nz+1) (one per run: runs initialised apart have their own dry-density profile,
      and the velocity is the mass flux over it) */
   int64_t vel_run_stride;
+  /*
+   * Persistent per-cell passes (0 = one workgroup per (run, cell); k > 0 = k workgroups per compute unit
+   * that pull (run, cell) items from the work list until it is empty), for the main collision pass and
+   * the main Delta m_l pass, each taking *_work_chunk consecutive items per atomic grab (0: a static
+   * round robin over the list, no atomics); set by an autotuner.  The work list holds the n_runs*nc
+   * global cells ordered by decreasing home-segment size (largest first), rebuilt by every reindex while a
+   * persistent pass is on; any permutation of the cells gives the same results, only the balance changes.
+   */
+  int32_t col_work_per_cu, dml_work_per_cu;
+  int32_t *work; /* n_runs*nc + LM_WORK_BINS + 2: work list, histogram scratch, 2 work counters (NULL: off) */
+  int32_t col_work_chunk, dml_work_chunk;
 } LmEngine;
+
+#define LM_WORK_BINS 1024 /* home-segment sizes are binned up to LM_WORK_BINS - 1 for the work-list order */
 
 /*
  * Parameters of the initial SIP ensemble and saturation adjustment (config.py:84-132 as
@@ -256,6 +269,15 @@ int lm_position_step(LmEngine *e, double dt, void *stream);
 
 /* Full reindex: counting sort of the SIPs by (run, current cell, id). */
 int lm_reindex(LmEngine *e, void *stream);
+
+/*
+ * (Re)builds the (run, cell) work list of the persistent per-cell passes from the current segments:
+ * cells by decreasing home-segment size.  lm_reindex calls it while a persistent pass is on; an
+ * autotuner calls it before timing persistent launch shapes.  The reference visits the cells of its
+ * all-cells loops in x-major order (collision/all_or_nothing.py:806-815, integration.py:915-938); the
+ * per-cell results do not depend on the visiting order, so the list only changes the load balance.
+ */
+int lm_build_work_list(LmEngine *e, void *stream);
 
 /*
  * Advective increments of r_v and Theta (integration.py:1483-1508 with

@@ -61,6 +61,8 @@ class LmEngine(C.Structure):
         ("tape_part", _p), ("tape_part_i", _p), ("tape_grid", _p), ("tape_fields", _p), ("ccd", _p),
         ("part_block", _i32), ("part_spt", _i32), ("col_cap", _i32), ("pad3", _i32),
         ("E_max", _d), ("vel_run_stride", _i64),
+        ("col_work_per_cu", _i32), ("dml_work_per_cu", _i32), ("work", _p),
+        ("col_work_chunk", _i32), ("dml_work_chunk", _i32),
     ]
 
 
@@ -90,12 +92,13 @@ LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
 
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
                  "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev",
-                 "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max", "vel_run_stride"]
+                 "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max", "vel_run_stride",
+                 "col_work_per_cu", "work", "dml_work_chunk"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",
     "lm_get_constants", "lm_num_constants", "lm_engine_init_cells", "lm_collision_step",
-    "lm_position_step", "lm_reindex", "lm_advection_increments", "lm_substep",
+    "lm_position_step", "lm_reindex", "lm_build_work_list", "lm_advection_increments", "lm_substep",
     "lm_adv_steps", "lm_engine_check", "lm_golovin_step", "lm_ecol_box_step", "lm_math_probe",
     "lm_prof_enable", "lm_prof_collect", "lm_prof_disable", "lm_particle_substep", "lm_grid_substep",
     "lm_box2d_step", "lm_box_check", "lm_divergence_upwind", "lm_update_r_l",
@@ -128,8 +131,8 @@ def lib():
     L.lm_engine_offsets.argtypes = [C.POINTER(_i64), _i32]
     L.lm_get_constants.argtypes = [C.POINTER(_d), _i32]
     L.lm_num_constants.restype = _i32
-    for name in ["lm_engine_init_cells", "lm_collision_step", "lm_reindex", "lm_engine_check", "lm_grid_substep",
-                 "lm_update_r_l"]:
+    for name in ["lm_engine_init_cells", "lm_collision_step", "lm_reindex", "lm_build_work_list", "lm_engine_check",
+                 "lm_grid_substep", "lm_update_r_l"]:
         getattr(L, name).argtypes = [E, _p]
         getattr(L, name).restype = C.c_int
     L.lm_position_step.argtypes = [E, _d, _p]

@@ -358,6 +358,8 @@ struct KView {
   int32_t *rem_cnt, *rem_slot;
   int rem_cap;
   int32_t *scan_buf, *scan_part, *err, *ovf_list, *ovf_cnt;
+  const int32_t *work; /* (run, cell) work list of the persistent passes */
+  int32_t *work_ctr;   /* its 2 counters: collision [0], Delta m_l [1] */
   const double *rnd_stream;
   long long *rnd_off;
   long long rnd_len;
@@ -447,6 +449,8 @@ static KView make_view(const LmEngine *e) {
   v.err = e->err;
   v.ovf_list = e->ovf_list;
   v.ovf_cnt = e->ovf_cnt;
+  v.work = e->work;
+  v.work_ctr = e->work ? e->work + (size_t)e->n_runs * e->nc + LM_WORK_BINS : nullptr;
   v.rnd_stream = e->rnd_stream;
   v.rnd_off = (long long *)e->rnd_off;
   v.rnd_len = e->rnd_len;
@@ -497,6 +501,49 @@ __global__ void k_init_cc(KView v) {
   double *C = v.cc + (size_t)gc * CC_N;
   fill_cc(P, v.scal + (size_t)r * 10 * v.nc, v.mat + (size_t)r * 7 * v.nc, c, v.nc, C);
   C[CC_TP] = C[CC_T];
+}
+
+/*
+ * Cell source of the one-wavefront-per-cell passes.  PASS_CELLS: one
+ * workgroup per global cell (blockIdx.x).  PASS_OVF: the cells queued in the
+ * overflow list, grid-strided.  PASS_WORK: persistent workgroups over the
+ * (run, cell) work list, either grid-strided (chunk 0: a static round robin
+ * over the size-ordered list, no atomics) or taking `chunk` consecutive items
+ * per atomic on the pass's counter until the list is exhausted (the counter
+ * only grows, so every wave exits).  The state is wave-uniform.
+ */
+#define PASS_CELLS 0
+#define PASS_OVF 1
+#define PASS_WORK 2
+struct CellSource {
+  int mode, chunk, ctr; /* pass mode, items per grab, counter index */
+  int it, next, end;    /* iteration; current chunk [next, end) */
+};
+
+__device__ static inline CellSource cell_source(int mode, int chunk, int ctr) {
+  CellSource s = {mode, chunk, ctr, 0, 0, 0};
+  return s;
+}
+
+/* next global cell of this workgroup, or -1 */
+__device__ static inline int next_cell(const KView &v, CellSource &s, const int32_t *ovf_cnt) {
+  const int it = s.it++;
+  if (s.mode == PASS_CELLS) return it == 0 ? (int)blockIdx.x : -1;
+  const int n = v.n_runs * v.nc;
+  if (s.mode == PASS_OVF || s.chunk <= 0) {
+    const int idx = (int)blockIdx.x + it * (int)gridDim.x;
+    if (s.mode == PASS_OVF) return idx < *ovf_cnt ? v.ovf_list[idx] : -1;
+    return idx < n ? v.work[idx] : -1;
+  }
+  if (s.next >= s.end) {
+    int b = 0;
+    if (threadIdx.x == 0) b = atomicAdd(v.work_ctr + s.ctr, s.chunk);
+    b = __builtin_amdgcn_readfirstlane(b);
+    if (b >= n) return -1;
+    s.next = b;
+    s.end = b + s.chunk < n ? b + s.chunk : n;
+  }
+  return v.work[s.next++];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -945,15 +992,16 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   }
 }
 
-/* main pass: one block per cell; overflow pass: the queued cells, grid-strided */
+/* main pass: one block per cell or persistent blocks over the work list; overflow pass: the queued cells,
+   grid-strided */
 template <int CAP, bool TAB, bool INJ>
-__global__ __launch_bounds__(WAVE) void k_collision(KView v, int from_ovf) {
-  if (!from_ovf) {
-    collide_cell<CAP, TAB, INJ>(v, blockIdx.x, 0);
-    return;
+__global__ __launch_bounds__(WAVE) void k_collision(KView v, int mode, int chunk) {
+  CellSource src = cell_source(mode, chunk, 0);
+  for (;;) {
+    const int gc = next_cell(v, src, v.ovf_cnt);
+    if (gc < 0) return;
+    collide_cell<CAP, TAB, INJ>(v, gc, mode == PASS_OVF);
   }
-  const int cnt = *v.ovf_cnt;
-  for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) collide_cell<CAP, TAB, INJ>(v, v.ovf_list[idx], 1);
 }
 
 /* member counts -> pair counts for the injected stream offsets */
@@ -1261,13 +1309,13 @@ __device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
 }
 
 template <int CAP>
-__global__ __launch_bounds__(WAVE) void k_dml(KView v, int from_ovf) {
-  if (!from_ovf) {
-    dml_cell<CAP>(v, blockIdx.x, 0);
-    return;
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_dml(KView v, int mode, int chunk) {
+  CellSource src = cell_source(mode, chunk, 1);
+  for (;;) {
+    const int gc = next_cell(v, src, v.ovf_cnt + 1);
+    if (gc < 0) return;
+    dml_cell<CAP>(v, gc, mode == PASS_OVF);
   }
-  const int cnt = v.ovf_cnt[1];
-  for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) dml_cell<CAP>(v, v.ovf_list[idx], 1);
 }
 
 /* grid.py:370-412 update_grid_r_l_np: r_l = 1e-18 m_dry^-1 sum(xi m_w), summed over the cell in ascending id */
@@ -1557,6 +1605,7 @@ __global__ void k_adv_incr(KView v, double dt_sub, int second, int act_relax) {
 /* ------------------------------------------------------------------------ */
 /* reindex: counting sort by (run, cell, id)                                */
 /* ------------------------------------------------------------------------ */
+
 /* scan layout per run: [cnt_0..cnt_{nc-1}, graveyard_old, dead_0..dead_{nc-1}] */
 __global__ __launch_bounds__(WAVE) void k_ri_count(KView v) {
   const int nc = v.nc, L = 2 * nc + 1;
@@ -1647,6 +1696,75 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(int32_t *a, const int32
   for (int q = threadIdx.x; q < SCAN_CHUNK; q += SCAN_BLOCK) {
     const int k = base + q;
     if (k < L) a[k] += add;
+  }
+}
+
+/*
+ * Work list of the persistent per-cell passes: the global cells ordered by
+ * decreasing home-segment size (the member count up to drift), so the most
+ * expensive cells start first and the small ones fill the tail.  Counting
+ * sort over LM_WORK_BINS size bins: histogram, exclusive scan, scatter.  The
+ * order within a bin follows the atomics; any order is a valid list.
+ */
+__device__ static inline int wl_bin(const KView &v, int gc) {
+  const int r = gc / v.nc, c = gc - r * v.nc;
+  const int32_t *s = v.seg + (size_t)r * (v.nc + 1);
+  const int len = s[c + 1] - s[c];
+  return LM_WORK_BINS - 1 - (len < LM_WORK_BINS - 1 ? len : LM_WORK_BINS - 1);
+}
+
+/* per-block histograms in LDS: one global atomic per (block, bin) instead of one per cell */
+#define WL_BLOCK 1024
+#define WL_PER 4
+__global__ __launch_bounds__(WL_BLOCK) void k_wl_hist(KView v, int32_t *hist) {
+  __shared__ int h[LM_WORK_BINS];
+  for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK) h[b] = 0;
+  __syncthreads();
+  const int n = v.n_runs * v.nc;
+  for (int q = 0; q < WL_PER; ++q) {
+    const int gc = (int)blockIdx.x * WL_BLOCK * WL_PER + q * WL_BLOCK + (int)threadIdx.x;
+    if (gc < n) atomicAdd(&h[wl_bin(v, gc)], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_wl_scan(int32_t *hist) {
+  __shared__ int tmp[SCAN_BLOCK];
+  int carry = 0;
+  for (int b0 = 0; b0 < LM_WORK_BINS; b0 += SCAN_BLOCK) {
+    const int k = b0 + threadIdx.x;
+    const int val = hist[k];
+    const int ex = block_exclusive_scan(val, tmp);
+    hist[k] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) tmp[0] = ex + val;
+    __syncthreads();
+    carry += tmp[0];
+    __syncthreads();
+  }
+}
+
+/* hist holds each bin's first slot; a block reserves one range per bin and places its cells by local rank */
+__global__ __launch_bounds__(WL_BLOCK) void k_wl_scatter(KView v, int32_t *hist, int32_t *work) {
+  __shared__ int h[LM_WORK_BINS];
+  for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK) h[b] = 0;
+  __syncthreads();
+  const int n = v.n_runs * v.nc;
+  int bin[WL_PER], rk[WL_PER];
+  for (int q = 0; q < WL_PER; ++q) {
+    const int gc = (int)blockIdx.x * WL_BLOCK * WL_PER + q * WL_BLOCK + (int)threadIdx.x;
+    bin[q] = gc < n ? wl_bin(v, gc) : 0;
+    rk[q] = gc < n ? atomicAdd(&h[bin[q]], 1) : 0;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK)
+    if (h[b]) h[b] = atomicAdd(&hist[b], h[b]);
+  __syncthreads();
+  for (int q = 0; q < WL_PER; ++q) {
+    const int gc = (int)blockIdx.x * WL_BLOCK * WL_PER + q * WL_BLOCK + (int)threadIdx.x;
+    if (gc < n) work[h[bin[q]] + rk[q]] = gc;
   }
 }
 
@@ -2153,20 +2271,49 @@ static int cell_capacity(const LmEngine *e) {
   return (int)(mean + 2.5 * sqrt(mean)) + 1;
 }
 
+/* compute units of the current device, cached per device */
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] <= 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+/* Main pass of a per-cell kernel: wpc persistent workgroups per compute unit over the work list (taking
+   chunk items per atomic on counter ctr, cleared by a fill kernel first; chunk 0: grid-strided), or one
+   workgroup per global cell.  Returns the pass mode. */
+static int main_pass(const LmEngine *e, int wpc, int chunk, int ctr, dim3 *grid, hipStream_t st) {
+  const int ncell = e->n_runs * e->nc;
+  if (wpc > 0 && e->work) {
+    const long long g = (long long)wpc * device_cus();
+    *grid = dim3((unsigned)(g < ncell ? g : ncell));
+    if (chunk > 0)
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, e->work + (size_t)ncell + LM_WORK_BINS + ctr, 0, 1);
+    return PASS_WORK;
+  }
+  *grid = dim3(ncell);
+  return PASS_CELLS;
+}
+
 #include "lm_adjoint.h"
 #include "lm_init.h"
 
 template <int CAP>
-static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int from_ovf) {
+static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int mode, int chunk) {
   const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
   if (tab && inj)
-    hipLaunchKernelGGL((k_collision<CAP, true, true>), grid, dim3(WAVE), 0, st, v, from_ovf);
+    hipLaunchKernelGGL((k_collision<CAP, true, true>), grid, dim3(WAVE), 0, st, v, mode, chunk);
   else if (tab)
-    hipLaunchKernelGGL((k_collision<CAP, true, false>), grid, dim3(WAVE), 0, st, v, from_ovf);
+    hipLaunchKernelGGL((k_collision<CAP, true, false>), grid, dim3(WAVE), 0, st, v, mode, chunk);
   else if (inj)
-    hipLaunchKernelGGL((k_collision<CAP, false, true>), grid, dim3(WAVE), 0, st, v, from_ovf);
+    hipLaunchKernelGGL((k_collision<CAP, false, true>), grid, dim3(WAVE), 0, st, v, mode, chunk);
   else
-    hipLaunchKernelGGL((k_collision<CAP, false, false>), grid, dim3(WAVE), 0, st, v, from_ovf);
+    hipLaunchKernelGGL((k_collision<CAP, false, false>), grid, dim3(WAVE), 0, st, v, mode, chunk);
 }
 
 extern "C" {
@@ -2191,7 +2338,8 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, ev),        (int64_t)offsetof(LmEngine, run_base),
       (int64_t)offsetof(LmEngine, tape_part), (int64_t)offsetof(LmEngine, ccd),
       (int64_t)offsetof(LmEngine, part_block), (int64_t)offsetof(LmEngine, E_max),
-      (int64_t)offsetof(LmEngine, vel_run_stride),
+      (int64_t)offsetof(LmEngine, vel_run_stride), (int64_t)offsetof(LmEngine, col_work_per_cu),
+      (int64_t)offsetof(LmEngine, work), (int64_t)offsetof(LmEngine, dml_work_chunk),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -2248,14 +2396,16 @@ int lm_collision_step(LmEngine *e, void *stream) {
     /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
        small capacity keeps the LDS footprint, and so the occupancy, low; larger cells go to the overflow pass */
     const int cap = e->col_cap > 0 ? e->col_cap : cell_capacity(e);
+    dim3 grid;
+    const int mode = main_pass(e, e->col_work_per_cu, e->col_work_chunk, 0, &grid, S(stream));
     if (cap <= 160)
-      launch_collision<160>(v, e, dim3(ncell), S(stream), 0);
+      launch_collision<160>(v, e, grid, S(stream), mode, e->col_work_chunk);
     else if (cap <= 256)
-      launch_collision<256>(v, e, dim3(ncell), S(stream), 0);
+      launch_collision<256>(v, e, grid, S(stream), mode, e->col_work_chunk);
     else
-      launch_collision<512>(v, e, dim3(ncell), S(stream), 0);
+      launch_collision<512>(v, e, grid, S(stream), mode, e->col_work_chunk);
   }
-  launch_collision<1024>(v, e, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), 1);
+  launch_collision<1024>(v, e, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), PASS_OVF, 0);
   LAUNCH_CHECK();
   if (e->ev_mode == 1) {
     /* the step's records in (cell, seq) order for the replay and the reverse sweep */
@@ -2285,6 +2435,22 @@ int lm_position_step(LmEngine *e, double dt, void *stream) {
   return 0;
 }
 
+int lm_build_work_list(LmEngine *e, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (!e->work) return set_err(LM_ERR_ARG, "work list buffer missing");
+  const KView w = make_view(e);
+  const int ncell = e->n_runs * e->nc;
+  int32_t *hist = e->work + ncell;
+  const dim3 g((unsigned)((ncell + WL_BLOCK * WL_PER - 1) / (WL_BLOCK * WL_PER)));
+  hipLaunchKernelGGL(k_fill_i32, dim3(LM_WORK_BINS / 256), dim3(256), 0, S(stream), hist, 0, LM_WORK_BINS);
+  hipLaunchKernelGGL(k_wl_hist, g, dim3(WL_BLOCK), 0, S(stream), w, hist);
+  hipLaunchKernelGGL(k_wl_scan, dim3(1), dim3(SCAN_BLOCK), 0, S(stream), hist);
+  hipLaunchKernelGGL(k_wl_scatter, g, dim3(WL_BLOCK), 0, S(stream), w, hist, e->work);
+  LAUNCH_CHECK();
+  return 0;
+}
+
 int lm_reindex(LmEngine *e, void *stream) {
   int rc = check_engine(e);
   if (rc) return rc;
@@ -2305,6 +2471,10 @@ int lm_reindex(LmEngine *e, void *stream) {
   LAUNCH_CHECK();
   e->cur = 1 - e->cur;
   e->ccur = 1 - e->ccur;
+  /* the work list of the persistent passes from the new segments (only when a persistent pass is on; the
+     list is a permutation of the cells at all times, a stale one only balances worse) */
+  if (e->work && (e->col_work_per_cu > 0 || e->dml_work_per_cu > 0) && (rc = lm_build_work_list(e, stream)))
+    return rc;
   if (e->ev_mode == 2) return lm_build_slot_of(e, stream);
   return 0;
 }
@@ -2367,11 +2537,14 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
   {
     ProfScope ps(e, 2, S(stream));
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 1, 0, 1);
+    dim3 grid;
+    const int mode = main_pass(e, e->dml_work_per_cu, e->dml_work_chunk, 1, &grid, S(stream));
     if (cell_capacity(e) <= 256)
-      hipLaunchKernelGGL(k_dml<256>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
+      hipLaunchKernelGGL(k_dml<256>, grid, dim3(WAVE), 0, S(stream), v, mode, e->dml_work_chunk);
     else
-      hipLaunchKernelGGL(k_dml<512>, dim3(ncell), dim3(WAVE), 0, S(stream), v, 0);
-    hipLaunchKernelGGL(k_dml<1024>, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), dim3(WAVE), 0, S(stream), v, 1);
+      hipLaunchKernelGGL(k_dml<512>, grid, dim3(WAVE), 0, S(stream), v, mode, e->dml_work_chunk);
+    hipLaunchKernelGGL(k_dml<1024>, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), dim3(WAVE), 0, S(stream), v,
+                       PASS_OVF, 0);
   }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();

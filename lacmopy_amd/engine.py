@@ -17,6 +17,7 @@ from ._native import LmEngine, check, lib
 
 SOLUTES = {"AS": 0, "NaCl": 1}
 CC_N, CC_TP = 16, 15  # per-cell constant table: width, slot of the pre-update cell temperature (lm_physics.h)
+WORK_BINS = 1024  # LM_WORK_BINS: size bins of the work-list order (include/lacmo_amd.h)
 
 
 def _ptr(t):
@@ -85,6 +86,10 @@ class Engine:
         mk("ovf_list", (nr * nc,), i32)
         mk("ovf_cnt", (2,), i32)
         mk("rnd_off", (nr * nc + 2,), i64)
+        # (run, cell) work list of the persistent passes (rebuilt by every reindex while one is on; starts as the
+        # identity order), its size-bin scratch and two work counters
+        mk("work", (nr * nc + WORK_BINS + 2,), i32)
+        self.buf["work"][:nr * nc].copy_(torch.arange(nr * nc, dtype=torch.int32))
         self.E_tab = None
         self.rnd_stream = None
 
@@ -101,7 +106,7 @@ class Engine:
                 getattr(e, f)[b] = self.buf[f"{f}{b}"].data_ptr()
         for f in ("dmxi", "away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
                   "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
-                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off"):
+                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off", "work"):
             setattr(e, f, self.buf[f].data_ptr())
         e.rem_cap = rem_cap
         e.vel_run_stride = 2 * (self.nx + 1) * (self.nz + 1)
@@ -264,6 +269,10 @@ class Engine:
     def reindex(self):
         check(self.L.lm_reindex(C.byref(self.e), self.stream))
 
+    def build_work_list(self):
+        """Orders the (run, cell) work list of the persistent passes by the current home-segment sizes."""
+        check(self.L.lm_build_work_list(C.byref(self.e), self.stream))
+
     def advection_increments(self, second):
         check(self.L.lm_advection_increments(C.byref(self.e), int(second), self.stream))
 
@@ -348,17 +357,40 @@ class Engine:
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
 
     # ------------------------------------------------------------ autotune
-    def set_launch_shape(self, part_block=0, part_spt=0, col_cap=0):
-        """Launch shape of the particle kernel (threads per block, SIPs per thread) and the member capacity
-        of the main collision pass; 0 restores the defaults (256, 1, from the mean SIPs per cell)."""
-        self.drop_graph()
-        self.e.part_block, self.e.part_spt, self.e.col_cap = int(part_block), int(part_spt), int(col_cap)
+    SHAPE_KEYS = ("part_block", "part_spt", "col_cap", "col_work_per_cu", "col_work_chunk", "dml_work_per_cu",
+                  "dml_work_chunk")
 
-    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), col_caps=(160, 256, 512), reps=3):
-        """Times each candidate launch shape on the engine's current state with HIP events on its stream
-        (the particle substep for every (block, SIPs per thread), the collision step for every main-pass
-        capacity), keeps the fastest and restores the state.  Results do not depend on the launch shape.
-        Returns {"particles": {(block, spt): ms}, "collision": {cap: ms}, "chosen": (block, spt, cap)}."""
+    def set_launch_shape(self, part_block=0, part_spt=0, col_cap=0, col_work_per_cu=None, dml_work_per_cu=None,
+                         col_work_chunk=None, dml_work_chunk=None):
+        """Launch shape of the particle kernel (threads per block, SIPs per thread), the member capacity
+        of the main collision pass, and the persistent main collision and Delta m_l passes over the
+        (run, cell) work list: workgroups per compute unit (0: one workgroup per cell) and items taken per
+        atomic grab (0: static round robin); None keeps the current value.  0 restores the defaults (256,
+        1, from the mean SIPs per cell, one workgroup per cell)."""
+        self.drop_graph()
+        e = self.e
+        e.part_block, e.part_spt, e.col_cap = int(part_block), int(part_spt), int(col_cap)
+        for k, val in (("col_work_per_cu", col_work_per_cu), ("dml_work_per_cu", dml_work_per_cu),
+                       ("col_work_chunk", col_work_chunk), ("dml_work_chunk", dml_work_chunk)):
+            if val is not None:
+                if int(val) < 0:
+                    raise ValueError(f"{k} must be >= 0")
+                setattr(e, k, int(val))
+
+    def launch_shape(self):
+        return {k: int(getattr(self.e, k)) for k in self.SHAPE_KEYS}
+
+    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), col_caps=(160, 256, 512),
+                 col_work=((0, 0), (16, 0), (16, 1), (16, 4), (32, 0), (32, 4)),
+                 dml_work=((0, 0), (32, 0), (32, 16), (64, 0), (64, 16)), reps=3):
+        """Times candidate launch shapes on the engine's current state with HIP events on its stream and
+        keeps the fastest: the particle substep (with its Delta m_l pass) for every (block, SIPs per
+        thread), then the Delta m_l pass's persistent shapes dml_work = (workgroups per CU, chunk) on the
+        substep, the collision step for every main-pass capacity, then its persistent shapes col_work.
+        (0, 0) is one workgroup per cell.  The state is restored before every candidate and at the end;
+        results do not depend on the launch shape.  Returns {"particles": {(block, spt): ms}, "dml":
+        {(wpc, chunk): ms}, "collision": {cap: ms}, "collision_work": {(wpc, chunk): ms}, "chosen":
+        launch_shape()}."""
         snap = self.snapshot()
         e = self.e
         stream = torch.cuda.current_stream(self.device)
@@ -376,23 +408,38 @@ class Engine:
             self.check()
             return t0.elapsed_time(t1) / reps
 
-        res = {"particles": {}, "collision": {}}
-        cap0 = int(e.col_cap)
+        def substep():
+            self.particle_substep(e.dt_sub, e.dt_sub)
+
+        res = {"particles": {}, "dml": {}, "collision": {}, "collision_work": {}}
+        shape = self.launch_shape()
+        self.build_work_list()
         for b in blocks:
             for spt in spts:
-                self.set_launch_shape(b, spt, cap0)
-                res["particles"][(b, spt)] = timed(lambda: self.particle_substep(e.dt_sub, e.dt_sub))
-        best_b, best_s = min(res["particles"], key=res["particles"].get)
+                shape.update(part_block=b, part_spt=spt, dml_work_per_cu=0, dml_work_chunk=0)
+                self.set_launch_shape(**shape)
+                res["particles"][(b, spt)] = timed(substep)
+        shape["part_block"], shape["part_spt"] = min(res["particles"], key=res["particles"].get)
+        for wpc, chunk in dml_work:
+            shape.update(dml_work_per_cu=wpc, dml_work_chunk=chunk)
+            self.set_launch_shape(**shape)
+            res["dml"][(wpc, chunk)] = timed(substep)
+        shape["dml_work_per_cu"], shape["dml_work_chunk"] = min(res["dml"], key=res["dml"].get)
         if e.act_collisions:
             for cap in col_caps:
-                self.set_launch_shape(best_b, best_s, cap)
+                shape.update(col_cap=cap, col_work_per_cu=0, col_work_chunk=0)
+                self.set_launch_shape(**shape)
                 res["collision"][cap] = timed(self.collision_step)
-            best_c = min(res["collision"], key=res["collision"].get)
-        else:
-            best_c = 0
-        self.set_launch_shape(best_b, best_s, best_c)
+            shape["col_cap"] = min(res["collision"], key=res["collision"].get)
+            for wpc, chunk in col_work:
+                shape.update(col_work_per_cu=wpc, col_work_chunk=chunk)
+                self.set_launch_shape(**shape)
+                res["collision_work"][(wpc, chunk)] = timed(self.collision_step)
+            shape["col_work_per_cu"], shape["col_work_chunk"] = min(res["collision_work"],
+                                                                    key=res["collision_work"].get)
+        self.set_launch_shape(**shape)
         self.restore(snap)
-        res["chosen"] = (best_b, best_s, best_c)
+        res["chosen"] = self.launch_shape()
         return res
 
     # ------------------------------------------------------- checkpoints

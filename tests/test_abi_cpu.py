@@ -33,6 +33,9 @@ def test_struct_layout_matches():
     L.lm_engine_offsets(off, len(N.OFFSET_FIELDS))
     for i, f in enumerate(N.OFFSET_FIELDS):
         assert off[i] == getattr(N.LmEngine, f).offset, f
+    from lacmopy_amd.engine import WORK_BINS
+    hdr = open(os.path.join(ROOT, "include", "lacmo_amd.h")).read()
+    assert int(re.search(r"#define LM_WORK_BINS (\d+)", hdr).group(1)) == WORK_BINS
 
 
 def test_kernel_constants_equal_oracle_and_reference_expressions():

@@ -49,7 +49,7 @@ def _engine(grid, parts, config, E, lnR0, lnf, seed, n_runs=1, rnd=None):
     return eng
 
 
-def _oracle_run(grid, parts, config, E, seed, n_steps, rnd=None, libm=False):
+def _oracle_run(grid, parts, config, E, seed, n_steps, rnd=None, libm=False, run=0):
     pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
     N = len(xi)
     scal, mat = grid.scalar_fields(), grid.mat_prop()
@@ -58,7 +58,7 @@ def _oracle_run(grid, parts, config, E, seed, n_steps, rnd=None, libm=False):
     T_p = np.ones(N)
     wr = np.zeros(1)
     no_cols = np.zeros(2, dtype=np.int64)
-    p = orc.adv_params(grid, N, config, seed=seed, E_col_grid=E)
+    p = orc.adv_params(grid, N, config, seed=seed, run=run, E_col_grid=E)
     orc.integrate_adv_steps(p, n_steps, pos, vel, cells, rel_pos, m_w, m_s, xi, active, T_p, wr, no_cols, scal,
                             mat, grid.velocity, grid.mass_flux_air_dry, E, prof_rv, prof_Th, t_relax,
                             rnd_stream=rnd, libm=libm)
@@ -230,6 +230,43 @@ def test_one_crowded_cell_takes_the_overflow_passes(gpu):
     _assert_same(eng.counters(0)[0], no_cols, "no_cols")
 
 
+@pytest.mark.parametrize("work_per_cu,chunk", [(1, 0), (1, 1), (4, 3)])
+def test_persistent_work_list_passes_match_oracle(gpu, work_per_cu, chunk):
+    """Persistent collision and Delta m_l passes over the (run, cell) work list: 6 runs of 8x8 cells (384
+    work items, more than one per workgroup at one workgroup per compute unit); run 0 stays bit-exact with
+    the oracle over three advection steps, the other runs equal the one-workgroup-per-cell launch, and
+    after each reindex the work list is a permutation of the cells ordered by decreasing home-segment size."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(nx=8, nz=8, spc=24, seed=23)
+    parts = _prep(grid, parts)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 3, 3)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 3, n_runs=6)
+    eng.set_launch_shape(0, 0, 0, work_per_cu, work_per_cu, chunk, chunk)
+    base = _engine(grid, parts, cfg, E, lnR0, lnf, 3, n_runs=6)
+    ncell = 6 * 64
+    for step in range(3):
+        eng.adv_steps(1)
+        base.adv_steps(1)
+        work = eng.buf["work"][:ncell].cpu().numpy()
+        assert np.array_equal(np.sort(work), np.arange(ncell)), "work list is not a permutation of the cells"
+        seg = eng.buf[f"seg{int(eng.e.cur)}"].cpu().numpy()
+        r, c = work // 64, work % 64
+        size = seg[r * 65 + c + 1] - seg[r * 65 + c]
+        assert np.all(np.diff(size) <= 0), "work list not ordered by decreasing home-segment size"
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, nm)
+    s, m = eng.export_grid(0)
+    _assert_same(s[:8], scal[:8], "grid_scalar_fields")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    for r in range(1, 6):
+        for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(r),
+                            base.export_particles(r)):
+            _assert_same(a, b, f"run {r} {nm}")
+        _assert_same(eng.counters(r)[0], base.counters(r)[0], f"run {r} no_cols")
+    assert no_cols.sum() > 0
+
+
 def test_launch_shapes_and_autotune_leave_results_unchanged(gpu):
     """The particle kernel's block size and SIPs per thread and the collision main-pass capacity change
     only how the work is launched: the state after two advection steps is the same bit for bit, and
@@ -239,7 +276,7 @@ def test_launch_shapes_and_autotune_leave_results_unchanged(gpu):
     grid, *parts = _case()
     parts = _prep(grid, parts)
     ref = None
-    for shape in [(0, 0, 0), (64, 2, 512), (512, 1, 160), (192, 2, 256)]:
+    for shape in [(0, 0, 0, 0, 0), (64, 2, 512, 1, 2, 2, 0), (512, 1, 160, 0, 8, 0, 5), (192, 2, 256, 16, 0, 1)]:
         eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
         eng.set_launch_shape(*shape)
         eng.adv_steps(2)
@@ -256,4 +293,6 @@ def test_launch_shapes_and_autotune_leave_results_unchanged(gpu):
     after = {k: v.cpu().numpy() for k, v in eng.snapshot()["buf"].items()}
     for k in before:
         _assert_same(after[k], before[k], f"{k} after autotune")
-    assert res["chosen"][0] in (64, 128, 256, 512) and res["chosen"][1] in (1, 2)
+    ch = res["chosen"]
+    assert ch["part_block"] in (64, 128, 256, 512) and ch["part_spt"] in (1, 2)
+    assert ch["col_work_per_cu"] in (0, 16, 32) and ch["dml_work_per_cu"] in (0, 32, 64)
