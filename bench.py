@@ -438,7 +438,7 @@ def main():
         tr = eng.autotune()
         ch = tr["chosen"]
         tune = {"chosen": ch,
-                "particle_ms": {f"{b}x{sp}": round(ms, 4) for (b, sp), ms in tr["particles"].items()},
+                "particle_ms": {f"{b}x{sp}/w{wp}": round(ms, 4) for (b, sp, wp), ms in tr["particles"].items()},
                 "dml_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["dml"].items()},
                 "collision_ms": {str(c): round(ms, 4) for c, ms in tr["collision"].items()},
                 "collision_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["collision_work"].items()},

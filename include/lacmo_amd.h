@@ -139,7 +139,8 @@ typedef struct LmEngine {
    * block (64..512, a multiple of 64) and SIPs per thread (1 or 2) of the particle kernel, and the
    * member capacity of the main collision pass (160, 256 or 512; larger cells take the overflow pass).
    */
-  int32_t part_block, part_spt, col_cap, pad3;
+  int32_t part_block, part_spt, col_cap;
+  int32_t tape_mode; /* taped substep of the adjoint: 0 = two passes, (T, p, S) then (m_w, m_s); 1 = one pass */
   /* largest entry of the E_col table (0: unknown); lets the collision kernel reject most pairs before it
      reads their table entry */
   double E_max;
@@ -158,6 +159,9 @@ typedef struct LmEngine {
   int32_t col_work_per_cu, dml_work_per_cu;
   int32_t *work; /* n_runs*nc + LM_WORK_BINS + 2: work list, histogram scratch, 2 work counters (NULL: off) */
   int32_t col_work_chunk, dml_work_chunk;
+  /* particle substep compiled for this many waves per SIMD (0: the default build; 5, 6 or 8: a smaller
+     register budget), one SIP per thread; set by an autotuner */
+  int32_t part_wpe, pad4;
 } LmEngine;
 
 #define LM_WORK_BINS 1024 /* home-segment sizes are binned up to LM_WORK_BINS - 1 for the work-list order */

@@ -59,10 +59,10 @@ class LmEngine(C.Structure):
         ("ev", _p), ("ev_cnt", _p), ("ev_count", _p), ("ev_off", _p), ("ev_order", _p), ("run_base", _p),
         ("slot_of", _p),
         ("tape_part", _p), ("tape_part_i", _p), ("tape_grid", _p), ("tape_fields", _p), ("ccd", _p),
-        ("part_block", _i32), ("part_spt", _i32), ("col_cap", _i32), ("pad3", _i32),
+        ("part_block", _i32), ("part_spt", _i32), ("col_cap", _i32), ("tape_mode", _i32),
         ("E_max", _d), ("vel_run_stride", _i64),
         ("col_work_per_cu", _i32), ("dml_work_per_cu", _i32), ("work", _p),
-        ("col_work_chunk", _i32), ("dml_work_chunk", _i32),
+        ("col_work_chunk", _i32), ("dml_work_chunk", _i32), ("part_wpe", _i32), ("pad4", _i32),
     ]
 
 
@@ -93,7 +93,7 @@ LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
                  "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev",
                  "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max", "vel_run_stride",
-                 "col_work_per_cu", "work", "dml_work_chunk"]
+                 "col_work_per_cu", "work", "dml_work_chunk", "part_wpe"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",

@@ -1068,45 +1068,70 @@ __device__ static inline bool near_home(const KView &v, int r, long long k, int 
 
 /*
  * Condensation update with its local Jacobian (adjoint tape): the Newton of
- * lmp_newton_dm on LmDual<5> values seeded in (m_w, m_s, T, p, S); the cell
+ * lmp_newton_dm on dual values seeded in (m_w, m_s, T, p, S); the cell
  * constants carry their derivatives in T and p from ccd.  The value part is
  * the double computation itself, so dm is the step's dm bit for bit.
+ * PART 0 carries all five directions (LmDual<5>); PART 1 only (m_w, m_s) and
+ * PART 2 only (T, p, S), so the tape can be written by two lighter passes
+ * (each direction's derivative arithmetic is independent of the others, so
+ * both ways write the same tape bit for bit).  PART 2 writes only its three
+ * columns; the others write the rest of the slot's tape entry.
  */
-template <int SOLUTE>
+template <int SOLUTE, int PART>
 __device__ static double newton_dm_tape(const LmPhys &P, const KView &v, long long k, long long n_tot, int gc,
                                         double dt_sub, double m_w, double m_s, double xi, const double *C) {
-  using D5 = LmDual<5>;
+  constexpr int N = PART == 0 ? 5 : (PART == 1 ? 2 : 3);
+  constexpr int oT = PART == 0 ? 2 : 0; /* direction of T (then p, S) when carried */
+  using DN = LmDual<N>;
   const double *G = v.ccd + (size_t)gc * 12;
-  D5 cx[CC_N];
-  cx[CC_T] = D5::seed(C[CC_T], 2);
-  cx[CC_S] = D5::seed(C[CC_S], 4);
+  DN cx[CC_N];
   const int ent[6] = {CC_LALPHA, CC_LBETA, CC_C1, CC_C2, CC_RHOW, CC_SIGW};
+  if constexpr (PART != 1) {
+    cx[CC_T] = DN::seed(C[CC_T], oT);
+    cx[CC_S] = DN::seed(C[CC_S], oT + 2);
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    D5 x(C[ent[q]]);
-    x.d[2] = G[2 * q];
-    x.d[3] = G[2 * q + 1];
-    cx[ent[q]] = x;
+    for (int q = 0; q < 6; ++q) {
+      DN x(C[ent[q]]);
+      x.d[oT] = G[2 * q];
+      x.d[oT + 1] = G[2 * q + 1];
+      cx[ent[q]] = x;
+    }
+  } else {
+    cx[CC_T] = DN(C[CC_T]);
+    cx[CC_S] = DN(C[CC_S]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cx[ent[q]] = DN(C[ent[q]]);
   }
-  const D5 mw = D5::seed(m_w, 0), ms = D5::seed(m_s, 1);
-  D5 R, w_s, rho;
+  DN mw(m_w), ms(m_s);
+  if constexpr (PART != 2) {
+    mw = DN::seed(m_w, 0);
+    ms = DN::seed(m_s, 1);
+  }
+  DN R, w_s, rho;
   lmp_R_ws_rho(P, SOLUTE, mw, ms, cx[CC_T], cx[CC_RHOW], R, w_s, rho);
-  const D5 sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, cx[CC_SIGW]);
-  const D5 dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, mw, ms, w_s, R, cx[CC_T], rho, cx, sigma_p);
+  const DN sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, cx[CC_SIGW]);
+  const DN dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, mw, ms, w_s, R, cx[CC_T], rho, cx, sigma_p);
   const size_t n = (size_t)n_tot;
   double *t = v.tape_part + k;
-  t[0] = 1.0 + dm.d[0];
+  if constexpr (PART != 2) {
+    t[0] = 1.0 + dm.d[0];
+    t[n] = dm.d[1];
+  }
+  if constexpr (PART != 1) {
 #pragma unroll
-  for (int q = 1; q < 5; ++q) t[q * n] = dm.d[q];
-  t[5 * n] = dm.v;
-  t[6 * n] = xi;
-  v.tape_part_i[k] = (int)(v.run_base[gc / v.nc] + v.id[k]);
-  v.tape_part_i[n + k] = gc;
+    for (int q = 0; q < 3; ++q) t[(2 + q) * n] = dm.d[oT + q];
+  }
+  if constexpr (PART != 2) {
+    t[5 * n] = dm.v;
+    t[6 * n] = xi;
+    v.tape_part_i[k] = (int)(v.run_base[gc / v.nc] + v.id[k]);
+    v.tape_part_i[n + k] = gc;
+  }
   return dm.v;
 }
 
 /* one SIP slot of the particle substep */
-template <int SOLUTE, bool POS_ONLY, bool TAPE>
+template <int SOLUTE, bool POS_ONLY, bool TAPE, int TPART = 0>
 __device__ __forceinline__ static void particle_update(const KView &v, long long k, long long n_tot, double dt_sub,
                                                        double dt_pos) {
   const LmPhys &P = c_P;
@@ -1131,7 +1156,7 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
     double R, w_s, rho;
     double dm;
     if (TAPE) {
-      dm = newton_dm_tape<SOLUTE>(P, v, k, n_tot, gc, dt_sub, m_w, m_s, xi, C);
+      dm = newton_dm_tape<SOLUTE, TPART>(P, v, k, n_tot, gc, dt_sub, m_w, m_s, xi, C);
     } else {
       lmp_R_ws_rho(P, SOLUTE, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
       const double sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, C[CC_SIGW]);
@@ -1218,6 +1243,50 @@ __global__ __launch_bounds__(TAPE ? TAPE_BLOCK : PART_BLOCK_MAX) void k_particle
     const long long k = base + (long long)s * blockDim.x;
     if (k < n_tot) particle_update<SOLUTE, POS_ONLY, TAPE>(v, k, n_tot, dt_sub, dt_pos);
   }
+}
+
+/*
+ * The particle substep (one SIP per thread) compiled for WPE waves per SIMD: the register budget drops
+ * from the 101 VGPRs of k_particles (4 waves) to 96, 80 or 64 at the price of a few spills; an autotuner
+ * picks (LmEngine.part_wpe).
+ */
+template <int SOLUTE, int WPE>
+__global__ __launch_bounds__(PART_BLOCK_MAX) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_particles_occ(
+    KView v, long long n_tot, double dt_sub, double dt_pos) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_tot) particle_update<SOLUTE, false, false>(v, k, n_tot, dt_sub, dt_pos);
+}
+
+template <int SOLUTE>
+static void launch_particles_occ(int wpe, dim3 grid, dim3 block, hipStream_t st, const KView &v, long long n,
+                                 double dt_sub, double dt_pos) {
+  if (wpe == 5)
+    hipLaunchKernelGGL((k_particles_occ<SOLUTE, 5>), grid, block, 0, st, v, n, dt_sub, dt_pos);
+  else if (wpe == 6)
+    hipLaunchKernelGGL((k_particles_occ<SOLUTE, 6>), grid, block, 0, st, v, n, dt_sub, dt_pos);
+  else
+    hipLaunchKernelGGL((k_particles_occ<SOLUTE, 8>), grid, block, 0, st, v, n, dt_sub, dt_pos);
+}
+
+/*
+ * Taped substep in two passes (LmEngine.tape_mode 0): k_tape_tps writes the (T, p, S) columns of each
+ * slot's condensation Jacobian from the pre-update state, then k_particles_tape<S, 1> runs the substep
+ * with the (m_w, m_s) columns.  Two narrower dual types need far fewer registers than one LmDual<5>.
+ */
+template <int SOLUTE>
+__global__ __launch_bounds__(TAPE_BLOCK) void k_tape_tps(KView v, long long n_tot, double dt_sub) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_tot) return;
+  const int gc = v.cell[k];
+  if (gc < 0) return;
+  newton_dm_tape<SOLUTE, 2>(c_P, v, k, n_tot, gc, dt_sub, v.m_w[k], v.m_s[k], v.xi[k], v.cc + (size_t)gc * CC_N);
+}
+
+template <int SOLUTE, int PART>
+__global__ __launch_bounds__(TAPE_BLOCK) void k_particles_tape(KView v, long long n_tot, double dt_sub,
+                                                               double dt_pos) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_tot) particle_update<SOLUTE, false, true, PART>(v, k, n_tot, dt_sub, dt_pos);
 }
 
 /* water_removed[r] += xi*m_w of this substep's removed SIPs, ascending id (integration.py:620-628) */
@@ -2246,6 +2315,11 @@ static int check_engine(const LmEngine *e) {
       e->part_spt > 2 || (e->col_cap && e->col_cap != 160 && e->col_cap != 256 && e->col_cap != 512))
     return set_err(LM_ERR_ARG, "bad launch shape (block 64..512 in steps of 64, 1 or 2 SIPs per thread, "
                                "collision capacity 160, 256 or 512)");
+  if (e->col_work_per_cu < 0 || e->dml_work_per_cu < 0 || e->col_work_chunk < 0 || e->dml_work_chunk < 0 ||
+      (e->part_wpe && e->part_wpe != 5 && e->part_wpe != 6 && e->part_wpe != 8) || e->tape_mode < 0 ||
+      e->tape_mode > 1)
+    return set_err(LM_ERR_ARG, "bad launch shape (negative work-list shape, particle waves per SIMD not 0, 5, 6 "
+                               "or 8, or tape mode not 0 or 1)");
   if (e->ev_mode && (!e->ev || !e->ev_off || !e->ev_order || !e->run_base || e->ev_cap <= 0))
     return set_err(LM_ERR_ARG, "event log buffers missing");
   if (e->ev_mode == 1 && (!e->ev_cnt || !e->ev_count)) return set_err(LM_ERR_ARG, "event counters missing");
@@ -2340,6 +2414,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, part_block), (int64_t)offsetof(LmEngine, E_max),
       (int64_t)offsetof(LmEngine, vel_run_stride), (int64_t)offsetof(LmEngine, col_work_per_cu),
       (int64_t)offsetof(LmEngine, work), (int64_t)offsetof(LmEngine, dml_work_chunk),
+      (int64_t)offsetof(LmEngine, part_wpe),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -2515,12 +2590,24 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
     if (e->tape_part) {
       hipLaunchKernelGGL(k_ccd, dim3((ncell + 255) / 256), dim3(256), 0, S(stream), v);
       const dim3 gt((unsigned)((n + TAPE_BLOCK - 1) / TAPE_BLOCK));
+      if (e->tape_mode == 1) {
+        /* one pass on LmDual<5> */
+        if (e->solute == 0)
+          hipLaunchKernelGGL((k_particles_tape<0, 0>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+        else
+          hipLaunchKernelGGL((k_particles_tape<1, 0>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+      } else if (e->solute == 0) {
+        hipLaunchKernelGGL(k_tape_tps<0>, gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub);
+        hipLaunchKernelGGL((k_particles_tape<0, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+      } else {
+        hipLaunchKernelGGL(k_tape_tps<1>, gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub);
+        hipLaunchKernelGGL((k_particles_tape<1, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+      }
+    } else if (spt != 2 && e->part_wpe > 0) {
       if (e->solute == 0)
-        hipLaunchKernelGGL((k_particles<0, false, true, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
-                           dt_sub_pos);
+        launch_particles_occ<0>(e->part_wpe, part_grid(e, 1), b, S(stream), v, n, dt_sub, dt_sub_pos);
       else
-        hipLaunchKernelGGL((k_particles<1, false, true, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
-                           dt_sub_pos);
+        launch_particles_occ<1>(e->part_wpe, part_grid(e, 1), b, S(stream), v, n, dt_sub, dt_sub_pos);
     } else if (spt == 2) {
       if (e->solute == 0)
         hipLaunchKernelGGL((k_particles<0, false, false, 2>), part_grid(e, 2), b, 0, S(stream), v, n, dt_sub,

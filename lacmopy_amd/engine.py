@@ -357,12 +357,13 @@ class Engine:
         check(self.L.lm_engine_check(C.byref(self.e), self.stream))
 
     # ------------------------------------------------------------ autotune
-    SHAPE_KEYS = ("part_block", "part_spt", "col_cap", "col_work_per_cu", "col_work_chunk", "dml_work_per_cu",
-                  "dml_work_chunk")
+    SHAPE_KEYS = ("part_block", "part_spt", "part_wpe", "col_cap", "col_work_per_cu", "col_work_chunk",
+                  "dml_work_per_cu", "dml_work_chunk")
 
     def set_launch_shape(self, part_block=0, part_spt=0, col_cap=0, col_work_per_cu=None, dml_work_per_cu=None,
-                         col_work_chunk=None, dml_work_chunk=None):
-        """Launch shape of the particle kernel (threads per block, SIPs per thread), the member capacity
+                         col_work_chunk=None, dml_work_chunk=None, part_wpe=None):
+        """Launch shape of the particle kernel (threads per block, SIPs per thread, and the waves per SIMD
+        it is compiled for: 0 the default build, 5, 6 or 8 with one SIP per thread), the member capacity
         of the main collision pass, and the persistent main collision and Delta m_l passes over the
         (run, cell) work list: workgroups per compute unit (0: one workgroup per cell) and items taken per
         atomic grab (0: static round robin); None keeps the current value.  0 restores the defaults (256,
@@ -370,6 +371,10 @@ class Engine:
         self.drop_graph()
         e = self.e
         e.part_block, e.part_spt, e.col_cap = int(part_block), int(part_spt), int(col_cap)
+        if part_wpe is not None:
+            if int(part_wpe) not in (0, 5, 6, 8):
+                raise ValueError("part_wpe must be 0, 5, 6 or 8")
+            e.part_wpe = int(part_wpe)
         for k, val in (("col_work_per_cu", col_work_per_cu), ("dml_work_per_cu", dml_work_per_cu),
                        ("col_work_chunk", col_work_chunk), ("dml_work_chunk", dml_work_chunk)):
             if val is not None:
@@ -380,15 +385,15 @@ class Engine:
     def launch_shape(self):
         return {k: int(getattr(self.e, k)) for k in self.SHAPE_KEYS}
 
-    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), col_caps=(160, 256, 512),
+    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), wpes=(0, 5, 6, 8), col_caps=(160, 256, 512),
                  col_work=((0, 0), (16, 0), (16, 1), (16, 4), (32, 0), (32, 4)),
                  dml_work=((0, 0), (32, 0), (32, 16), (64, 0), (64, 16)), reps=3):
         """Times candidate launch shapes on the engine's current state with HIP events on its stream and
         keeps the fastest: the particle substep (with its Delta m_l pass) for every (block, SIPs per
-        thread), then the Delta m_l pass's persistent shapes dml_work = (workgroups per CU, chunk) on the
+        thread, waves per SIMD of the build; the non-default builds with one SIP per thread), then the Delta m_l pass's persistent shapes dml_work = (workgroups per CU, chunk) on the
         substep, the collision step for every main-pass capacity, then its persistent shapes col_work.
         (0, 0) is one workgroup per cell.  The state is restored before every candidate and at the end;
-        results do not depend on the launch shape.  Returns {"particles": {(block, spt): ms}, "dml":
+        results do not depend on the launch shape.  Returns {"particles": {(block, spt, wpe): ms}, "dml":
         {(wpc, chunk): ms}, "collision": {cap: ms}, "collision_work": {(wpc, chunk): ms}, "chosen":
         launch_shape()}."""
         snap = self.snapshot()
@@ -416,10 +421,13 @@ class Engine:
         self.build_work_list()
         for b in blocks:
             for spt in spts:
-                shape.update(part_block=b, part_spt=spt, dml_work_per_cu=0, dml_work_chunk=0)
-                self.set_launch_shape(**shape)
-                res["particles"][(b, spt)] = timed(substep)
-        shape["part_block"], shape["part_spt"] = min(res["particles"], key=res["particles"].get)
+                for wpe in wpes:
+                    if wpe and spt != 1:
+                        continue
+                    shape.update(part_block=b, part_spt=spt, part_wpe=wpe, dml_work_per_cu=0, dml_work_chunk=0)
+                    self.set_launch_shape(**shape)
+                    res["particles"][(b, spt, wpe)] = timed(substep)
+        shape["part_block"], shape["part_spt"], shape["part_wpe"] = min(res["particles"], key=res["particles"].get)
         for wpc, chunk in dml_work:
             shape.update(dml_work_per_cu=wpc, dml_work_chunk=chunk)
             self.set_launch_shape(**shape)

@@ -94,6 +94,25 @@ def test_taped_rerun_reproduces_window(gpu):
     w.gradient(w_nc=1.0, w_lwp=1.0, verify=True)
 
 
+def test_two_pass_and_one_pass_tapes_give_the_same_gradient(gpu):
+    """The taped substep in two passes ((T, p, S) columns, then (m_w, m_s); the default) and in one pass on
+    LmDual<5> write the same tape bit for bit (each direction's derivative arithmetic is independent of the
+    others); the gradients then agree up to the order of the reverse sweep's floating-point atomics."""
+    eng, w, _ = _window(n_runs=2, n_steps=4, K=2)
+    w.run(record=True)
+    grads, tapes = [], []
+    for mode in (0, 1):
+        eng.e.tape_mode = mode
+        grads.append([g.cpu().numpy() for g in w.gradient(w_nc=1.0, w_lwp=1.0, verify=True)])
+        tapes.append({k: t.cpu().numpy() for k, t in w.tape.items()})
+    eng.e.tape_mode = 0
+    for k in ("part", "part_i"):
+        assert np.array_equal(tapes[0][k], tapes[1][k]), f"tape {k} differs between the two-pass and one-pass runs"
+    for a, b, nm in zip(grads[0], grads[1], ("xi", "m_s")):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * np.abs(b).max(), err_msg=f"d J / d {nm}")
+    assert np.abs(grads[0][0]).max() > 0
+
+
 def test_gradient_vs_finite_differences_sips(gpu):
     eng, w, _ = _window(n_runs=2)
     slots = w.slot_map()

@@ -276,7 +276,8 @@ def test_launch_shapes_and_autotune_leave_results_unchanged(gpu):
     grid, *parts = _case()
     parts = _prep(grid, parts)
     ref = None
-    for shape in [(0, 0, 0, 0, 0), (64, 2, 512, 1, 2, 2, 0), (512, 1, 160, 0, 8, 0, 5), (192, 2, 256, 16, 0, 1)]:
+    for shape in [(0, 0, 0, 0, 0), (64, 2, 512, 1, 2, 2, 0), (512, 1, 160, 0, 8, 0, 5, 5), (192, 2, 256, 16, 0, 1),
+                  (128, 1, 0, 0, 0, 0, 0, 6), (256, 1, 0, 0, 0, 0, 0, 8)]:
         eng = _engine(grid, parts, cfg, E, lnR0, lnf, 11)
         eng.set_launch_shape(*shape)
         eng.adv_steps(2)
