@@ -700,6 +700,58 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   return 2;
 }
 
+/*
+ * Row bounds of the collision pass filter.  With the cell-wide X = max xi,
+ * Rc = max |R| and velocity ranges [u_lo, u_hi], [w_lo, w_hi] (NaN entries
+ * skipped: a pair with a NaN in xi_i, R, u or w never fires, and xi_max is
+ * xi_i when xi_j is NaN), every rounded factor of a pair's Pl and d2 in row i
+ * is at most the same expression evaluated on the bounds, because IEEE
+ * rounding is monotone: xi_max <= X, |fl(R_i + R_j)| <= fl(|R_i| + Rc),
+ * E <= Eb, |fl(u_i - u_j)| <= max(fl(u_i - u_lo), fl(u_hi - u_i)).  So every
+ * pair of the row has Pl^2 d2 <= B_i = (P_i^2) d2_i, and a quad whose smallest
+ * threshold thr has thr^2 (1 - 2^-40) > B_i holds no pair with p >= thr (the
+ * per-pair filter's argument).  P_i <= 1e-140 with d2_i < 1e100 bounds p below
+ * 1e-89 and is stored as B = 1e-178.  B is kept as a float rounded up (a
+ * larger B only rejects less); NaN or inf never rejects.  X only decreases
+ * with events; Rc can grow, so the bounds are recomputed after each event.
+ */
+template <int CAP>
+__device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n, const double *sX, const double *sR,
+                                  const double *sU, const double *sW, float *sB) {
+  const int lane = threadIdx.x;
+  double X = -INFINITY, Rc = 0.0, ulo = INFINITY, uhi = -INFINITY, wlo = INFINITY, whi = -INFINITY;
+  for (int p = lane; p < n; p += WAVE) {
+    X = fmax(X, sX[p]);
+    Rc = fmax(Rc, fabs(sR[p]));
+    ulo = fmin(ulo, sU[p]);
+    uhi = fmax(uhi, sU[p]);
+    wlo = fmin(wlo, sW[p]);
+    whi = fmax(whi, sW[p]);
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    X = fmax(X, __shfl_xor(X, off));
+    Rc = fmax(Rc, __shfl_xor(Rc, off));
+    ulo = fmin(ulo, __shfl_xor(ulo, off));
+    uhi = fmax(uhi, __shfl_xor(uhi, off));
+    wlo = fmin(wlo, __shfl_xor(wlo, off));
+    whi = fmax(whi, __shfl_xor(whi, off));
+  }
+  for (int p = lane; p < n; p += WAVE) {
+    const double du = fmax(sU[p] - ulo, uhi - sU[p]), dw = fmax(sW[p] - wlo, whi - sW[p]);
+    const double d2 = du * du + dw * dw;
+    const double sr = fabs(sR[p]) + Rc;
+    const double Pb = (((X * dtdV) * ((P.pi * sr) * sr)) * Eb) * 1.0E-12;
+    double B = INFINITY;
+    if (Pb > 1.0E-140)
+      B = (Pb * Pb) * d2;
+    else if ((Pb <= 1.0E-140) & (d2 < 1.0E100))
+      B = 1.0E-178;
+    float f = (float)B; /* B >= 0 or NaN: rounded down, the next float up is above B */
+    if ((double)f < B) f = __int_as_float(__float_as_int(f) + 1);
+    sB[p] = f;
+  }
+}
+
 /* pair number of (i, j) in the reference's row-major loop */
 __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
 
@@ -736,6 +788,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
      gather scratch lives inside the u/w arrays until the ranks are known. */
   __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
   __shared__ int sIND[CAP], sSL[CAP], sQ[CAP + 1];
+  __shared__ float sB[CAP];
   __shared__ NbrTab tb;
   __shared__ int sSeq;
   int *sSlot = (int *)sU;
@@ -791,6 +844,10 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   const bool use_tab = TAB;
   const long long roff = INJ ? v.rnd_off[gc] : 0;
   const double dtdV = v.dt_col_over_dV, pi = P.pi;
+  /* the E bound of the row filter: the table's largest entry (unknown: no row filter) or the constant */
+  const double Eb = TAB ? (v.E_max > 0.0 ? v.E_max : INFINITY) : v.E_const;
+  row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB);
+  __syncthreads();
   const uint32_t key_run = (uint32_t)r, key_step = *v.col_step_dev, key_cell = (uint32_t)c;
   double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
   const int nQ = sQ[n - 1];
@@ -811,6 +868,22 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
       if (pair_cnt(i, dlast - i, n) < k_next) continue;
       lm_u32x4 blk = {{0u, 0u, 0u, 0u}};
       if (!INJ) blk = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)(4 * b), 0);
+      {
+        /* row filter: the quad's smallest threshold against the row bound (row_bounds) */
+        double thq = 1.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int jr = 4 * b + t - i;
+          const bool valid = (jr > i) & (jr < n) & (pair_cnt(i, jr, n) >= k_next);
+          double th;
+          if (INJ)
+            th = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 1.0;
+          else
+            th = (double)(lm_word(blk, (uint32_t)t) >> 5) * (1.0 / 134217728.0);
+          if (valid) thq = fmin(thq, th);
+        }
+        if ((thq > 0.0) & ((thq * thq) * (1.0 - 0x1p-40) > (double)sB[i])) continue;
+      }
       const double xi_i = sX[i], R_i = sR[i], u_i = sU[i], w_i = sW[i];
       const double *Erow = TAB ? v.E + (size_t)sIND[i] * v.nb : nullptr;
       /*
@@ -818,8 +891,8 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
        * square root, Pl^2 d2 (Pl = (xi_max dt/dV) (((pi s) s) E) 1e-12) equals p^2 within 10 roundings,
        * so Pl^2 d2 < thr^2 (1 - 2^-40) proves p < thr, thr = the 27-bit lower bound of the pair's
        * uniform (or the injected uniform): the pair does not fire and is no multiple collection.  For
-       * Pl <= 1e-140 (and d2 < 1e100) p < 1e-89 < thr whenever thr > 0.  Everything else -- thr = 0,
-       * overflow, NaN -- takes the exact path below.
+       * Pl <= 1e-140 (and d2 < 1e100) p < 1e-89 < thr whenever thr^2 > 1e-178.  Everything else --
+       * thr = 0, overflow, NaN -- takes the exact path below.
        */
       unsigned maybe = 0;
 #pragma unroll
@@ -853,7 +926,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
           const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
           const double Pl = (geo * Ec) * 1.0E-12;
           const bool big = Pl > 1.0E-140;
-          no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 0.0));
+          no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 1.0E-178));
         }
         maybe |= (unsigned)(valid & !no) << t;
       }
@@ -920,6 +993,11 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     __syncthreads();
     ++n_events;
     k_next = ecnt + 1;
+    if (n_events < SPEC_MAX_EVENTS) {
+      /* the event may have grown a radius: fresh row bounds for the next pass */
+      row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB);
+      __syncthreads();
+    }
   }
 
   if (!finished) {
@@ -995,7 +1073,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
 /* main pass: one block per cell or persistent blocks over the work list; overflow pass: the queued cells,
    grid-strided */
 template <int CAP, bool TAB, bool INJ>
-__global__ __launch_bounds__(WAVE) void k_collision(KView v, int mode, int chunk) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_collision(KView v, int mode, int chunk) {
   CellSource src = cell_source(mode, chunk, 0);
   for (;;) {
     const int gc = next_cell(v, src, v.ovf_cnt);
