@@ -197,6 +197,7 @@ def adjoint_cost(seed0, spc, window, every, spinup, check=0, case="icmw75"):
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
     # aerosol-mode parameters (N, r_med, sigma of both modes) -> window-start xi, m_s -> J_nc, by the chain rule
+    eng.restore(w.start)
     slots = w.slot_map()
     cur = int(eng.e.cur)
     theta = torch.tensor([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6], dtype=torch.float64, device="cuda",

@@ -829,14 +829,21 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     sR[p] = R;
     sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
   }
-  /* quads per row: row i holds diagonals d = 2i+1 .. i+n-1, blocks (2i+1)>>2 .. (i+n-1)>>2 */
-  if (lane == 0) {
-    int acc = 0;
-    for (int i = 0; i < n - 1; ++i) {
-      sQ[i] = acc;
-      acc += ((i + n - 1) >> 2) - ((2 * i + 1) >> 2) + 1;
+  /* quads per row: row i holds diagonals d = 2i+1 .. i+n-1, blocks (2i+1)>>2 .. (i+n-1)>>2; sQ[i] = quads
+     of the rows before i (wave-wide exclusive scan, 64 rows at a time), sQ[n-1] = all quads */
+  {
+    int carry = 0;
+    for (int r0 = 0; r0 < n; r0 += WAVE) {
+      const int r = r0 + lane;
+      const int c = r < n - 1 ? ((r + n - 1) >> 2) - ((2 * r + 1) >> 2) + 1 : 0;
+      int incl = c;
+      for (int off = 1; off < WAVE; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+      }
+      if (r < n) sQ[r] = carry + incl - c;
+      carry += __shfl(incl, WAVE - 1);
     }
-    sQ[n - 1] = acc;
   }
   __syncthreads();
 
