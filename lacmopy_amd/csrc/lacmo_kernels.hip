@@ -1168,10 +1168,18 @@ __device__ static double newton_dm_tape(const LmPhys &P, const KView &v, long lo
   constexpr int N = PART == 0 ? 5 : (PART == 1 ? 2 : 3);
   constexpr int oT = PART == 0 ? 2 : 0; /* direction of T (then p, S) when carried */
   using DN = LmDual<N>;
-  const double *G = v.ccd + (size_t)gc * 12;
-  DN cx[CC_N];
-  const int ent[6] = {CC_LALPHA, CC_LBETA, CC_C1, CC_C2, CC_RHOW, CC_SIGW};
-  if constexpr (PART != 1) {
+  DN dm;
+  if constexpr (PART == 1) {
+    /* (m_w, m_s) columns: the cell constants enter as doubles (no derivative arithmetic on them) */
+    const DN mw = DN::seed(m_w, 0), ms = DN::seed(m_s, 1);
+    DN R, w_s, rho;
+    lmp_R_ws_rho(P, SOLUTE, mw, ms, C[CC_T], C[CC_RHOW], R, w_s, rho);
+    const DN sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, C[CC_SIGW]);
+    dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, mw, ms, w_s, R, C[CC_T], rho, C, sigma_p);
+  } else {
+    const double *G = v.ccd + (size_t)gc * 12;
+    DN cx[CC_N];
+    const int ent[6] = {CC_LALPHA, CC_LBETA, CC_C1, CC_C2, CC_RHOW, CC_SIGW};
     cx[CC_T] = DN::seed(C[CC_T], oT);
     cx[CC_S] = DN::seed(C[CC_S], oT + 2);
 #pragma unroll
@@ -1181,21 +1189,21 @@ __device__ static double newton_dm_tape(const LmPhys &P, const KView &v, long lo
       x.d[oT + 1] = G[2 * q + 1];
       cx[ent[q]] = x;
     }
-  } else {
-    cx[CC_T] = DN(C[CC_T]);
-    cx[CC_S] = DN(C[CC_S]);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) cx[ent[q]] = DN(C[ent[q]]);
+    if constexpr (PART == 2) {
+      /* (T, p, S) columns: the masses enter as doubles */
+      DN R, rho;
+      double w_s;
+      lmp_R_ws_rho(P, SOLUTE, m_w, m_s, cx[CC_T], cx[CC_RHOW], R, w_s, rho);
+      const DN sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, cx[CC_SIGW]);
+      dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, m_w, m_s, w_s, R, cx[CC_T], rho, cx, sigma_p);
+    } else {
+      const DN mw = DN::seed(m_w, 0), ms = DN::seed(m_s, 1);
+      DN R, w_s, rho;
+      lmp_R_ws_rho(P, SOLUTE, mw, ms, cx[CC_T], cx[CC_RHOW], R, w_s, rho);
+      const DN sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, cx[CC_SIGW]);
+      dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, mw, ms, w_s, R, cx[CC_T], rho, cx, sigma_p);
+    }
   }
-  DN mw(m_w), ms(m_s);
-  if constexpr (PART != 2) {
-    mw = DN::seed(m_w, 0);
-    ms = DN::seed(m_s, 1);
-  }
-  DN R, w_s, rho;
-  lmp_R_ws_rho(P, SOLUTE, mw, ms, cx[CC_T], cx[CC_RHOW], R, w_s, rho);
-  const DN sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, cx[CC_SIGW]);
-  const DN dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, mw, ms, w_s, R, cx[CC_T], rho, cx, sigma_p);
   const size_t n = (size_t)n_tot;
   double *t = v.tape_part + k;
   if constexpr (PART != 2) {

@@ -43,6 +43,22 @@ struct LmDual {
 
 #define LMDUAL template <int N> LM_FN
 
+/* expressions mixing doubles and LmDual<N> are LmDual<N> (LmCommonT of lm_physics.h) */
+template <class A, class B>
+struct LmCommon2;
+template <int N>
+struct LmCommon2<LmDual<N>, double> {
+  using type = LmDual<N>;
+};
+template <int N>
+struct LmCommon2<double, LmDual<N>> {
+  using type = LmDual<N>;
+};
+template <int N>
+struct LmCommon2<LmDual<N>, LmDual<N>> {
+  using type = LmDual<N>;
+};
+
 LMDUAL LmDual<N> operator+(const LmDual<N> &a, const LmDual<N> &b) {
   LmDual<N> r;
   r.v = a.v + b.v;

@@ -43,6 +43,31 @@ enum {
 /* templates over the value type: double for the step, LmDual<N> (lm_dual.h) for its local Jacobians */
 #define LMDT template <class R> LMD
 
+/*
+ * The condensation functions below take each argument's type on its own, so a caller can mix doubles
+ * (no derivative) with dual numbers; LmCommonT<...> is the type of an expression over the arguments
+ * (double if all are double; lm_dual.h adds the dual cases).  The value of every operation is the same
+ * double operation in the same order whatever the types.
+ */
+template <class A, class B>
+struct LmCommon2;
+template <>
+struct LmCommon2<double, double> {
+  using type = double;
+};
+template <class... T>
+struct LmCommon;
+template <class A>
+struct LmCommon<A> {
+  using type = A;
+};
+template <class A, class B, class... T>
+struct LmCommon<A, B, T...> {
+  using type = typename LmCommon<typename LmCommon2<A, B>::type, T...>::type;
+};
+template <class... T>
+using LmCommonT = typename LmCommon<T...>::type;
+
 LMDT R lmp_poly(const double *par, int n, R x) { /* algebra.py:11-34 */
   R res = par[0] * x + par[1];
   for (int k = 2; k < n; ++k) res = res * x + par[k];
@@ -55,7 +80,8 @@ LMDT R lmp_density_water(const LmPhys &P, R T) {
   return P.par_water_dens[0] + P.par_water_dens[1] * (d * d);
 }
 /* material_properties.py:60-84 */
-LMDT R lmp_density_NaCl(const LmPhys &P, R w, R T) {
+template <class W, class TT>
+LMD LmCommonT<W, TT> lmp_density_NaCl(const LmPhys &P, W w, TT T) {
   const double *p = P.par_sol_dens_NaCl;
   return p[0] + p[1] * w + p[2] * T + p[3] * w * w + p[4] * w * T + p[5] * T * T;
 }
@@ -65,7 +91,8 @@ LMDT R lmp_sigma_water(R T) {
   return 0.2358 * lm_pow(tau, 1.256) * (1 - 0.625 * tau);
 }
 /* microphysics.py:83-100 (the exponent is one_third = RN(1/3), lm_pow_third's) */
-LMDT R lmp_radius(const LmPhys &P, R m, R rho) {
+template <class M, class RH>
+LMD LmCommonT<M, RH> lmp_radius(const LmPhys &P, M m, RH rho) {
   return lm_pow_third(P.four_pi_over_three_inv * m / rho);
 }
 /* microphysics.py:989-1006 */
@@ -78,21 +105,24 @@ LMDT R lmp_l_beta(const LmPhys &P, R T, R D_v) {
 }
 
 /* Solution density at the cell temperature: AS (material_properties.py:89-115) or NaCl. */
-LMDT R lmp_density_solution(const LmPhys &P, int solute, R w, R T, R rho_w_T) {
-  return solute == 0 ? rho_w_T * lmp_poly(P.par_rho_AS, 4, w) : lmp_density_NaCl(P, w, T);
+template <class W, class TT, class RW>
+LMD LmCommonT<W, TT, RW> lmp_density_solution(const LmPhys &P, int solute, W w, TT T, RW rho_w_T) {
+  using RT = LmCommonT<W, TT, RW>;
+  return solute == 0 ? RT(rho_w_T * lmp_poly(P.par_rho_AS, 4, w)) : RT(lmp_density_NaCl(P, w, T));
 }
 
 /* microphysics.py:179-234 */
-template <class V>
-LMD void lmp_R_ws_rho(const LmPhys &P, int solute, V m_w, V m_s, V T, V rho_w_T, V &R, V &w_s, V &rho) {
-  const V m_p = m_w + m_s;
+template <class MW, class MS, class TT, class RW, class RR, class WS, class RH>
+LMD void lmp_R_ws_rho(const LmPhys &P, int solute, MW m_w, MS m_s, TT T, RW rho_w_T, RR &R, WS &w_s, RH &rho) {
+  const LmCommonT<MW, MS> m_p = m_w + m_s;
   w_s = m_s / m_p;
   rho = lmp_density_solution(P, solute, w_s, T, rho_w_T);
   R = lmp_radius(P, m_p, rho);
 }
 
 /* Surface tension of the solution: material_properties.py:274-292 (NaCl), 346-366 (AS). */
-LMDT R lmp_sigma_solution(const LmPhys &P, int solute, R w, R sigma_w_T) {
+template <class W, class SW>
+LMD LmCommonT<W, SW> lmp_sigma_solution(const LmPhys &P, int solute, W w, SW sigma_w_T) {
   return sigma_w_T * (1.0 + (solute == 0 ? P.par_sigma_AS : P.par_sigma_NaCl) * w / (1. - w));
 }
 
@@ -100,37 +130,39 @@ LMDT R lmp_sigma_solution(const LmPhys &P, int solute, R w, R sigma_w_T) {
  * Condensation mass rate gamma and d gamma / d m_w (Fukuta-Walter):
  * microphysics.py:1359-1469 (AS), 1237-1353 (NaCl).
  */
-LMDT void lmp_mass_rate(const LmPhys &P, int solute, R m_w, R m_s, R w_s, R R_p, R T_p, R rho_p, R S_amb,
-                        const R *cc, R sigma_p, R &gamma, R &dgamma) {
-  const R l_alpha_plus_R_p = 1.0E-6 * (R_p + cc[CC_LALPHA]);
-  const R l_beta_plus_R_p = 1.0E-6 * (R_p + cc[CC_LBETA]);
-  const R m_p_inv_SI = 1.0E18 / (m_w + m_s);
-  R drho_dm_over_rho;
+template <class MW, class MS, class WS, class RP, class TP, class RH, class SA, class CT, class SG, class G>
+LMD void lmp_mass_rate(const LmPhys &P, int solute, MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p, SA S_amb,
+                       const CT *cc, SG sigma_p, G &gamma, G &dgamma) {
+  const LmCommonT<RP, CT> l_alpha_plus_R_p = 1.0E-6 * (R_p + cc[CC_LALPHA]);
+  const LmCommonT<RP, CT> l_beta_plus_R_p = 1.0E-6 * (R_p + cc[CC_LBETA]);
+  const LmCommonT<MW, MS> m_p_inv_SI = 1.0E18 / (m_w + m_s);
+  LmCommonT<CT, MW, MS, RH, WS, TP> drho_dm_over_rho;
   if (solute == 0) {
     drho_dm_over_rho = -cc[CC_RHOW] * m_p_inv_SI / rho_p * w_s * lmp_poly(P.par_rho_deriv_AS, 3, w_s);
   } else {
     const double *p = P.par_sol_dens_NaCl;
     drho_dm_over_rho = -w_s * m_p_inv_SI / rho_p * (p[1] + 2.0 * p[3] * w_s + p[4] * T_p);
   }
-  const R dR_p_dm_over_R_p = P.one_third * (m_p_inv_SI - drho_dm_over_rho);
-  const R dR_p_dm = 1.0E-6 * dR_p_dm_over_R_p * R_p;
-  const R eps_k = 2.0E6 * sigma_p / (P.R_v * T_p * rho_p * R_p);
-  const R kelvin_term = lm_exp(eps_k);
+  const auto dR_p_dm_over_R_p = P.one_third * (m_p_inv_SI - drho_dm_over_rho);
+  const auto dR_p_dm = 1.0E-6 * dR_p_dm_over_R_p * R_p;
+  const auto eps_k = 2.0E6 * sigma_p / (P.R_v * T_p * rho_p * R_p);
+  const auto kelvin_term = lm_exp(eps_k);
   const double *aw = solute == 0 ? P.par_wat_act_AS : P.par_wat_act_NaCl;
   const double *daw = solute == 0 ? P.par_wat_act_deriv_AS : P.par_wat_act_deriv_NaCl;
   const double psig = solute == 0 ? P.par_sigma_AS : P.par_sigma_NaCl;
-  const R a_w = lmp_poly(aw, 5, w_s);
-  const R da_w_dm = -m_p_inv_SI * w_s * lmp_poly(daw, 4, w_s);
-  const R dsigma_dm = -psig * cc[CC_SIGW] * m_p_inv_SI * (w_s / ((1. - w_s) * (1. - w_s)));
-  R S_eq = a_w * kelvin_term;
-  const R dSeq_dm = da_w_dm * kelvin_term + S_eq * eps_k * (dsigma_dm / sigma_p - drho_dm_over_rho - dR_p_dm_over_R_p);
-  const R c1 = cc[CC_C1], c2 = cc[CC_C2];
-  const R f3 = 1.0 / ((l_alpha_plus_R_p)*S_eq * c1 + (l_beta_plus_R_p)*c2);
-  const R f1f3 = 4.0 * P.pi * R_p * R_p * f3;
-  const R dg1_dm = (dSeq_dm * (l_alpha_plus_R_p) + S_eq * dR_p_dm) * c1 + dR_p_dm * c2;
-  S_eq = S_amb - S_eq;
-  gamma = 1.0E6 * f1f3 * S_eq;
-  dgamma = 1.0E-12 * f1f3 * (S_eq * (2.0 * dR_p_dm_over_R_p - f3 * dg1_dm) - dSeq_dm);
+  const auto a_w = lmp_poly(aw, 5, w_s);
+  const auto da_w_dm = -m_p_inv_SI * w_s * lmp_poly(daw, 4, w_s);
+  const auto dsigma_dm = -psig * cc[CC_SIGW] * m_p_inv_SI * (w_s / ((1. - w_s) * (1. - w_s)));
+  const auto S_eq = a_w * kelvin_term;
+  const auto dSeq_dm =
+      da_w_dm * kelvin_term + S_eq * eps_k * (dsigma_dm / sigma_p - drho_dm_over_rho - dR_p_dm_over_R_p);
+  const CT c1 = cc[CC_C1], c2 = cc[CC_C2];
+  const auto f3 = 1.0 / ((l_alpha_plus_R_p)*S_eq * c1 + (l_beta_plus_R_p)*c2);
+  const auto f1f3 = 4.0 * P.pi * R_p * R_p * f3;
+  const auto dg1_dm = (dSeq_dm * (l_alpha_plus_R_p) + S_eq * dR_p_dm) * c1 + dR_p_dm * c2;
+  const auto S_sup = S_amb - S_eq; /* the reference reuses the name S_eq for S - S_eq */
+  gamma = 1.0E6 * f1f3 * S_sup;
+  dgamma = 1.0E-12 * f1f3 * (S_sup * (2.0 * dR_p_dm_over_R_p - f3 * dg1_dm) - dSeq_dm);
 }
 
 /*
@@ -139,15 +171,18 @@ LMDT void lmp_mass_rate(const LmPhys &P, int solute, R m_w, R m_s, R w_s, R R_p,
  * and surface tension for both solutes, as the reference does).
  * Returns dm = mass_new - m_w.
  */
-LMDT R lmp_newton_dm(const LmPhys &P, int solute, double dt_sub, int no_iter, R m_w, R m_s, R w_s, R R_p, R T_p,
-                      R rho_p, const R *cc, R sigma_p) {
+template <class MW, class MS, class WS, class RP, class TP, class RH, class CT, class SG>
+LMD LmCommonT<MW, MS, WS, RP, TP, RH, CT, SG> lmp_newton_dm(const LmPhys &P, int solute, double dt_sub, int no_iter,
+                                                            MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p,
+                                                            const CT *cc, SG sigma_p) {
+  using R = LmCommonT<MW, MS, WS, RP, TP, RH, CT, SG>;
   const R m_w_effl = m_s * (P.w_s_max_AS_inv - 1.0);
-  const R S_amb = cc[CC_S];
+  const CT S_amb = cc[CC_S];
   R gamma, dgamma;
   lmp_mass_rate(P, solute, m_w, m_s, w_s, R_p, T_p, rho_p, S_amb, cc, sigma_p, gamma, dgamma);
   R d = dt_sub * dgamma;
   R denom_inv = d < 0.9 ? R(1.0 / (1.0 - d)) : R(10.0);
-  R mass_new = lm_np_max(m_w_effl, m_w + dt_sub * gamma * denom_inv);
+  R mass_new = lm_np_max(m_w_effl, R(m_w + dt_sub * gamma * denom_inv));
   for (int it = 0; it < no_iter - 1; ++it) {
     const R m_p = mass_new + m_s;
     const R ws = m_s / m_p;

@@ -111,6 +111,43 @@ def test_adv_steps_bit_exact(gpu, n_steps):
     assert no_cols.sum() > 0, "case should produce collisions"
 
 
+@pytest.mark.parametrize("variant", ["NaCl", "stress150", "one_collision_step", "four_collision_steps_no_relaxation"])
+def test_adv_steps_configurations(gpu, variant):
+    """Two advection steps, bit-exact vs the oracle, for the other configurations the step supports: NaCl
+    solute (its mass rate and density laws, integration.py:728-836 keeping the AS constants inside the
+    iterations); the stress case's parameters (10 m cells, Hall table, dt 0.5 s, 20 substeps, mass-flux
+    amplitude 1.0); one collision step per advection step (placed at each subloop start,
+    integration.py:1247-1251); four per step (one per substep, integration.py:1257-1262) without
+    relaxation."""
+    kernel = "Hall_Bott" if variant == "stress150" else "Long_Bott"
+    E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
+    kw, case_kw = {}, {}
+    if variant == "NaCl":
+        kw = dict(solute_type="NaCl")
+    elif variant == "stress150":
+        kw = dict(dt_adv=0.5, dt_col=0.25, scale_dt_cond=10)
+        case_kw = dict(dx=10.0, dz=10.0, spc=48, j_max_ref=1.0)
+    elif variant == "one_collision_step":
+        kw = dict(no_col_per_adv=1, dt_col=1.0)
+    else:
+        kw = dict(no_col_per_adv=4, dt_col=0.25, act_relaxation=False)
+    cfg = _config(E, lnR0, lnf, nb, **kw)
+    grid, *parts = _case(seed=29, **case_kw)
+    parts = _prep(grid, parts)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 13, 2)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 13)
+    eng.adv_steps(2)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, f"{variant} {nm}")
+    s, m = eng.export_grid(0)
+    _assert_same(s[:8], scal[:8], f"{variant} grid_scalar_fields")
+    _assert_same(m, mat, f"{variant} grid_mat_prop")
+    nc_got, wr_got = eng.counters(0)
+    _assert_same(nc_got, no_cols, f"{variant} no_cols")
+    _assert_same(wr_got, wr, f"{variant} water_removed")
+    assert no_cols.sum() > 0, "case should produce collisions"
+
+
 def test_collision_step_injected_stream(gpu):
     """All-cells collision step with the reference's draw stream injected (x-major cells, row-major pairs)."""
     from lacmopy_amd.engine import Engine  # noqa: F401
