@@ -1366,8 +1366,9 @@ static void launch_particles_occ(int wpe, dim3 grid, dim3 block, hipStream_t st,
  * slot's condensation Jacobian from the pre-update state, then k_particles_tape<S, 1> runs the substep
  * with the (m_w, m_s) columns.  Two narrower dual types need far fewer registers than one LmDual<5>.
  */
-template <int SOLUTE>
-__global__ __launch_bounds__(TAPE_BLOCK) void k_tape_tps(KView v, long long n_tot, double dt_sub) {
+template <int SOLUTE, int WPE>
+__global__ __launch_bounds__(TAPE_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_tape_tps(
+    KView v, long long n_tot, double dt_sub) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_tot) return;
   const int gc = v.cell[k];
@@ -1375,9 +1376,9 @@ __global__ __launch_bounds__(TAPE_BLOCK) void k_tape_tps(KView v, long long n_to
   newton_dm_tape<SOLUTE, 2>(c_P, v, k, n_tot, gc, dt_sub, v.m_w[k], v.m_s[k], v.xi[k], v.cc + (size_t)gc * CC_N);
 }
 
-template <int SOLUTE, int PART>
-__global__ __launch_bounds__(TAPE_BLOCK) void k_particles_tape(KView v, long long n_tot, double dt_sub,
-                                                               double dt_pos) {
+template <int SOLUTE, int PART, int WPE>
+__global__ __launch_bounds__(TAPE_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_particles_tape(
+    KView v, long long n_tot, double dt_sub, double dt_pos) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n_tot) particle_update<SOLUTE, false, true, PART>(v, k, n_tot, dt_sub, dt_pos);
 }
@@ -2686,15 +2687,19 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
       if (e->tape_mode == 1) {
         /* one pass on LmDual<5> */
         if (e->solute == 0)
-          hipLaunchKernelGGL((k_particles_tape<0, 0>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+          hipLaunchKernelGGL((k_particles_tape<0, 0, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
+                             dt_sub_pos);
         else
-          hipLaunchKernelGGL((k_particles_tape<1, 0>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+          hipLaunchKernelGGL((k_particles_tape<1, 0, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
+                             dt_sub_pos);
       } else if (e->solute == 0) {
-        hipLaunchKernelGGL(k_tape_tps<0>, gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub);
-        hipLaunchKernelGGL((k_particles_tape<0, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+        hipLaunchKernelGGL((k_tape_tps<0, 2>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub);
+        hipLaunchKernelGGL((k_particles_tape<0, 1, 2>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
+                           dt_sub_pos);
       } else {
-        hipLaunchKernelGGL(k_tape_tps<1>, gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub);
-        hipLaunchKernelGGL((k_particles_tape<1, 1>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub, dt_sub_pos);
+        hipLaunchKernelGGL((k_tape_tps<1, 2>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub);
+        hipLaunchKernelGGL((k_particles_tape<1, 1, 2>), gt, dim3(TAPE_BLOCK), 0, S(stream), v, n, dt_sub,
+                           dt_sub_pos);
       }
     } else if (spt != 2 && e->part_wpe > 0) {
       if (e->solute == 0)

@@ -43,6 +43,20 @@ struct LmDual {
 
 #define LMDUAL template <int N> LM_FN
 
+/*
+ * 1/x for the derivative parts only (never for a value): on the GPU the hardware reciprocal estimate
+ * refined by one Newton step (three instructions instead of a correctly rounded division, error a few
+ * units in the last place), on the host the division.
+ */
+LM_FN double lm_drcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(x);
+  return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
 /* expressions mixing doubles and LmDual<N> are LmDual<N> (LmCommonT of lm_physics.h) */
 template <class A, class B>
 struct LmCommon2;
@@ -126,7 +140,7 @@ LMDUAL LmDual<N> operator*(double a, const LmDual<N> &b) {
 LMDUAL LmDual<N> operator/(const LmDual<N> &a, const LmDual<N> &b) {
   LmDual<N> r;
   r.v = a.v / b.v;
-  const double rb = 1.0 / b.v;
+  const double rb = lm_drcp(b.v);
 #pragma unroll
   for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * rb;
   return r;
@@ -134,7 +148,7 @@ LMDUAL LmDual<N> operator/(const LmDual<N> &a, const LmDual<N> &b) {
 LMDUAL LmDual<N> operator/(const LmDual<N> &a, double b) {
   LmDual<N> r;
   r.v = a.v / b;
-  const double rb = 1.0 / b;
+  const double rb = lm_drcp(b);
 #pragma unroll
   for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * rb;
   return r;
@@ -142,7 +156,7 @@ LMDUAL LmDual<N> operator/(const LmDual<N> &a, double b) {
 LMDUAL LmDual<N> operator/(double a, const LmDual<N> &b) {
   LmDual<N> r;
   r.v = a / b.v;
-  const double s = -r.v / b.v;
+  const double s = -r.v * lm_drcp(b.v);
 #pragma unroll
   for (int k = 0; k < N; ++k) r.d[k] = s * b.d[k];
   return r;
@@ -178,15 +192,15 @@ LMDUAL LmDual<N> lm_exp(const LmDual<N> &x) {
   const double e = lm_exp(x.v);
   return lm_chain(x, e, e);
 }
-LMDUAL LmDual<N> lm_log(const LmDual<N> &x) { return lm_chain(x, lm_log(x.v), 1.0 / x.v); }
+LMDUAL LmDual<N> lm_log(const LmDual<N> &x) { return lm_chain(x, lm_log(x.v), lm_drcp(x.v)); }
 /* x^y with a constant exponent */
 LMDUAL LmDual<N> lm_pow(const LmDual<N> &x, double y) {
   const double p = lm_pow(x.v, y);
-  return lm_chain(x, p, y * (p / x.v));
+  return lm_chain(x, p, y * (p * lm_drcp(x.v)));
 }
 LMDUAL LmDual<N> lm_pow_third(const LmDual<N> &x) {
   const double p = lm_pow_third(x.v);
-  return lm_chain(x, p, p / (3.0 * x.v));
+  return lm_chain(x, p, p * lm_drcp(3.0 * x.v));
 }
 /* np.maximum / np.minimum / np.fmin: the selected operand, with its derivatives */
 LMDUAL LmDual<N> lm_np_max(const LmDual<N> &a, const LmDual<N> &b) {
