@@ -222,6 +222,41 @@ def test_collision_many_events(gpu, kernel):
     assert np.isfinite(xi).all()
 
 
+@pytest.mark.parametrize("kernel", ["Long_Bott", "const"])
+def test_equal_multiplicities_follow_the_tie_rule(gpu, kernel):
+    """Cells whose SIPs all share one multiplicity: every event is a near-equal collection
+    (all_or_nothing.py:597-614) on an exact tie, where xi_i <= xi_j makes i the smaller SIP
+    (all_or_nothing.py:565-568) and decides which SIP keeps 0.35 and which 0.15 of the summed
+    multiplicity; three collision steps bit-exact vs the oracle, row filter included."""
+    if kernel == "const":
+        E, lnR0, lnf, nb = 0.9, 0.0, 1.0, 0
+    else:
+        E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = make_case(nx=6, nz=6, spc=50, seed=31, cloud_base=0.0, cloud_top=120.0, R_drop=30.0,
+                             sigma_drop=1.6)
+    parts = list(_prep(grid, parts))
+    rng = np.random.default_rng(8)
+    parts[1] = np.array([parts[1][0], -rng.uniform(0.0, 3.0, parts[1].shape[1])])  # fall speeds
+    parts[5] = np.full(parts[5].size, 4.0E9)                                       # one multiplicity
+    parts = tuple(parts)
+    pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 41)
+    no_cols = np.zeros(2, dtype=np.int64)
+    for step in range(3):
+        orc.collision_all_cells(xi, m_w, m_s, vel, grid.temperature.copy(), cells, grid.no_cells,
+                                cfg["dt_col"] / grid.volume_cell, E, nb, lnR0, lnf, no_cols, seed=41, run=0,
+                                step=step)
+        eng.collision_step()
+    got = eng.export_particles(0)
+    _assert_same(got[5], xi, "xi")
+    _assert_same(got[3], m_w, "m_w")
+    _assert_same(got[4], m_s, "m_s")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    assert no_cols[0] > 36, f"expected many events, got {no_cols}"
+    assert int(eng.buf["n_equal_xi"][0]) > 36, "expected near-equal collections on the ties"
+
+
 def test_dense_cells_take_the_large_capacity_passes(gpu):
     """Cells of ~300 SIPs exceed the 256-member fast passes of the collision and Delta m_l kernels and go
     through their 1024-member overflow passes; results stay bit-identical to the oracle."""
