@@ -149,10 +149,10 @@ typedef struct LmEngine {
      and the velocity is the mass flux over it) */
   int64_t vel_run_stride;
   /*
-   * Persistent per-cell passes (0 = one workgroup per (run, cell); k > 0 = k workgroups per compute unit
+   * Persistent per-cell passes (0 = one workgroup per (run, cell); k = 1..256 workgroups per compute unit
    * that pull (run, cell) items from the work list until it is empty), for the main collision pass and
-   * the main Delta m_l pass, each taking *_work_chunk consecutive items per atomic grab (0: a static
-   * round robin over the list, no atomics); set by an autotuner.  The work list holds the n_runs*nc
+   * the main Delta m_l pass, each taking *_work_chunk (0..4096) consecutive items per atomic grab (0: a
+   * static round robin over the list, no atomics); set by an autotuner.  The work list holds the n_runs*nc
    * global cells ordered by decreasing home-segment size (largest first), rebuilt by every reindex while a
    * persistent pass is on; any permutation of the cells gives the same results, only the balance changes.
    */

@@ -2410,10 +2410,11 @@ static int check_engine(const LmEngine *e) {
     return set_err(LM_ERR_ARG, "bad launch shape (block 64..512 in steps of 64, 1 or 2 SIPs per thread, "
                                "collision capacity 160, 256 or 512)");
   if (e->col_work_per_cu < 0 || e->dml_work_per_cu < 0 || e->col_work_chunk < 0 || e->dml_work_chunk < 0 ||
+      e->col_work_per_cu > 256 || e->dml_work_per_cu > 256 || e->col_work_chunk > 4096 || e->dml_work_chunk > 4096 ||
       (e->part_wpe && e->part_wpe != 5 && e->part_wpe != 6 && e->part_wpe != 8) || e->tape_mode < 0 ||
       e->tape_mode > 1)
-    return set_err(LM_ERR_ARG, "bad launch shape (negative work-list shape, particle waves per SIMD not 0, 5, 6 "
-                               "or 8, or tape mode not 0 or 1)");
+    return set_err(LM_ERR_ARG, "bad launch shape (work-list shape outside 0..256 workgroups per CU or 0..4096 "
+                               "items per grab, particle waves per SIMD not 0, 5, 6 or 8, or tape mode not 0 or 1)");
   if (e->ev_mode && (!e->ev || !e->ev_off || !e->ev_order || !e->run_base || e->ev_cap <= 0))
     return set_err(LM_ERR_ARG, "event log buffers missing");
   if (e->ev_mode == 1 && (!e->ev_cnt || !e->ev_count)) return set_err(LM_ERR_ARG, "event counters missing");

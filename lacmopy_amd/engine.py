@@ -378,8 +378,9 @@ class Engine:
         for k, val in (("col_work_per_cu", col_work_per_cu), ("dml_work_per_cu", dml_work_per_cu),
                        ("col_work_chunk", col_work_chunk), ("dml_work_chunk", dml_work_chunk)):
             if val is not None:
-                if int(val) < 0:
-                    raise ValueError(f"{k} must be >= 0")
+                hi = 256 if k.endswith("per_cu") else 4096
+                if not 0 <= int(val) <= hi:
+                    raise ValueError(f"{k} must be in 0..{hi}")
                 setattr(e, k, int(val))
 
     def launch_shape(self):
