@@ -35,6 +35,10 @@ extern "C" {
 #define LM_ERR_RND_SHORT 1006  /* the injected uniform stream ran out */
 #define LM_ERR_EV_CAP 1007     /* the collision-event log of an adjoint window is full */
 
+/* SIPs that may reach the ground per run in one particle substep (LmEngine.rem_cap is 1..this; more raise
+   LM_ERR_REM_CAP): water_removed sums them in ascending id in LDS */
+#define LM_REM_CAP_MAX 1024
+
 /*
  * One collision event of an adjoint window, recorded by lm_collision_step
  * (ev_mode 1) and replayed as a fixed event (ev_mode 2): the pair, the branch
@@ -96,7 +100,7 @@ typedef struct LmEngine {
   int64_t *n_equal_xi;   /* n_runs: near-equal-xi collision events (the reference prints them) */
   int32_t *rem_cnt;      /* n_runs */
   int32_t *rem_slot;     /* n_runs*rem_cap */
-  int32_t rem_cap, pad0;
+  int32_t rem_cap, pad0; /* removal slots per run and substep, 1..LM_REM_CAP_MAX */
   /* scratch */
   int32_t *scan_buf;  /* n_runs*(2nc+1)+1 */
   int32_t *scan_part; /* scan partials */

@@ -42,16 +42,8 @@ def _ptr(t):
 
 def collision_steps_per_adv(e):
     """Collision steps of one advection step, as lm_adv_steps places them (integration.py:1515-1568, 1095-1294)."""
-    if not e.act_collisions:
-        return 0
-
-    def subloop(no_cond, no_col):
-        return (no_col == 1) + (no_cond if no_col > 1 else 0) + (no_col > no_cond)
-
-    sdc, npa = int(e.scale_dt_cond), int(e.no_col_per_adv)
-    n1 = 0 if npa == 1 else (1 if npa == 2 else sdc)
-    n2 = 1 if npa in (1, 2) else sdc
-    return int(subloop(sdc, n1) + subloop(sdc - 1, n2))
+    from .integration import col_steps_per_adv
+    return col_steps_per_adv(e.no_col_per_adv, e.scale_dt_cond, e.act_collisions)
 
 
 class AdjointWindow:

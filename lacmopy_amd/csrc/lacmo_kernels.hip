@@ -1384,14 +1384,17 @@ __global__ __launch_bounds__(TAPE_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE,
 }
 
 /* water_removed[r] += xi*m_w of this substep's removed SIPs, ascending id (integration.py:620-628) */
-#define REM_LDS 1024
+#define REM_LDS LM_REM_CAP_MAX
 __global__ __launch_bounds__(WAVE) void k_water(KView v) {
   const int r = blockIdx.x;
   __shared__ int sId[REM_LDS], sSlot[REM_LDS], sRank[REM_LDS], sOrd[REM_LDS];
   const int n0 = v.rem_cnt[r];
   if (n0 == 0) return;
-  const int n = n0 < REM_LDS ? n0 : REM_LDS;
-  if (n0 > REM_LDS && threadIdx.x == 0) raise_err(v.err, LM_ERR_REM_CAP, r);
+  /* rem_slot holds rem_cap slots per run (1 <= rem_cap <= REM_LDS, checked on the host); particle_update
+     flagged LM_ERR_REM_CAP for removals beyond it, and only the recorded ones are read */
+  const int lim = v.rem_cap < REM_LDS ? v.rem_cap : REM_LDS;
+  const int n = n0 < lim ? n0 : lim;
+  if (n0 > lim && threadIdx.x == 0) raise_err(v.err, LM_ERR_REM_CAP, r);
   for (int t = threadIdx.x; t < n; t += WAVE) {
     const int k = v.rem_slot[(size_t)r * v.rem_cap + t];
     sSlot[t] = k;
@@ -2405,6 +2408,8 @@ static int check_engine(const LmEngine *e) {
   if (e->vel_run_stride != 0 && e->vel_run_stride != (int64_t)2 * (e->nx + 1) * (e->nz + 1))
     return set_err(LM_ERR_ARG, "vel_run_stride must be 0 or 2*(nx+1)*(nz+1)");
   if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
+  if (e->rem_cap < 1 || e->rem_cap > LM_REM_CAP_MAX || !e->rem_slot || !e->rem_cnt)
+    return set_err(LM_ERR_ARG, "rem_cap must be in 1..LM_REM_CAP_MAX (1024) with rem_slot and rem_cnt set");
   if (e->part_block < 0 || e->part_block > PART_BLOCK_MAX || (e->part_block % WAVE) || e->part_spt < 0 ||
       e->part_spt > 2 || (e->col_cap && e->col_cap != 160 && e->col_cap != 256 && e->col_cap != 512))
     return set_err(LM_ERR_ARG, "bad launch shape (block 64..512 in steps of 64, 1 or 2 SIPs per thread, "

@@ -18,6 +18,7 @@ from ._native import LmEngine, check, lib
 SOLUTES = {"AS": 0, "NaCl": 1}
 CC_N, CC_TP = 16, 15  # per-cell constant table: width, slot of the pre-update cell temperature (lm_physics.h)
 WORK_BINS = 1024  # LM_WORK_BINS: size bins of the work-list order (include/lacmo_amd.h)
+REM_CAP_MAX = 1024  # LM_REM_CAP_MAX: ground removals per run and substep (include/lacmo_amd.h)
 
 
 def _ptr(t):
@@ -29,6 +30,8 @@ class Engine:
 
     def __init__(self, nx, nz, n_per_run, ranges, steps, volume_cell, p_ref=1.0E5, p_ref_inv=1.0E-5,
                  device="cuda", rem_cap=1024):
+        if not 1 <= int(rem_cap) <= REM_CAP_MAX:
+            raise ValueError(f"rem_cap must be in 1..{REM_CAP_MAX} (removals per run and substep)")
         if not torch.cuda.is_available():
             raise RuntimeError("lacmopy_amd.Engine needs a GPU (HIP device); there is no CPU fallback")
         self.L = lib()
@@ -108,7 +111,7 @@ class Engine:
                   "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
                   "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off", "work"):
             setattr(e, f, self.buf[f].data_ptr())
-        e.rem_cap = rem_cap
+        e.rem_cap = int(rem_cap)
         e.vel_run_stride = 2 * (self.nx + 1) * (self.nz + 1)
         e.e_const_mode = 1
         e.E_const = 0.5

@@ -334,13 +334,26 @@ def _run_interval(res, no_adv_steps, dump_every, trace_ids, traced_vectors, trac
         cnt += n
 
 
-def _col_steps_per_adv(no_col_per_adv, scale_dt_cond):
-    """Collision steps of one advection step (integration.py:1515-1519, 1570-1572, 1247-1292)."""
-    if no_col_per_adv == 1:
-        return 1
-    if no_col_per_adv == 2:
-        return 2
-    return 2 * scale_dt_cond
+def col_steps_per_adv(no_col_per_adv, scale_dt_cond, act_collisions=True):
+    """Collision steps of one advection step as integrate_adv_step_np places them.
+
+    Subloop 1 runs scale_dt_cond substeps with 0, 1 or scale_dt_cond collision steps
+    (integration.py:1515-1519), subloop 2 runs scale_dt_cond - 1 substeps with 1 or scale_dt_cond
+    (:1570-1572); a subloop with no_col collision steps over no_cond substeps performs one at its start if
+    no_col == 1, one before each substep if no_col > 1, and one more at its end if no_col > no_cond
+    (:1247-1292).  With scale_dt_cond == 1 the second subloop has no substep, so its single collision step
+    is counted twice (start and end), as the reference runs it.
+    """
+    if not act_collisions:
+        return 0
+
+    def subloop(no_cond, no_col):
+        return int(no_col == 1) + (no_cond if no_col > 1 else 0) + int(no_col > no_cond)
+
+    sdc, npa = int(scale_dt_cond), int(no_col_per_adv)
+    n1 = 0 if npa == 1 else (1 if npa == 2 else sdc)
+    n2 = 1 if npa in (1, 2) else sdc
+    return subloop(sdc, n1) + subloop(sdc - 1, n2)
 
 
 def simulate_interval(grid_scalar_fields, grid_mat_prop, grid_velocity, grid_mass_flux_air_dry, p_ref, p_ref_inv,
@@ -483,7 +496,7 @@ def simulate(grid, pos, vel, cells, m_w, m_s, xi, active_ids, water_removed, no_
     N = len(xi)
     eng = _engine(grid.no_cells, grid.ranges, grid.steps, grid.volume_cell, grid.p_ref, grid.p_ref_inv, N)
     if act_collisions:
-        rng.seed(seed_sim, int(round(t_start / dt)) * _col_steps_per_adv(no_col_per_adv, scale_dt_cond))
+        rng.seed(seed_sim, int(round(t_start / dt)) * col_steps_per_adv(no_col_per_adv, scale_dt_cond))
     _set_step_params(eng, solute_type, dt, dt_sub, dt_sub_half, dt_col / grid.volume_cell, scale_dt_cond,
                      no_col_per_adv, act_collisions, act_relaxation, no_iter_impl_mass, g_set)
     _set_kernel(eng, E_col_grid, R_kernel_low_log, bin_factor_R_log)

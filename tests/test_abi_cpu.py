@@ -84,3 +84,37 @@ def test_host_layout_order_and_inverse():
         for x, y in zip(back, (pos, vel, cells, m_w, m_s, xi, act)):
             assert np.array_equal(x, y)
     assert seg[-1] == 2 * len(xi)
+
+
+def _checked_engine(**over):
+    """An LmEngine that passes every shape check, with placeholder (never dereferenced) buffer pointers."""
+    e = N.LmEngine()
+    e.n_runs, e.nx, e.nz, e.nc, e.n_tot = 1, 4, 4, 16, 10
+    fake = 0x1000
+    for f in ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "cell", "seg"):
+        getattr(e, f)[0] = getattr(e, f)[1] = fake
+    for f in ("cc", "err", "away", "col_step_dev", "rem_slot", "rem_cnt"):
+        setattr(e, f, fake)
+    e.rem_cap = 1024
+    for k, v in over.items():
+        setattr(e, k, v)
+    return e
+
+
+def test_engine_check_rejects_bad_removal_capacity():
+    """rem_cap sizes rem_slot per run and bounds what k_water reads: outside 1..LM_REM_CAP_MAX, or without
+    its buffers, every entry point refuses the engine before any device work."""
+    L = N.lib()
+    hdr = open(os.path.join(ROOT, "include", "lacmo_amd.h")).read()
+    cap_max = int(re.search(r"#define LM_REM_CAP_MAX (\d+)", hdr).group(1))
+    from lacmopy_amd.engine import REM_CAP_MAX
+    assert cap_max == REM_CAP_MAX == 1024
+    for bad in (dict(rem_cap=0), dict(rem_cap=-5), dict(rem_cap=cap_max + 1), dict(rem_slot=None)):
+        e = _checked_engine(**bad)
+        rc = L.lm_reindex(C.byref(e), None)
+        assert rc == 1001, (bad, rc)
+        assert b"rem_cap" in L.lm_last_error_message()
+    import pytest
+    from lacmopy_amd.engine import Engine
+    with pytest.raises(ValueError, match="rem_cap"):
+        Engine(4, 4, [10], [[0, 80], [0, 80]], [20, 20], 400.0, rem_cap=0)

@@ -129,3 +129,28 @@ def test_dual_newton_jacobian(tmp_path):
     err = np.array([float(r[1]) for r in rows])
     assert same.all(), f"{(same == 0).sum()} dual values differ from the double Newton"
     assert np.median(err) < 1.0E-8 and err.max() < 1.0E-4, (np.median(err), err.max())
+
+
+@pytest.mark.parametrize("scale_dt_cond", [1, 2, 3])
+@pytest.mark.parametrize("no_col_per_adv", [1, 2, 5])
+def test_collision_steps_per_adv_step_match_the_subloop_schedule(scale_dt_cond, no_col_per_adv):
+    """The collision-step counter simulate() resumes from (integration.col_steps_per_adv, also the adjoint's
+    count) equals the number of collision steps the oracle's restatement of integrate_adv_step_np
+    (integration.py:1515-1585 with the subloop placement of :1247-1292) takes per advection step."""
+    from lacmopy_amd.integration import col_steps_per_adv
+    from oracle import oracle as orc
+    from lacmopy_amd.setup_case import init_profiles
+    grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=3, nz=3, spc=2, seed=1)
+    cfg = dict(dt_adv=1.0, dt_col=1.0 / no_col_per_adv, scale_dt_cond=scale_dt_cond, no_col_per_adv=no_col_per_adv,
+               act_collisions=True, act_relaxation=False, no_iter_impl_mass=3, solute_type="AS", g_set=9.80665)
+    N = len(xi)
+    p = orc.adv_params(grid, N, cfg, seed=1, col_step=7, E_col_grid=0.5)
+    rel = np.zeros_like(pos)
+    scal, mat = grid.scalar_fields(), grid.mat_prop()
+    prof = init_profiles(grid)
+    n_steps = 2
+    orc.integrate_adv_steps(p, n_steps, pos, vel, cells, rel, m_w, m_s, xi, act, np.ones(N), np.zeros(1),
+                            np.zeros(2, dtype=np.int64), scal, mat, grid.velocity, grid.mass_flux_air_dry, 0.5,
+                            *prof)
+    assert p.col_step - 7 == n_steps * col_steps_per_adv(no_col_per_adv, scale_dt_cond)
+    assert col_steps_per_adv(no_col_per_adv, scale_dt_cond, act_collisions=False) == 0
