@@ -31,16 +31,7 @@ B_SUBSTEP = 104.0    # algorithmic bytes per SIP per particle substep (60 read +
 B_COLLISION = 64.0   # per SIP per collision step (40 read + 24 write)
 B_REINDEX = 140.0    # per SIP per reindex
 
-# Workloads.  icmw75: the 2-D ICMW kinematic stratocumulus on 75x75 cells of 20 m, Poisson(128) SIPs per
-# cell, Long E_col, dt 1 s, 10 substeps, 64 seeds.  stress150: the same 1500 m domain on 150x150 cells of
-# 10 m, Poisson(256) SIPs per cell, Hall E_col, mass-flux amplitude 1.0 kg m^-2 s^-1, dt 0.5 s,
-# 20 substeps, 16 seeds.  Both: 2 collision steps per advection step, relaxation on.
-CASES = {
-    "icmw75": dict(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, runs=64, kernel="Long_Bott", dt=1.0, sdc=5,
-                   j_max=0.6),
-    "stress150": dict(nx=150, nz=150, dx=10.0, dz=10.0, spc=256.0, runs=16, kernel="Hall_Bott", dt=0.5, sdc=10,
-                      j_max=1.0),
-}
+from lacmopy_amd.setup_case import WORKLOADS as CASES  # noqa: E402
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_icmw75_64seeds.md")
@@ -85,7 +76,7 @@ def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
     import torch  # noqa: F401
     from lacmopy_amd import load_kernel_data_Ecol
     from lacmopy_amd.engine import Engine
-    from lacmopy_amd.setup_case import init_profiles, make_case
+    from lacmopy_amd.setup_case import init_profiles, workload_case
     c = CASES[case]
     kernel = kernel or c["kernel"]
     if kernel == "const":
@@ -94,9 +85,8 @@ def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
         E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
     cases = []
     for r in range(n_runs):
-        grid, pos, vel, cells, m_w, m_s, xi, act, draws = make_case(nx=c["nx"], nz=c["nz"], dx=c["dx"], dz=c["dz"],
-                                                                    spc=spc, seed=seed0 + 2 * r,
-                                                                    j_max_ref=c["j_max"], return_draws=True)
+        grid, pos, vel, cells, m_w, m_s, xi, act, draws = workload_case(case, seed0 + 2 * r, spc=spc,
+                                                                        return_draws=True)
         cases.append((grid, (pos, vel, cells, m_w, m_s, xi, act), draws))
     grid0 = cases[0][0]
     eng = Engine(c["nx"], c["nz"], [len(k[1][5]) for k in cases], grid0.ranges, grid0.steps, grid0.volume_cell)

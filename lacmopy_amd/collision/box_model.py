@@ -15,6 +15,76 @@ from .._native import check, lib
 from .all_or_nothing import _p, _stream, _torch
 
 
+FOUR_PI_OVER_THREE = 4.0 * math.pi / 3.0  # constants.py:28
+
+
+def gen_mass_ensemble_weights_SinSIP_expo(m_mean, mass_density, dV, kappa, eta, weak_threshold, r_critmin,
+                                          m_high_over_m_low=1.0E6, seed=3711, setseed=True, rng=None):
+    """generation_SIP_ensemble.py:31-146: SinSIP ensemble of the exponential mass distribution
+    f(m) = exp(-m/m_mean)/m_mean (Unterstrasser 2017): one mass drawn uniformly in each of the
+    l_max = int(kappa log10(m_high/m_low)) logarithmic bins above m_low = 4/3 pi rho r_critmin^3, weight
+    f(mu) * bin width; weights below eta * max are dropped (fixed threshold) or, weak threshold, raised to
+    eta * max with probability w / (eta * max).  Returns (masses (fg), weights, m_low, bins).
+
+    The draws are the reference's own: its jitted generator calls np.random.seed(seed) inside Numba code
+    (:98), which seeds Numba's MT19937 by Knuth's init_genrand, and np.random.rand gives the 53-bit
+    genrand_res53 doubles -- the same generator, seeding and construction as NumPy's legacy
+    RandomState(seed).random_sample.  setseed=False continues `rng` (a RandomState), as the reference
+    continues Numba's state.  Scalar exp is the C library's (Numba lowers np.exp of a float to libm).
+    """
+    if setseed:
+        rng = np.random.RandomState(seed)
+    elif rng is None:
+        raise ValueError("setseed=False needs the RandomState to continue (rng=)")
+    m_low = FOUR_PI_OVER_THREE * mass_density * r_critmin * r_critmin * r_critmin
+    m_mean_inv = 1.0 / m_mean
+    bin_factor = 10 ** (1.0 / kappa)
+    m_high = m_low * m_high_over_m_low
+    l_max = int(kappa * np.log10(m_high_over_m_low))
+    rnd = rng.random_sample(l_max)
+    rnd2 = rng.random_sample(l_max) if weak_threshold else None
+    weights = np.zeros(l_max)
+    masses = np.zeros(l_max)
+    bins = np.zeros(l_max + 1)
+    m_left = m_low
+    bins[0] = m_left
+    bin_n = 0
+    while m_left < m_high:
+        if bin_n >= l_max:
+            # the reference's loop runs one bin past its arrays here (unchecked in Numba): no defined result
+            raise ValueError(f"kappa = {kappa}: the bin loop of the reference overruns its {l_max} bins")
+        m_right = m_left * bin_factor
+        bin_width = m_right - m_left
+        mu = m_left + rnd[bin_n] * bin_width
+        weights[bin_n] = m_mean_inv * math.exp(-mu * m_mean_inv) * bin_width  # distributions.py:26-46
+        masses[bin_n] = mu
+        m_left = m_right
+        bin_n += 1
+        bins[bin_n] = m_left
+    weight_critmin = weights.max() * eta
+    valid = np.full(l_max, True)
+    for b in range(l_max):
+        if weights[b] < weight_critmin:
+            if weak_threshold and rnd2[b] < weights[b] / weight_critmin:
+                weights[b] = weight_critmin
+            else:
+                valid[b] = False
+    return masses[valid], weights[valid], m_low, bins
+
+
+def golovin_expo_ensemble(seed, kappa=40, eta=1.0E-9, weak_threshold=False, LWC0=1.0E-3, R_mean=9.3, r_critmin=0.6,
+                          m_high_over_m_low=1.0E6, mass_density=1.0E3, dV=1.0):
+    """(xis, masses in fg, DNC0) of one seed of the Golovin box-model case as run_box_model.py sets it up
+    (:133-146, 236-245, 306-328): LWC0 = 1 g m^-3, R_mean = 9.3 um, DNC0 = LWC0 / m_mean, xi = DNC0 dV w."""
+    c_radius_to_mass = 4.0E-18 * math.pi * mass_density / 3.0
+    m_mean = c_radius_to_mass * R_mean ** 3
+    DNC0 = LWC0 / m_mean
+    masses, weights, _, _ = gen_mass_ensemble_weights_SinSIP_expo(1.0E18 * m_mean, mass_density, dV, kappa, eta,
+                                                                   weak_threshold, r_critmin, m_high_over_m_low, seed)
+    # run_box_model.py:321-323 stores 1e-18 * masses and simulates with 1e18 * (the stored kg masses)
+    return (DNC0 * dV) * weights, 1.0E18 * (1.0E-18 * masses), DNC0
+
+
 def _save_schedule(dt, t_end, dt_save):
     """box_model.py:152-157: steps, snapshots (t = 0, every dt_save, and t_end) and steps between snapshots."""
     no_steps = int(math.ceil(t_end / dt))

@@ -44,6 +44,37 @@ def _profiles_without_liquid(z, z0, p0, p_ref, Theta_l, r_tot):
     return T, p, rho_dry
 
 
+# Benchmark workloads.  icmw75: the 2-D ICMW kinematic stratocumulus on 75x75 cells of 20 m, Poisson(128) SIPs
+# per cell, Long E_col, dt 1 s, 10 substeps, 64 seeds.  stress150: the same 1500 m domain on 150x150 cells of
+# 10 m, Poisson(256) SIPs per cell, Hall E_col, mass-flux amplitude 1.0 kg m^-2 s^-1, dt 0.5 s, 20 substeps,
+# 16 seeds.  Both: 2 collision steps per advection step, relaxation on.  Run r of a batch is seeded
+# seed0 + 2 r (the reference's run_lacmo.sh:7-16 seed spacing).
+WORKLOADS = {
+    "icmw75": dict(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, runs=64, kernel="Long_Bott", dt=1.0, sdc=5,
+                   j_max=0.6),
+    "stress150": dict(nx=150, nz=150, dx=10.0, dz=10.0, spc=256.0, runs=16, kernel="Hall_Bott", dt=0.5, sdc=10,
+                      j_max=1.0),
+}
+
+
+def workload_config(case, E_col_grid=None, R_kernel_low_log=None, bin_factor_R_log=None):
+    """Step configuration of a workload, with the keys simulate() reads (integration.py:1908-1923)."""
+    c = WORKLOADS[case]
+    cfg = dict(solute_type="AS", dt_adv=c["dt"], dt_col=c["dt"] / 2, scale_dt_cond=c["sdc"], no_col_per_adv=2,
+               no_iter_impl_mass=3, g_set=9.80665, act_collisions=True, act_relaxation=True)
+    if E_col_grid is not None and not isinstance(E_col_grid, float):
+        cfg.update(no_kernel_bins=int(np.asarray(E_col_grid).shape[0]), R_kernel_low_log=R_kernel_low_log,
+                   bin_factor_R_log=bin_factor_R_log)
+    return cfg
+
+
+def workload_case(case, seed, spc=None, return_draws=False):
+    """make_case for one run of a workload (spc: mean SIPs per cell, default the workload's)."""
+    c = WORKLOADS[case]
+    return make_case(nx=c["nx"], nz=c["nz"], dx=c["dx"], dz=c["dz"], spc=c["spc"] if spc is None else spc,
+                     seed=seed, j_max_ref=c["j_max"], return_draws=return_draws)
+
+
 def make_case(nx=75, nz=75, dx=20.0, dz=20.0, spc=128.0, seed=1001, p_0=101500.0, p_ref=1.0E5,
               Theta_l=289.0, r_tot_0=7.5E-3, j_max_ref=0.6, cloud_base=600.0, cloud_top=1000.0,
               DNC0=(60.0E6, 40.0E6), mu_R=(0.02, 0.075), sigma_R=(1.4, 1.6), R_drop=9.0, sigma_drop=1.35,
