@@ -1,0 +1,119 @@
+"""Collection-efficiency tables of the collision kernels (collision/kernel.py of LacmoPy), host side.
+
+The tables are inputs of the collision steps, built once (as the reference builds and saves them before a
+run, run_box_model.py:264-273): E_col(R_i, R_j) on a logarithmic radius grid, for the Long kernel from its
+closed form (Long 1974 as modified by Bott 1998) and for the Hall kernel by bilinear interpolation of the
+Hall (1980) / Seesselberg (1996) table that Bott (1998) uses, in radius of the collector and radius ratio.
+The raw Hall table is the reference's data file (lacmopy_amd/data/kernel_data/Hall).  The grids here are
+written in the reference's evaluation order, so a regenerated table equals the reference's saved one.
+"""
+import math
+import os
+
+import numpy as np
+
+_HALL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data", "kernel_data", "Hall")
+
+
+def _hall_raw():
+    E = np.load(os.path.join(_HALL, "Hall_Bott_E_col_table_raw_corr.npy"))
+    R = np.load(os.path.join(_HALL, "Hall_Bott_R_col_table_raw.npy"))
+    q = np.load(os.path.join(_HALL, "Hall_Bott_R_ratio_table_raw.npy"))
+    return E, R, q
+
+
+def compute_E_col_Long_Bott(R_i, R_j):
+    """kernel.py:350-386: E = 4.5e-4 R^2 (1 - 3 / (max(3, r) + 0.01)) for a collector R <= 50 um, else 1
+    (R the larger, r the smaller radius, um)."""
+    if R_j >= R_i:
+        R_max, R_min = R_j, R_i
+    else:
+        R_max, R_min = R_i, R_j
+    if R_max <= 50:
+        return 4.5E-4 * R_max * R_max * (1.0 - 3.0 / (max(3.0, R_min) + 1.0E-2))
+    return 1.0
+
+
+def radius_grid_log(R_low, R_high, no_bins_10):
+    """The logarithmic radius grid of kernel.py:550-558, 626-634: R_low times 10^(1/no_bins_10) per bin, by
+    repeated multiplication, ceil(no_bins_10 log10(R_high/R_low)) + 1 bins."""
+    bin_factor = 10 ** (1.0 / no_bins_10)
+    no_bins = int(math.ceil(no_bins_10 * math.log10(R_high / R_low))) + 1
+    g = np.zeros(no_bins)
+    g[0] = R_low
+    for b in range(1, no_bins):
+        g[b] = g[b - 1] * bin_factor
+    return g
+
+
+def generate_E_col_grid_R_Long_Bott_np(R_low, R_high, no_bins_10, radius_grid_in=None):
+    """kernel.py:512-577."""
+    rg = radius_grid_log(R_low, R_high, no_bins_10) if radius_grid_in is None else np.asarray(radius_grid_in)
+    n = rg.shape[0]
+    E = np.zeros((n, n))
+    for j in range(n):
+        for i in range(j + 1):
+            E[j, i] = compute_E_col_Long_Bott(float(rg[i]), float(rg[j]))
+            E[i, j] = E[j, i]
+    return E, rg
+
+
+def generate_E_col_grid_R_Hall_Bott_np(R_low, R_high, no_bins_10, radius_grid_in=None):
+    """kernel.py:581-694: per collector radius R (the larger) the floor interval of the raw collector radii
+    (below the first: its row; above the last: the last row, linear in the ratio only and capped at 1), per
+    ratio r/R the floor interval of the raw ratios (ratio 1 uses the last interval), bilinear weights."""
+    Er, Rr, qr = _hall_raw()
+    nR, nq = Rr.shape[0], qr.shape[0]
+    rg = radius_grid_log(R_low, R_high, no_bins_10) if radius_grid_in is None else np.asarray(radius_grid_in)
+    n = rg.shape[0]
+    E = np.zeros((n, n))
+    for iR, R_col in enumerate(rg):
+        R_col = float(R_col)
+        if R_col <= Rr[0]:
+            iRt = -1
+        elif R_col > Rr[-1]:
+            iRt = nR - 1
+        else:
+            for t in range(nR - 1):
+                if Rr[t] < R_col <= Rr[t + 1]:
+                    iRt = t
+                    break
+        for ir in range(iR + 1):
+            ratio = float(rg[ir]) / R_col
+            if ratio >= 1.0:
+                iq = nq - 2
+            else:
+                for t in range(nq - 1):
+                    if qr[t] <= ratio < qr[t + 1]:
+                        iq = t
+                        break
+            q = (ratio - qr[iq]) / (qr[iq + 1] - qr[iq])
+            if iRt == nR - 1:
+                e = (1. - q) * Er[iRt, iq] + q * Er[iRt, iq + 1]
+                E[iR, ir] = e if e <= 1.0 else 1.0
+            elif iRt == -1:
+                E[iR, ir] = (1. - q) * Er[0, iq] + q * Er[0, iq + 1]
+            else:
+                p = (R_col - Rr[iRt]) / (Rr[iRt + 1] - Rr[iRt])
+                # kernel.py:485-508 interpol_bilin, in its order of terms
+                E[iR, ir] = ((1. - p) * (1. - q) * Er[iRt, iq] + (1. - p) * q * Er[iRt, iq + 1]
+                             + p * (1. - q) * Er[iRt + 1, iq] + p * q * Er[iRt + 1, iq + 1])
+            E[ir, iR] = E[iR, ir]
+    return E, rg
+
+
+def generate_E_col_grid_R_np(R_low, R_high, no_bins_10, kernel_name):
+    """kernel.py:698-742."""
+    if kernel_name == "Hall_Bott":
+        return generate_E_col_grid_R_Hall_Bott_np(R_low, R_high, no_bins_10)
+    if kernel_name == "Long_Bott":
+        return generate_E_col_grid_R_Long_Bott_np(R_low, R_high, no_bins_10)
+    raise ValueError(f"unknown kernel {kernel_name!r} (Hall_Bott or Long_Bott)")
+
+
+def generate_and_save_E_col_grid_R(R_low, R_high, no_bins_10, kernel_name, save_dir):
+    """kernel.py:744-789: the grid and table as radius_grid_out.npy / E_col_grid.npy in save_dir."""
+    E, rg = generate_E_col_grid_R_np(R_low, R_high, no_bins_10, kernel_name)
+    np.save(save_dir + "radius_grid_out.npy", rg)
+    np.save(save_dir + "E_col_grid.npy", E)
+    return E, rg
