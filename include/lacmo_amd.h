@@ -441,6 +441,25 @@ int lm_box2d_step(int32_t n_boxes, const int32_t *off, double *xis, double *m_w,
 int lm_box_check(void *stream);
 
 /*
+ * Ensemble statistics of box-model snapshots, replacing the sums of analyze_sim_data
+ * (collision/box_model.py:1354-1585).  The snapshots of all seeds are (n_times, n_tot) arrays of
+ * multiplicities and masses (kg), seed s in the columns [off[s], off[s+1]).
+ * lm_box_minmax: per time (m_min, m_max, xi_min, xi_max) into out (n_times, 4).
+ * lm_box_stats: per (seed, time) the moments sum(xi m^n)/dV, n = 0..3 (moments (n_sims, n_times, 4)), and
+ * over the no_bins (<= 64) mass bins edges (n_times, no_bins+1) of np.histogram's convention the spectra
+ * f_m = sum xi / (width_m dV), g_m = sum xi m / (width_m dV), g_ln_r = sum xi m / (width_lnr dV)
+ * (each (n_sims, n_times, no_bins)); per-bin sums in ascending SIP order.
+ * lm_mean_sem: over the first axis of fields (n_sims, n_vals), the mean and std(ddof=1)/sqrt(n_sims).
+ */
+int lm_box_minmax(int32_t n_times, int64_t n_tot, const double *xis, const double *masses, double *out,
+                  void *stream);
+int lm_box_stats(int32_t n_times, int32_t n_sims, const int64_t *off, int64_t n_tot, const double *xis,
+                 const double *masses, const double *edges, const double *width_m, const double *width_lnr,
+                 int32_t no_bins, double dV, double *moments, double *f_m, double *g_m, double *g_ln_r,
+                 void *stream);
+int lm_mean_sem(const double *fields, int32_t n_sims, int64_t n_vals, double *mean, double *sem, void *stream);
+
+/*
  * integration.py:150-286 compute_divergence_upwind_np for n_fields stacked
  * (nx, nz) fields with one (2, nx+1, nz+1) flux field: limited third-order
  * upwind flux divergence, boundary type 0 = periodic, 1 = fixed per axis.

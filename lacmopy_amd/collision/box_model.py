@@ -85,6 +85,136 @@ def golovin_expo_ensemble(seed, kappa=40, eta=1.0E-9, weak_threshold=False, LWC0
     return (DNC0 * dV) * weights, 1.0E18 * (1.0E-18 * masses), DNC0
 
 
+def compute_radius_from_mass_vec(mass, density):
+    """microphysics.py:83-122: (3/(4 pi) m / rho)^(1/3) (um from fg), with the C library's pow as Numba's
+    vectorized form calls it."""
+    m = np.broadcast_to(np.asarray(mass, dtype=np.float64), np.broadcast(mass, density).shape)
+    d = np.broadcast_to(np.asarray(density, dtype=np.float64), m.shape)
+    c, third = 0.75 / math.pi, 1.0 / 3.0
+    return np.array([math.pow(c * float(a) / float(b), third) for a, b in zip(m.ravel(), d.ravel())]).reshape(m.shape)
+
+
+def ecol_expo_ensemble(seed, mass_density=1.0E3, **kw):
+    """SIP quantities (xis, masses, radii, vel, mass_densities) of one seed of the expo ensemble for the
+    hydrodynamic kernels, as run_box_model.py:325-333 builds them: radii from the masses, Beard velocities."""
+    from .kernel import compute_terminal_velocity_Beard_vec
+    xis, masses, _ = golovin_expo_ensemble(seed, mass_density=mass_density, **kw)
+    mass_densities = np.ones_like(masses) * mass_density
+    radii = compute_radius_from_mass_vec(masses, mass_densities)
+    vel = compute_terminal_velocity_Beard_vec(radii)
+    return xis, masses, radii, vel, mass_densities
+
+
+def ensemble_statistics(xis_vs_time, masses_vs_time, mass_density, dV, no_bins):
+    """The statistics of analyze_sim_data (box_model.py:1354-1585) for snapshots held in memory:
+    xis_vs_time[s], masses_vs_time[s] of seed s with shape (no_times, N_s), masses in kg.
+
+    Per time the mass bins span the smallest to the largest mass of all seeds (no_bins logarithmic bins, the
+    outer edges widened by 1e-4), and per seed and time the moments 0-3 and the spectra f_m, g_m, g_ln_r are
+    summed on the GPU (lm_box_stats), then averaged over the seeds with their standard errors
+    (lm_mean_sem).  Returns a dict with the reference's arrays: moments_vs_time_avg/_std (no_times, 4),
+    f_m_num_avg/std_vs_time, g_m_num_..., g_ln_r_num_... (no_times, no_bins), bins_mass_centers,
+    bins_rad_centers (no_times, 3, no_bins), moments_vs_time_Unt, and the bins themselves.
+    """
+    torch = _torch()
+    L = lib()
+    no_sims = len(xis_vs_time)
+    no_times = int(np.asarray(xis_vs_time[0]).shape[0])
+    sizes = [int(np.asarray(x).shape[1]) for x in xis_vs_time]
+    off_h = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n_tot = int(off_h[-1])
+    X = np.concatenate([np.asarray(x, dtype=np.float64) for x in xis_vs_time], axis=1)
+    M = np.concatenate([np.asarray(m, dtype=np.float64) for m in masses_vs_time], axis=1)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    st = _stream(torch)
+    dX, dM = dev(X), dev(M)
+    mm = torch.empty((no_times, 4), dtype=torch.float64, device="cuda")
+    check(L.lm_box_minmax(no_times, n_tot, _p(dX), _p(dM), _p(mm), st))
+    mm = mm.cpu().numpy()
+    # box_model.py:1460-1506, per time: bins, widths and the three kinds of bin centres
+    bins_mass_vs_time = np.zeros((no_times, no_bins + 1))
+    bins_mass_width_vs_time = np.zeros((no_times, no_bins))
+    bins_rad_width_log_vs_time = np.zeros((no_times, no_bins))
+    bin_factors_vs_time = np.zeros(no_times)
+    bins_mass_centers, bins_rad_centers = [], []
+    for t in range(no_times):
+        m_min, m_max = mm[t, 0], mm[t, 1]
+        bin_factor = (m_max / m_min) ** (1.0 / no_bins)
+        bin_factors_vs_time[t] = bin_factor
+        bins_mass = np.zeros(no_bins + 1)
+        bins_mass[0] = m_min
+        for b in range(1, no_bins + 1):
+            bins_mass[b] = bins_mass[b - 1] * bin_factor
+        bins_mass[0] *= 0.9999
+        bins_mass[-1] *= 1.0001
+        bins_mass_vs_time[t] = bins_mass
+        bins_rad = compute_radius_from_mass_vec(1E18 * bins_mass, mass_density)
+        bins_mass_log, bins_rad_log = np.log(bins_mass), np.log(bins_rad)
+        width = bins_mass[1:] - bins_mass[:-1]
+        bins_mass_width_vs_time[t] = width
+        bins_rad_width_log_vs_time[t] = bins_rad_log[1:] - bins_rad_log[:-1]
+        c_lin = 0.5 * (bins_mass[:-1] + bins_mass[1:])
+        r_lin = 0.5 * (bins_rad[:-1] + bins_rad[1:])
+        c_log = np.exp(0.5 * (bins_mass_log[:-1] + bins_mass_log[1:]))
+        r_log = np.exp(0.5 * (bins_rad_log[:-1] + bins_rad_log[1:]))
+        m_avg = M[t].sum() / X[t].sum()
+        c_ex = bins_mass[:-1] + m_avg * np.log(width / (m_avg * (1 - np.exp(-width / m_avg))))
+        r_ex = compute_radius_from_mass_vec(1E18 * c_ex, mass_density)
+        bins_mass_centers.append(np.array((c_lin, c_log, c_ex)))
+        bins_rad_centers.append(np.array((r_lin, r_log, r_ex)))
+    off = torch.from_numpy(off_h).cuda()
+    ed, wm, wr = dev(bins_mass_vs_time), dev(bins_mass_width_vs_time), dev(bins_rad_width_log_vs_time)
+    mom = torch.empty((no_sims, no_times, 4), dtype=torch.float64, device="cuda")
+    fm = torch.empty((no_sims, no_times, no_bins), dtype=torch.float64, device="cuda")
+    gm, gr = torch.empty_like(fm), torch.empty_like(fm)
+    check(L.lm_box_stats(no_times, no_sims, _p(off), n_tot, _p(dX), _p(dM), _p(ed), _p(wm), _p(wr), no_bins,
+                         float(dV), _p(mom), _p(fm), _p(gm), _p(gr), st))
+    out = {}
+
+    def avg_sem(a, name):
+        mean, sem = torch.empty(a.shape[1:], dtype=torch.float64, device="cuda"), torch.empty(
+            a.shape[1:], dtype=torch.float64, device="cuda")
+        check(L.lm_mean_sem(_p(a), no_sims, int(a[0].numel()), _p(mean), _p(sem), st))
+        out[name + "_avg"], out[name + "_std"] = mean.cpu().numpy(), sem.cpu().numpy()
+
+    avg_sem(mom, "moments_vs_time")
+    avg_sem(fm, "f_m_num_vs_time")
+    avg_sem(gm, "g_m_num_vs_time")
+    avg_sem(gr, "g_ln_r_num_vs_time")
+    out["moments_vs_time"] = np.transpose(mom.cpu().numpy(), (1, 2, 0))  # (no_times, 4, no_sims)
+    # box_model.py:1543-1551
+    unt = np.zeros((no_times, 4))
+    for t in range(no_times):
+        for n in range(4):
+            unt[t, n] = math.log(bin_factors_vs_time[t]) / 3.0 * np.sum(
+                out["g_ln_r_num_vs_time_avg"][t] * bins_mass_centers[t][1] ** (n - 1))
+    out.update(moments_vs_time_Unt=unt, bins_mass_vs_time=bins_mass_vs_time,
+               bins_mass_width_vs_time=bins_mass_width_vs_time, bins_rad_width_log_vs_time=bins_rad_width_log_vs_time,
+               bins_mass_centers=np.array(bins_mass_centers), bins_rad_centers=np.array(bins_rad_centers),
+               m_min_vs_time=mm[:, 0], m_max_vs_time=mm[:, 1], xi_min_vs_time=mm[:, 2], xi_max_vs_time=mm[:, 3],
+               bin_factors_vs_time=bin_factors_vs_time)
+    return out
+
+
+def analyze_sim_data(kappa, mass_density, dV, no_sims, start_seed, no_bins, load_dir):
+    """box_model.py:1354-1585: loads the snapshots xis_vs_time_{seed}.npy / masses_vs_time_{seed}.npy (fg) of
+    the seeds start_seed, start_seed + 2, ... that simulate_collisions wrote, computes their statistics
+    (ensemble_statistics) and writes the reference's files into load_dir."""
+    seeds = np.arange(start_seed, start_seed + no_sims * 2, 2)
+    xs = [np.load(load_dir + f"xis_vs_time_{seed}.npy") for seed in seeds]
+    ms = [1E-18 * np.load(load_dir + f"masses_vs_time_{seed}.npy") for seed in seeds]
+    r = ensemble_statistics(xs, ms, mass_density, dV, no_bins)
+    end = f"no_sims_{no_sims}_no_bins_{no_bins}"
+    for key, name in (("moments_vs_time_avg", "moments_vs_time_avg"), ("moments_vs_time_std", "moments_vs_time_std"),
+                      ("f_m_num_vs_time_avg", "f_m_num_avg_vs_time"), ("f_m_num_vs_time_std", "f_m_num_std_vs_time"),
+                      ("g_m_num_vs_time_avg", "g_m_num_avg_vs_time"), ("g_m_num_vs_time_std", "g_m_num_std_vs_time"),
+                      ("g_ln_r_num_vs_time_avg", "g_ln_r_num_avg_vs_time"),
+                      ("g_ln_r_num_vs_time_std", "g_ln_r_num_std_vs_time"),
+                      ("bins_mass_centers", "bins_mass_centers"), ("bins_rad_centers", "bins_rad_centers")):
+        np.save(load_dir + f"{name}_" + end + ".npy", r[key])
+    return r
+
+
 def _save_schedule(dt, t_end, dt_save):
     """box_model.py:152-157: steps, snapshots (t = 0, every dt_save, and t_end) and steps between snapshots."""
     no_steps = int(math.ceil(t_end / dt))

@@ -22,6 +22,51 @@ def _hall_raw():
     return E, R, q
 
 
+# Beard (1976) terminal velocity, kernel.py:100-167 (polynomials in compute_polynom's order, algebra.py:11-34)
+_P1_BEARD = (-0.318657e1, 0.992696, -0.153193e-2, -0.987059e-3, -0.578878e-3, 0.855176e-4, -0.327815e-5)[::-1]
+_P2_BEARD = (-0.500015e1, 0.523778e1, -0.204914e1, 0.475294, -0.542819e-1, 0.238449e-2)[::-1]
+_V_MAX_BEARD = 9.04929248
+
+
+def _polynom(par, x):
+    res = par[0] * x + par[1]
+    for a in par[2:]:
+        res = res * x + a
+    return res
+
+
+def compute_terminal_velocity_Beard(R):
+    """kernel.py:107-167: terminal velocity (m/s) of a water drop of radius R (um) in air, Beard (1976) with
+    Bott's (1998) constants: Stokes-Cunningham below 10 um, the Davies-number fit below 535 um, the Bond-number
+    fit below 3.5 mm, 9.049 m/s above.  Host side (the box steps update velocities on the device); the C
+    library's log, exp and pow, as Numba lowers them, and viscosity^4 by squaring as Numba's integer power."""
+    rho_w, rho_a, visc, sigma_w, g = 1.0E3, 1.225, 1.818E-5, 73.0E-3, 9.80665
+    drho = rho_w - rho_a
+    if R < 10.0:
+        l0 = 6.62E-2
+        C1 = drho * g / (4.5 * visc)
+        C_sc = 1.0 + 1.257 * l0 / R
+        return C1 * C_sc * R * R * 1.0E-12
+    if R < 535.0:
+        N_Da = 32.0E-18 * R * R * R * rho_a * drho * g / (3.0 * visc * visc)
+        Y = _polynom(_P1_BEARD, math.log(N_Da))
+        C_sc = 1.0 + 1.257 * 6.62E-2 / R
+        return visc * C_sc * math.exp(Y) / (rho_a * R * 2.0E-6)
+    if R < 3500.0:
+        N_Bo = 16.0E-12 * R * R * drho * g / (3.0 * sigma_w)
+        v2 = visc * visc
+        N_P16 = math.pow(sigma_w * sigma_w * sigma_w * rho_a * rho_a / ((v2 * v2) * drho * g), 1.0 / 6.0)
+        Y = _polynom(_P2_BEARD, math.log(N_Bo * N_P16))
+        return visc * N_P16 * math.exp(Y) / (rho_a * R * 2.0E-6)
+    return _V_MAX_BEARD
+
+
+def compute_terminal_velocity_Beard_vec(R):
+    """kernel.py:169-190, elementwise."""
+    R = np.asarray(R, dtype=np.float64)
+    return np.array([compute_terminal_velocity_Beard(float(r)) for r in R.ravel()]).reshape(R.shape)
+
+
 def compute_E_col_Long_Bott(R_i, R_j):
     """kernel.py:350-386: E = 4.5e-4 R^2 (1 - 3 / (max(3, r) + 0.01)) for a collector R <= 50 um, else 1
     (R the larger, r the smaller radius, um)."""

@@ -1997,6 +1997,135 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* box-model ensemble statistics (collision/box_model.py:1354-1585)          */
+/* ------------------------------------------------------------------------ */
+/*
+ * Snapshots of all seeds live as (time, SIP) arrays: time t holds the SIPs of seed s in the columns
+ * [off[s], off[s+1]).  k_box_minmax: per time the smallest and largest mass and multiplicity over all seeds
+ * (exact: fmin/fmax do not round).
+ */
+__global__ __launch_bounds__(256) void k_box_minmax(long long n_tot, const double *xis, const double *masses,
+                                                    double *out /* (n_times, 4): m_min, m_max, xi_min, xi_max */) {
+  const int t = blockIdx.x;
+  const double *X = xis + (size_t)t * n_tot, *M = masses + (size_t)t * n_tot;
+  double a = INFINITY, b = -INFINITY, c = INFINITY, d = -INFINITY;
+  for (long long k = threadIdx.x; k < n_tot; k += blockDim.x) {
+    a = fmin(a, M[k]);
+    b = fmax(b, M[k]);
+    c = fmin(c, X[k]);
+    d = fmax(d, X[k]);
+  }
+  __shared__ double s[4][256];
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  s[2][threadIdx.x] = c;
+  s[3][threadIdx.x] = d;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + h]);
+      s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + h]);
+      s[2][threadIdx.x] = fmin(s[2][threadIdx.x], s[2][threadIdx.x + h]);
+      s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[(size_t)t * 4 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+/*
+ * One wavefront per (seed, time): the moments sum(xi m^n)/dV, n = 0..3 (box_model.py:1521-1524), and the
+ * histograms of box_model.py:1508-1519 over the time's nb mass bins (np.histogram's bins: [e_b, e_b+1),
+ * the last one closed; masses outside are not counted): f = sum xi / (width_m dV), g = sum xi m /
+ * (width_m dV), g_ln_r = sum xi m / (width_lnr dV) (binning the radius on the radius edges of the mass
+ * edges is binning the mass: the radius is increasing in the mass).  Lane b sums bin b over the seed's SIPs
+ * in ascending order, staged through LDS, so the sums are reproducible.  Outputs are (n_sims, n_times, .).
+ */
+#define BOXSTAT_CHUNK 1024
+#define BOXSTAT_BINS 64
+__global__ __launch_bounds__(WAVE) void k_box_stats(int n_times, int n_sims, const long long *off, long long n_tot,
+                                                    const double *xis, const double *masses, const double *edges,
+                                                    const double *width_m, const double *width_lnr, int nb,
+                                                    double dV, double *mom, double *f_m, double *g_m,
+                                                    double *g_lnr) {
+  const int s = blockIdx.x / n_times, t = blockIdx.x % n_times, lane = threadIdx.x;
+  const long long k0 = off[s], k1 = off[s + 1];
+  const double *X = xis + (size_t)t * n_tot, *M = masses + (size_t)t * n_tot, *E = edges + (size_t)t * (nb + 1);
+  __shared__ double sX[BOXSTAT_CHUNK], sM[BOXSTAT_CHUNK];
+  __shared__ int sB[BOXSTAT_CHUNK];
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, hn = 0.0, hm = 0.0;
+  for (long long c0 = k0; c0 < k1; c0 += BOXSTAT_CHUNK) {
+    const int cn = (int)(k1 - c0 < BOXSTAT_CHUNK ? k1 - c0 : BOXSTAT_CHUNK);
+    __syncthreads();
+    for (int q = lane; q < cn; q += WAVE) {
+      const double x = X[c0 + q], m = M[c0 + q];
+      sX[q] = x;
+      sM[q] = m;
+      /* searchsorted(edges, m, 'right') - 1, the closed last bin, -1 outside */
+      int lo = 0, hi = nb + 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (E[mid] <= m)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      int b = lo - 1;
+      if (m == E[nb]) b = nb - 1;
+      sB[q] = (b >= 0 && b < nb) ? b : -1;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      /* moments in ascending SIP order */
+      for (int q = 0; q < cn; ++q) {
+        const double x = sX[q], m = sM[q];
+        m0 += x;
+        m1 += x * m;
+        m2 += x * (m * m);
+        m3 += x * (m * m * m);
+      }
+    }
+    if (lane < nb) {
+      for (int q = 0; q < cn; ++q) {
+        if (sB[q] == lane) {
+          hn += sX[q];
+          hm += sX[q] * sM[q];
+        }
+      }
+    }
+  }
+  const size_t base = (size_t)s * n_times + t;
+  if (lane == 0) {
+    mom[base * 4 + 0] = m0 / dV;
+    mom[base * 4 + 1] = m1 / dV;
+    mom[base * 4 + 2] = m2 / dV;
+    mom[base * 4 + 3] = m3 / dV;
+  }
+  if (lane < nb) {
+    const double wm = width_m[(size_t)t * nb + lane] * dV, wr = width_lnr[(size_t)t * nb + lane] * dV;
+    f_m[base * nb + lane] = hn / wm;
+    g_m[base * nb + lane] = hm / wm;
+    g_lnr[base * nb + lane] = hm / wr;
+  }
+}
+
+/* mean over seeds and its standard error std(ddof = 1) / sqrt(n) (box_model.py:1526-1541), two passes */
+__global__ void k_mean_sem(const double *f, int n_sims, long long n_vals, double *mean, double *sem) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_vals) return;
+  double s = 0.0;
+  for (int r = 0; r < n_sims; ++r) s += f[(size_t)r * n_vals + k];
+  const double m = s / (double)n_sims;
+  double q = 0.0;
+  for (int r = 0; r < n_sims; ++r) {
+    const double d = f[(size_t)r * n_vals + k] - m;
+    q += d * d;
+  }
+  mean[k] = m;
+  sem[k] = n_sims > 1 ? sqrt(q / (double)(n_sims - 1)) / sqrt((double)n_sims) : 0.0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* box models: one wavefront per box                                        */
 /* ------------------------------------------------------------------------ */
 #define BOX_CAP 1024
@@ -3054,6 +3183,38 @@ int lm_box_check(void *stream) {
     snprintf(msg, sizeof msg, "box %d holds more than %d SIPs (box kernels keep a box in LDS)", h[1], BOX_CAP);
     return set_err(h[0], msg);
   }
+  return 0;
+}
+
+int lm_box_minmax(int32_t n_times, int64_t n_tot, const double *xis, const double *masses, double *out,
+                  void *stream) {
+  if (n_times <= 0 || n_tot <= 0 || !xis || !masses || !out) return set_err(LM_ERR_ARG, "bad box min/max arguments");
+  hipLaunchKernelGGL(k_box_minmax, dim3(n_times), dim3(256), 0, S(stream), (long long)n_tot, xis, masses, out);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_box_stats(int32_t n_times, int32_t n_sims, const int64_t *off, int64_t n_tot, const double *xis,
+                 const double *masses, const double *edges, const double *width_m, const double *width_lnr,
+                 int32_t no_bins, double dV, double *moments, double *f_m, double *g_m, double *g_ln_r,
+                 void *stream) {
+  if (n_times <= 0 || n_sims <= 0 || n_tot <= 0 || !off || !xis || !masses || !edges || !width_m || !width_lnr ||
+      !moments || !f_m || !g_m || !g_ln_r)
+    return set_err(LM_ERR_ARG, "bad box statistics arguments");
+  if (no_bins < 1 || no_bins > BOXSTAT_BINS) return set_err(LM_ERR_ARG, "no_bins must be in 1..64");
+  if ((int64_t)n_times * n_sims > 0x7fffffff) return set_err(LM_ERR_ARG, "too many (time, seed) pairs");
+  hipLaunchKernelGGL(k_box_stats, dim3((unsigned)(n_times * n_sims)), dim3(WAVE), 0, S(stream), n_times, n_sims,
+                     (const long long *)off, (long long)n_tot, xis, masses, edges, width_m, width_lnr, no_bins, dV,
+                     moments, f_m, g_m, g_ln_r);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_mean_sem(const double *fields, int32_t n_sims, int64_t n_vals, double *mean, double *sem, void *stream) {
+  if (!fields || !mean || !sem || n_sims <= 0 || n_vals <= 0) return set_err(LM_ERR_ARG, "bad mean/sem arguments");
+  hipLaunchKernelGGL(k_mean_sem, dim3((unsigned)((n_vals + 255) / 256)), dim3(256), 0, S(stream), fields, n_sims,
+                     (long long)n_vals, mean, sem);
+  LAUNCH_CHECK();
   return 0;
 }
 

@@ -63,3 +63,70 @@ def oracle_golovin_box(seed, box, box_seed, t_end, dt_save, out_path):
     ms.append(m.copy())
     np.savez(out_path, xs=np.array(xs), ms=np.array(ms), no_cols=no_cols)
     return out_path
+
+
+def oracle_ecol_box(kernel, seed, box, box_seed, t_end, dt_save, E_path, out_path):
+    """An Ecol_grid_R box model (all_or_nothing.py:269-390; Beard velocities updated after every step,
+    box_model.py:165-176) of one seed of the expo ensemble with the oracle, draws keyed (seed, box, step)."""
+    from lacmopy_amd.collision.box_model import ecol_expo_ensemble, _save_schedule
+    from oracle import oracle as orc
+    d = np.load(E_path)
+    E, rg = d["E"], d["rg"]
+    lnR0, lnf, nb = np.log(rg[0]), np.log(rg[1] / rg[0]), len(rg)
+    xi, m, R, vel, rho = ecol_expo_ensemble(box_seed)
+    L = orc.lib()
+    no_steps, no_saves, dn_save = _save_schedule(1.0, t_end, dt_save)
+    xs, ms = [], []
+    no_cols = np.zeros(2, dtype=np.int64)
+    for step in range(no_steps):
+        if step % dn_save == 0:
+            xs.append(xi.copy())
+            ms.append(m.copy())
+        L.or_collision_ecol_box(len(xi), orc.P(xi), orc.P(m), orc.P(R), orc.P(vel), orc.P(rho), 1.0, orc.P(E), nb,
+                                lnR0, lnf, orc.P(no_cols), None, seed, box, step)
+        L.or_update_velocity_Beard(len(xi), orc.P(vel), orc.P(R))
+    xs.append(xi.copy())
+    ms.append(m.copy())
+    np.savez(out_path, xs=np.array(xs), ms=np.array(ms), no_cols=no_cols, R=R, vel=vel)
+    return out_path
+
+
+def numpy_box_statistics(xis_vs_time, masses_vs_time, mass_density, dV, no_bins):
+    """analyze_sim_data's sums (box_model.py:1390-1541) with NumPy's histogram, mean and std, masses in kg:
+    the checker of the GPU statistics."""
+    no_sims, no_times = len(xis_vs_time), xis_vs_time[0].shape[0]
+    mom = np.zeros((no_times, 4, no_sims))
+    f_all, g_all, r_all = [], [], []
+    for t in range(no_times):
+        ms = [m[t] for m in masses_vs_time]
+        xs = [x[t] for x in xis_vs_time]
+        m_min, m_max = np.concatenate(ms).min(), np.concatenate(ms).max()
+        bf = (m_max / m_min) ** (1.0 / no_bins)
+        bins = np.zeros(no_bins + 1)
+        bins[0] = m_min
+        for b in range(1, no_bins + 1):
+            bins[b] = bins[b - 1] * bf
+        bins[0] *= 0.9999
+        bins[-1] *= 1.0001
+        rad_bins = (0.75 / np.pi * 1E18 * bins / mass_density) ** (1.0 / 3.0)
+        w = bins[1:] - bins[:-1]
+        wl = np.log(rad_bins[1:]) - np.log(rad_bins[:-1])
+        f, g, r = [], [], []
+        for s in range(no_sims):
+            rad = (0.75 / np.pi * 1E18 * ms[s] / mass_density) ** (1.0 / 3.0)
+            f.append(np.histogram(ms[s], bins, weights=xs[s])[0] / (w * dV))
+            g.append(np.histogram(ms[s], bins, weights=xs[s] * ms[s])[0] / (w * dV))
+            r.append(np.histogram(rad, rad_bins, weights=xs[s] * ms[s])[0] / (wl * dV))
+            mom[t, 0, s] = xs[s].sum() / dV
+            for n in range(1, 4):
+                mom[t, n, s] = np.sum(xs[s] * ms[s] ** n) / dV
+        f_all.append(f)
+        g_all.append(g)
+        r_all.append(r)
+    f_all, g_all, r_all = (np.array(a) for a in (f_all, g_all, r_all))  # (no_times, no_sims, no_bins)
+    out = {"moments_vs_time_avg": mom.mean(axis=2),
+           "moments_vs_time_std": np.std(mom, axis=2, ddof=1) / np.sqrt(no_sims)}
+    for name, a in (("f_m_num_vs_time", f_all), ("g_m_num_vs_time", g_all), ("g_ln_r_num_vs_time", r_all)):
+        out[name + "_avg"] = a.mean(axis=1)
+        out[name + "_std"] = np.std(a, axis=1, ddof=1) / np.sqrt(no_sims)
+    return out

@@ -123,6 +123,64 @@ def test_golovin_box_model_50_seeds_3600_s(gpu, tmp_path):
     assert np.max(np.abs(rel[times <= 900.0, 2])) < 0.08, rel[:, 2]
     assert -0.5 < rel[-1, 2] < -0.1
     assert nc.sum() > 0
+    _check_box_statistics(xs, ms)
+
+
+def _check_box_statistics(xs, ms, no_bins=50):
+    """analyze_sim_data's statistics on the GPU (lm_box_stats, lm_mean_sem) against NumPy's histogram, mean
+    and std of the same snapshots: the per-bin and moment sums differ only in summation order (tolerance
+    1e-12 of the largest value of each array)."""
+    from lacmopy_amd.collision.box_model import ensemble_statistics
+    mk = [1.0E-18 * m for m in ms]
+    got = ensemble_statistics(xs, mk, 1.0E3, 1.0, no_bins)
+    ref = fc.numpy_box_statistics(xs, mk, 1.0E3, 1.0, no_bins)
+    for key, r in ref.items():
+        g = got[key]
+        assert g.shape == r.shape, key
+        if key.startswith("moments"):
+            np.testing.assert_allclose(g, r, rtol=1e-10, atol=0, err_msg=key)
+        else:  # NumPy's weighted histogram differences cumulative sums: absolute errors on the row's scale
+            scale = np.max(np.abs(r), axis=-1, keepdims=True)
+            assert np.all(np.abs(g - r) <= 1e-12 * scale), (key, float(np.max(np.abs(g - r) / scale)))
+
+
+@pytest.mark.parametrize("kernel", ["Long_Bott", "Hall_Bott"])
+def test_hydrodynamic_box_models_50_seeds_3600_s(gpu, tmp_path, kernel):
+    """Long and Hall box models (Ecol_grid_R on the box model's table, 0.6-6000 um at 200 bins per decade,
+    Beard velocities updated after every step) of the 50 seeds: every snapshot, radius and velocity of every
+    box bit-exact against the oracle; ensemble moments against the Wang et al. (2007) moments the reference
+    holds (lambda_0 within 4% to 1800 s, lambda_2 within 5% to 1200 s, lambda_1 within 0.5%)."""
+    from lacmopy_amd.collision.box_model import ecol_expo_ensemble, simulate_collisions_batched
+    from lacmopy_amd.collision.kernel import generate_E_col_grid_R_np
+    n_seeds, t_end, dt_save = 50, 3600.0, 200.0
+    seeds = [SEED0 + 2 * k for k in range(n_seeds)]
+    E, rg = generate_E_col_grid_R_np(0.6, 6.0E3, 200, kernel)
+    E_path = str(tmp_path / "E.npz")
+    np.savez(E_path, E=E, rg=rg)
+    print(f"C1 {kernel}: {n_seeds} boxes x {int(t_end)} steps", flush=True)
+    with _pool(16) as ex:
+        futs = [ex.submit(fc.oracle_ecol_box, kernel, SEED0, k, s, t_end, dt_save, E_path,
+                          str(tmp_path / f"box{k}.npz")) for k, s in enumerate(seeds)]
+        boxes = [ecol_expo_ensemble(s) for s in seeds]
+        kq = (E, len(rg), float(np.log(rg[0])), float(np.log(rg[1] / rg[0])))
+        xs, ms, times, nc = simulate_collisions_batched(boxes, kq, kernel, "Ecol_grid_R", 1.0, 1.0, t_end, dt_save,
+                                                        SEED0)
+        paths = [f.result() for f in futs]
+    for k, p in enumerate(paths):
+        d = np.load(p)
+        _assert_same(xs[k], d["xs"], f"box {k} xi snapshots")
+        _assert_same(ms[k], d["ms"], f"box {k} mass snapshots")
+        _assert_same(boxes[k][2], d["R"], f"box {k} radii")
+        _assert_same(boxes[k][3], d["vel"], f"box {k} velocities")
+        _assert_same(nc[k], d["no_cols"], f"box {k} no_cols")
+    ref = np.loadtxt(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_data", kernel,
+                                  "Wang_2007_moments.txt"))
+    mom = np.array([[(xs[k] * (1.0E-18 * ms[k]) ** n).sum(axis=1) for n in range(4)] for k in range(n_seeds)])
+    rel = mom.mean(axis=0) / ref - 1.0
+    assert np.max(np.abs(rel[0, times <= 1800.0])) < 0.04, rel[0]
+    assert np.max(np.abs(rel[1])) < 0.005, rel[1]
+    assert np.max(np.abs(rel[2, times <= 1200.0])) < 0.05, rel[2]
+    _check_box_statistics(xs, ms)
 
 
 def test_icmw75_plain_run_one_step_autotuned(gpu, tmp_path):
