@@ -166,7 +166,19 @@ typedef struct LmEngine {
   /* particle substep compiled for this many waves per SIMD (0: the default build; 5, 6 or 8: a smaller
      register budget), one SIP per thread; set by an autotuner */
   int32_t part_wpe, pad4;
+  /*
+   * Migrant lists (NULL: off): per (run, cell) the slots of the SIPs that entered the cell from another home
+   * segment since the last reindex, appended by the position updates (mig_cnt counts every entry, also
+   * past mig_cap; an entry may be stale or repeated -- readers keep the slots whose cell is the cell, once).
+   * Per-cell passes take a cell's members as its home segment plus this list while the list is complete
+   * (mig_cnt <= mig_cap), else from the 3x3 home segments the away mask names.  Cleared by every reindex.
+   */
+  int32_t *mig_cnt;  /* n_runs*nc */
+  int32_t *mig_slot; /* n_runs*nc*mig_cap */
+  int32_t mig_cap, pad5; /* 1..LM_MIG_CAP_MAX */
 } LmEngine;
+
+#define LM_MIG_CAP_MAX 64 /* largest migrant list per cell */
 
 #define LM_WORK_BINS 1024 /* home-segment sizes are binned up to LM_WORK_BINS - 1 for the work-list order */
 

@@ -358,6 +358,8 @@ struct KView {
   int32_t *rem_cnt, *rem_slot;
   int rem_cap;
   int32_t *scan_buf, *scan_part, *err, *ovf_list, *ovf_cnt;
+  int32_t *mig_cnt, *mig_slot; /* migrant lists (NULL: off) */
+  int mig_cap;
   const int32_t *work; /* (run, cell) work list of the persistent passes */
   int32_t *work_ctr;   /* its 2 counters: collision [0], Delta m_l [1] */
   const double *rnd_stream;
@@ -449,6 +451,9 @@ static KView make_view(const LmEngine *e) {
   v.err = e->err;
   v.ovf_list = e->ovf_list;
   v.ovf_cnt = e->ovf_cnt;
+  v.mig_cnt = e->mig_cnt;
+  v.mig_slot = e->mig_slot;
+  v.mig_cap = e->mig_cap;
   v.work = e->work;
   v.work_ctr = e->work ? e->work + (size_t)e->n_runs * e->nc + LM_WORK_BINS : nullptr;
   v.rnd_stream = e->rnd_stream;
@@ -467,6 +472,92 @@ static KView make_view(const LmEngine *e) {
   }
   v.ccd = e->ccd;
   return v;
+}
+
+/*
+ * Members of global cell gc from its home segment and its migrant list, valid while the list is complete
+ * (mig_cnt[gc] <= mig_cap): the home segment [s0, s1) holds its SIPs in ascending id, the list the slots of
+ * SIPs that entered from other segments since the reindex (some stale: moved on, or entries repeated).
+ * Writes the members' slots in ascending id into sSL (up to cap) and returns their number (above cap: the
+ * cell is over capacity), or -1 when the list is incomplete (the caller takes the 3x3 gather instead).
+ * LDS scratch: sOwn, sOid (cap ints each), sM (3 * LM_MIG_CAP_MAX ints).  Ends with a barrier.
+ */
+__device__ static int fast_members(const KView &v, int gc, int *sSL, int cap, int *sOwn, int *sOid, int *sM) {
+  const int lane = threadIdx.x;
+  const int mc = v.mig_cnt[gc];
+  if (mc > v.mig_cap) return -1;
+  const int nc = v.nc, r = gc / nc, c = gc % nc;
+  const int32_t *seg = v.seg + (size_t)r * (nc + 1);
+  const int s0 = seg[c], T0 = seg[c + 1] - s0;
+  /* home segment: members in slot order, which is id order */
+  int n_own = 0;
+  for (int base = 0; base < T0; base += WAVE) {
+    const int t = base + lane;
+    const bool mem = t < T0 && v.cell[s0 + t] == gc;
+    const unsigned long long bal = __ballot(mem);
+    const int pos = n_own + __popcll(bal & lane_mask_lt());
+    if (mem && pos < cap) sOwn[pos] = s0 + t;
+    n_own += __popcll(bal);
+  }
+  int *sMk = sM, *sMs = sM + LM_MIG_CAP_MAX, *sMid = sM + 2 * LM_MIG_CAP_MAX;
+  int nm = 0;
+  if (mc > 0) {
+    /* the list's members, each once (an earlier entry with the same slot wins), then sorted by id */
+    int k = -1;
+    bool mem = false;
+    if (lane < mc) {
+      k = v.mig_slot[(size_t)gc * v.mig_cap + lane];
+      mem = v.cell[k] == gc;
+      sMk[lane] = k;
+    }
+    __syncthreads();
+    if (mem)
+      for (int e = 0; e < lane; ++e)
+        if (sMk[e] == k) mem = false;
+    const unsigned long long bal = __ballot(mem);
+    nm = __popcll(bal);
+    const int idk = mem ? v.id[k] : 0;
+    int rk = 0;
+    __syncthreads();
+    if (mem) sMid[__popcll(bal & lane_mask_lt())] = idk;
+    __syncthreads();
+    if (mem)
+      for (int e = 0; e < nm; ++e) rk += sMid[e] < idk;
+    __syncthreads();
+    if (mem) {
+      sMs[rk] = k;
+      sMid[rk] = idk;
+    }
+  }
+  const int n = n_own + nm;
+  __syncthreads();
+  if (n > cap) return n;
+  if (nm == 0) {
+    for (int p = lane; p < n_own; p += WAVE) sSL[p] = sOwn[p];
+  } else {
+    for (int p = lane; p < n_own; p += WAVE) sOid[p] = v.id[sOwn[p]];
+    __syncthreads();
+    for (int p = lane; p < n_own; p += WAVE) {
+      const int idp = sOid[p];
+      int lo = 0;
+      while (lo < nm && sMid[lo] < idp) ++lo; /* at most LM_MIG_CAP_MAX */
+      sSL[p + lo] = sOwn[p];
+    }
+    for (int q = lane; q < nm; q += WAVE) {
+      const int idq = sMid[q];
+      int lo = 0, hi = n_own;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sOid[mid] < idq)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      sSL[q + lo] = sMs[q];
+    }
+  }
+  __syncthreads();
+  return n;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -556,14 +647,14 @@ __device__ static inline int next_cell(const KView &v, CellSource &s, const int3
  * of the pair loop, so a lane walks one row through consecutive diagonals
  * and consecutive pair numbers: one Philox block serves two diagonals.
  */
-/* p > u(i, j), deciding from the A word when possible (see lm_philox.h) */
-__device__ static inline bool pair_fires(double p, uint32_t aw, unsigned long long seed, uint32_t run, uint32_t step,
-                                         uint32_t cell, uint32_t i, uint32_t d) {
-  const double lo = (double)(aw >> 5) * (1.0 / 134217728.0); /* (A>>5) * 2^-27 */
+/* p > u(i, j), deciding from the guard byte G and the A word when possible (see lm_philox.h) */
+__device__ static inline bool pair_fires(double p, uint32_t G, uint32_t aw, unsigned long long seed, uint32_t run,
+                                         uint32_t step, uint32_t cell, uint32_t i, uint32_t d) {
+  const double lo = lm_pair_lo40(G, aw); /* (G 2^32 + A) 2^-40 <= u < lo + 2^-40 */
   if (p < lo) return false;
-  if (p >= lo + (1.0 / 134217728.0)) return true;
+  if (p >= lo + (1.0 / 1099511627776.0)) return true;
   const lm_u32x4 b = lm_pair_block(seed, run, step, cell, i, d, 1);
-  return p > lm_res53(aw, lm_word(b, d & 3));
+  return p > lm_pair_u53(G, aw, lm_word(b, d & 3));
 }
 
 /* State of one cell's collision step in LDS (id order) plus slot-indexed globals. */
@@ -573,6 +664,7 @@ struct CollCell {
   int n;
   int gc;   /* global cell */
   int *seq; /* LDS: events of this cell so far (event log) */
+  const float *sB; /* LDS: row bounds of the pass filter */
 };
 
 /* global SIP id of slot k: runs occupy the slot ranges [run_base[r], run_base[r+1]) */
@@ -717,7 +809,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
  */
 template <int CAP>
 __device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n, const double *sX, const double *sR,
-                                  const double *sU, const double *sW, float *sB) {
+                                  const double *sU, const double *sW, float *sB, unsigned char *sTG) {
   const int lane = threadIdx.x;
   double X = -INFINITY, Rc = 0.0, ulo = INFINITY, uhi = -INFINITY, wlo = INFINITY, whi = -INFINITY;
   for (int p = lane; p < n; p += WAVE) {
@@ -749,13 +841,142 @@ __device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n
     float f = (float)B; /* B >= 0 or NaN: rounded down, the next float up is above B */
     if ((double)f < B) f = __int_as_float(__float_as_int(f) + 1);
     sB[p] = f;
+    /* tg = the smallest guard byte G with (G/256)^2 (1 - 2^-40) > B, the quad filter's own test: a quad of
+       row p whose valid pairs all have leading bytes >= tg holds no pair that can fire (G/256 is a lower bound
+       of each uniform); tg >= 256 (NaN, or B too large for any byte) never rejects.  Stored as
+       sTG[p] = min(tg, 256) - 1: reject when the smallest byte exceeds sTG[p] */
+    const double Bf = (double)f;
+    int tg = 257;
+    if (Bf == Bf && Bf < 1.0) {
+      tg = (int)(256.0 * sqrt(Bf));
+      if (tg > 256) tg = 256;
+      for (int k = 0; k < 4 && tg <= 256 && !(((tg * (1.0 / 256.0)) * (tg * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf);
+           ++k)
+        ++tg;
+      if (!(((tg * (1.0 / 256.0)) * (tg * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf)) tg = 257;
+      while (tg > 1 && (((tg - 1) * (1.0 / 256.0)) * ((tg - 1) * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf) --tg;
+    }
+    sTG[p] = (unsigned char)((tg < 256 ? tg : 256) - 1);
   }
 }
 
 /* pair number of (i, j) in the reference's row-major loop */
 __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
 
-#define SPEC_MAX_EVENTS 4
+#define SPEC_MAX_EVENTS 16
+#define QCAP 128 /* quads the guard pass can queue per pass */
+
+/* State of one cell's collision step in LDS (id order) plus slot-indexed globals. */
+
+/*
+ * Quad (i, b) of the speculative pass -- four consecutive pairs (i, 4b + t - i) of row i sharing one Philox
+ * block -- under the current state: the row-major key (cnt << 32 | i << 16 | j) of its first pair with
+ * cnt >= k_next that fires, or ~0.  gw is the quad's guard word (Philox draws), unused with an injected
+ * stream.
+ */
+template <int CAP, bool TAB, bool INJ>
+__device__ static inline unsigned long long eval_quad(const LmPhys &P, const KView &v, const CollCell &cc, int i,
+                                                      int b, int k_next, uint32_t gw, long long roff,
+                                                      uint32_t key_run, uint32_t key_step, uint32_t key_cell) {
+  const int n = cc.n;
+  const double *sX = cc.sX, *sR = cc.sR, *sU = cc.sU, *sW = cc.sW;
+  const int *sIND = cc.sIND;
+  const float *sB = cc.sB;
+  const double dtdV = v.dt_col_over_dV, pi = P.pi;
+  unsigned long long best = ~0ull;
+  lm_u32x4 blk = {{0u, 0u, 0u, 0u}};
+  if (!INJ) blk = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)(4 * b), 0);
+  {
+    /* row filter: the quad's smallest threshold against the row bound (row_bounds) */
+    double thq = 1.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int jr = 4 * b + t - i;
+      const bool valid = (jr > i) & (jr < n) & (pair_cnt(i, jr, n) >= k_next);
+      double th;
+      if (INJ)
+        th = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 1.0;
+      else
+        th = lm_pair_lo40(lm_guard_byte(gw, (uint32_t)t), lm_word(blk, (uint32_t)t));
+      if (valid) thq = fmin(thq, th);
+    }
+    if ((thq > 0.0) & ((thq * thq) * (1.0 - 0x1p-40) > (double)sB[i])) return ~0ull;
+  }
+  const double xi_i = sX[i], R_i = sR[i], u_i = sU[i], w_i = sW[i];
+  const double *Erow = TAB ? v.E + (size_t)sIND[i] * v.nb : nullptr;
+  /*
+   * Filter.  p = (xi_max dt/dV) * ((((pi s) s) E) sqrt(d2)) 1e-12 with s = R_i + R_j.  Without the
+   * square root, Pl^2 d2 (Pl = (xi_max dt/dV) (((pi s) s) E) 1e-12) equals p^2 within 10 roundings,
+   * so Pl^2 d2 < thr^2 (1 - 2^-40) proves p < thr, thr = the 40-bit lower bound of the pair's
+   * uniform (or the injected uniform): the pair does not fire and is no multiple collection.  For
+   * Pl <= 1e-140 (and d2 < 1e100) p < 1e-89 < thr whenever thr^2 > 1e-178.  Everything else --
+   * thr = 0, overflow, NaN -- takes the exact path below.
+   */
+  unsigned maybe = 0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int d = 4 * b + t;
+    const int jr = d - i;
+    const bool valid = (jr > i) & (jr < n) & (pair_cnt(i, jr, n) >= k_next);
+    /* clamped into [0, n-1] (jr is -1 at the start of some rows, >= n at the end): every lane runs the
+       same straight-line code on in-range LDS entries, and invalid pairs are masked out below */
+    const int j = jr < 0 ? 0 : (jr < n ? jr : n - 1);
+    const double xi_j = sX[j];
+    const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
+    const double du = u_i - sU[j], dw = w_i - sW[j];
+    const double d2 = du * du + dw * dw;
+    const double sR2 = R_i + sR[j];
+    double thr;
+    if (INJ)
+      thr = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 0.0;
+    else
+      thr = lm_pair_lo40(lm_guard_byte(gw, (uint32_t)t), lm_word(blk, (uint32_t)t));
+    const double thr2 = (thr * thr) * (1.0 - 0x1p-40);
+    const double geo = (xi_max * dtdV) * ((pi * sR2) * sR2);
+    /* first with the table's largest entry: every rounding step is monotone in E, so a pair rejected
+       with E_max is rejected with its own entry; only the others read the table */
+    bool no = false;
+    if (TAB && v.E_max > 0.0) {
+      const double Pm = (geo * v.E_max) * 1.0E-12;
+      no = (Pm > 1.0E-140) & ((Pm * Pm) * d2 < thr2);
+    }
+    if (!no) {
+      const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
+      const double Pl = (geo * Ec) * 1.0E-12;
+      const bool big = Pl > 1.0E-140;
+      no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 1.0E-178));
+    }
+    maybe |= (unsigned)(valid & !no) << t;
+  }
+  if (maybe) {
+    /* rare: the reference's exact p_crit and decision, first firing pair of the quad */
+    for (int t = 0; t < 4; ++t) {
+      if (!((maybe >> t) & 1u)) continue;
+      const int d = 4 * b + t;
+      const int j = d - i;
+      const int cnt = pair_cnt(i, j, n);
+      const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
+      const double xi_j = sX[j];
+      const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
+      const double du = u_i - sU[j], dw = w_i - sW[j];
+      const double p_crit = xi_max * dtdV * lmp_kernel_hydro(P, R_i, sR[j], Ec, sqrt(du * du + dw * dw));
+      bool fires;
+      if (p_crit > 1.0)
+        fires = true;
+      else if (INJ)
+        fires = p_crit > rnd_at(v, roff + cnt);
+      else
+        fires = pair_fires(p_crit, lm_guard_byte(gw, (uint32_t)t), lm_word(blk, (uint32_t)t), v.seed, key_run,
+                           key_step, key_cell, (uint32_t)i, (uint32_t)d);
+      if (fires) {
+        const unsigned long long key = ((unsigned long long)cnt << 32) | ((unsigned)i << 16) | (unsigned)j;
+        best = key < best ? key : best;
+        break;
+      }
+    }
+  }
+  return best;
+}
 
 /*
  * One all-cells collision step, one wavefront per cell.
@@ -789,6 +1010,9 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
   __shared__ int sIND[CAP], sSL[CAP], sQ[CAP + 1];
   __shared__ float sB[CAP];
+  __shared__ unsigned char sTG[CAP];
+  __shared__ int sQueue[QCAP];
+  __shared__ int sQn;
   __shared__ NbrTab tb;
   __shared__ int sSeq;
   int *sSlot = (int *)sU;
@@ -796,8 +1020,14 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   int *sRank = (int *)sW;
   unsigned char *sSub = (unsigned char *)(sRank + CAP);
 
-  nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
-  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
+  /* members: from the home segment and the migrant list (slots in id order straight into sSL), or, with
+     an incomplete list, gathered from the 3x3 home segments and ranked by id */
+  int n = v.mig_cnt ? fast_members(v, gc, sSL, CAP, sSlot, sId, (int *)sW) : -1;
+  const bool fast = n >= 0;
+  if (!fast) {
+    nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
+    n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
+  }
   if (n > CAP) {
     if (lane == 0) {
       if (from_ovf) {
@@ -812,10 +1042,12 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   if (n < 2) return;
   if (lane == 0) sSeq = 0;
   __syncthreads();
-  rank_members(n, sId, sSub, &tb, sRank);
-  __syncthreads();
-  for (int m = lane; m < n; m += WAVE) sSL[sRank[m]] = sSlot[m];
-  __syncthreads();
+  if (!fast) {
+    rank_members(n, sId, sSub, &tb, sRank);
+    __syncthreads();
+    for (int m = lane; m < n; m += WAVE) sSL[sRank[m]] = sSlot[m];
+    __syncthreads();
+  }
 
   const double *C = v.cc + (size_t)gc * CC_N;
   const double T = C[CC_T], rho_w_T = C[CC_RHOW];
@@ -847,123 +1079,126 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   }
   __syncthreads();
 
-  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n, gc, &sSeq};
+  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n, gc, &sSeq, sB};
   const bool use_tab = TAB;
   const long long roff = INJ ? v.rnd_off[gc] : 0;
   const double dtdV = v.dt_col_over_dV, pi = P.pi;
   /* the E bound of the row filter: the table's largest entry (unknown: no row filter) or the constant */
   const double Eb = TAB ? (v.E_max > 0.0 ? v.E_max : INFINITY) : v.E_const;
-  row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB);
+  row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB, sTG);
   __syncthreads();
   const uint32_t key_run = (uint32_t)r, key_step = *v.col_step_dev, key_cell = (uint32_t)c;
+  const uint32_t gkey = INJ ? 0u : lm_guard_key(v.seed, key_run, key_step, key_cell);
   double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
   const int nQ = sQ[n - 1];
   long long c_single = 0, c_multi = 0, c_eq = 0;
   int k_next = 0;  /* first pair (row-major number) not yet decided */
+  unsigned long long prev_event = 0; /* key of the last applied event */
   int n_events = 0;
   bool rho_ready = false;
   bool finished = false;
 
   while (n_events < SPEC_MAX_EVENTS) {
-    /* parallel pass: first firing pair with cnt >= k_next under the current state */
+    /* parallel pass: first firing pair with cnt >= k_next under the current state.  The first quad holding
+       pair k_next: the pair after the last event (or pair 0); quad order is row-major pair order. */
     unsigned long long best = ~0ull;
-    int i = 0; /* row of quad q: rows only grow with q, so each lane walks forward */
-    for (int q = lane; q < nQ; q += WAVE) {
-      while (sQ[i + 1] <= q) ++i;
-      const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
-      const int dlast = (4 * b + 3 < i + n - 1) ? 4 * b + 3 : i + n - 1;
-      if (pair_cnt(i, dlast - i, n) < k_next) continue;
-      lm_u32x4 blk = {{0u, 0u, 0u, 0u}};
-      if (!INJ) blk = lm_pair_block(v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)(4 * b), 0);
-      {
-        /* row filter: the quad's smallest threshold against the row bound (row_bounds) */
-        double thq = 1.0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int jr = 4 * b + t - i;
-          const bool valid = (jr > i) & (jr < n) & (pair_cnt(i, jr, n) >= k_next);
-          double th;
-          if (INJ)
-            th = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 1.0;
-          else
-            th = (double)(lm_word(blk, (uint32_t)t) >> 5) * (1.0 / 134217728.0);
-          if (valid) thq = fmin(thq, th);
-        }
-        if ((thq > 0.0) & ((thq * thq) * (1.0 - 0x1p-40) > (double)sB[i])) continue;
+    int i0 = 0, q_first = 0;
+    if (k_next > 0) {
+      int ni = (int)((prev_event >> 16) & 0xffff), nj = (int)(prev_event & 0xffff) + 1;
+      if (nj >= n) {
+        ++ni;
+        nj = ni + 1;
       }
-      const double xi_i = sX[i], R_i = sR[i], u_i = sU[i], w_i = sW[i];
-      const double *Erow = TAB ? v.E + (size_t)sIND[i] * v.nb : nullptr;
-      /*
-       * Filter.  p = (xi_max dt/dV) * ((((pi s) s) E) sqrt(d2)) 1e-12 with s = R_i + R_j.  Without the
-       * square root, Pl^2 d2 (Pl = (xi_max dt/dV) (((pi s) s) E) 1e-12) equals p^2 within 10 roundings,
-       * so Pl^2 d2 < thr^2 (1 - 2^-40) proves p < thr, thr = the 27-bit lower bound of the pair's
-       * uniform (or the injected uniform): the pair does not fire and is no multiple collection.  For
-       * Pl <= 1e-140 (and d2 < 1e100) p < 1e-89 < thr whenever thr^2 > 1e-178.  Everything else --
-       * thr = 0, overflow, NaN -- takes the exact path below.
-       */
-      unsigned maybe = 0;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int d = 4 * b + t;
-        const int jr = d - i;
-        const bool valid = (jr > i) & (jr < n) & (pair_cnt(i, jr, n) >= k_next);
-        /* clamped into [0, n-1] (jr is -1 at the start of some rows, >= n at the end): every lane runs the
-           same straight-line code on in-range LDS entries, and invalid pairs are masked out below */
-        const int j = jr < 0 ? 0 : (jr < n ? jr : n - 1);
-        const double xi_j = sX[j];
-        const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
-        const double du = u_i - sU[j], dw = w_i - sW[j];
-        const double d2 = du * du + dw * dw;
-        const double sR2 = R_i + sR[j];
-        double thr;
-        if (INJ)
-          thr = valid ? rnd_at(v, roff + pair_cnt(i, jr, n)) : 0.0;
-        else
-          thr = (double)(lm_word(blk, (uint32_t)t) >> 5) * (1.0 / 134217728.0);
-        const double thr2 = (thr * thr) * (1.0 - 0x1p-40);
-        const double geo = (xi_max * dtdV) * ((pi * sR2) * sR2);
-        /* first with the table's largest entry: every rounding step is monotone in E, so a pair rejected
-           with E_max is rejected with its own entry; only the others read the table */
-        bool no = false;
-        if (TAB && v.E_max > 0.0) {
-          const double Pm = (geo * v.E_max) * 1.0E-12;
-          no = (Pm > 1.0E-140) & ((Pm * Pm) * d2 < thr2);
-        }
-        if (!no) {
-          const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
-          const double Pl = (geo * Ec) * 1.0E-12;
-          const bool big = Pl > 1.0E-140;
-          no = (big & ((Pl * Pl) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 1.0E-178));
-        }
-        maybe |= (unsigned)(valid & !no) << t;
+      if (ni >= n - 1) {
+        finished = true;
+        break;
       }
-      if (maybe) {
-        /* rare: the reference's exact p_crit and decision, first firing pair of the quad */
-        for (int t = 0; t < 4; ++t) {
-          if (!((maybe >> t) & 1u)) continue;
-          const int d = 4 * b + t;
-          const int j = d - i;
-          const int cnt = pair_cnt(i, j, n);
-          const double Ec = TAB ? Erow[sIND[j]] : v.E_const;
-          const double xi_j = sX[j];
-          const double xi_max = xi_i <= xi_j ? xi_j : xi_i;
-          const double du = u_i - sU[j], dw = w_i - sW[j];
-          const double p_crit = xi_max * dtdV * lmp_kernel_hydro(P, R_i, sR[j], Ec, sqrt(du * du + dw * dw));
-          bool fires;
-          if (p_crit > 1.0)
-            fires = true;
-          else if (INJ)
-            fires = p_crit > rnd_at(v, roff + cnt);
-          else
-            fires = pair_fires(p_crit, lm_word(blk, (uint32_t)t), v.seed, key_run, key_step, key_cell, (uint32_t)i,
-                               (uint32_t)d);
-          if (fires) {
-            const unsigned long long key = ((unsigned long long)cnt << 32) | ((unsigned)i << 16) | (unsigned)j;
-            best = key < best ? key : best;
-            break;
+      i0 = ni;
+      q_first = sQ[ni] + ((ni + nj) >> 2) - ((2 * ni + 1) >> 2);
+    }
+    if (INJ) {
+      int i = i0; /* row of quad q: rows only grow with q, so each lane walks forward */
+      for (int q = q_first + lane; q < nQ; q += WAVE) {
+        while (sQ[i + 1] <= q) ++i;
+        const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
+        const unsigned long long kq = eval_quad<CAP, TAB, INJ>(P, v, cc, i, b, k_next, 0u, roff, key_run,
+                                                               key_step, key_cell);
+        best = kq < best ? kq : best;
+      }
+    } else {
+      /* Guard pass, row-parallel: lane l takes rows i0 + l, i0 + l + 64, ...; per quad the guard word and an
+         integer compare of its valid pairs' smallest leading byte with the row's bound sTG (no Philox, no
+         memory access), survivors appended to an LDS queue (any order: the pass takes the minimum).  The
+         queue is then evaluated by the whole wave; a queue overflow (cells whose bounds reject little) takes
+         the plain pass over all quads from q_first. */
+      if (lane == 0) sQn = 0;
+      __syncthreads();
+      /* rows i0 .. n-2 in pairs (i0 + p, n - 2 - p): every lane gets about the same number of quads */
+      const int n_rows = n - 1 - i0;
+      for (int pr = lane; 2 * pr < n_rows; pr += WAVE) {
+#pragma unroll 1
+        for (int side = 0; side < 2; ++side) {
+          const int i = side == 0 ? i0 + pr : n - 2 - pr;
+          if (side == 1 && i <= i0 + pr) break;
+          const int base = i * n - i * (i + 1) / 2;
+          const int jmin = (k_next - base + i + 1 > i + 1) ? k_next - base + i + 1 : i + 1;
+          if (jmin >= n) continue;
+          const uint32_t st = sTG[i]; /* keep a quad when a valid pair's byte is <= st */
+          const int b0 = (i + jmin) >> 2, b1 = (i + n - 1) >> 2;
+          /* four quads per iteration: their hash chains are independent and overlap in the pipeline */
+          for (int bb = b0; bb <= b1; bb += 4) {
+            uint32_t gq[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) gq[u] = lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * (bb + u)));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int b = bb + u;
+              if (b > b1) break;
+              uint32_t g = gq[u];
+              if ((b == b0) | (b == b1)) {
+                /* bytes of pairs outside [jmin, n-1] set to 0xFF, which no bound below 255 keeps; the quad
+                   holds at least one valid pair (j = jmin in the first, j = n-1 in the last) */
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  const int j = 4 * b + t - i;
+                  if ((j < jmin) | (j >= n)) g |= 0xFFu << (8 * t);
+                }
+              }
+              bool keep;
+              if (st < 127u) /* a byte below st + 1 (exact word test, valid for bounds up to 128) */
+                keep = ((g - 0x01010101u * (st + 1u)) & ~g & 0x80808080u) != 0u;
+              else
+                keep = true; /* the bound rejects (almost) nothing: every quad holds a valid pair */
+              if (keep) {
+                const int pos = atomicAdd(&sQn, 1);
+                if (pos < QCAP) sQueue[pos] = (i << 16) | b;
+              }
+            }
           }
         }
       }
+      __syncthreads();
+      const int nq = sQn;
+      if (nq <= QCAP) {
+        for (int e = lane; e < nq; e += WAVE) {
+          const int pk = sQueue[e], qi = pk >> 16, qb = pk & 0xffff;
+          const unsigned long long kq = eval_quad<CAP, TAB, INJ>(
+              P, v, cc, qi, qb, k_next, lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
+              key_step, key_cell);
+          best = kq < best ? kq : best;
+        }
+      } else {
+        int i = i0;
+        for (int q = q_first + lane; q < nQ; q += WAVE) {
+          while (sQ[i + 1] <= q) ++i;
+          const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
+          const unsigned long long kq = eval_quad<CAP, TAB, INJ>(
+              P, v, cc, i, b, k_next, lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * b)), roff, key_run, key_step,
+              key_cell);
+          best = kq < best ? kq : best;
+        }
+      }
+      __syncthreads();
     }
     for (int off = WAVE / 2; off > 0; off >>= 1) {
       const unsigned long long o = __shfl_xor(best, off);
@@ -1000,9 +1235,10 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     __syncthreads();
     ++n_events;
     k_next = ecnt + 1;
+    prev_event = best;
     if (n_events < SPEC_MAX_EVENTS) {
       /* the event may have grown a radius: fresh row bounds for the next pass */
-      row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB);
+      row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB, sTG);
       __syncthreads();
     }
   }
@@ -1020,11 +1256,12 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
         const int j = d - i;
         const int cnt = pair_cnt(i, j, n);
         if (cnt < k_next) continue;
-        uint32_t aw = 0;
+        uint32_t aw = 0, G = 0;
         double rnd_inj = 0.0;
         if (INJ) {
           rnd_inj = rnd_at(v, roff + cnt);
         } else {
+          G = lm_guard_byte(lm_guard_word(gkey, (uint32_t)i, (uint32_t)d), (uint32_t)d);
           const int tag = (i << 16) | (d >> 2);
           const bool cls = (i >> 6) & 1;
           if (tag != (cls ? tag1 : tag0)) {
@@ -1047,7 +1284,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
         else if (INJ)
           fires = p_crit > rnd_inj;
         else
-          fires = pair_fires(p_crit, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d);
+          fires = pair_fires(p_crit, G, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d);
         if (fires) {
           const int kind = apply_event(P, v, cc, rho_g, i, j, p_crit);
           if (kind == 1)
@@ -1308,10 +1545,15 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
       gc_new = r * nc + (int)ii * v.nz + (int)jj;
       if (gc_new != gc) { /* rare: the SIP changes cell */
         int home;
-        if (!near_home(v, r, k, (int)ii, (int)jj, &home))
+        if (!near_home(v, r, k, (int)ii, (int)jj, &home)) {
           raise_err(v.err, LM_ERR_LOST_SIP, (int)k);
-        else if (home != (int)ii * v.nz + (int)jj)
+        } else if (home != (int)ii * v.nz + (int)jj) {
           atomicOr(&v.away[(size_t)r * nc + home], dir_bit(home / v.nz, home % v.nz, (int)ii, (int)jj, v.nx));
+          if (v.mig_cnt) {
+            const int q = atomicAdd(&v.mig_cnt[gc_new], 1);
+            if (q < v.mig_cap) v.mig_slot[(size_t)gc_new * v.mig_cap + q] = (int)k;
+          }
+        }
       }
     }
   }
@@ -1435,8 +1677,106 @@ __device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
   __shared__ int sSlot[CAP], sIdRank[2 * CAP];
   __shared__ unsigned char sSub[CAP];
   __shared__ NbrTab tb;
+  __shared__ int sMig[3 * LM_MIG_CAP_MAX];
   int *sId = sIdRank, *sRank = sIdRank + CAP;
   double *sVal = (double *)sIdRank; /* reuses the id/rank storage once the ranks are in registers */
+  constexpr int PER = CAP / WAVE;
+  if (v.mig_cnt) {
+    /*
+     * Complete migrant list: the sum over the home segment in slot order (= id order), non-members adding
+     * +0.0 (exact: a sum that starts at +0.0 is never -0.0, and s + 0.0 = s), merged by id with the list's
+     * members (each slot once).  Each home slot's cell, value and id load in one round, the migrants' in
+     * the next.
+     */
+    const int mc = v.mig_cnt[gc];
+    const int32_t *seg = v.seg + (size_t)r * (nc + 1);
+    const int c = gc % nc, s0 = seg[c], T0 = seg[c + 1] - s0;
+    if (mc <= v.mig_cap && T0 + mc <= CAP) {
+      const int lane = threadIdx.x;
+      int *sMk = sMig, *sMid = sMig + LM_MIG_CAP_MAX;
+      double ov[PER];
+      int oid[PER];
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int t = lane + q * WAVE;
+        ov[q] = 0.0;
+        oid[q] = 0;
+        if (t < T0) {
+          const int k = s0 + t;
+          const double d = v.dmxi[k];
+          ov[q] = v.cell[k] == gc ? d : 0.0;
+          oid[q] = v.id[k];
+        }
+      }
+      int mk = -1, mid_ = 0;
+      double mval = 0.0;
+      bool mem = false;
+      if (lane < mc) {
+        mk = v.mig_slot[(size_t)gc * v.mig_cap + lane];
+        mem = v.cell[mk] == gc;
+        mval = v.dmxi[mk];
+        mid_ = v.id[mk];
+        sMk[lane] = mk;
+      }
+      __syncthreads();
+      if (mem)
+        for (int e = 0; e < lane; ++e)
+          if (sMk[e] == mk) mem = false;
+      const unsigned long long bal = __ballot(mem);
+      const int nm = __popcll(bal);
+      int opos[PER];
+      if (nm > 0) {
+        if (mem) sMid[__popcll(bal & lane_mask_lt())] = mid_;
+        __syncthreads();
+        /* home entry: its index plus the migrants with smaller ids; migrant: its rank among them plus the
+           home entries with smaller ids (ascending in the segment) */
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          int lo = 0; /* sMid is in list order: count all */
+          if (lane + q * WAVE < T0)
+            for (int e = 0; e < nm; ++e) lo += sMid[e] < oid[q];
+          opos[q] = lane + q * WAVE + lo;
+        }
+        int rk = 0;
+        if (mem)
+          for (int e = 0; e < nm; ++e) rk += sMid[e] < mid_;
+        __syncthreads();
+        /* home ids to LDS for the migrants' lower bounds */
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+          if (lane + q * WAVE < T0) sSlot[lane + q * WAVE] = oid[q];
+        __syncthreads();
+        if (mem) {
+          int lo = 0, hi = T0;
+          while (lo < hi) {
+            const int m2 = (lo + hi) >> 1;
+            if (sSlot[m2] < mid_)
+              lo = m2 + 1;
+            else
+              hi = m2;
+          }
+          rk += lo;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+          if (lane + q * WAVE < T0) sVal[opos[q]] = ov[q];
+        if (mem) sVal[rk] = mval;
+      } else {
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+          if (lane + q * WAVE < T0) sVal[lane + q * WAVE] = ov[q];
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const int tot = T0 + nm;
+        double sum = 0.0;
+        for (int p = 0; p < tot; ++p) sum += sVal[p];
+        v.dml[gc] = sum * 1.0E-18;
+      }
+      return;
+    }
+  }
   nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
   if (n > CAP) {
@@ -1453,7 +1793,6 @@ __device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
   __syncthreads();
   rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
-  constexpr int PER = CAP / WAVE;
   int rk[PER];
   double val[PER];
 #pragma unroll
@@ -2537,6 +2876,8 @@ static int check_engine(const LmEngine *e) {
   if (e->vel_run_stride != 0 && e->vel_run_stride != (int64_t)2 * (e->nx + 1) * (e->nz + 1))
     return set_err(LM_ERR_ARG, "vel_run_stride must be 0 or 2*(nx+1)*(nz+1)");
   if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
+  if (e->mig_cnt && (!e->mig_slot || e->mig_cap < 1 || e->mig_cap > LM_MIG_CAP_MAX))
+    return set_err(LM_ERR_ARG, "migrant lists need mig_slot and mig_cap in 1..LM_MIG_CAP_MAX (64)");
   if (e->rem_cap < 1 || e->rem_cap > LM_REM_CAP_MAX || !e->rem_slot || !e->rem_cnt)
     return set_err(LM_ERR_ARG, "rem_cap must be in 1..LM_REM_CAP_MAX (1024) with rem_slot and rem_cnt set");
   if (e->part_block < 0 || e->part_block > PART_BLOCK_MAX || (e->part_block % WAVE) || e->part_spt < 0 ||
@@ -2643,7 +2984,8 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, part_block), (int64_t)offsetof(LmEngine, E_max),
       (int64_t)offsetof(LmEngine, vel_run_stride), (int64_t)offsetof(LmEngine, col_work_per_cu),
       (int64_t)offsetof(LmEngine, work), (int64_t)offsetof(LmEngine, dml_work_chunk),
-      (int64_t)offsetof(LmEngine, part_wpe),
+      (int64_t)offsetof(LmEngine, part_wpe), (int64_t)offsetof(LmEngine, mig_cnt),
+      (int64_t)offsetof(LmEngine, mig_cap),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -2772,6 +3114,8 @@ int lm_reindex(LmEngine *e, void *stream) {
   }
   /* every SIP is in its home cell again (a fill kernel, not a memset: the step stays graph-capturable) */
   hipLaunchKernelGGL(k_fill_i32, dim3((R * nc + 255) / 256), dim3(256), 0, S(stream), e->away, 0, R * nc);
+  if (e->mig_cnt)
+    hipLaunchKernelGGL(k_fill_i32, dim3((R * nc + 255) / 256), dim3(256), 0, S(stream), e->mig_cnt, 0, R * nc);
   LAUNCH_CHECK();
   e->cur = 1 - e->cur;
   e->ccur = 1 - e->ccur;

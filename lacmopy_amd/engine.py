@@ -19,6 +19,7 @@ SOLUTES = {"AS": 0, "NaCl": 1}
 CC_N, CC_TP = 16, 15  # per-cell constant table: width, slot of the pre-update cell temperature (lm_physics.h)
 WORK_BINS = 1024  # LM_WORK_BINS: size bins of the work-list order (include/lacmo_amd.h)
 REM_CAP_MAX = 1024  # LM_REM_CAP_MAX: ground removals per run and substep (include/lacmo_amd.h)
+MIG_CAP = 32  # migrant-list entries per cell (LmEngine.mig_cap, at most LM_MIG_CAP_MAX = 64)
 
 
 def _ptr(t):
@@ -64,6 +65,8 @@ class Engine:
             mk(f"seg{b}", (nr * (nc + 1) + 1,), i32)
         mk("dmxi", (n,), f64)
         mk("away", (nr * nc,), i32)
+        mk("mig_cnt", (nr * nc,), i32)
+        mk("mig_slot", (nr * nc * MIG_CAP,), i32)
         mk("col_step_dev", (1,), i32)
         mk("scal", (nr, 10, nc), f64)
         mk("mat", (nr, 7, nc), f64)
@@ -109,9 +112,11 @@ class Engine:
                 getattr(e, f)[b] = self.buf[f"{f}{b}"].data_ptr()
         for f in ("dmxi", "away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
                   "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
-                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off", "work"):
+                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off", "work", "mig_cnt",
+                  "mig_slot"):
             setattr(e, f, self.buf[f].data_ptr())
         e.rem_cap = int(rem_cap)
+        e.mig_cap = MIG_CAP
         e.vel_run_stride = 2 * (self.nx + 1) * (self.nz + 1)
         e.e_const_mode = 1
         e.E_const = 0.5
@@ -238,6 +243,7 @@ class Engine:
         self.buf[f"cell{self.e.ccur}"].copy_(torch.from_numpy(hcell))
         self.buf[f"seg{cur}"].copy_(torch.from_numpy(seg))
         self.buf["away"].zero_()  # every SIP sits in its home cell
+        self.buf["mig_cnt"].zero_()
 
     def set_counters(self, run, no_cols=None, water_removed=None):
         if no_cols is not None:
@@ -456,8 +462,8 @@ class Engine:
 
     # ------------------------------------------------------- checkpoints
     _SNAP_SIP = ("x", "z", "u", "w", "m_w", "m_s", "xi", "id", "seg")
-    _SNAP_GRID = ("away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "no_cols", "water_removed",
-                  "n_equal_xi", "rem_cnt", "err")
+    _SNAP_GRID = ("away", "mig_cnt", "mig_slot", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "no_cols",
+                  "water_removed", "n_equal_xi", "rem_cnt", "err")
 
     def snapshot(self):
         """Device copy of the evolving state (current SIP buffer and cells, grid, counters)."""

@@ -53,8 +53,10 @@ def _window(n_runs=1, n_steps=6, K=2, seed=5):
 
 
 def _state(eng):
+    """The engine's evolving state; migrant-list entries are left out (appended by atomics in any order, the
+    readers sort them by id)."""
     s = eng.snapshot()
-    return {k: v.cpu().numpy() for k, v in s["buf"].items()}
+    return {k: v.cpu().numpy() for k, v in s["buf"].items() if k != "mig_slot"}
 
 
 def _close(adj, fd, J, x, h, rtol, what, seen=None):

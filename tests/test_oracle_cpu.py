@@ -45,6 +45,70 @@ def test_pair_uniforms_are_uniform_and_distinct():
     assert np.all(np.round(u * 2.0 ** 53) == u * 2.0 ** 53)
 
 
+def _cell_uniforms(seed, run, step, cell, n):
+    """Uniforms of all pairs of a cell of n SIPs as an (n, n) array (upper triangle)."""
+    U = np.full((n, n), np.nan)
+    for i in range(n):
+        for j in range(i + 1, n):
+            U[i, j] = L.or_pair_uniform(seed, run, step, cell, i, j)
+    return U
+
+
+def test_guard_bytes_are_uniform_and_independent():
+    """The leading byte of every pair's uniform comes from the guard word (MurmurHash3's fmix32 of a keyed
+    Weyl sequence over the blocks), the next 32 bits from Philox.  Over 12 cells of 128 SIPs (97,536 pairs):
+    chi-square tests of the byte's uniformity, of the joint distribution of the bytes of neighbouring
+    pairs in a block and across blocks (upper nibbles), of the byte against Philox's next byte, and of the
+    bytes of one pair in consecutive collision steps and neighbouring cells; the uniforms pass
+    Kolmogorov-Smirnov."""
+    from scipy.stats import chi2, kstest
+    n = 128
+    G, A8, Gstep, Gcell = [], [], [], []
+    pairs = [(i, j) for i in range(n) for j in range(i + 1, n)]
+    ii = np.array([p[0] for p in pairs])
+    jj = np.array([p[1] for p in pairs])
+    us = []
+    for c in range(12):
+        U = _cell_uniforms(2024, c % 3, 7 + c, 100 + c, n)
+        u = U[ii, jj]
+        us.append(u)
+        G.append(np.floor(u * 256).astype(int))
+        A8.append(np.floor(u * 65536).astype(int) % 256)
+        Gstep.append(np.floor(_cell_uniforms(2024, c % 3, 8 + c, 100 + c, n)[ii, jj] * 256).astype(int))
+        Gcell.append(np.floor(_cell_uniforms(2024, c % 3, 7 + c, 101 + c, n)[ii, jj] * 256).astype(int))
+    u = np.concatenate(us)
+    assert kstest(u, "uniform").pvalue > 1e-3
+    Gc = np.concatenate(G)
+
+    def uniform_p(x, k):
+        obs = np.bincount(x, minlength=k)
+        e = len(x) / k
+        return chi2.sf(((obs - e) ** 2 / e).sum(), k - 1)
+
+    def indep_p(a, b, k=16):
+        obs = np.zeros((k, k))
+        np.add.at(obs, (a, b), 1)
+        e = np.outer(obs.sum(1), obs.sum(0)) / obs.sum()
+        return chi2.sf(((obs - e) ** 2 / e).sum(), (k - 1) ** 2)
+
+    assert uniform_p(Gc, 256) > 1e-3
+    # pairs (i, j) and (i, j + 1): same block or adjacent blocks of the row
+    nxt = {(i, j): k for k, (i, j) in enumerate(pairs)}
+    a_idx = [k for k, (i, j) in enumerate(pairs) if (i, j + 1) in nxt]
+    b_idx = [nxt[(pairs[k][0], pairs[k][1] + 1)] for k in a_idx]
+    a = np.concatenate([g[a_idx] >> 4 for g in G])
+    b = np.concatenate([g[b_idx] >> 4 for g in G])
+    assert indep_p(a, b) > 1e-3
+    # the next row, same j
+    c_idx = [k for k, (i, j) in enumerate(pairs) if (i + 1, j) in nxt]
+    d_idx = [nxt[(pairs[k][0] + 1, pairs[k][1])] for k in c_idx]
+    assert indep_p(np.concatenate([g[c_idx] >> 4 for g in G]), np.concatenate([g[d_idx] >> 4 for g in G])) > 1e-3
+    # guard byte against the Philox byte that follows it; against the next step and the next cell
+    assert indep_p(Gc >> 4, np.concatenate(A8) >> 4) > 1e-3
+    assert indep_p(Gc >> 4, np.concatenate(Gstep) >> 4) > 1e-3
+    assert indep_p(Gc >> 4, np.concatenate(Gcell) >> 4) > 1e-3
+
+
 # ---------------------------------------------------------------- lm_math vs libm
 def test_lm_math_against_glibc_and_mpmath():
     import mpmath as mp
@@ -385,10 +449,12 @@ def test_stokes_drag_limit():
 
 def test_water_budget_conserved_without_relaxation():
     """Total water (vapour + liquid + removed) is conserved through full advection steps with
-    condensation, sedimentation, advection and collisions (relaxation off)."""
+    condensation, sedimentation, advection and single and multiple collections (relaxation off).  The drops
+    (50 um) stay in the regime where the multiple collection keeps multiplicities positive (xi_min K dt/dV
+    < 1): beyond it the reference's all_or_nothing.py:577-592 drives xi_max negative, as the oracle does."""
     E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
     grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=8, nz=8, spc=16, seed=4, cloud_base=0.0,
-                                                           cloud_top=60.0, R_drop=80.0, sigma_drop=1.4)
+                                                           cloud_top=60.0, R_drop=50.0, sigma_drop=1.4)
     N = xi.size
     cfg = dict(dt_adv=1.0, scale_dt_cond=5, dt_col=0.5, no_col_per_adv=2, act_collisions=True,
                act_relaxation=False, no_iter_impl_mass=3, solute_type="AS", g_set=9.80665, no_kernel_bins=nb,
@@ -409,7 +475,8 @@ def test_water_budget_conserved_without_relaxation():
     nco = np.zeros(2, dtype=np.int64)
     orc.integrate_adv_steps(p, 8, pos, vel, cells, rel, m_w, m_s, xi, act, np.ones(N), wr, nco, scal, mat,
                             grid.velocity, grid.mass_flux_air_dry, E, *prof)
-    assert wr[0] > 0 and nco.sum() > 0, "case should sediment and collide"
+    assert wr[0] > 0 and nco[0] > 0 and nco[1] > 0, "case should sediment and collide (single and multiple)"
+    assert np.all(xi > 0)
     assert abs(total() / W0 - 1) < 1e-12
 
 
