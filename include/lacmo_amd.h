@@ -453,6 +453,28 @@ int lm_box2d_step(int32_t n_boxes, const int32_t *off, double *xis, double *m_w,
 int lm_box_check(void *stream);
 
 /*
+ * One all-cells collision step on the reference's own device arrays, replacing
+ * collision_step_Ecol_grid_R_all_cells_2D_multicomp_np (collision/all_or_nothing.py:754-846): xis, m_w, m_s
+ * (n_sips, updated in place), vel (2, n_sips), grid_T (nx, nz), cells (2, n_sips) int64; the members of cell
+ * (i, j) are the SIPs with cells == (i, j) in ascending index (inactive SIPs, cells[1] = -1, are in no cell).
+ * E_col (nb x nb, the Ecol_grid_R table with lnR0 = R_kernel_low_log, lnf = bin_factor_R_log) or NULL for the
+ * constant efficiency E_const (Ecol_const); E_max the table's largest entry (0: unknown, the pair filter then
+ * evaluates every pair).  no_cols (2) int64 += (single, multiple) collections.  Draws keyed (seed, run 0, step,
+ * cell) as lm_collision_step's, or rnd_stream (rnd_len values, consumed in the reference's order).  Cells of
+ * up to 1024 members.  Stream-ordered, no host synchronisation; the library keeps a device workspace (grown
+ * on demand).  Errors of the kernels (a cell over capacity, a short stream) are reported by
+ * lm_collision_all_cells_check(0, stream).
+ */
+int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s, const double *vel,
+                           const double *grid_T, const int64_t *cells, int32_t nx, int32_t nz, double dt_over_dV,
+                           const double *E_col, int32_t nb, double lnR0, double lnf, double E_const, double E_max,
+                           int64_t *no_cols, int32_t solute, uint64_t seed, uint32_t step, const double *rnd_stream,
+                           int64_t rnd_len, void *stream);
+/* Error word of the last all-cells calls on this device (no_alloc 0: lm_collision_all_cells, 1: its
+   no-allocation form); synchronises the stream, clears the word. */
+int lm_collision_all_cells_check(int32_t no_alloc, void *stream);
+
+/*
  * Ensemble statistics of box-model snapshots, replacing the sums of analyze_sim_data
  * (collision/box_model.py:1354-1585).  The snapshots of all seeds are (n_times, n_tot) arrays of
  * multiplicities and masses (kg), seed s in the columns [off[s], off[s+1]).

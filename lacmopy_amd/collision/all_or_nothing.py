@@ -141,6 +141,39 @@ def collision_step_Ecol_const_2D_multicomp_np(xis, m_w, m_s, radii, vel, mass_de
 collision_step_Ecol_const_2D_multicomp = collision_step_Ecol_const_2D_multicomp_np
 
 
+_tables = {}
+
+
+def _device_table(torch, E):
+    """Device copy and largest entry of an E_col table, kept while the caller's array is alive and unchanged."""
+    if isinstance(E, torch.Tensor):
+        t = E.to(device="cuda", dtype=torch.float64).contiguous()
+        return t, float(t.max()) if bool(torch.isfinite(t).all()) and float(t.min()) >= 0.0 else 0.0
+    a = np.ascontiguousarray(E, dtype=np.float64)
+    key = (a.__array_interface__["data"][0], a.shape, hash(a[::max(1, a.shape[0] // 8)].tobytes()))
+    hit = _tables.get(key)
+    if hit is None or not np.array_equal(hit[2], a):
+        t = torch.from_numpy(a).cuda()
+        emax = float(a.max()) if np.isfinite(a).all() and a.min() >= 0.0 else 0.0
+        _tables.clear()
+        hit = _tables[key] = (t, emax, a.copy())
+    return hit[0], hit[1]
+
+
+def _as_device(torch, a, dtype, name, shape=None):
+    """A caller's CUDA tensor as it is (zero-copy; it is updated in place), or a device copy of an array."""
+    if isinstance(a, torch.Tensor):
+        if a.device.type != "cuda" or a.dtype != dtype or not a.is_contiguous():
+            raise TypeError(f"{name}: a CUDA tensor must be contiguous {dtype} (got {a.device}, {a.dtype})")
+        t = a
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a, dtype={torch.float64: np.float64, torch.int64: np.int64}[dtype]))
+        t = t.cuda()
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    return t
+
+
 def collision_step_Ecol_grid_R_all_cells_2D_multicomp_np(xis, m_w, m_s, vel, grid_temperature, cells, no_cells,
                                                          dt_over_dV, E_col_grid, no_kernel_bins, R_kernel_low_log,
                                                          bin_factor_R_log, no_cols, solute_type, *, rnd=None):
@@ -149,41 +182,42 @@ def collision_step_Ecol_grid_R_all_cells_2D_multicomp_np(xis, m_w, m_s, vel, gri
     Members of cell (i, j) are the SIPs with cells == (i, j) in ascending
     index; radii and densities from m_w, m_s at the cell temperature; a float
     E_col_grid selects the constant-efficiency kernel.  Updates xis, m_w, m_s
-    and no_cols in place.
+    and no_cols in place.  The arrays may be NumPy arrays (copied to the GPU
+    and back) or CUDA tensors of the reference's shapes and dtypes (float64;
+    cells and no_cols int64), which the step reads and updates where they are
+    (lm_collision_all_cells: no copy, no per-call engine).
     """
-    from ..engine import Engine
-    from ..integration import _set_kernel
+    torch = _torch()
+    L = lib()
     nx, nz = int(no_cells[0]), int(no_cells[1])
-    N = len(xis)
-    cells = np.asarray(cells, dtype=np.int64)
-    active = (cells[0] >= 0) & (cells[0] < nx) & (cells[1] >= 0) & (cells[1] < nz)
-    eng = Engine(nx, nz, [N], [[0.0, float(nx)], [0.0, float(nz)]], [1.0, 1.0], 1.0)
-    eng.set_params(solute=solute_type, act_collisions=True, act_relaxation=False, seed=rng.get_seed())
-    eng.e.dt_col_over_dV = float(dt_over_dV)
-    _set_kernel(eng, E_col_grid, R_kernel_low_log, bin_factor_R_log)
-    # only the temperature matters to the collision step; the other fields keep the per-cell table finite
-    scal = np.ones((10, nx, nz))
-    scal[0] = grid_temperature
-    scal[1] = 1.0E5
-    eng.set_grid(0, scal, np.ones((7, nx, nz)))
-    eng.init_cells()
-    zeros = np.zeros((2, N))
-    parked = np.where(active, cells, np.clip(cells, 0, [[nx - 1], [nz - 1]]))
-    eng.load_particles([(zeros, vel, parked, m_w, m_s, xis, active)])
-    eng.set_counters(0, no_cols)
+    N = int(xis.shape[0])
+    f64, i64 = torch.float64, torch.int64
+    x = _as_device(torch, xis, f64, "xis", (N,))
+    mw = _as_device(torch, m_w, f64, "m_w", (N,))
+    ms = _as_device(torch, m_s, f64, "m_s", (N,))
+    v = _as_device(torch, vel, f64, "vel", (2, N))
+    c = _as_device(torch, cells, i64, "cells", (2, N))
+    T = _as_device(torch, grid_temperature, f64, "grid_temperature", (nx, nz))
+    nc = _as_device(torch, no_cols, i64, "no_cols", (2,))
+    if isinstance(E_col_grid, float):
+        E, emax, E_const, nb, l0, lf = None, 0.0, float(E_col_grid), 0, 0.0, 1.0
+    else:
+        E, emax = _device_table(torch, E_col_grid)
+        E_const, nb, l0, lf = 0.0, int(no_kernel_bins), float(R_kernel_low_log), float(bin_factor_R_log)
+        if tuple(E.shape) != (nb, nb):
+            raise ValueError(f"E_col_grid has shape {tuple(E.shape)}, no_kernel_bins = {nb}")
+    r = None
     if rnd is not None:
-        counts = np.bincount(cells[0, active] * nz + cells[1, active], minlength=nx * nz)
-        need = int((counts * (counts - 1) // 2).sum())
-        if np.asarray(rnd).size < need:
-            raise ValueError(f"rnd holds {np.asarray(rnd).size} uniforms, the step needs {need}")
-        eng.set_rnd_stream(rnd)
-    eng.col_step = rng.take_steps(1)
-    eng.collision_step()
-    _, _, _, mw, ms, x, _ = eng.export_particles(0)
-    no_cols[...] = eng.counters(0)[0]
-    xis[...] = x
-    m_w[...] = mw
-    m_s[...] = ms
+        r = rnd if isinstance(rnd, torch.Tensor) and rnd.device.type == "cuda" else _dev(torch, np.asarray(rnd))
+        r = r.reshape(-1).to(f64).contiguous()
+    st = _stream(torch)
+    check(L.lm_collision_all_cells(N, _p(x), _p(mw), _p(ms), _p(v), _p(T), _p(c), nx, nz, float(dt_over_dV), _p(E), nb,
+                                   l0, lf, E_const, emax, _p(nc), 0 if solute_type == "AS" else 1, rng.get_seed(),
+                                   rng.take_steps(1), _p(r), int(r.numel()) if r is not None else 0, st))
+    check(L.lm_collision_all_cells_check(0, st))
+    for host, dev in ((xis, x), (m_w, mw), (m_s, ms), (no_cols, nc)):
+        if not isinstance(host, torch.Tensor):
+            host[...] = dev.cpu().numpy()
 
 
 collision_step_Ecol_grid_R_all_cells_2D_multicomp = collision_step_Ecol_grid_R_all_cells_2D_multicomp_np

@@ -360,6 +360,9 @@ struct KView {
   int32_t *scan_buf, *scan_part, *err, *ovf_list, *ovf_cnt;
   int32_t *mig_cnt, *mig_slot; /* migrant lists (NULL: off) */
   int mig_cap;
+  /* member lists of the reference-layout collision step (NULL: engine layout): the SIP indices of cell c
+     are mlist[moff[c] .. moff[c+1]), in any order */
+  const int32_t *mlist, *moff;
   const int32_t *work; /* (run, cell) work list of the persistent passes */
   int32_t *work_ctr;   /* its 2 counters: collision [0], Delta m_l [1] */
   const double *rnd_stream;
@@ -1022,8 +1025,28 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
 
   /* members: from the home segment and the migrant list (slots in id order straight into sSL), or, with
      an incomplete list, gathered from the 3x3 home segments and ranked by id */
-  int n = v.mig_cnt ? fast_members(v, gc, sSL, CAP, sSlot, sId, (int *)sW) : -1;
-  const bool fast = n >= 0;
+  int n;
+  bool fast;
+  if (v.mlist) {
+    /* the cell's member list, sorted by SIP index (the reference's mask order, all_or_nothing.py:812-829) */
+    const int o0 = v.moff[gc];
+    n = v.moff[gc + 1] - o0;
+    fast = true;
+    if (n <= CAP) {
+      for (int p = lane; p < n; p += WAVE) sSlot[p] = v.mlist[o0 + p];
+      __syncthreads();
+      for (int p = lane; p < n; p += WAVE) {
+        const int k = sSlot[p];
+        int rk = 0;
+        for (int q = 0; q < n; ++q) rk += sSlot[q] < k;
+        sSL[rk] = k;
+      }
+      __syncthreads();
+    }
+  } else {
+    n = v.mig_cnt ? fast_members(v, gc, sSL, CAP, sSlot, sId, (int *)sW) : -1;
+    fast = n >= 0;
+  }
   if (!fast) {
     nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
     n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
@@ -2336,6 +2359,75 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* all-cells collision step on the reference's arrays                       */
+/* ------------------------------------------------------------------------ */
+/* cell temperature and water density of the per-cell table (the slots the collision step reads) */
+__global__ void k_ref_cc(int nc, const double *grid_T, double *cc) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const double T = grid_T[c];
+  cc[(size_t)c * CC_N + CC_T] = T;
+  cc[(size_t)c * CC_N + CC_RHOW] = lmp_density_water(c_P, T);
+}
+
+/* cell of SIP i from cells (2, N) int64, -1 outside the grid (inactive SIPs carry cells[1] = -1) */
+__device__ static inline int ref_cell(const int64_t *cells, long long N, long long i, int nx, int nz) {
+  const long long a = cells[i], b = cells[N + i];
+  return (a >= 0 && a < nx && b >= 0 && b < nz) ? (int)(a * nz + b) : -1;
+}
+
+__global__ void k_ref_count(long long N, const int64_t *cells, int nx, int nz, int32_t *cnt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int c = ref_cell(cells, N, i, nx, nz);
+  if (c >= 0) atomicAdd(&cnt[c], 1);
+}
+
+/* in-place exclusive scan of a[0..L) by one workgroup, total into a[L] */
+__global__ __launch_bounds__(1024) void k_scan1_i32(int32_t *a, int L) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (L + 1023) / 1024;
+  const int b = t * per, e = b + per < L ? b + per : L;
+  int s = 0;
+  for (int k = b; k < e; ++k) s += a[k];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int q = 0; q < 1024; ++q) {
+      const int x = part[q];
+      part[q] = acc;
+      acc += x;
+    }
+    a[L] = acc;
+  }
+  __syncthreads();
+  int acc = part[t];
+  for (int k = b; k < e; ++k) {
+    const int x = a[k];
+    a[k] = acc;
+    acc += x;
+  }
+}
+
+__global__ void k_ref_scatter(long long N, const int64_t *cells, int nx, int nz, const int32_t *off, int32_t *fill,
+                              int32_t *list) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int c = ref_cell(cells, N, i, nx, nz);
+  if (c >= 0) list[off[c] + atomicAdd(&fill[c], 1)] = (int)i;
+}
+
+/* pairs per cell for the offsets of an injected stream (reference order: cells x-major) */
+__global__ void k_ref_count_pairs(int nc, const int32_t *off, long long *rnd_off) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const long long n = off[c + 1] - off[c];
+  rnd_off[c] = n * (n - 1) / 2;
+}
+
+/* ------------------------------------------------------------------------ */
 /* box-model ensemble statistics (collision/box_model.py:1354-1585)          */
 /* ------------------------------------------------------------------------ */
 /*
@@ -3528,6 +3620,200 @@ int lm_box_check(void *stream) {
     return set_err(h[0], msg);
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* all-cells collision step on the reference's arrays (all_or_nothing.py:754-846) */
+/* ------------------------------------------------------------------------ */
+/* device scratch of one call: member lists, pre-step densities, the per-cell table and counters */
+struct RefWork {
+  int32_t *cnt;      /* nc + 1: counts, then offsets (exclusive scan, total at nc) */
+  int32_t *fill;     /* nc: scatter cursors */
+  int32_t *list;     /* n: SIP indices grouped by cell */
+  double *rho;       /* n: pre-step density of each SIP (events only) */
+  double *cc;        /* nc * CC_N: cell temperature and water density */
+  int32_t *ovf_list; /* nc */
+  int32_t *small;    /* 8: ovf_cnt[2], err[4], step counter, pad */
+  long long *n_eq;   /* 1: near-equal collections */
+  long long *rnd_off; /* nc + 2: injected-stream offsets */
+};
+
+static size_t ref_bytes(long long n, long long nc) {
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return up(4 * (nc + 1)) + up(4 * nc) + up(4 * n) + up(8 * n) + up(8 * CC_N * nc) + up(4 * nc) + up(4 * 8) +
+         up(8) + up(8 * (nc + 2));
+}
+
+static RefWork ref_carve(char *base, long long n, long long nc) {
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  RefWork w;
+  size_t o = 0;
+  w.cnt = (int32_t *)(base + o);
+  o += up(4 * (nc + 1));
+  w.fill = (int32_t *)(base + o);
+  o += up(4 * nc);
+  w.list = (int32_t *)(base + o);
+  o += up(4 * n);
+  w.rho = (double *)(base + o);
+  o += up(8 * n);
+  w.cc = (double *)(base + o);
+  o += up(8 * CC_N * nc);
+  w.ovf_list = (int32_t *)(base + o);
+  o += up(4 * nc);
+  w.small = (int32_t *)(base + o);
+  o += up(4 * 8);
+  w.n_eq = (long long *)(base + o);
+  o += up(8);
+  w.rnd_off = (long long *)(base + o);
+  return w;
+}
+
+/* error words of the last reference-layout calls (device pointers), per device: allocating, no-alloc */
+static int32_t *g_ref_err[64][2];
+
+static int ref_args(long long n_sips, const double *xis, const double *m_w, const double *m_s, const double *vel,
+                    const double *grid_T, const int64_t *cells, int nx, int nz, const double *E, int nb,
+                    const int64_t *no_cols, int solute, const double *rnd_stream, long long rnd_len) {
+  if (n_sips < 0 || nx <= 0 || nz <= 0 || (n_sips > 0 && (!xis || !m_w || !m_s || !vel || !cells)) || !grid_T ||
+      !no_cols || solute < 0 || solute > 1 || (E && nb <= 0) || (rnd_stream && rnd_len < 0))
+    return set_err(LM_ERR_ARG, "bad all-cells collision arguments");
+  if ((long long)nx * nz > 0x3fffffff || n_sips > 0x7ffffff0) return set_err(LM_ERR_ARG, "grid or SIP count too large");
+  return 0;
+}
+
+/* the step on caller-owned arrays with the scratch w; no host synchronisation, no allocation */
+static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w, double *m_s, const double *vel,
+                    const double *grid_T, const int64_t *cells, int nx, int nz, double dt_over_dV, const double *E,
+                    int nb, double lnR0, double lnf, double E_const, double E_max, int64_t *no_cols, int solute,
+                    uint64_t seed, uint32_t step, const double *rnd_stream, long long rnd_len, hipStream_t st) {
+  const int nc = nx * nz;
+  const long long N = n_sips;
+  const unsigned gN = (unsigned)((N + 255) / 256), gC = (unsigned)((nc + 255) / 256);
+  /* what must start at zero: counts, cursors, overflow counters, the near-equal counter */
+  hipLaunchKernelGGL(k_fill_i32, dim3((nc + 1 + 255) / 256), dim3(256), 0, st, w.cnt, 0, nc + 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, st, w.fill, 0, nc);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, w.small, 0, 2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)w.n_eq, 0, 2);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, (uint32_t *)(w.small + 6), step);
+  hipLaunchKernelGGL(k_ref_cc, dim3(gC), dim3(256), 0, st, nc, grid_T, w.cc);
+  if (N > 0) hipLaunchKernelGGL(k_ref_count, dim3(gN), dim3(256), 0, st, N, cells, nx, nz, w.cnt);
+  hipLaunchKernelGGL(k_scan1_i32, dim3(1), dim3(1024), 0, st, w.cnt, nc);
+  if (N > 0) hipLaunchKernelGGL(k_ref_scatter, dim3(gN), dim3(256), 0, st, N, cells, nx, nz, w.cnt, w.fill, w.list);
+  KView v;
+  memset(&v, 0, sizeof(v));
+  v.n_runs = 1;
+  v.nx = nx;
+  v.nz = nz;
+  v.nc = nc;
+  v.solute = solute;
+  v.dt_col_over_dV = dt_over_dV;
+  v.E = E;
+  v.nb = nb;
+  v.e_const_mode = E ? 0 : 1;
+  v.E_const = E_const;
+  v.lnR0 = lnR0;
+  v.lnf = lnf;
+  v.E_max = E ? E_max : 0.0;
+  v.seed = seed;
+  v.col_step_dev = (const uint32_t *)(w.small + 6);
+  v.xi = xis;
+  v.m_w = m_w;
+  v.m_s = m_s;
+  v.u = (double *)vel;
+  v.w = (double *)vel + N;
+  v.cc = w.cc;
+  v.dmxi = w.rho;
+  v.no_cols = (long long *)no_cols;
+  v.n_equal_xi = w.n_eq;
+  v.err = w.small + 2;
+  v.ovf_list = w.ovf_list;
+  v.ovf_cnt = w.small;
+  v.mlist = w.list;
+  v.moff = w.cnt;
+  v.rnd_stream = rnd_stream;
+  v.rnd_len = rnd_len;
+  v.rnd_off = w.rnd_off;
+  if (rnd_stream) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)(w.rnd_off + nc), 0, 4);
+    hipLaunchKernelGGL(k_ref_count_pairs, dim3(gC), dim3(256), 0, st, nc, w.cnt, w.rnd_off);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.rnd_off, nc);
+  }
+  const bool tab = E != nullptr, inj = rnd_stream != nullptr;
+  /* main pass at the 256-member capacity, larger cells queued for the 1024-member pass */
+  if (tab && inj)
+    hipLaunchKernelGGL((k_collision<256, true, true>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
+  else if (tab)
+    hipLaunchKernelGGL((k_collision<256, true, false>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
+  else if (inj)
+    hipLaunchKernelGGL((k_collision<256, false, true>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
+  else
+    hipLaunchKernelGGL((k_collision<256, false, false>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
+  const dim3 go(nc < OVF_GRID ? nc : OVF_GRID);
+  if (tab && inj)
+    hipLaunchKernelGGL((k_collision<1024, true, true>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
+  else if (tab)
+    hipLaunchKernelGGL((k_collision<1024, true, false>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
+  else if (inj)
+    hipLaunchKernelGGL((k_collision<1024, false, true>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
+  else
+    hipLaunchKernelGGL((k_collision<1024, false, false>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+/* the allocating form's workspace, per device, grown on demand */
+static char *g_ref_ws[64];
+static size_t g_ref_ws_bytes[64];
+
+int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s, const double *vel,
+                           const double *grid_T, const int64_t *cells, int32_t nx, int32_t nz, double dt_over_dV,
+                           const double *E_col, int32_t nb, double lnR0, double lnf, double E_const, double E_max,
+                           int64_t *no_cols, int32_t solute, uint64_t seed, uint32_t step, const double *rnd_stream,
+                           int64_t rnd_len, void *stream) {
+  int rc = ref_args(n_sips, xis, m_w, m_s, vel, grid_T, cells, nx, nz, E_col, nb, no_cols, solute, rnd_stream,
+                    rnd_len);
+  if (rc) return rc;
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(LM_ERR_ARG, "device index out of range");
+  const size_t need = ref_bytes(n_sips, (long long)nx * nz);
+  if (g_ref_ws_bytes[dev] < need) {
+    /* a larger workspace: the previous one may still be in use by queued work on this stream */
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    if (g_ref_ws[dev]) HIP_TRY(hipFree(g_ref_ws[dev]));
+    g_ref_ws[dev] = nullptr;
+    g_ref_ws_bytes[dev] = 0;
+    HIP_TRY(hipMalloc((void **)&g_ref_ws[dev], need));
+    g_ref_ws_bytes[dev] = need;
+    HIP_TRY(hipMemset(g_ref_ws[dev], 0, need));
+  }
+  const RefWork w = ref_carve(g_ref_ws[dev], n_sips, (long long)nx * nz);
+  g_ref_err[dev][0] = w.small + 2;
+  return ref_step(w, n_sips, xis, m_w, m_s, vel, grid_T, cells, nx, nz, dt_over_dV, E_col, nb, lnR0, lnf, E_const,
+                  E_max, no_cols, solute, seed, step, rnd_stream, rnd_len, S(stream));
+}
+
+int lm_collision_all_cells_check(int32_t no_alloc, void *stream) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64 || no_alloc < 0 || no_alloc > 1) return set_err(LM_ERR_ARG, "bad check arguments");
+  int32_t *e = g_ref_err[dev][no_alloc];
+  if (!e) return 0;
+  int32_t h[4];
+  HIP_TRY(hipMemcpyAsync(h, e, sizeof(h), hipMemcpyDeviceToHost, S(stream)));
+  HIP_TRY(hipStreamSynchronize(S(stream)));
+  if (!h[0]) return 0;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e, 0, 4);
+  HIP_TRY(hipStreamSynchronize(S(stream)));
+  char msg[200];
+  if (h[0] == LM_ERR_CELL_CAP)
+    snprintf(msg, sizeof msg, "cell %d holds more than 1024 SIPs (the all-cells collision kernels' capacity)", h[1]);
+  else if (h[0] == LM_ERR_RND_SHORT)
+    snprintf(msg, sizeof msg, "the injected uniform stream is shorter than the step's pairs");
+  else
+    snprintf(msg, sizeof msg, "all-cells collision step error %d (detail %d)", h[0], h[1]);
+  return set_err(h[0], msg);
 }
 
 int lm_box_minmax(int32_t n_times, int64_t n_tot, const double *xis, const double *masses, double *out,

@@ -204,6 +204,47 @@ def test_collision_all_cells_np(gpu, kernel):
     assert o_nc.sum() > 0 and N > 0
 
 
+@pytest.mark.parametrize("variant", ["device_tensors", "injected_stream", "crowded_cell"])
+def test_collision_all_cells_on_device_tensors(gpu, variant):
+    """The all-cells drop-in on CUDA tensors of the reference's shapes updates them where they are (same
+    storage, no copies) and matches the oracle bit for bit; with an injected uniform stream (consumed in the
+    reference's order); with one cell of 700 SIPs (the 1024-member pass)."""
+    import torch
+    from lacmopy_amd.collision.all_or_nothing import collision_step_Ecol_grid_R_all_cells_2D_multicomp_np as step
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    grid, *parts = _case(nx=6, nz=12, spc=40, seed=19)
+    pos, vel, cells, m_w, m_s, xi, active = _prep(grid, parts)
+    cells = cells.copy()
+    if variant == "crowded_cell":
+        cells[:, :700] = np.array([[2], [5]])
+    cells[1, ::23] = -1
+    N = len(xi)
+    vel = vel.copy()
+    vel[1] -= np.random.default_rng(4).uniform(0.0, 3.0, N)
+    T = grid.temperature.copy()
+    rnd = None
+    if variant == "injected_stream":
+        act = cells[1] >= 0
+        cnt = np.bincount(cells[0, act] * 12 + cells[1, act], minlength=72)
+        rnd = np.random.default_rng(8).random(int((cnt * (cnt - 1) // 2).sum()))
+    o_xi, o_mw, o_ms, o_nc = xi.copy(), m_w.copy(), m_s.copy(), np.array([3, 1], dtype=np.int64)
+    orc.collision_all_cells(o_xi, o_mw, o_ms, vel, T, cells, grid.no_cells, 0.5 / grid.volume_cell, E, nb, lnR0, lnf,
+                            o_nc, rnd_stream=rnd, seed=7, run=0, step=4)
+    dev = {k: torch.from_numpy(np.ascontiguousarray(a)).cuda() for k, a in
+           (("xi", xi), ("m_w", m_w), ("m_s", m_s), ("vel", vel), ("cells", cells))}
+    nc = torch.tensor([3, 1], dtype=torch.int64, device="cuda")
+    ptrs = {k: t.data_ptr() for k, t in dev.items()}
+    rng.seed(7, 4)
+    step(dev["xi"], dev["m_w"], dev["m_s"], dev["vel"], T, dev["cells"], grid.no_cells, 0.5 / grid.volume_cell, E, nb,
+         lnR0, lnf, nc, "AS", rnd=rnd)
+    assert {k: t.data_ptr() for k, t in dev.items()} == ptrs
+    _assert_same(dev["xi"].cpu().numpy(), o_xi, "xi")
+    _assert_same(dev["m_w"].cpu().numpy(), o_mw, "m_w")
+    _assert_same(dev["m_s"].cpu().numpy(), o_ms, "m_s")
+    _assert_same(nc.cpu().numpy(), o_nc, "no_cols")
+    assert o_nc.sum() > 4
+
+
 def _cell_sips(n, seed):
     rs = np.random.default_rng(seed)
     xi = rs.uniform(1e4, 1e7, n)
