@@ -470,6 +470,21 @@ int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s
                            const double *E_col, int32_t nb, double lnR0, double lnf, double E_const, double E_max,
                            int64_t *no_cols, int32_t solute, uint64_t seed, uint32_t step, const double *rnd_stream,
                            int64_t rnd_len, void *stream);
+/*
+ * lm_collision_all_cells without any allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results; its scratch (member lists, densities, the per-cell table, counters) is one fixed
+ * device array of the library (about 17 MB per GPU, held until the process ends), so it takes at most
+ * LM_NOALLOC_MAX_SIPS SIPs on at most LM_NOALLOC_MAX_CELLS cells (larger calls fail with LM_ERR_ARG) and
+ * calls that overlap on one GPU race on it.  Kernel errors: lm_collision_all_cells_check(1, stream).
+ */
+#define LM_NOALLOC_MAX_SIPS (1 << 20)
+#define LM_NOALLOC_MAX_CELLS (1 << 15)
+int lm_collision_all_cells_noalloc(int64_t n_sips, double *xis, double *m_w, double *m_s, const double *vel,
+                                   const double *grid_T, const int64_t *cells, int32_t nx, int32_t nz,
+                                   double dt_over_dV, const double *E_col, int32_t nb, double lnR0, double lnf,
+                                   double E_const, double E_max, int64_t *no_cols, int32_t solute, uint64_t seed,
+                                   uint32_t step, const double *rnd_stream, int64_t rnd_len, void *stream);
 /* Error word of the last all-cells calls on this device (no_alloc 0: lm_collision_all_cells, 1: its
    no-allocation form); synchronises the stream, clears the word. */
 int lm_collision_all_cells_check(int32_t no_alloc, void *stream);

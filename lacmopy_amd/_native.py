@@ -107,6 +107,7 @@ EXPORTED = [
     "lm_derived_fields", "lm_ensemble_stats", "lm_build_slot_of", "lm_objective", "lm_adj_seed",
     "lm_adj_adv_steps", "lm_init_par_sizeof", "lm_init_level_profile", "lm_init_level_sips", "lm_init_positions",
     "lm_box_minmax", "lm_box_stats", "lm_mean_sem", "lm_collision_all_cells", "lm_collision_all_cells_check",
+    "lm_collision_all_cells_noalloc",
 ]
 
 _lib = None
@@ -173,6 +174,7 @@ def lib():
     L.lm_collision_all_cells.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _i32, _i32, _d, _p, _i32, _d, _d, _d, _d, _p,
                                          _i32, C.c_uint64, C.c_uint32, _p, _i64, _p]
     L.lm_collision_all_cells_check.argtypes = [_i32, _p]
+    L.lm_collision_all_cells_noalloc.argtypes = L.lm_collision_all_cells.argtypes
     L.lm_prof_enable.argtypes = [E, _i32]
     L.lm_prof_collect.argtypes = [E, C.POINTER(_d), C.POINTER(_i32), _i32]
     L.lm_prof_disable.argtypes = [E]
@@ -183,7 +185,7 @@ def lib():
                  "lm_graph_launch", "lm_graph_destroy", "lm_derived_fields", "lm_ensemble_stats",
                  "lm_build_slot_of", "lm_objective", "lm_adj_seed", "lm_adj_adv_steps", "lm_init_level_profile",
                  "lm_init_level_sips", "lm_init_positions", "lm_box_minmax", "lm_box_stats", "lm_mean_sem",
-                 "lm_collision_all_cells", "lm_collision_all_cells_check"]:
+                 "lm_collision_all_cells", "lm_collision_all_cells_check", "lm_collision_all_cells_noalloc"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -3638,10 +3638,10 @@ struct RefWork {
   long long *rnd_off; /* nc + 2: injected-stream offsets */
 };
 
-static size_t ref_bytes(long long n, long long nc) {
-  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return up(4 * (nc + 1)) + up(4 * nc) + up(4 * n) + up(8 * n) + up(8 * CC_N * nc) + up(4 * nc) + up(4 * 8) +
-         up(8) + up(8 * (nc + 2));
+static constexpr size_t ref_up(size_t b) { return (b + 255) & ~(size_t)255; }
+static constexpr size_t ref_bytes(long long n, long long nc) {
+  return ref_up(4 * (nc + 1)) + ref_up(4 * nc) + ref_up(4 * n) + ref_up(8 * n) + ref_up(8 * CC_N * nc) +
+         ref_up(4 * nc) + ref_up(4 * 8) + ref_up(8) + ref_up(8 * (nc + 2));
 }
 
 static RefWork ref_carve(char *base, long long n, long long nc) {
@@ -3689,10 +3689,11 @@ static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w
   const int nc = nx * nz;
   const long long N = n_sips;
   const unsigned gN = (unsigned)((N + 255) / 256), gC = (unsigned)((nc + 255) / 256);
-  /* what must start at zero: counts, cursors, overflow counters, the near-equal counter */
+  /* what must start at zero: counts, cursors, overflow counters, the error word (so the check reports this
+     call's errors), the near-equal counter */
   hipLaunchKernelGGL(k_fill_i32, dim3((nc + 1 + 255) / 256), dim3(256), 0, st, w.cnt, 0, nc + 1);
   hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, st, w.fill, 0, nc);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, w.small, 0, 2);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, w.small, 0, 6); /* overflow counters, error word */
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)w.n_eq, 0, 2);
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, (uint32_t *)(w.small + 6), step);
   hipLaunchKernelGGL(k_ref_cc, dim3(gC), dim3(256), 0, st, nc, grid_T, w.cc);
@@ -3790,6 +3791,52 @@ int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s
   }
   const RefWork w = ref_carve(g_ref_ws[dev], n_sips, (long long)nx * nz);
   g_ref_err[dev][0] = w.small + 2;
+  return ref_step(w, n_sips, xis, m_w, m_s, vel, grid_T, cells, nx, nz, dt_over_dV, E_col, nb, lnR0, lnf, E_const,
+                  E_max, no_cols, solute, seed, step, rnd_stream, rnd_len, S(stream));
+}
+
+/*
+ * The same step with its scratch in `scratch`, a device array of fixed size that lives as long as the process
+ * (one copy per GPU): for callers that cannot allocate (graph capture, serving loops).  Host code never
+ * passes the array's name to a kernel: it is a host-side stand-in, and its device address (per GPU) comes
+ * from hipGetSymbolAddress on the first call with that GPU current.  Calls that overlap on one GPU share
+ * the array and race.
+ */
+static __device__ __align__(256) char scratch[ref_bytes(LM_NOALLOC_MAX_SIPS, LM_NOALLOC_MAX_CELLS)];
+static char *g_scratch_ptr[64];
+
+int lm_collision_all_cells_noalloc(int64_t n_sips, double *xis, double *m_w, double *m_s, const double *vel,
+                                   const double *grid_T, const int64_t *cells, int32_t nx, int32_t nz,
+                                   double dt_over_dV, const double *E_col, int32_t nb, double lnR0, double lnf,
+                                   double E_const, double E_max, int64_t *no_cols, int32_t solute, uint64_t seed,
+                                   uint32_t step, const double *rnd_stream, int64_t rnd_len, void *stream) {
+  int rc = ref_args(n_sips, xis, m_w, m_s, vel, grid_T, cells, nx, nz, E_col, nb, no_cols, solute, rnd_stream,
+                    rnd_len);
+  if (rc) return rc;
+  if (n_sips > LM_NOALLOC_MAX_SIPS || (long long)nx * nz > LM_NOALLOC_MAX_CELLS) {
+    char msg[200];
+    snprintf(msg, sizeof msg,
+             "lm_collision_all_cells_noalloc: %lld SIPs on %lld cells exceed its fixed scratch (at most %d SIPs and "
+             "%d cells); use lm_collision_all_cells",
+             (long long)n_sips, (long long)nx * nz, LM_NOALLOC_MAX_SIPS, LM_NOALLOC_MAX_CELLS);
+    return set_err(LM_ERR_ARG, msg);
+  }
+  if (ensure_phys()) return set_err(LM_ERR_ARG, "cannot upload constants to the device");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_err(LM_ERR_ARG, "device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      char msg[160];
+      snprintf(msg, sizeof msg, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+      return set_err(e != hipSuccess ? (int)e : LM_ERR_ARG, msg);
+    }
+    g_scratch_ptr[dev] = (char *)ptr;
+  }
+  const RefWork w = ref_carve(g_scratch_ptr[dev], n_sips, (long long)nx * nz);
+  g_ref_err[dev][1] = w.small + 2;
   return ref_step(w, n_sips, xis, m_w, m_s, vel, grid_T, cells, nx, nz, dt_over_dV, E_col, nb, lnR0, lnf, E_const,
                   E_max, no_cols, solute, seed, step, rnd_stream, rnd_len, S(stream));
 }
