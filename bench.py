@@ -6,20 +6,30 @@ kinematic stratocumulus, 75x75 cells of 20 m, Poisson(128) SIPs per cell, bimoda
 Long E_col table, dt_adv = 1 s, 10 condensation substeps, 2 collision steps per advection step,
 relaxation on.  --case stress150: 150x150 cells of 10 m, Poisson(256), Hall E_col, dt 0.5 s,
 20 substeps, 16 seeds.
-  * batch: 64 (16) independent seeds in one engine (the headline `value`)
-  * plain: one run (reported next to it; ratio = batch / plain)
-A "step" is one advection step of every SIP of every run.
-SIP-steps/s = sum over runs of SIPs x timed steps / wall seconds, with all inputs resident in HBM;
-the timed region is bracketed by torch.cuda.synchronize().
 
-Also measured: the dominant kernel's duration with HIP events (roofline vs HBM 8 TB/s, algorithmic
-bytes per SIP), and the CPU oracle on a bounded sample of the plain 75x75 case (cpu_baseline).
+A "step" is one advection step of every SIP of every run.  SIP-steps/s = sum over runs of SIPs x
+timed steps / wall seconds, with all inputs resident in HBM; every timed region is bracketed by
+torch.cuda.synchronize().  Legs (--legs, default all):
+  * early:    the 64-seed batch a few steps after the synthetic start (eager and HIP-graph replays)
+  * plain:    one run, same state, eager and graph (batch / plain ratio)
+  * drizzle:  the collision-active state: one run spun up 7200 s without collisions, replicated to the
+              64 runs, then 900 s with collisions (every run its own draws) before the timed steps;
+              eager and graph, kernel times, and the plain run in the same state.  The headline `value`
+              is this leg's graph-replay rate (the 64-seed batch in its stated regime, hipGraph step)
+  * adjoint:  cost of the adjoint window after the spin-up and its finite-difference check
+  * box:      the Golovin box model, 50 seeds x 3600 s (the oracle's time for one box beside it)
+  * stress:   the 150x150 Poisson(256) 16-seed batch
+  * init:     GPU initial states of 64 seeds
+  * cpu:      the oracle (C restatement) on the host cores: 5 warm-up, 3 x 60 timed advection steps
+              (median of 3) per process, one process per core of the box's share and one alone
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -30,6 +40,8 @@ HBM_PEAK_GBS = 8000.0
 B_SUBSTEP = 104.0    # algorithmic bytes per SIP per particle substep (60 read + 44 write)
 B_COLLISION = 64.0   # per SIP per collision step (40 read + 24 write)
 B_REINDEX = 140.0    # per SIP per reindex
+HOST_THREADS = 16    # the box's CPU share for one GPU
+LEGS = ("early", "plain", "drizzle", "adjoint", "box", "stress", "init", "cpu")
 
 from lacmopy_amd.setup_case import WORKLOADS as CASES  # noqa: E402
 
@@ -56,44 +68,53 @@ def profile_value(path, tag, kernel):
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on the 64-seed workload, from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy); None if absent."""
-    try:
-        with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                parts = line.split()
-                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
-                    return float(parts[2])
-    except OSError:
-        pass
-    return None
+    return profile_value(TRAFFIC_PROFILE, "TRAFFIC", kernel)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
-    """kernel: 'Long_Bott' / 'Hall_Bott' table, or 'const' (Hydro_E_const, E = 0.5); default: the case's."""
-    import torch  # noqa: F401
+def _kernel_data(kernel):
     from lacmopy_amd import load_kernel_data_Ecol
-    from lacmopy_amd.engine import Engine
-    from lacmopy_amd.setup_case import init_profiles, workload_case
-    c = CASES[case]
-    kernel = kernel or c["kernel"]
     if kernel == "const":
-        E, lnR0, lnf = 0.5, None, None
-    else:
-        E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
-    cases = []
-    for r in range(n_runs):
+        return 0.5, None, None
+    E, lnR0, lnf, _ = load_kernel_data_Ecol(kernel)
+    return E, lnR0, lnf
+
+
+def build_cases(n_runs, spc, seed0, case):
+    """Host inputs of n_runs seeds (seed0 + 2 r) of `case`, built on the host threads."""
+    from lacmopy_amd.setup_case import workload_case
+
+    def one(r):
         grid, pos, vel, cells, m_w, m_s, xi, act, draws = workload_case(case, seed0 + 2 * r, spc=spc,
                                                                         return_draws=True)
-        cases.append((grid, (pos, vel, cells, m_w, m_s, xi, act), draws))
-    grid0 = cases[0][0]
-    eng = Engine(c["nx"], c["nz"], [len(k[1][5]) for k in cases], grid0.ranges, grid0.steps, grid0.volume_cell)
+        return grid, (pos, vel, cells, m_w, m_s, xi, act), draws
+
+    with ThreadPoolExecutor(min(HOST_THREADS, max(n_runs, 1))) as ex:
+        return list(ex.map(one, range(n_runs)))
+
+
+def new_engine(case, kernel, seed0, n_per_run, grid0):
+    from lacmopy_amd.engine import Engine
+    c = CASES[case]
+    E, lnR0, lnf = _kernel_data(kernel)
+    eng = Engine(c["nx"], c["nz"], n_per_run, grid0.ranges, grid0.steps, grid0.volume_cell)
     eng.set_params(solute="AS", no_iter_impl_mass=3, scale_dt_cond=c["sdc"], no_col_per_adv=2, act_collisions=True,
                    act_relaxation=True, dt=c["dt"], dt_col=c["dt"] / 2, g_set=9.80665, seed=seed0)
     eng.set_collision_kernel(E, lnR0, lnf)
     eng.set_flow(grid0.velocity, grid0.mass_flux_air_dry)
+    return eng
+
+
+def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
+    """kernel: 'Long_Bott' / 'Hall_Bott' table, or 'const' (Hydro_E_const, E = 0.5); default: the case's."""
+    import torch  # noqa: F401
+    from lacmopy_amd.setup_case import init_profiles
+    kernel = kernel or CASES[case]["kernel"]
+    cases = build_cases(n_runs, spc, seed0, case)
+    eng = new_engine(case, kernel, seed0, [len(k[1][5]) for k in cases], cases[0][0])
     for r, (g, _, _) in enumerate(cases):
         eng.set_relaxation(r, *init_profiles(g))
         eng.set_grid(r, g.scalar_fields(), g.mat_prop())
@@ -101,6 +122,36 @@ def build_engine(n_runs, spc, seed0, case="icmw75", kernel=None):
     eng.load_particles([k[1] for k in cases])
     n_sips = sum(len(k[1][5]) for k in cases)
     return eng, n_sips, cases
+
+
+def replicate_engine(src, grid0, n_runs, case, kernel, seed0):
+    """An n_runs engine whose runs all start from run 0 of `src` (particles, grid fields, step counter);
+    the collision draws are keyed per run, so the runs evolve independently from there."""
+    from lacmopy_amd.setup_case import init_profiles
+    parts = src.export_particles(0)
+    scal, mat = src.export_grid(0)
+    eng = new_engine(case, kernel, seed0, [len(parts[5])] * n_runs, grid0)
+    prof = init_profiles(grid0)
+    for r in range(n_runs):
+        eng.set_relaxation(r, *prof)
+        eng.set_grid(r, scal, mat)
+    eng.init_cells()
+    eng.load_particles([parts] * n_runs)
+    eng.col_step = src.col_step
+    return eng
+
+
+def tune(eng):
+    t = time.perf_counter()
+    tr = eng.autotune()
+    out = {"chosen": tr["chosen"],
+           "particle_ms": {f"{b}x{sp}/w{wp}": round(ms, 4) for (b, sp, wp), ms in tr["particles"].items()},
+           "dml_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["dml"].items()},
+           "collision_ms": {str(c): round(ms, 4) for c, ms in tr["collision"].items()},
+           "collision_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["collision_work"].items()},
+           "seconds": time.perf_counter() - t}
+    log(f"autotune: {out}")
+    return out
 
 
 def time_steps(eng, steps, warmup, graph=False):
@@ -121,6 +172,16 @@ def time_steps(eng, steps, warmup, graph=False):
     return t1 - t0
 
 
+def rates(eng, n_sips, steps, warmup):
+    """Eager and graph-replay SIP-steps/s over `steps` timed steps each."""
+    s_e = time_steps(eng, steps, warmup, graph=False)
+    s_g = time_steps(eng, steps, warmup, graph=True)
+    eng.drop_graph()
+    return {"eager_sip_steps_per_s": n_sips * steps / s_e, "eager_ms_per_step": s_e / steps * 1e3,
+            "graph_sip_steps_per_s": n_sips * steps / s_g, "graph_ms_per_step": s_g / steps * 1e3,
+            "steps": steps, "warmup": warmup}
+
+
 def kernel_times(eng, steps):
     """HIP-event durations of the step's kernels over `steps` profiled advection steps.
 
@@ -137,34 +198,46 @@ def kernel_times(eng, steps):
     return {k: (ms / max(n, 1), n) for k, (ms, n) in res.items()}
 
 
-# adjoint window per case: (window steps, checkpoint interval, spin-up steps without collisions, event capacity).
-# icmw75: 600 s after a 2 h spin-up, checkpoints every 60 steps; stress150: the last 300 s (600 steps of 0.5 s)
-# of 1800 s, checkpoints every 10 steps (the tape of 10 steps of 5.8M SIPs x 20 substeps is 74 GB).
-ADJ_DEFAULTS = {"icmw75": (600, 60, 7200, 1 << 20), "stress150": (600, 10, 3000, 1 << 23)}
+def collision_events(eng):
+    return eng.buf["no_cols"].sum(0).cpu().tolist()
 
 
-def adjoint_cost(seed0, spc, window, every, spinup, check=0, case="icmw75"):
-    """Adjoint of a `window`-step window of one run of `case` after `spinup` steps without collisions:
-    wall time of the plain forward window, of the recording forward (events + checkpoints every `every`
-    steps) and of one gradient (taped re-runs + reverse sweep), for the cloud-droplet-number objective,
-    and the gradient with respect to the six aerosol-mode parameters through the SIP initialisation map
-    (the window-start multiplicities and dry masses are still the map's: no collision before the window).
-    check > 0: also compare the gradient with central finite differences on `check` random droplets
-    (multiplicity, h = 1e-3 relative) and on the six aerosol-mode parameters (fourth order, h = 1e-3)."""
+# adjoint window per case: (window steps, checkpoint interval, spin-up steps, collisions in the spin-up,
+# event capacity).  icmw75: 600 s after a 2 h spin-up without collisions, checkpoints every 60 steps;
+# stress150: the last 300 s (600 steps of 0.5 s) of 1800 s with collisions, checkpoints every 10 steps (the
+# tape of 10 steps of 5.8M SIPs x 20 substeps is 74 GB).
+ADJ_DEFAULTS = {"icmw75": (600, 60, 7200, False, 1 << 20), "stress150": (600, 10, 3000, True, 1 << 23)}
+
+
+def spin_up(eng, steps, collisions):
+    """`steps` advection steps of the engine, with or without collisions (untimed preparation)."""
+    import torch
+    e = eng.e
+    e.act_collisions = int(collisions)
+    t = time.perf_counter()
+    eng.adv_steps(steps)
+    torch.cuda.synchronize()
+    e.act_collisions = 1
+    eng.check()
+    log(f"spin-up: {steps} steps ({'with' if collisions else 'without'} collisions) in "
+        f"{time.perf_counter() - t:.1f} s")
+
+
+def adjoint_cost(eng, cases, start, window, every, check=0, case="icmw75", spinup=0, spinup_collisions=False):
+    """Adjoint of a `window`-step window of one run of `case` from the state `start` (a snapshot after
+    `spinup` steps): wall time of the plain forward window, of the recording forward (events + checkpoints
+    every `every` steps) and of one gradient (taped re-runs + reverse sweep), for the cloud-droplet-number
+    objective.  Without collisions before the window the window-start multiplicities and dry masses are
+    still the SIP initialisation map's, and the gradient with respect to the six aerosol-mode parameters
+    follows by the chain rule through that map.  check > 0: also compare the gradient with central finite
+    differences on `check` random droplets (multiplicity, h = 1e-3 relative) and on the six aerosol-mode
+    parameters (fourth order, h = 1e-3)."""
     import torch
     from lacmopy_amd.adjoint import AdjointWindow
     from lacmopy_amd.setup_case import aerosol_sips
-    eng, n1, cases = build_engine(1, spc, seed0, case=case)
     c = CASES[case]
-    e = eng.e
-    e.act_collisions = 0
-    t = time.perf_counter()
-    eng.adv_steps(spinup)
-    torch.cuda.synchronize()
-    log(f"adjoint: spin-up {spinup} steps in {time.perf_counter() - t:.1f} s")
-    e.act_collisions = 1
-    eng.check()
-    start = eng.snapshot()
+    n1 = eng.n_tot
+    eng.restore(start)
     eng.adv_steps(2)  # warm-up of the collision path
     eng.restore(start)
     torch.cuda.synchronize()
@@ -174,7 +247,7 @@ def adjoint_cost(seed0, spc, window, every, spinup, check=0, case="icmw75"):
     t_fwd = time.perf_counter() - t0
     eng.check()
     eng.restore(start)
-    w = AdjointWindow(eng, window, checkpoint_every=every, ev_cap=ADJ_DEFAULTS[case][3])
+    w = AdjointWindow(eng, window, checkpoint_every=every, ev_cap=ADJ_DEFAULTS[case][4])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J = w.run(record=True)
@@ -186,31 +259,38 @@ def adjoint_cost(seed0, spc, window, every, spinup, check=0, case="icmw75"):
     lam_xi, lam_ms = w.gradient(w_nc=1.0, w_lwp=0.0)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
-    # aerosol-mode parameters (N, r_med, sigma of both modes) -> window-start xi, m_s -> J_nc, by the chain rule
-    eng.restore(w.start)
-    slots = w.slot_map()
-    cur = int(eng.e.cur)
-    theta = torch.tensor([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6], dtype=torch.float64, device="cuda",
-                         requires_grad=True)
-    xi_m, ms_m = aerosol_sips(theta, cases[0][2])
-    # the map restated in torch rounds differently from make_case's numpy in the last bits
-    map_dev = max(float(((xi_m.detach() - eng.buf[f"xi{cur}"][slots]).abs() / xi_m.detach().abs()).max()),
-                  float(((ms_m.detach() - eng.buf[f"m_s{cur}"][slots]).abs() / ms_m.detach().abs()).max()))
-    (g_theta,) = torch.autograd.grad((xi_m * lam_xi).sum() + (ms_m * lam_ms).sum(), theta)
     res = {"workload": f"one {case} run ({c['nx']}x{c['nz']} cells, {n1} SIPs), {window}-step window (dt "
                        f"{c['dt']:g} s, {2 * c['sdc']} substeps, 2 collision steps) after a {spinup}-step spin-up "
-                       f"without collisions; fixed collision events; checkpoints every {every} steps",
+                       f"{'with' if spinup_collisions else 'without'} collisions; fixed collision events; "
+                       f"checkpoints every {every} steps",
            "sips": n1, "window_steps": window, "checkpoint_every": every, "events": w.n_events,
            "J_nc": float(J[0][0]), "J_lwp": float(J[1][0]),
            "forward_s": t_fwd, "record_s": t_rec, "gradient_s": t_grad,
            "cost_ratio": (t_rec + t_grad) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
-           "dJnc_dmodes": g_theta.tolist(), "mode_map_rel_dev_from_window_start": map_dev,
            "tape_gb": sum(t.numel() * t.element_size() for t in w.tape.values()) / 1e9}
+    if not spinup_collisions:
+        # aerosol-mode parameters (N, r_med, sigma of both modes) -> window-start xi, m_s -> J_nc
+        eng.restore(w.start)
+        slots = w.slot_map()
+        cur = int(eng.e.cur)
+        theta = torch.tensor([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6], dtype=torch.float64, device="cuda",
+                             requires_grad=True)
+        xi_m, ms_m = aerosol_sips(theta, cases[0][2])
+        # the map restated in torch rounds differently from make_case's numpy in the last bits
+        res["mode_map_rel_dev_from_window_start"] = max(
+            float(((xi_m.detach() - eng.buf[f"xi{cur}"][slots]).abs() / xi_m.detach().abs()).max()),
+            float(((ms_m.detach() - eng.buf[f"m_s{cur}"][slots]).abs() / ms_m.detach().abs()).max()))
+        (g_theta,) = torch.autograd.grad((xi_m * lam_xi).sum() + (ms_m * lam_ms).sum(), theta)
+        res["dJnc_dmodes"] = g_theta.tolist()
+    else:
+        res["dJnc_dmodes"] = None
+        res["dJnc_dmodes_note"] = ("collisions before the window change multiplicities and dry masses, so the "
+                                   "window-start state is not the initialisation map's; per-SIP gradients only")
     log(f"adjoint: {res}")
     if check:
-        res["fd_check"] = adjoint_fd_check(w, eng, cases, lam_xi, check)
+        res["fd_check"] = adjoint_fd_check(w, eng, cases if not spinup_collisions else None, lam_xi, check)
         log(f"adjoint fd check: {res['fd_check']}")
-    del w, eng
+    del w
     torch.cuda.empty_cache()
     return res
 
@@ -238,8 +318,9 @@ def adjoint_fd_check(w, eng, cases, lam_xi, n_check):
         a = float(lam_xi[gid])
         out["sips"].append({"gid": int(gid), "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300),
                             "floor_rel": 256 * eps * abs(J0) / (2 * h * x) / max(abs(fd), 1e-300)})
+    out["max_rel_sips"] = max(s["rel"] for s in out["sips"])
     # mode parameters through the initialisation map (fourth-order quotient, h = 1e-3)
-    draws = cases[0][2] if len(cases[0]) > 2 else None
+    draws = cases[0][2] if cases is not None else None
     if draws is not None:
         theta0 = np.array([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6])
         theta = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
@@ -257,37 +338,85 @@ def adjoint_fd_check(w, eng, cases, lam_xi, n_check):
             fd = (8.0 * (J_at(h) - J_at(-h)) - (J_at(2 * h) - J_at(-2 * h))) / (12 * h * theta0[k])
             a = float(grad[k])
             out["modes"].append({"k": k, "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300)})
+        out["max_rel_modes"] = max(m["rel"] for m in out["modes"])
+    return out
+
+
+def box_model_leg(seed0, n_seeds=50, t_end=3600.0, dt_save=150.0, b=1.5, LWC0=1.0E-3):
+    """The Golovin box model of the reference (run_box_model.py:133-153, 306-328): n_seeds boxes of the
+    expo SinSIP ensemble (kappa 40), dt 1 s, batched on the GPU (one launch per step for all boxes; wall
+    time includes the snapshot copies), the ensemble moments against Golovin's closed forms, and the
+    oracle's time for the first box on one host core."""
+    import math
+    import torch
+    from lacmopy_amd.collision.box_model import golovin_expo_ensemble, simulate_collisions_batched
+    seeds = [seed0 + 2 * k for k in range(n_seeds)]
+    boxes, DNC0 = [], None
+    for s in seeds:
+        xi, m, DNC0 = golovin_expo_ensemble(s)
+        boxes.append((xi, m))
+    host = [(x.copy(), m.copy()) for x, m in boxes[:1]]
+    n_sips = sum(len(x) for x, _ in boxes)
+    simulate_collisions_batched([(x.copy(), m.copy()) for x, m in boxes], None, "Golovin", "analytic", 1.0, 1.0,
+                                10.0, 10.0, seed0)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    xs, ms, times, nc = simulate_collisions_batched(boxes, None, "Golovin", "analytic", 1.0, 1.0, t_end, dt_save,
+                                                    seed0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    mom = np.zeros((len(times), 3, n_seeds))
+    for k in range(n_seeds):
+        mk = 1.0E-18 * ms[k]
+        for n in range(3):
+            mom[:, n, k] = (xs[k] * mk ** n).sum(axis=1)
+    avg = mom.mean(axis=2)
+    an = np.array([[DNC0 * math.exp(-b * LWC0 * t), LWC0, 2 * LWC0 ** 2 / DNC0 * math.exp(2 * b * LWC0 * t)]
+                   for t in times])
+    rel = avg / an - 1.0
+    out = {"workload": f"Golovin box model, {n_seeds} seeds (expo SinSIP, kappa 40), dt 1 s, {int(t_end)} s",
+           "sips": n_sips, "steps": int(t_end), "wall_s": wall, "sip_steps_per_s": n_sips * t_end / wall,
+           "collision_events": nc.sum(0).tolist(),
+           "lambda0_max_rel_dev": float(np.max(np.abs(rel[:, 0]))),
+           "lambda2_rel_dev_600s": float(rel[times == 600.0, 2][0]),
+           "lambda2_rel_dev_end": float(rel[-1, 2])}
+    try:
+        from oracle import oracle as orc
+        x, m = host[0]
+        no = np.zeros(2, dtype=np.int64)
+        t = time.perf_counter()
+        for step in range(int(t_end)):
+            orc.golovin_step(x, m, 1.0, no, seed=seed0, run=0, step=step)
+        cpu_s = time.perf_counter() - t
+        out["cpu_oracle_s_per_box"] = cpu_s
+        out["gpu_over_cpu_1_core"] = cpu_s * n_seeds / wall
+        # the oracle's first box against the GPU's (the same keyed draws)
+        out["box0_bit_exact"] = bool(np.array_equal(x, boxes[0][0]) and np.array_equal(m, boxes[0][1]))
+    except (ImportError, OSError) as ex:
+        out["cpu_oracle_s_per_box"] = None
+        out["cpu_note"] = f"oracle not loaded: {ex}"
+    log(f"box model: {out}")
     return out
 
 
 def _cpu_run(args):
-    """One CPU worker of the all-cores baseline: its own seed of the plain case, SIP-steps/s of its steps."""
-    seed0, spc, max_steps, max_s = args
-    return cpu_baseline(seed0, spc, max_steps=max_steps, max_s=max_s)["value"]
+    """One process of the CPU baseline: the protocol on its own seed of the reduced case."""
+    seed0, spc, nx, warm, timed, reps = args
+    return cpu_protocol(seed0, spc, nx, warm, timed, reps)
 
 
-def cpu_baseline_all_cores(seed0, spc, workers):
-    """The reference's own pattern for many runs (one single-threaded process per run, run_lacmo.sh:7-16):
-    `workers` processes, each the oracle on its own seed of the plain 75x75 case, started together; the sum
-    of their SIP-steps/s."""
-    import multiprocessing as mp
-    ctx = mp.get_context("spawn")
-    with ctx.Pool(workers) as pool:
-        rates = pool.map(_cpu_run, [(seed0 + 2 * k, spc, 2, 8.0) for k in range(workers)])
-    return float(np.sum(rates))
-
-
-def cpu_baseline(seed0, spc, max_steps=3, max_s=10.0):
-    """Oracle (C restatement, 1 thread) on the plain 75x75 case: bounded sample of adv steps."""
+def cpu_protocol(seed0, spc, nx=2, warm=5, timed=60, reps=3):
+    """Oracle (C restatement, 1 thread) on a reduced domain of the 75x75 case: `nx` columns x 75 levels of
+    20 m cells (the whole vertical profile and its cloud layer), Poisson(spc) SIPs per cell, the same step
+    (Long table, dt 1 s, 10 substeps, 2 collision steps, relaxation); `warm` untimed advection steps, then
+    `reps` repetitions of `timed` steps; returns the median SIP-steps/s and all three."""
     from lacmopy_amd import load_kernel_data_Ecol
-    from lacmopy_amd.setup_case import init_profiles, make_case
+    from lacmopy_amd.setup_case import init_profiles, make_case, workload_config
     from oracle import oracle as orc
     E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
-    grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=75, nz=75, spc=spc, seed=seed0)
+    grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=nx, nz=75, spc=spc, seed=seed0)
     N = len(xi)
-    cfg = dict(solute_type="AS", dt_adv=1.0, dt_col=0.5, scale_dt_cond=5, no_col_per_adv=2, no_iter_impl_mass=3,
-               g_set=9.80665, act_collisions=True, act_relaxation=True, no_kernel_bins=nb, R_kernel_low_log=lnR0,
-               bin_factor_R_log=lnf)
+    cfg = workload_config("icmw75", E, lnR0, lnf)
     p = orc.adv_params(grid, N, cfg, seed=seed0)
     scal, mat = grid.scalar_fields(), grid.mat_prop()
     prof = init_profiles(grid)
@@ -298,17 +427,84 @@ def cpu_baseline(seed0, spc, max_steps=3, max_s=10.0):
     q = orc.adv_params(grid, N, cfg)
     orc.lib().or_update_pos_and_cells(orc.C.byref(q), 0.0, orc.P(pos), orc.P(vel), orc.P(cells), orc.P(rel),
                                       orc.P(m_w), orc.P(xi), orc.P(act.view(np.uint8)), orc.P(wr))
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        orc.integrate_adv_steps(p, 1, pos, vel, cells, rel, m_w, m_s, xi, act, T_p, wr, nco, scal, mat,
+
+    def steps(n):
+        orc.integrate_adv_steps(p, n, pos, vel, cells, rel, m_w, m_s, xi, act, T_p, wr, nco, scal, mat,
                                 grid.velocity, grid.mass_flux_air_dry, E, prof[0], prof[1], prof[2])
-        steps += 1
-        if time.perf_counter() - t0 > max_s or steps >= max_steps:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": N * steps / dt, "unit": "SIP-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} advection steps of the plain 75x75 case ({N} SIPs), oracle C restatement, 1 thread"}
+
+    steps(warm)
+    r = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        steps(timed)
+        r.append(N * timed / (time.perf_counter() - t0))
+    return {"median": float(np.median(r)), "reps": r, "sips": N, "collision_events": nco.tolist()}
+
+
+def cpu_full_domain_sample(seed0, spc, n_steps=1):
+    """The oracle on the full 75x75 case for n_steps advection steps (one thread): the per-SIP rate at the
+    workload's own size, beside the reduced-domain protocol."""
+    from lacmopy_amd import load_kernel_data_Ecol
+    from lacmopy_amd.setup_case import init_profiles, make_case, workload_config
+    from oracle import oracle as orc
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    grid, pos, vel, cells, m_w, m_s, xi, act = make_case(nx=75, nz=75, spc=spc, seed=seed0)
+    N = len(xi)
+    cfg = workload_config("icmw75", E, lnR0, lnf)
+    p = orc.adv_params(grid, N, cfg, seed=seed0)
+    scal, mat = grid.scalar_fields(), grid.mat_prop()
+    prof = init_profiles(grid)
+    rel = np.zeros((2, N))
+    q = orc.adv_params(grid, N, cfg)
+    orc.lib().or_update_pos_and_cells(orc.C.byref(q), 0.0, orc.P(pos), orc.P(vel), orc.P(cells), orc.P(rel),
+                                      orc.P(m_w), orc.P(xi), orc.P(act.view(np.uint8)), orc.P(np.zeros(1)))
+    t0 = time.perf_counter()
+    orc.integrate_adv_steps(p, n_steps, pos, vel, cells, rel, m_w, m_s, xi, act, np.ones(N), np.zeros(1),
+                            np.zeros(2, dtype=np.int64), scal, mat, grid.velocity, grid.mass_flux_air_dry, E,
+                            prof[0], prof[1], prof[2])
+    return {"sips": N, "steps": n_steps, "sip_steps_per_s": N * n_steps / (time.perf_counter() - t0)}
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model or platform.processor(), "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(seed0, spc, workers, nx=2):
+    """BASELINE protocol for the CPU path: single core (one process alone), then all cores of the share (one
+    independent run per core, started together, the reference's one process per run, run_lacmo.sh:7-16);
+    each process 5 warm-up + 3 x 60 timed advection steps, median of the 3; plus one step of the full 75x75
+    case on one core."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    t = time.perf_counter()
+    single = cpu_protocol(seed0, spc, nx)
+    if workers > 1:
+        with ctx.Pool(workers) as pool:
+            per = pool.map(_cpu_run, [(seed0 + 2 * k, spc, nx, 5, 60, 3) for k in range(workers)])
+        agg = float(np.sum([r["median"] for r in per]))
+    else:
+        per, agg = [single], single["median"]
+    full = cpu_full_domain_sample(seed0, spc)
+    out = {"value": agg, "unit": "SIP-steps/s", "cores": workers, "kind": "port",
+           "sample": f"oracle C restatement, 1 thread per process, on {nx} columns x 75 levels of the 75x75 case "
+                     f"(20 m cells, Poisson({spc:g}), the same step); per process 5 warm-up and 3 x 60 timed "
+                     f"advection steps, median of 3; {workers} processes started together, one seed each; "
+                     f"value = sum of their medians",
+           "single_core": single["median"], "single_core_reps": single["reps"],
+           "per_process_medians": [r["median"] for r in per], "sips_per_process": single["sips"],
+           "full_domain_1_core": full, "host": host_info(), "seconds": time.perf_counter() - t}
+    log(f"cpu baseline: {out}")
+    return out
 
 
 def copy_probe(mib=1024, reps=3):
@@ -362,49 +558,71 @@ def init_cost(n_seeds, cpu=True):
     return out
 
 
+def roofline(ktimes, n_sips, default_workload):
+    """The particle substep kernel (one thread per SIP, the step's dominant kernel) against HBM: algorithmic
+    bytes of one launch over its HIP-event duration; its VALU co-bound from the committed counter summary."""
+    if not ktimes or "particles" not in ktimes:
+        return None
+    ms = ktimes["particles"][0]
+    achieved = n_sips * B_SUBSTEP / (ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic("k_particles") if default_workload else None,
+            "kernel": "k_particles", "bytes_per_launch": n_sips * B_SUBSTEP, "launch_ms": ms}
+    valu = profile_value(VALU_PROFILE, "VALU", "k_particles") if default_workload else None
+    flops = profile_value(VALU_PROFILE, "FP64FLOPS", "k_particles") if default_workload else None
+    if valu and flops:
+        roof["valu"] = {"achieved": valu / (ms * 1e-3), "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
+                        "frac": valu / (ms * 1e-3) / VALU_ISSUE_PEAK,
+                        "fp64_tflops": flops / (ms * 1e-3) / 1e12,
+                        "fp64_frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--case", default="icmw75", choices=sorted(CASES))
     ap.add_argument("--runs", type=int, default=None, help="seeds in the batch (default: the case's)")
     ap.add_argument("--spc", type=float, default=None, help="mean SIPs per cell (default: the case's)")
     ap.add_argument("--seed", type=int, default=1001)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-workers", type=int, default=min(16, len(os.sched_getaffinity(0))),
+    ap.add_argument("--legs", default=",".join(LEGS), help=f"comma-separated subset of {','.join(LEGS)}")
+    ap.add_argument("--cpu-workers", type=int, default=min(HOST_THREADS, len(os.sched_getaffinity(0))),
                     help="processes of the all-cores CPU baseline (1: single core only)")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-kernels", action="store_true")
     ap.add_argument("--kernel", default=None, choices=["Long_Bott", "Hall_Bott", "const"])
-    ap.add_argument("--graph", action="store_true",
-                    help="replay the step captured in a two-step HIP graph instead of launching kernel by kernel")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
-    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--spinup", type=int, default=None,
+                    help="spin-up steps without collisions before the collision-active leg and the adjoint "
+                         "(default 7200; stress150: 3000, with collisions)")
+    ap.add_argument("--col-steps", type=int, default=900,
+                    help="advection steps with collisions between the spin-up and the collision-active timing")
     ap.add_argument("--adj-window", type=int, default=None,
                     help="advection steps of the adjoint window (default 600 for both cases)")
     ap.add_argument("--adj-every", type=int, default=None,
                     help="checkpoint interval of the adjoint window (default 60; stress150: 10)")
-    ap.add_argument("--adj-spinup", type=int, default=None,
-                    help="spin-up steps before the adjoint window (default 7200; stress150: 3000)")
-    ap.add_argument("--adj-check", type=int, default=0,
-                    help="also check the adjoint against finite differences on this many droplets and the "
-                         "six aerosol-mode parameters")
-    ap.add_argument("--no-init", action="store_true", help="skip timing the GPU initial states")
+    ap.add_argument("--adj-check", type=int, default=20,
+                    help="droplets of the finite-difference check of the adjoint (plus the six aerosol-mode "
+                         "parameters); 0: no check")
     ap.add_argument("--init-seeds", type=int, default=64, help="seeds of the timed GPU initialisation")
-    ap.add_argument("--init-only", action="store_true", help="only time the GPU initialisation (no CPU leg)")
+    ap.add_argument("--stress-runs", type=int, default=16, help="seeds of the stress150 batch leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibration of the HBM byte counters)")
     args = ap.parse_args()
+    legs = [x for x in args.legs.split(",") if x]
+    bad = [x for x in legs if x not in LEGS]
+    if bad:
+        ap.error(f"unknown legs {bad}")
     case = CASES[args.case]
     args.runs = args.runs or case["runs"]
     args.spc = args.spc or case["spc"]
     args.kernel = args.kernel or case["kernel"]
     n_sub = 2 * case["sdc"]
-    adj_w, adj_k, adj_s, _ = ADJ_DEFAULTS[args.case]
+    adj_w, adj_k, adj_s, adj_col, _ = ADJ_DEFAULTS[args.case]
     args.adj_window = args.adj_window or adj_w
     args.adj_every = args.adj_every or adj_k
-    args.adj_spinup = args.adj_spinup if args.adj_spinup is not None else adj_s
+    args.spinup = args.spinup if args.spinup is not None else adj_s
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): independent seeds shard "
               "trivially across GPUs and no multi-GPU path is built", file=sys.stderr)
@@ -414,102 +632,140 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
-    if args.init_only:
-        print(json.dumps({"init": init_cost(args.init_seeds, cpu=False)}), flush=True)
-        return
+    icmw = args.case == "icmw75"
+    default_workload = (icmw and args.runs == 64 and args.spc == 128.0 and args.seed == 1001
+                        and args.kernel == "Long_Bott")
+    out = {}
 
-    log(f"building batch of {args.runs} runs ...")
-    t = time.perf_counter()
-    eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, case=args.case, kernel=args.kernel)
-    log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
-    tune = None
-    if not args.no_tune:
+    # ---- early state: the batch a few steps after the synthetic start
+    shape = None
+    if "early" in legs:
+        log(f"building batch of {args.runs} runs ...")
         t = time.perf_counter()
-        eng.adv_steps(1)  # tune on an evolved state
-        tr = eng.autotune()
-        ch = tr["chosen"]
-        tune = {"chosen": ch,
-                "particle_ms": {f"{b}x{sp}/w{wp}": round(ms, 4) for (b, sp, wp), ms in tr["particles"].items()},
-                "dml_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["dml"].items()},
-                "collision_ms": {str(c): round(ms, 4) for c, ms in tr["collision"].items()},
-                "collision_work_ms": {f"{w}/{c}": round(ms, 4) for (w, c), ms in tr["collision_work"].items()},
-                "seconds": time.perf_counter() - t}
-        log(f"autotune: {tune}")
-    secs = time_steps(eng, args.steps, args.warmup, graph=args.graph)
-    batch_rate = n_sips * args.steps / secs
-    log(f"batch: {secs / args.steps * 1e3:.2f} ms/step, {batch_rate:.4g} SIP-steps/s")
-    ncols = eng.buf["no_cols"].sum(0).cpu().tolist()
-    log(f"collision events so far (single, multiple): {ncols}")
-    ktimes = {} if args.no_kernels else kernel_times(eng, max(1, min(args.steps, 3)))
-    log(f"kernel times (ms): {ktimes}")
-    n_batch_sips = n_sips
-    del eng
-    torch.cuda.empty_cache()
+        eng, n_sips, _ = build_engine(args.runs, args.spc, args.seed, case=args.case, kernel=args.kernel)
+        log(f"  {n_sips} SIPs loaded in {time.perf_counter() - t:.1f} s")
+        tn = None
+        if not args.no_tune:
+            eng.adv_steps(1)  # tune on an evolved state
+            tn = tune(eng)
+            shape = tn["chosen"]
+        r = rates(eng, n_sips, args.steps, args.warmup)
+        r.update(sips=n_sips, collision_events=collision_events(eng), autotune=tn,
+                 kernel_ms=kernel_times(eng, max(1, min(args.steps, 3))))
+        out["early"] = r
+        log(f"early batch: {r}")
+        del eng
+        torch.cuda.empty_cache()
 
-    plain = None
-    if not args.no_plain:
+    if "plain" in legs:
         eng1, n1, _ = build_engine(1, args.spc, args.seed, case=args.case, kernel=args.kernel)
         shape1 = None
         if not args.no_tune:
             eng1.adv_steps(1)
             shape1 = eng1.autotune()["chosen"]
-        s1 = time_steps(eng1, max(args.steps, 10), args.warmup, graph=args.graph)
-        plain = {"sip_steps_per_s": n1 * max(args.steps, 10) / s1, "ms_per_step": s1 / max(args.steps, 10) * 1e3,
-                 "sips": n1, "launch_shape": shape1}
-        log(f"plain: {plain}")
+        r = rates(eng1, n1, max(args.steps, 10), args.warmup)
+        r.update(sips=n1, launch_shape=shape1)
+        out["plain_early"] = r
+        log(f"plain early: {r}")
         del eng1
 
-    roof = None
-    if ktimes:
-        # dominant per-SIP kernel: the particle substep kernel (one thread per SIP)
-        ms = ktimes["particles"][0]
-        achieved = n_batch_sips * B_SUBSTEP / (ms * 1e-3) / 1e9
-        default_workload = (args.case == "icmw75" and args.runs == 64 and args.spc == 128.0 and args.seed == 1001
-                            and args.kernel == "Long_Bott")
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic("k_particles") if default_workload else None,
-                "kernel": "k_particles", "bytes_per_launch": n_batch_sips * B_SUBSTEP, "launch_ms": ms}
-        valu = profile_value(VALU_PROFILE, "VALU", "k_particles") if default_workload else None
-        flops = profile_value(VALU_PROFILE, "FP64FLOPS", "k_particles") if default_workload else None
-        if valu and flops:
-            # the co-bound of this kernel: VALU instruction issue (counts per launch from the committed --pmc
-            # summary of the same workload, over the launch time measured here)
-            roof["valu"] = {"achieved": valu / (ms * 1e-3), "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
-                            "frac": valu / (ms * 1e-3) / VALU_ISSUE_PEAK,
-                            "fp64_tflops": flops / (ms * 1e-3) / 1e12,
-                            "fp64_frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    # ---- spin-up of one run: the collision-active state and the adjoint window start
+    spin = None
+    if {"drizzle", "adjoint"} & set(legs):
+        spin, n1, cases1 = build_engine(1, args.spc, args.seed, case=args.case, kernel=args.kernel)
+        spin_up(spin, args.spinup, adj_col)
+        spun = spin.snapshot()
 
-    adj = None
-    if not args.no_adjoint:
-        adj = adjoint_cost(args.seed, args.spc, args.adj_window, args.adj_every, args.adj_spinup, args.adj_check,
-                           case=args.case)
-
-    init = None if (args.no_init or args.case != "icmw75") else init_cost(args.init_seeds, cpu=not args.no_cpu)
-
-    # the CPU baseline is the oracle on the plain 75x75 case (the metric's workload)
-    cpu = None if (args.no_cpu or args.case != "icmw75") else cpu_baseline(args.seed, args.spc)
-    if cpu and args.cpu_workers > 1:
+    if "drizzle" in legs:
+        spin.adv_steps(args.col_steps)
+        torch.cuda.synchronize()
+        spin.check()
+        drz = {"state": f"seed {args.seed}: {args.spinup} steps {'with' if adj_col else 'without'} collisions, "
+                        f"then {args.col_steps} steps with collisions",
+               "plain_collision_events": collision_events(spin)}
         t = time.perf_counter()
-        single = dict(cpu)
-        agg = cpu_baseline_all_cores(args.seed, args.spc, args.cpu_workers)
-        cpu = {"value": agg, "unit": "SIP-steps/s", "cores": args.cpu_workers, "kind": "port",
-               "sample": f"{args.cpu_workers} processes started together, each the oracle C restatement (1 thread) "
-                         f"on its own seed of the plain 75x75 case for 2 advection steps (the reference's one "
-                         f"single-threaded process per run); sum of their SIP-steps/s",
-               "single_core": single, "seconds": time.perf_counter() - t}
-    if cpu:
-        log(f"cpu baseline: {cpu}")
+        eng = replicate_engine(spin, cases1[0][0], args.runs, args.case, args.kernel, args.seed)
+        log(f"collision-active batch: {args.runs} copies loaded in {time.perf_counter() - t:.1f} s")
+        n_sips_d = eng.n_tot
+        if not args.no_tune:
+            eng.adv_steps(1)
+            drz["autotune"] = tune(eng)
+        # the runs diverge under their own draws for col_steps more steps before the timing
+        eng.adv_steps_graph(args.col_steps)
+        torch.cuda.synchronize()
+        eng.check()
+        eng.drop_graph()
+        c0 = collision_events(eng)
+        r = rates(eng, n_sips_d, args.steps, args.warmup)
+        drz.update(r)
+        drz["sips"] = n_sips_d
+        drz["collision_events_in_timed_steps"] = [a - b for a, b in zip(collision_events(eng), c0)]
+        drz["kernel_ms"] = kernel_times(eng, max(1, min(args.steps, 3)))
+        drz["state"] += (f"; replicated to {args.runs} runs, {args.col_steps} more steps with per-run draws "
+                         f"before the timing")
+        del eng
+        torch.cuda.empty_cache()
+        if not args.no_tune:
+            spin.autotune()
+        pr = rates(spin, spin.n_tot, max(args.steps, 10), args.warmup)
+        drz["plain"] = pr
+        drz["batch_over_plain_graph"] = drz["graph_sip_steps_per_s"] / pr["graph_sip_steps_per_s"]
+        out["drizzle"] = drz
+        log(f"collision-active: {drz}")
+
+    if "adjoint" in legs:
+        out["adjoint"] = adjoint_cost(spin, cases1, spun, args.adj_window, args.adj_every, args.adj_check,
+                                      case=args.case, spinup=args.spinup, spinup_collisions=adj_col)
+    if spin is not None:
+        del spin, spun
+        torch.cuda.empty_cache()
+
+    if "box" in legs and icmw:
+        out["box_model"] = box_model_leg(args.seed)
+
+    if "stress" in legs and icmw:
+        t = time.perf_counter()
+        eng, n_s, _ = build_engine(args.stress_runs, None, args.seed, case="stress150")
+        log(f"stress150 batch: {n_s} SIPs loaded in {time.perf_counter() - t:.1f} s")
+        st = {"workload": f"stress150: 150x150 cells of 10 m, Poisson(256), {args.stress_runs} seeds, Hall E_col, "
+                          f"dt 0.5 s, 20 substeps, 2 collision steps; a few steps after the synthetic start"}
+        if not args.no_tune:
+            eng.adv_steps(1)
+            st["launch_shape"] = eng.autotune()["chosen"]
+        st.update(rates(eng, n_s, 5, 2))
+        st["sips"] = n_s
+        st["kernel_ms"] = kernel_times(eng, 1)
+        out["stress"] = st
+        log(f"stress: {st}")
+        del eng
+        torch.cuda.empty_cache()
+
+    if "init" in legs and icmw:
+        out["initial_states"] = init_cost(args.init_seeds, cpu="cpu" in legs)
+
+    cpu = cpu_baseline(args.seed, args.spc, args.cpu_workers) if ("cpu" in legs and icmw) else None
+
+    # headline: the batch in the collision-active state with the hipGraph step; else the early state
+    if "drizzle" in out:
+        head, kt, n_head = out["drizzle"], out["drizzle"]["kernel_ms"], out["drizzle"]["sips"]
+        head_desc = "collision-active state (" + out["drizzle"]["state"] + "), two-step HIP graph replays"
+        value, ms_step = head["graph_sip_steps_per_s"], head["graph_ms_per_step"]
+    elif "early" in out:
+        head, kt, n_head = out["early"], out["early"]["kernel_ms"], out["early"]["sips"]
+        head_desc = f"{args.warmup} steps after the synthetic start, eager launches"
+        value, ms_step = head["eager_sip_steps_per_s"], head["eager_ms_per_step"]
+    else:
+        head, kt, n_head, head_desc, value, ms_step = None, None, None, None, None, None
+    plain = out.get("drizzle", {}).get("plain") or out.get("plain_early")
     res = {
         "metric": "SIP-steps/s, 2D ICMW 75x75: 64-seed batch vs plain GPU; adjoint/forward cost"
-        if args.case == "icmw75"
-        else f"SIP-steps/s, {args.case}",
-        "value": batch_rate,
+        if icmw else f"SIP-steps/s, {args.case}",
+        "value": value,
         "unit": "SIP-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": secs / args.steps * 1e3,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -517,17 +773,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"ICMW 2D kinematic Sc ({args.case}), {case['nx']}x{case['nz']} cells of "
                                f"{case['dx']:g} m, Poisson({args.spc:g}) SIPs/cell, {args.runs} seeds batched, "
-                               f"{args.kernel} E_col, dt={case['dt']:g} s, {n_sub} substeps, 2 collision steps",
-                   "runs": args.runs, "sips_total": n_batch_sips, "cells": case["nx"] * case["nz"],
+                               f"{args.kernel} E_col, dt={case['dt']:g} s, {n_sub} substeps, 2 collision steps; "
+                               f"{head_desc}",
+                   "runs": args.runs, "sips_total": n_head, "cells": case["nx"] * case["nz"],
                    "algorithmic_bytes_per_sip_step": n_sub * B_SUBSTEP + 2 * B_COLLISION + B_REINDEX,
-                   "launch": "two-step HIP graph replays" if args.graph else "eager",
-                   "autotune": tune,
-                   "plain_single_run": plain,
-                   "batch_over_plain": (batch_rate / plain["sip_steps_per_s"]) if plain else None,
-                   "kernel_ms": ktimes, "collision_events": ncols,
-                   "adjoint": adj,
-                   "initial_states": init},
-        "roofline": roof,
+                   "batch_over_plain": (head["graph_sip_steps_per_s"] / plain["graph_sip_steps_per_s"])
+                   if (head and plain) else None,
+                   **out},
+        "roofline": roofline(kt, n_head, default_workload) if kt else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

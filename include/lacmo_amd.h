@@ -105,8 +105,10 @@ typedef struct LmEngine {
   int32_t *scan_buf;  /* n_runs*(2nc+1)+1 */
   int32_t *scan_part; /* scan partials */
   int32_t *err;       /* 4 error words on the device */
-  int32_t *ovf_list;  /* cells over the fast-path LDS capacity */
-  int32_t *ovf_cnt;   /* 2: cells queued for the large-capacity collision [0] and Delta m_l [1] passes */
+  int32_t *ovf_list;  /* 2*n_runs*nc: cells over the fast-path LDS capacity (collision: list A in the first
+                         half, its cells over 256 members in the second) */
+  int32_t *ovf_cnt;   /* 3: cells queued for the large-capacity collision [0] and Delta m_l [1] passes, and for
+                         the collision's 1024-member pass after its 256-member pass [2] */
   /* injected uniform stream (parity mode; NULL = Philox) */
   const double *rnd_stream;
   int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */
@@ -161,7 +163,8 @@ typedef struct LmEngine {
    * persistent pass is on; any permutation of the cells gives the same results, only the balance changes.
    */
   int32_t col_work_per_cu, dml_work_per_cu;
-  int32_t *work; /* n_runs*nc + LM_WORK_BINS + 2: work list, histogram scratch, 2 work counters (NULL: off) */
+  int32_t *work; /* n_runs*nc + LM_WORK_BINS + 6: work list, histogram scratch, 2 work counters, the int64 cost
+                    sum of the list build at the next 8-byte boundary (NULL: off) */
   int32_t col_work_chunk, dml_work_chunk;
   /* particle substep compiled for this many waves per SIMD (0: the default build; 5, 6 or 8: a smaller
      register budget), one SIP per thread; set by an autotuner */
@@ -175,10 +178,18 @@ typedef struct LmEngine {
    */
   int32_t *mig_cnt;  /* n_runs*nc */
   int32_t *mig_slot; /* n_runs*nc*mig_cap */
-  int32_t mig_cap, pad5; /* 1..LM_MIG_CAP_MAX */
+  int32_t mig_cap; /* 1..LM_MIG_CAP_MAX */
+  /* events a cell's collision step takes one at a time through its candidate queue before the anti-diagonal
+     schedule finishes it: 0 = the default (48), else 1..LM_SPEC_EVENTS_MAX; results do not depend on it */
+  int32_t spec_events;
+  /* n_runs*nc or NULL: wall-clock ticks (100 MHz) each cell's last collision step took, written by the
+     collision passes; with it the work list starts with the cells whose cost exceeds half the expected
+     length of a persistent collision pass (their cost order), then the others by home-segment size */
+  uint32_t *col_cost;
 } LmEngine;
 
-#define LM_MIG_CAP_MAX 64 /* largest migrant list per cell */
+#define LM_MIG_CAP_MAX 64
+#define LM_SPEC_EVENTS_MAX 1024 /* largest migrant list per cell */
 
 #define LM_WORK_BINS 1024 /* home-segment sizes are binned up to LM_WORK_BINS - 1 for the work-list order */
 

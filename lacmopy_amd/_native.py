@@ -63,7 +63,7 @@ class LmEngine(C.Structure):
         ("E_max", _d), ("vel_run_stride", _i64),
         ("col_work_per_cu", _i32), ("dml_work_per_cu", _i32), ("work", _p),
         ("col_work_chunk", _i32), ("dml_work_chunk", _i32), ("part_wpe", _i32), ("pad4", _i32),
-        ("mig_cnt", _p), ("mig_slot", _p), ("mig_cap", _i32), ("pad5", _i32),
+        ("mig_cnt", _p), ("mig_slot", _p), ("mig_cap", _i32), ("spec_events", _i32), ("col_cost", _p),
     ]
 
 
@@ -94,7 +94,7 @@ LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
                  "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev",
                  "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max", "vel_run_stride",
-                 "col_work_per_cu", "work", "dml_work_chunk", "part_wpe", "mig_cnt", "mig_cap"]
+                 "col_work_per_cu", "work", "dml_work_chunk", "part_wpe", "mig_cnt", "mig_cap", "spec_events", "col_cost"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",

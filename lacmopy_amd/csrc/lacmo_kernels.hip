@@ -360,6 +360,8 @@ struct KView {
   int32_t *scan_buf, *scan_part, *err, *ovf_list, *ovf_cnt;
   int32_t *mig_cnt, *mig_slot; /* migrant lists (NULL: off) */
   int mig_cap;
+  int spec_events; /* events through the candidate queue before the anti-diagonal schedule */
+  uint32_t *col_cost; /* per-cell cost of the last collision step (ticks), or null */
   /* member lists of the reference-layout collision step (NULL: engine layout): the SIP indices of cell c
      are mlist[moff[c] .. moff[c+1]), in any order */
   const int32_t *mlist, *moff;
@@ -376,6 +378,8 @@ struct KView {
   double *tape_part, *ccd;
   int32_t *tape_part_i;
 };
+
+#define SPEC_MAX_EVENTS 48 /* default of LmEngine.spec_events */
 
 static KView make_view(const LmEngine *e) {
   KView v;
@@ -457,6 +461,8 @@ static KView make_view(const LmEngine *e) {
   v.mig_cnt = e->mig_cnt;
   v.mig_slot = e->mig_slot;
   v.mig_cap = e->mig_cap;
+  v.spec_events = e->spec_events > 0 ? e->spec_events : SPEC_MAX_EVENTS;
+  v.col_cost = e->col_cost;
   v.work = e->work;
   v.work_ctr = e->work ? e->work + (size_t)e->n_runs * e->nc + LM_WORK_BINS : nullptr;
   v.rnd_stream = e->rnd_stream;
@@ -619,14 +625,15 @@ __device__ static inline CellSource cell_source(int mode, int chunk, int ctr) {
   return s;
 }
 
-/* next global cell of this workgroup, or -1 */
-__device__ static inline int next_cell(const KView &v, CellSource &s, const int32_t *ovf_cnt) {
+/* next global cell of this workgroup, or -1 (PASS_OVF: from ovf_list[0 .. *ovf_cnt)) */
+__device__ static inline int next_cell(const KView &v, CellSource &s, const int32_t *ovf_list,
+                                       const int32_t *ovf_cnt) {
   const int it = s.it++;
   if (s.mode == PASS_CELLS) return it == 0 ? (int)blockIdx.x : -1;
   const int n = v.n_runs * v.nc;
   if (s.mode == PASS_OVF || s.chunk <= 0) {
     const int idx = (int)blockIdx.x + it * (int)gridDim.x;
-    if (s.mode == PASS_OVF) return idx < *ovf_cnt ? v.ovf_list[idx] : -1;
+    if (s.mode == PASS_OVF) return idx < *ovf_cnt ? ovf_list[idx] : -1;
     return idx < n ? v.work[idx] : -1;
   }
   if (s.next >= s.end) {
@@ -810,64 +817,203 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
  * larger B only rejects less); NaN or inf never rejects.  X only decreases
  * with events; Rc can grow, so the bounds are recomputed after each event.
  */
-template <int CAP>
-__device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n, const double *sX, const double *sR,
-                                  const double *sU, const double *sW, float *sB, unsigned char *sTG) {
+/* Largest multiplicity and radius and the velocity ranges over the members outside big[0..m) (wave-wide). */
+struct CellMax {
+  double X, Rc, ulo, uhi, wlo, whi;
+};
+
+#define BIG_MAX 8 /* members the guard pass can leave to the per-pair pass */
+
+/* Largest multiplicity and radius and the velocity ranges over the members whose rank byte is 0 or above m
+   (rank = 1 + position in the cell's list of largest radii, 0 if not listed; wave-wide). */
+__device__ static CellMax cell_max(int n, const double *sX, const double *sR, const double *sU, const double *sW,
+                                   const unsigned char *sRank, int m) {
   const int lane = threadIdx.x;
-  double X = -INFINITY, Rc = 0.0, ulo = INFINITY, uhi = -INFINITY, wlo = INFINITY, whi = -INFINITY;
+  CellMax M = {-INFINITY, 0.0, INFINITY, -INFINITY, INFINITY, -INFINITY};
   for (int p = lane; p < n; p += WAVE) {
-    X = fmax(X, sX[p]);
-    Rc = fmax(Rc, fabs(sR[p]));
-    ulo = fmin(ulo, sU[p]);
-    uhi = fmax(uhi, sU[p]);
-    wlo = fmin(wlo, sW[p]);
-    whi = fmax(whi, sW[p]);
+    const int rk = sRank[p];
+    if (rk != 0 && rk <= m) continue;
+    M.X = fmax(M.X, sX[p]);
+    M.Rc = fmax(M.Rc, fabs(sR[p]));
+    M.ulo = fmin(M.ulo, sU[p]);
+    M.uhi = fmax(M.uhi, sU[p]);
+    M.wlo = fmin(M.wlo, sW[p]);
+    M.whi = fmax(M.whi, sW[p]);
   }
   for (int off = WAVE / 2; off > 0; off >>= 1) {
-    X = fmax(X, __shfl_xor(X, off));
-    Rc = fmax(Rc, __shfl_xor(Rc, off));
-    ulo = fmin(ulo, __shfl_xor(ulo, off));
-    uhi = fmax(uhi, __shfl_xor(uhi, off));
-    wlo = fmin(wlo, __shfl_xor(wlo, off));
-    whi = fmax(whi, __shfl_xor(whi, off));
+    M.X = fmax(M.X, __shfl_xor(M.X, off));
+    M.Rc = fmax(M.Rc, __shfl_xor(M.Rc, off));
+    M.ulo = fmin(M.ulo, __shfl_xor(M.ulo, off));
+    M.uhi = fmax(M.uhi, __shfl_xor(M.uhi, off));
+    M.wlo = fmin(M.wlo, __shfl_xor(M.wlo, off));
+    M.whi = fmax(M.whi, __shfl_xor(M.whi, off));
   }
+  return M;
+}
+
+/*
+ * Bound B of row p against the members counted in M: for every such j, p(p, j)^2 <= B within the roundings of
+ * the pair filter (xi_max <= X, R_p + R_j <= |R_p| + Rc, |v_p - v_j| within the velocity ranges), rounded up
+ * to a float; and the row's guard-byte bound st: a pair whose leading byte G exceeds st has
+ * (G/256)^2 (1 - 2^-40) > B, the quad filter's own test, so it cannot fire (G/256 is a lower bound of its
+ * uniform).  st = 255 (NaN, or B too large for any byte) never rejects.
+ */
+__device__ static inline float row_bound(const LmPhys &P, double dtdV, double Eb, const CellMax &M, double R,
+                                         double u, double w, int &st) {
+  const double du = fmax(u - M.ulo, M.uhi - u), dw = fmax(w - M.wlo, M.whi - w);
+  const double d2 = du * du + dw * dw;
+  const double sr = fabs(R) + M.Rc;
+  const double Pb = (((M.X * dtdV) * ((P.pi * sr) * sr)) * Eb) * 1.0E-12;
+  double B = INFINITY;
+  if (Pb > 1.0E-140)
+    B = (Pb * Pb) * d2;
+  else if ((Pb <= 1.0E-140) & (d2 < 1.0E100))
+    B = 1.0E-178;
+  float f = (float)B; /* B >= 0 or NaN: rounded down, the next float up is above B */
+  if ((double)f < B) f = __int_as_float(__float_as_int(f) + 1);
+  /* tg = the smallest byte G with (G/256)^2 (1 - 2^-40) > B; st = min(tg, 256) - 1 */
+  const double Bf = (double)f;
+  int tg = 257;
+  if (Bf == Bf && Bf < 1.0) {
+    tg = (int)(256.0 * sqrt(Bf));
+    if (tg > 256) tg = 256;
+    for (int k = 0; k < 4 && tg <= 256 && !(((tg * (1.0 / 256.0)) * (tg * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf);
+         ++k)
+      ++tg;
+    if (!(((tg * (1.0 / 256.0)) * (tg * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf)) tg = 257;
+    while (tg > 1 && (((tg - 1) * (1.0 / 256.0)) * ((tg - 1) * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf) --tg;
+  }
+  st = (tg < 256 ? tg : 256) - 1;
+  return f;
+}
+
+/* quads of row p (diagonals 2p+1 .. p+n-1 in blocks of four) */
+__device__ static inline int row_quads(int p, int n) {
+  return p < n - 1 ? ((p + n - 1) >> 2) - ((2 * p + 1) >> 2) + 1 : 0;
+}
+
+/* expected quads of row p that the guard pass keeps with bound st (uniform bytes, four per quad) */
+__device__ static inline double row_kept(int p, int n, int st) {
+  const double q = 1.0 - (st + 1) * (1.0 / 256.0);
+  return row_quads(p, n) * (1.0 - (q * q) * (q * q));
+}
+
+/*
+ * Row bounds of the pass filters.  sB[p]: the row bound over all members (eval_quad's row filter, valid for
+ * every pair of row p).  sTG[p]: the guard pass's byte bound, over the members outside the cell's exclusion
+ * list sBig[0..*sNbig): a few members of the largest radii (drops among cloud droplets), whose pairs the
+ * per-pair pass tests instead.  One large drop widens the radius and velocity ranges of the whole cell and
+ * with them every row's bound; the list is used when the expected work of the guard pass's queue drops by
+ * more than the per-pair tests cost (quads of nq_all in total; a queue over QCAP quads costs a pass over all).
+ * Any list is exact: every pair is covered by its row's bound or by the per-pair pass.
+ */
+template <int CAP>
+__device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n, const double *sX, const double *sR,
+                                  const double *sU, const double *sW, float *sB, unsigned char *sTG, int *sBig,
+                                  int *sNbig, unsigned char *sRank, int nq_all) {
+  const int lane = threadIdx.x;
+  for (int p = lane; p < n; p += WAVE) sRank[p] = 0;
+  __syncthreads();
+  const CellMax M0 = cell_max(n, sX, sR, sU, sW, sRank, 0);
+  double kept0 = 0.0;
   for (int p = lane; p < n; p += WAVE) {
-    const double du = fmax(sU[p] - ulo, uhi - sU[p]), dw = fmax(sW[p] - wlo, whi - sW[p]);
-    const double d2 = du * du + dw * dw;
-    const double sr = fabs(sR[p]) + Rc;
-    const double Pb = (((X * dtdV) * ((P.pi * sr) * sr)) * Eb) * 1.0E-12;
-    double B = INFINITY;
-    if (Pb > 1.0E-140)
-      B = (Pb * Pb) * d2;
-    else if ((Pb <= 1.0E-140) & (d2 < 1.0E100))
-      B = 1.0E-178;
-    float f = (float)B; /* B >= 0 or NaN: rounded down, the next float up is above B */
-    if ((double)f < B) f = __int_as_float(__float_as_int(f) + 1);
-    sB[p] = f;
-    /* tg = the smallest guard byte G with (G/256)^2 (1 - 2^-40) > B, the quad filter's own test: a quad of
-       row p whose valid pairs all have leading bytes >= tg holds no pair that can fire (G/256 is a lower bound
-       of each uniform); tg >= 256 (NaN, or B too large for any byte) never rejects.  Stored as
-       sTG[p] = min(tg, 256) - 1: reject when the smallest byte exceeds sTG[p] */
-    const double Bf = (double)f;
-    int tg = 257;
-    if (Bf == Bf && Bf < 1.0) {
-      tg = (int)(256.0 * sqrt(Bf));
-      if (tg > 256) tg = 256;
-      for (int k = 0; k < 4 && tg <= 256 && !(((tg * (1.0 / 256.0)) * (tg * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf);
-           ++k)
-        ++tg;
-      if (!(((tg * (1.0 / 256.0)) * (tg * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf)) tg = 257;
-      while (tg > 1 && (((tg - 1) * (1.0 / 256.0)) * ((tg - 1) * (1.0 / 256.0))) * (1.0 - 0x1p-40) > Bf) --tg;
-    }
-    sTG[p] = (unsigned char)((tg < 256 ? tg : 256) - 1);
+    int st;
+    sB[p] = row_bound(P, dtdV, Eb, M0, sR[p], sU[p], sW[p], st);
+    sTG[p] = (unsigned char)st;
+    kept0 += row_kept(p, n, st);
   }
+  for (int off = WAVE / 2; off > 0; off >>= 1) kept0 += __shfl_xor(kept0, off);
+  const double over = 1.25 * nq_all; /* a queue overflow: the anti-diagonal schedule over all pairs */
+  /* the floor: quads holding a zero byte, which no bound rejects (1 - (255/256)^4 of all) */
+  const double floor_kept = 0.0155 * nq_all;
+  int best_m = 0;
+  if (kept0 > floor_kept + 24.0 && n >= 3 * BIG_MAX) {
+    /* the BIG_MAX largest radii, in decreasing order (ties by position): sRank[p] = 1 + place */
+    for (int k = 0; k < BIG_MAX; ++k) {
+      double br = -1.0;
+      int bp = n;
+      for (int p = lane; p < n; p += WAVE) {
+        const double r = fabs(sR[p]);
+        if (sRank[p] == 0 && (r > br || (r == br && p < bp))) {
+          br = r;
+          bp = p;
+        }
+      }
+      for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const double orr = __shfl_xor(br, off);
+        const int op = __shfl_xor(bp, off);
+        if (orr > br || (orr == br && op < bp)) {
+          br = orr;
+          bp = op;
+        }
+      }
+      if (lane == 0) {
+        sRank[bp] = (unsigned char)(k + 1);
+        sBig[k] = bp;
+      }
+      __syncthreads();
+    }
+    double best = kept0 + (kept0 > 100.0 ? over : 0.0);
+    for (int m = 2; m <= BIG_MAX; m *= 2) {
+      const CellMax Mm = cell_max(n, sX, sR, sU, sW, sRank, m);
+      double kept = 0.0;
+      for (int p = lane; p < n; p += WAVE) {
+        const int rk = sRank[p];
+        if (rk != 0 && rk <= m) continue;
+        int st;
+        (void)row_bound(P, dtdV, Eb, Mm, sR[p], sU[p], sW[p], st);
+        kept += row_kept(p, n, st);
+      }
+      for (int off = WAVE / 2; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
+      const double cost = kept + 0.18 * n * m + (kept > 100.0 ? over : 0.0);
+      if (cost < best) {
+        best = cost;
+        best_m = m;
+      }
+    }
+    if (best_m > 0) {
+      const CellMax Mm = cell_max(n, sX, sR, sU, sW, sRank, best_m);
+      for (int p = lane; p < n; p += WAVE) {
+        int st;
+        (void)row_bound(P, dtdV, Eb, Mm, sR[p], sU[p], sW[p], st);
+        sTG[p] = (unsigned char)st; /* rows of the list are left to the per-pair pass */
+      }
+    }
+    __syncthreads();
+    /* only the chosen part of the list stays marked */
+    for (int p = lane; p < n; p += WAVE)
+      if (sRank[p] > best_m) sRank[p] = 0;
+  }
+  if (lane == 0) *sNbig = best_m;
+}
+
+/*
+ * Whether pair (i, j) can fire given only its guard byte G (u >= G/256): the pair filter of eval_quad with
+ * the E bound Eb and the threshold G/256 in place of the pair's 40-bit lower bound.  false proves p < G/256
+ * <= u: no event and no multiple collection.
+ */
+__device__ static inline bool pair_may_fire8(const LmPhys &P, double dtdV, double Eb, const double *sX,
+                                             const double *sR, const double *sU, const double *sW, int i, int j,
+                                             uint32_t G) {
+  if (G == 0u) return true;
+  const double xi_max = sX[i] <= sX[j] ? sX[j] : sX[i];
+  const double du = sU[i] - sU[j], dw = sW[i] - sW[j];
+  const double d2 = du * du + dw * dw;
+  const double sR2 = sR[i] + sR[j];
+  const double thr = (double)G * (1.0 / 256.0);
+  const double thr2 = (thr * thr) * (1.0 - 0x1p-40);
+  const double Pm = (((xi_max * dtdV) * ((P.pi * sR2) * sR2)) * Eb) * 1.0E-12;
+  const bool big = Pm > 1.0E-140;
+  const bool no = (big & ((Pm * Pm) * d2 < thr2)) | ((!big) & (d2 < 1.0E100) & (thr2 > 1.0E-178));
+  return !no;
 }
 
 /* pair number of (i, j) in the reference's row-major loop */
 __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
 
-#define SPEC_MAX_EVENTS 16
-#define QCAP 128 /* quads the guard pass can queue per pass */
+#define QCAP 128 /* quads the candidate queue holds: entries lane and lane + WAVE of the evaluating wave */
+#define KEY_STALE 0xFFFFFFFFFFFFFFFEull /* queue entry to evaluate (keys of real pairs are below 2^63) */
+static_assert(QCAP == 2 * WAVE, "the queue evaluation takes two entries per lane");
 
 /* State of one cell's collision step in LDS (id order) plus slot-indexed globals. */
 
@@ -880,7 +1026,8 @@ __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * 
 template <int CAP, bool TAB, bool INJ>
 __device__ static inline unsigned long long eval_quad(const LmPhys &P, const KView &v, const CollCell &cc, int i,
                                                       int b, int k_next, uint32_t gw, long long roff,
-                                                      uint32_t key_run, uint32_t key_step, uint32_t key_cell) {
+                                                      uint32_t key_run, uint32_t key_step, uint32_t key_cell,
+                                                      bool row_filter) {
   const int n = cc.n;
   const double *sX = cc.sX, *sR = cc.sR, *sU = cc.sU, *sW = cc.sW;
   const int *sIND = cc.sIND;
@@ -903,7 +1050,7 @@ __device__ static inline unsigned long long eval_quad(const LmPhys &P, const KVi
         th = lm_pair_lo40(lm_guard_byte(gw, (uint32_t)t), lm_word(blk, (uint32_t)t));
       if (valid) thq = fmin(thq, th);
     }
-    if ((thq > 0.0) & ((thq * thq) * (1.0 - 0x1p-40) > (double)sB[i])) return ~0ull;
+    if (row_filter & (thq > 0.0) & ((thq * thq) * (1.0 - 0x1p-40) > (double)sB[i])) return ~0ull;
   }
   const double xi_i = sX[i], R_i = sR[i], u_i = sU[i], w_i = sW[i];
   const double *Erow = TAB ? v.E + (size_t)sIND[i] * v.nb : nullptr;
@@ -991,7 +1138,7 @@ __device__ static inline unsigned long long eval_quad(const LmPhys &P, const KVi
  * The kernel therefore evaluates all remaining pairs in parallel on the
  * current state (full lane use, no barriers), takes the first firing pair in
  * row-major order (wave minimum), applies it and repeats from the next pair.
- * Cells with more than SPEC_MAX_EVENTS events finish with the anti-diagonal
+ * Cells with more than v.spec_events events finish with the anti-diagonal
  * schedule: pairs grouped by d = i + j are mutually independent and every
  * dependence points to a smaller d, so processing the remaining pairs by
  * increasing d reproduces the sequential loop too.
@@ -1000,8 +1147,15 @@ __device__ static inline unsigned long long eval_quad(const LmPhys &P, const KVi
  * (diagonals 4b..4b+3), i.e. four consecutive pairs of the row sharing one
  * block of A words.
  */
+/* the lists of a collision pass: the cells it reads (PASS_OVF) and where it queues cells over its capacity
+   (out_list null: such a cell is an error) */
+struct ColLists {
+  const int32_t *in_list, *in_cnt;
+  int32_t *out_list, *out_cnt;
+};
+
 template <int CAP, bool TAB, bool INJ>
-__device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
+__device__ static void collide_cell(const KView &v, int gc, const ColLists &io) {
   const LmPhys &P = c_P;
   __syncthreads(); /* the LDS of the previous cell of this block is free */
   const int nc = v.nc;
@@ -1015,7 +1169,10 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   __shared__ float sB[CAP];
   __shared__ unsigned char sTG[CAP];
   __shared__ int sQueue[QCAP];
+  __shared__ unsigned long long sQkey[QCAP]; /* each entry's first firing pair (key), ~0: none, KEY_STALE */
   __shared__ int sQn;
+  __shared__ int sBig[BIG_MAX], sNbig;
+  __shared__ unsigned char sIsBig[CAP]; /* rank in the list of largest radii the guard pass leaves out, or 0 */
   __shared__ NbrTab tb;
   __shared__ int sSeq;
   int *sSlot = (int *)sU;
@@ -1053,16 +1210,17 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   }
   if (n > CAP) {
     if (lane == 0) {
-      if (from_ovf) {
+      if (!io.out_list) {
         raise_err(v.err, LM_ERR_CELL_CAP, gc);
       } else {
-        const int q = atomicAdd(v.ovf_cnt, 1);
-        v.ovf_list[q] = gc;
+        const int q = atomicAdd(io.out_cnt, 1);
+        io.out_list[q] = gc;
       }
     }
     return;
   }
   if (n < 2) return;
+  const unsigned long long t_cell = v.col_cost ? wall_clock64() : 0ull;
   if (lane == 0) sSeq = 0;
   __syncthreads();
   if (!fast) {
@@ -1108,7 +1266,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   const double dtdV = v.dt_col_over_dV, pi = P.pi;
   /* the E bound of the row filter: the table's largest entry (unknown: no row filter) or the constant */
   const double Eb = TAB ? (v.E_max > 0.0 ? v.E_max : INFINITY) : v.E_const;
-  row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB, sTG);
+  row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB, sTG, sBig, &sNbig, sIsBig, sQ[n - 1]);
   __syncthreads();
   const uint32_t key_run = (uint32_t)r, key_step = *v.col_step_dev, key_cell = (uint32_t)c;
   const uint32_t gkey = INJ ? 0u : lm_guard_key(v.seed, key_run, key_step, key_cell);
@@ -1121,7 +1279,12 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
   bool rho_ready = false;
   bool finished = false;
 
-  while (n_events < SPEC_MAX_EVENTS) {
+  bool build = true; /* the candidate queue is built from the row bounds by the first pass */
+  bool fresh = true; /* row bounds describe the current state (no event yet) */
+  const int spec_events = v.spec_events > 0 ? v.spec_events : SPEC_MAX_EVENTS;
+  while (n_events < spec_events) {
+    unsigned long long kq0 = ~0ull, kq1 = ~0ull; /* this pass's keys of queue entries lane, lane + 64 */
+    int nq = 0;
     /* parallel pass: first firing pair with cnt >= k_next under the current state.  The first quad holding
        pair k_next: the pair after the last event (or pair 0); quad order is row-major pair order. */
     unsigned long long best = ~0ull;
@@ -1145,80 +1308,119 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
         while (sQ[i + 1] <= q) ++i;
         const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
         const unsigned long long kq = eval_quad<CAP, TAB, INJ>(P, v, cc, i, b, k_next, 0u, roff, key_run,
-                                                               key_step, key_cell);
+                                                               key_step, key_cell, true);
         best = kq < best ? kq : best;
       }
     } else {
-      /* Guard pass, row-parallel: lane l takes rows i0 + l, i0 + l + 64, ...; per quad the guard word and an
-         integer compare of its valid pairs' smallest leading byte with the row's bound sTG (no Philox, no
-         memory access), survivors appended to an LDS queue (any order: the pass takes the minimum).  The
-         queue is then evaluated by the whole wave; a queue overflow (cells whose bounds reject little) takes
-         the plain pass over all quads from q_first. */
-      if (lane == 0) sQn = 0;
-      __syncthreads();
-      /* rows i0 .. n-2 in pairs (i0 + p, n - 2 - p): every lane gets about the same number of quads */
-      const int n_rows = n - 1 - i0;
-      for (int pr = lane; 2 * pr < n_rows; pr += WAVE) {
+      /* The candidate queue: quads that may hold a firing pair with cnt >= k_next.  Built by the first pass
+         with the guard pass, row-parallel (lane l takes rows l, l + 64, ...; per quad the guard word and an
+         integer compare of its valid pairs' smallest leading byte with the row's bound sTG: no Philox, no
+         memory access) and the per-pair pass over the pairs of the listed members; then kept across events
+         (an event changes two members only, see below).  A queue over QCAP quads hands the cell to the
+         anti-diagonal schedule. */
+      if (build) {
+        if (lane == 0) sQn = 0;
+        __syncthreads();
+        const int nbig = sNbig;
+        /* rows i0 .. n-2 in pairs (i0 + p, n - 2 - p): every lane gets about the same number of quads */
+        const int n_rows = n - 1 - i0;
+        for (int pr = lane; 2 * pr < n_rows; pr += WAVE) {
 #pragma unroll 1
-        for (int side = 0; side < 2; ++side) {
-          const int i = side == 0 ? i0 + pr : n - 2 - pr;
-          if (side == 1 && i <= i0 + pr) break;
-          const int base = i * n - i * (i + 1) / 2;
-          const int jmin = (k_next - base + i + 1 > i + 1) ? k_next - base + i + 1 : i + 1;
-          if (jmin >= n) continue;
-          const uint32_t st = sTG[i]; /* keep a quad when a valid pair's byte is <= st */
-          const int b0 = (i + jmin) >> 2, b1 = (i + n - 1) >> 2;
-          /* four quads per iteration: their hash chains are independent and overlap in the pipeline */
-          for (int bb = b0; bb <= b1; bb += 4) {
-            uint32_t gq[4];
+          for (int side = 0; side < 2; ++side) {
+            const int i = side == 0 ? i0 + pr : n - 2 - pr;
+            if (side == 1 && i <= i0 + pr) break;
+            if (sIsBig[i]) continue; /* the per-pair pass takes the rows of the list */
+            const int base = i * n - i * (i + 1) / 2;
+            const int jmin = (k_next - base + i + 1 > i + 1) ? k_next - base + i + 1 : i + 1;
+            if (jmin >= n) continue;
+            const uint32_t st = sTG[i]; /* keep a quad when a valid pair's byte is <= st */
+            const int b0 = (i + jmin) >> 2, b1 = (i + n - 1) >> 2;
+            /* four quads per iteration: their hash chains are independent and overlap in the pipeline */
+            for (int bb = b0; bb <= b1; bb += 4) {
+              uint32_t gq[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) gq[u] = lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * (bb + u)));
+              for (int u = 0; u < 4; ++u) gq[u] = lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * (bb + u)));
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int b = bb + u;
-              if (b > b1) break;
-              uint32_t g = gq[u];
-              if ((b == b0) | (b == b1)) {
-                /* bytes of pairs outside [jmin, n-1] set to 0xFF, which no bound below 255 keeps; the quad
-                   holds at least one valid pair (j = jmin in the first, j = n-1 in the last) */
+              for (int u = 0; u < 4; ++u) {
+                const int b = bb + u;
+                if (b > b1) break;
+                uint32_t g = gq[u];
+                if ((b == b0) | (b == b1)) {
+                  /* bytes of pairs outside [jmin, n-1] set to 0xFF, which no bound below 255 keeps; the quad
+                     holds at least one valid pair (j = jmin in the first, j = n-1 in the last) */
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                  const int j = 4 * b + t - i;
-                  if ((j < jmin) | (j >= n)) g |= 0xFFu << (8 * t);
+                  for (int t = 0; t < 4; ++t) {
+                    const int j = 4 * b + t - i;
+                    if ((j < jmin) | (j >= n)) g |= 0xFFu << (8 * t);
+                  }
                 }
-              }
-              bool keep;
-              if (st < 127u) /* a byte below st + 1 (exact word test, valid for bounds up to 128) */
-                keep = ((g - 0x01010101u * (st + 1u)) & ~g & 0x80808080u) != 0u;
-              else
-                keep = true; /* the bound rejects (almost) nothing: every quad holds a valid pair */
-              if (keep) {
-                const int pos = atomicAdd(&sQn, 1);
-                if (pos < QCAP) sQueue[pos] = (i << 16) | b;
+                bool keep;
+                if (st < 127u) /* a byte below st + 1 (exact word test, valid for bounds up to 128) */
+                  keep = ((g - 0x01010101u * (st + 1u)) & ~g & 0x80808080u) != 0u;
+                else
+                  keep = true; /* the bound rejects (almost) nothing: every quad holds a valid pair */
+                if (keep) {
+                  const int pos = atomicAdd(&sQn, 1);
+                  if (pos < QCAP) {
+                  sQueue[pos] = (i << 16) | b;
+                  sQkey[pos] = KEY_STALE;
+                }
+                }
               }
             }
           }
         }
-      }
-      __syncthreads();
-      const int nq = sQn;
-      if (nq <= QCAP) {
-        for (int e = lane; e < nq; e += WAVE) {
-          const int pk = sQueue[e], qi = pk >> 16, qb = pk & 0xffff;
-          const unsigned long long kq = eval_quad<CAP, TAB, INJ>(
-              P, v, cc, qi, qb, k_next, lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
-              key_step, key_cell);
-          best = kq < best ? kq : best;
+        /* per-pair pass: every pair of a listed member (its row and its column) with cnt >= k_next, tested with
+           its guard byte alone; the quads of the pairs that can fire join the queue (a pair of two listed
+           members may be queued twice: the pass takes the minimum) */
+        for (int kb = 0; kb < nbig; ++kb) {
+          const int bm = sBig[kb];
+          for (int o = lane; o < n; o += WAVE) {
+            if (o == bm) continue;
+            const int i = o < bm ? o : bm, j = o < bm ? bm : o;
+            if (pair_cnt(i, j, n) < k_next) continue;
+            const int d = i + j;
+            const uint32_t G = lm_guard_byte(lm_guard_word(gkey, (uint32_t)i, (uint32_t)d), (uint32_t)d);
+            if (pair_may_fire8(P, dtdV, Eb, sX, sR, sU, sW, i, j, G)) {
+              const int pos = atomicAdd(&sQn, 1);
+              if (pos < QCAP) {
+              sQueue[pos] = (i << 16) | (d >> 2);
+              sQkey[pos] = KEY_STALE;
+            }
+            }
+          }
         }
-      } else {
-        int i = i0;
-        for (int q = q_first + lane; q < nQ; q += WAVE) {
-          while (sQ[i + 1] <= q) ++i;
-          const int b = ((2 * i + 1) >> 2) + (q - sQ[i]);
-          const unsigned long long kq = eval_quad<CAP, TAB, INJ>(
-              P, v, cc, i, b, k_next, lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * b)), roff, key_run, key_step,
-              key_cell);
-          best = kq < best ? kq : best;
+        __syncthreads();
+        build = false;
+      }
+      nq = sQn;
+      /* too many candidates (the bounds reject little and events are many): the anti-diagonal schedule, exact
+         from any k_next, finishes the cell */
+      if (nq > QCAP) break;
+      {
+        /* each lane evaluates queue entries lane and lane + 64; the row filter of eval_quad needs bounds of
+           the current state (fresh: no event since the queue was built) */
+        if (lane < nq) {
+          kq0 = sQkey[lane];
+          if (kq0 == KEY_STALE) {
+            const int pk = sQueue[lane], qi = pk >> 16, qb = pk & 0xffff;
+            kq0 = eval_quad<CAP, TAB, INJ>(P, v, cc, qi, qb, k_next,
+                                           lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
+                                           key_step, key_cell, fresh);
+            sQkey[lane] = kq0;
+          }
+          best = kq0 < best ? kq0 : best;
+        }
+        if (lane + WAVE < nq) {
+          kq1 = sQkey[lane + WAVE];
+          if (kq1 == KEY_STALE) {
+            const int pk = sQueue[lane + WAVE], qi = pk >> 16, qb = pk & 0xffff;
+            kq1 = eval_quad<CAP, TAB, INJ>(P, v, cc, qi, qb, k_next,
+                                           lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
+                                           key_step, key_cell, fresh);
+            sQkey[lane + WAVE] = kq1;
+          }
+          best = kq1 < best ? kq1 : best;
         }
       }
       __syncthreads();
@@ -1259,15 +1461,87 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     ++n_events;
     k_next = ecnt + 1;
     prev_event = best;
-    if (n_events < SPEC_MAX_EVENTS) {
-      /* the event may have grown a radius: fresh row bounds for the next pass */
-      row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB, sTG);
-      __syncthreads();
+    if (n_events < spec_events) {
+      if (INJ) {
+        /* the event may have grown a radius: fresh row bounds for the next pass */
+        row_bounds<CAP>(P, dtdV, Eb, n, sX, sR, sU, sW, sB, sTG, sBig, &sNbig, sIsBig, sQ[n - 1]);
+        __syncthreads();
+      } else {
+        /*
+         * Keep the queue.  The event changed members ei and ej only, so every pair without them keeps its p and
+         * its uniform: a quad that held no firing pair (key ~0) holds none now except through pairs of ei or
+         * ej, and a quad with a firing pair is kept with its key (evaluated again only if it holds a pair of ei
+         * or ej).  The undecided pairs of ei and ej -- (ei, y) and (ej, y) for y > ej, (x, ej) for ei < x < ej --
+         * are tested again with their guard bytes on the new state and their quads queued.
+         */
+        const int pk0 = lane < nq ? sQueue[lane] : 0, pk1 = lane + WAVE < nq ? sQueue[lane + WAVE] : 0;
+        const bool al0 = (lane < nq) & (kq0 != ~0ull), al1 = (lane + WAVE < nq) & (kq1 != ~0ull);
+        /* a kept quad's key stays exact unless the quad holds a pair of ei or ej (pairs (i, 4b - i .. 4b + 3 - i)):
+           its pairs before the key still do not fire and the key's pair still fires */
+        const int i_0 = pk0 >> 16, j_0 = 4 * (pk0 & 0xffff) - i_0, i_1 = pk1 >> 16, j_1 = 4 * (pk1 & 0xffff) - i_1;
+        const bool t0 = (i_0 == ei) | (i_0 == ej) | ((unsigned)(ei - j_0) < 4u) | ((unsigned)(ej - j_0) < 4u);
+        const bool t1 = (i_1 == ei) | (i_1 == ej) | ((unsigned)(ei - j_1) < 4u) | ((unsigned)(ej - j_1) < 4u);
+        const unsigned long long nk0 = t0 ? KEY_STALE : kq0, nk1 = t1 ? KEY_STALE : kq1;
+        const unsigned long long b0 = __ballot(al0), b1 = __ballot(al1);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int c0 = __popcll(b0);
+        __syncthreads();
+        if (al0) {
+          const int q = __popcll(b0 & below);
+          sQueue[q] = pk0;
+          sQkey[q] = nk0;
+        }
+        if (al1) {
+          const int q = c0 + __popcll(b1 & below);
+          sQueue[q] = pk1;
+          sQkey[q] = nk1;
+        }
+        if (lane == 0) sQn = c0 + __popcll(b1);
+        __syncthreads();
+        const int la = n - 1 - ej, lb = ej - ei - 1;
+        for (int t = lane; t < 2 * la + lb; t += WAVE) {
+          int i, j;
+          if (t < la) {
+            i = ei;
+            j = ej + 1 + t;
+          } else if (t < la + lb) {
+            i = ei + 1 + (t - la);
+            j = ej;
+          } else {
+            i = ej;
+            j = ej + 1 + (t - la - lb);
+          }
+          const int d = i + j;
+          const uint32_t G = lm_guard_byte(lm_guard_word(gkey, (uint32_t)i, (uint32_t)d), (uint32_t)d);
+          if (pair_may_fire8(P, dtdV, Eb, sX, sR, sU, sW, i, j, G)) {
+            const int pos = atomicAdd(&sQn, 1);
+            if (pos < QCAP) {
+              sQueue[pos] = (i << 16) | (d >> 2);
+              sQkey[pos] = KEY_STALE;
+            }
+          }
+        }
+        __syncthreads();
+        fresh = false;
+        if (sQn > QCAP) break; /* the anti-diagonal schedule takes over */
+      }
     }
   }
 
+  if (!finished && !rho_ready) {
+    /* the pre-step densities (the candidate queue overflowed before the cell's first event) */
+    for (int p = lane; p < n; p += WAVE) {
+      const int k = sSL[p];
+      double R, ws, rho;
+      lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
+      rho_g[k] = rho;
+    }
+    rho_ready = true;
+    __threadfence_block();
+    __syncthreads();
+  }
   if (!finished) {
-    /* many events: anti-diagonal schedule over the pairs with cnt >= k_next */
+    /* many events or candidates: anti-diagonal schedule over the pairs with cnt >= k_next */
     int tag0 = -1, tag1 = -1;
     lm_u32x4 blk0, blk1;
     blk0.v[0] = blk0.v[1] = blk0.v[2] = blk0.v[3] = 0;
@@ -1285,6 +1559,8 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
           rnd_inj = rnd_at(v, roff + cnt);
         } else {
           G = lm_guard_byte(lm_guard_word(gkey, (uint32_t)i, (uint32_t)d), (uint32_t)d);
+          /* the pair's bound on the current state against its guard byte: most pairs end here */
+          if (!pair_may_fire8(P, dtdV, Eb, sX, sR, sU, sW, i, j, G)) continue;
           const int tag = (i << 16) | (d >> 2);
           const bool cls = (i >> 6) & 1;
           if (tag != (cls ? tag1 : tag0)) {
@@ -1322,7 +1598,7 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     }
   }
 
-  if (n_events > 0)
+  if (n_events > 0 || !finished)
     for (int m = lane; m < n; m += WAVE) v.xi[sSL[m]] = sX[m];
   for (int off = WAVE / 2; off > 0; off >>= 1) {
     c_single += __shfl_down(c_single, off);
@@ -1334,18 +1610,23 @@ __device__ static void collide_cell(const KView &v, int gc, int from_ovf) {
     if (c_single) atomicAdd((unsigned long long *)&v.no_cols[2 * r], (unsigned long long)c_single);
     if (c_multi) atomicAdd((unsigned long long *)&v.no_cols[2 * r + 1], (unsigned long long)c_multi);
     if (c_eq) atomicAdd((unsigned long long *)&v.n_equal_xi[r], (unsigned long long)c_eq);
+    if (v.col_cost) {
+      const unsigned long long dt = wall_clock64() - t_cell;
+      v.col_cost[gc] = dt < 0xFFFFFFFFull ? (uint32_t)dt : 0xFFFFFFFFu;
+    }
   }
 }
 
-/* main pass: one block per cell or persistent blocks over the work list; overflow pass: the queued cells,
+/* main pass: one block per cell or persistent blocks over the work list; overflow passes: the queued cells,
    grid-strided */
 template <int CAP, bool TAB, bool INJ>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_collision(KView v, int mode, int chunk) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_collision(KView v, int mode, int chunk,
+                                                                                           ColLists io) {
   CellSource src = cell_source(mode, chunk, 0);
   for (;;) {
-    const int gc = next_cell(v, src, v.ovf_cnt);
+    const int gc = next_cell(v, src, io.in_list, io.in_cnt);
     if (gc < 0) return;
-    collide_cell<CAP, TAB, INJ>(v, gc, mode == PASS_OVF);
+    collide_cell<CAP, TAB, INJ>(v, gc, io);
   }
 }
 
@@ -1840,7 +2121,7 @@ template <int CAP>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_dml(KView v, int mode, int chunk) {
   CellSource src = cell_source(mode, chunk, 1);
   for (;;) {
-    const int gc = next_cell(v, src, v.ovf_cnt + 1);
+    const int gc = next_cell(v, src, v.ovf_list, v.ovf_cnt + 1);
     if (gc < 0) return;
     dml_cell<CAP>(v, gc, mode == PASS_OVF);
   }
@@ -2227,31 +2508,69 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(int32_t *a, const int32
   }
 }
 
+#define WL_BLOCK 1024
+#define WL_PER 4
+
 /*
- * Work list of the persistent per-cell passes: the global cells ordered by
- * decreasing home-segment size (the member count up to drift), so the most
- * expensive cells start first and the small ones fill the tail.  Counting
- * sort over LM_WORK_BINS size bins: histogram, exclusive scan, scatter.  The
- * order within a bin follows the atomics; any order is a valid list.
+ * Work list of the persistent per-cell passes: the global cells ordered by decreasing home-segment size (the
+ * member count up to drift), so the larger cells start first and the small ones fill the tail.  With col_cost
+ * kept, the cells whose last collision step took more than half the expected length of a persistent pass
+ * (sum of all cells' costs over the workers, thr) come first, by decreasing cost (log-scale bins, 16 per
+ * octave): a cell that long has to start at once, while cells well below it keep the SIMDs issuing best when
+ * mixed (they are latency-bound when many events, throughput-bound otherwise).  Counting sort over
+ * LM_WORK_BINS bins (the lower half for costs, the upper half for sizes): histogram, exclusive scan, scatter.
+ * The order within a bin follows the atomics; any order is a valid list.
  */
-__device__ static inline int wl_bin(const KView &v, int gc) {
+__device__ static inline int wl_bin(const KView &v, int gc, unsigned long long thr) {
+  if (v.col_cost) {
+    const uint32_t t = v.col_cost[gc];
+    if (t > thr) {
+      const int lg = 31 - __clz((int)t); /* t > thr >= 16: 4 .. 31 */
+      const int b = (lg - 3) * 16 + (int)((t >> (lg - 4)) & 15u); /* 16 .. 463 */
+      return LM_WORK_BINS / 2 - 1 - (b < LM_WORK_BINS / 2 - 1 ? b : LM_WORK_BINS / 2 - 1);
+    }
+  }
   const int r = gc / v.nc, c = gc - r * v.nc;
   const int32_t *s = v.seg + (size_t)r * (v.nc + 1);
   const int len = s[c + 1] - s[c];
-  return LM_WORK_BINS - 1 - (len < LM_WORK_BINS - 1 ? len : LM_WORK_BINS - 1);
+  const int span = LM_WORK_BINS - 1 - (v.col_cost ? LM_WORK_BINS / 2 : 0); /* size bins: [1023 - span, 1023] */
+  return LM_WORK_BINS - 1 - (len < span ? len : span);
+}
+
+/* sum of the cells' last collision costs (ticks) into *sum */
+__global__ __launch_bounds__(WL_BLOCK) void k_wl_cost_sum(KView v, unsigned long long *sum) {
+  __shared__ unsigned long long part[WL_BLOCK / WAVE];
+  const int n = v.n_runs * v.nc;
+  unsigned long long a = 0;
+  for (int gc = (int)(blockIdx.x * WL_BLOCK + threadIdx.x); gc < n; gc += (int)(gridDim.x * WL_BLOCK))
+    a += v.col_cost[gc];
+  for (int off = WAVE / 2; off > 0; off >>= 1) a += __shfl_down(a, off);
+  if ((threadIdx.x & (WAVE - 1)) == 0) part[threadIdx.x / WAVE] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < WL_BLOCK / WAVE; ++w) t += part[w];
+    atomicAdd(sum, t);
+  }
 }
 
 /* per-block histograms in LDS: one global atomic per (block, bin) instead of one per cell */
-#define WL_BLOCK 1024
-#define WL_PER 4
-__global__ __launch_bounds__(WL_BLOCK) void k_wl_hist(KView v, int32_t *hist) {
+/* the threshold of the cost-ordered head of the list: half the expected length of a persistent collision pass
+   over `workers` workgroups (no head: ~0) */
+__device__ static inline unsigned long long wl_thr(const unsigned long long *sum, long long workers) {
+  return (sum && workers > 0) ? *sum / (unsigned long long)(2 * workers) : ~0ull;
+}
+
+__global__ __launch_bounds__(WL_BLOCK) void k_wl_hist(KView v, int32_t *hist, const unsigned long long *sum,
+                                                      long long workers) {
+  const unsigned long long thr = wl_thr(sum, workers);
   __shared__ int h[LM_WORK_BINS];
   for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK) h[b] = 0;
   __syncthreads();
   const int n = v.n_runs * v.nc;
   for (int q = 0; q < WL_PER; ++q) {
     const int gc = (int)blockIdx.x * WL_BLOCK * WL_PER + q * WL_BLOCK + (int)threadIdx.x;
-    if (gc < n) atomicAdd(&h[wl_bin(v, gc)], 1);
+    if (gc < n) atomicAdd(&h[wl_bin(v, gc, thr)], 1);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK)
@@ -2275,7 +2594,9 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_wl_scan(int32_t *hist) {
 }
 
 /* hist holds each bin's first slot; a block reserves one range per bin and places its cells by local rank */
-__global__ __launch_bounds__(WL_BLOCK) void k_wl_scatter(KView v, int32_t *hist, int32_t *work) {
+__global__ __launch_bounds__(WL_BLOCK) void k_wl_scatter(KView v, int32_t *hist, int32_t *work,
+                                                         const unsigned long long *sum, long long workers) {
+  const unsigned long long thr = wl_thr(sum, workers);
   __shared__ int h[LM_WORK_BINS];
   for (int b = threadIdx.x; b < LM_WORK_BINS; b += WL_BLOCK) h[b] = 0;
   __syncthreads();
@@ -2283,7 +2604,7 @@ __global__ __launch_bounds__(WL_BLOCK) void k_wl_scatter(KView v, int32_t *hist,
   int bin[WL_PER], rk[WL_PER];
   for (int q = 0; q < WL_PER; ++q) {
     const int gc = (int)blockIdx.x * WL_BLOCK * WL_PER + q * WL_BLOCK + (int)threadIdx.x;
-    bin[q] = gc < n ? wl_bin(v, gc) : 0;
+    bin[q] = gc < n ? wl_bin(v, gc, thr) : 0;
     rk[q] = gc < n ? atomicAdd(&h[bin[q]], 1) : 0;
   }
   __syncthreads();
@@ -2292,7 +2613,9 @@ __global__ __launch_bounds__(WL_BLOCK) void k_wl_scatter(KView v, int32_t *hist,
   __syncthreads();
   for (int q = 0; q < WL_PER; ++q) {
     const int gc = (int)blockIdx.x * WL_BLOCK * WL_PER + q * WL_BLOCK + (int)threadIdx.x;
-    if (gc < n) work[h[bin[q]] + rk[q]] = gc;
+    if (gc < n) {
+      work[h[bin[q]] + rk[q]] = gc;
+    }
   }
 }
 
@@ -2970,6 +3293,8 @@ static int check_engine(const LmEngine *e) {
   if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
   if (e->mig_cnt && (!e->mig_slot || e->mig_cap < 1 || e->mig_cap > LM_MIG_CAP_MAX))
     return set_err(LM_ERR_ARG, "migrant lists need mig_slot and mig_cap in 1..LM_MIG_CAP_MAX (64)");
+  if (e->spec_events < 0 || e->spec_events > LM_SPEC_EVENTS_MAX)
+    return set_err(LM_ERR_ARG, "spec_events must be 0 (default) or 1..LM_SPEC_EVENTS_MAX (1024)");
   if (e->rem_cap < 1 || e->rem_cap > LM_REM_CAP_MAX || !e->rem_slot || !e->rem_cnt)
     return set_err(LM_ERR_ARG, "rem_cap must be in 1..LM_REM_CAP_MAX (1024) with rem_slot and rem_cnt set");
   if (e->part_block < 0 || e->part_block > PART_BLOCK_MAX || (e->part_block % WAVE) || e->part_spt < 0 ||
@@ -3040,16 +3365,16 @@ static int main_pass(const LmEngine *e, int wpc, int chunk, int ctr, dim3 *grid,
 #include "lm_init.h"
 
 template <int CAP>
-static void launch_collision(const KView &v, const LmEngine *e, dim3 grid, hipStream_t st, int mode, int chunk) {
-  const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
+static void launch_collision(const KView &v, bool tab, bool inj, dim3 grid, hipStream_t st, int mode, int chunk,
+                             const ColLists &io) {
   if (tab && inj)
-    hipLaunchKernelGGL((k_collision<CAP, true, true>), grid, dim3(WAVE), 0, st, v, mode, chunk);
+    hipLaunchKernelGGL((k_collision<CAP, true, true>), grid, dim3(WAVE), 0, st, v, mode, chunk, io);
   else if (tab)
-    hipLaunchKernelGGL((k_collision<CAP, true, false>), grid, dim3(WAVE), 0, st, v, mode, chunk);
+    hipLaunchKernelGGL((k_collision<CAP, true, false>), grid, dim3(WAVE), 0, st, v, mode, chunk, io);
   else if (inj)
-    hipLaunchKernelGGL((k_collision<CAP, false, true>), grid, dim3(WAVE), 0, st, v, mode, chunk);
+    hipLaunchKernelGGL((k_collision<CAP, false, true>), grid, dim3(WAVE), 0, st, v, mode, chunk, io);
   else
-    hipLaunchKernelGGL((k_collision<CAP, false, false>), grid, dim3(WAVE), 0, st, v, mode, chunk);
+    hipLaunchKernelGGL((k_collision<CAP, false, false>), grid, dim3(WAVE), 0, st, v, mode, chunk, io);
 }
 
 extern "C" {
@@ -3078,6 +3403,8 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, work), (int64_t)offsetof(LmEngine, dml_work_chunk),
       (int64_t)offsetof(LmEngine, part_wpe), (int64_t)offsetof(LmEngine, mig_cnt),
       (int64_t)offsetof(LmEngine, mig_cap),
+      (int64_t)offsetof(LmEngine, spec_events),
+      (int64_t)offsetof(LmEngine, col_cost),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -3127,23 +3454,38 @@ int lm_collision_step(LmEngine *e, void *stream) {
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, S(stream), (long long *)e->rnd_off, ncell);
     LAUNCH_CHECK();
   }
+  /* list A: cells over the main pass's capacity; list B (second half of ovf_list, counter ovf_cnt[2]): cells
+     over 256 members, after a main pass below 256 */
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt, 0, 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 2, 0, 1);
   KView v = make_view(e);
+  const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
+  /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
+     small capacity keeps the LDS footprint, and so the occupancy, low; larger cells go to the overflow passes */
+  const int cap = e->col_cap > 0 ? e->col_cap : cell_capacity(e);
+  const ColLists main_io = {nullptr, nullptr, e->ovf_list, e->ovf_cnt};
   {
     ProfScope ps(e, 1, S(stream));
-    /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
-       small capacity keeps the LDS footprint, and so the occupancy, low; larger cells go to the overflow pass */
-    const int cap = e->col_cap > 0 ? e->col_cap : cell_capacity(e);
     dim3 grid;
     const int mode = main_pass(e, e->col_work_per_cu, e->col_work_chunk, 0, &grid, S(stream));
     if (cap <= 160)
-      launch_collision<160>(v, e, grid, S(stream), mode, e->col_work_chunk);
+      launch_collision<160>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
     else if (cap <= 256)
-      launch_collision<256>(v, e, grid, S(stream), mode, e->col_work_chunk);
+      launch_collision<256>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
     else
-      launch_collision<512>(v, e, grid, S(stream), mode, e->col_work_chunk);
+      launch_collision<512>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
   }
-  launch_collision<1024>(v, e, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), PASS_OVF, 0);
+  ColLists last_io = {e->ovf_list, e->ovf_cnt, nullptr, nullptr};
+  if (cap <= 160) {
+    /* the 256-member pass over list A at its own occupancy (a few resident workgroups per compute unit,
+       grid-strided), the larger cells on to list B */
+    const ColLists mid_io = {e->ovf_list, e->ovf_cnt, e->ovf_list + ncell, e->ovf_cnt + 2};
+    const long long gm = 12LL * device_cus();
+    launch_collision<256>(v, tab, inj, dim3((unsigned)(gm < ncell ? gm : ncell)), S(stream), PASS_OVF, 0, mid_io);
+    last_io.in_list = e->ovf_list + ncell;
+    last_io.in_cnt = e->ovf_cnt + 2;
+  }
+  launch_collision<1024>(v, tab, inj, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), PASS_OVF, 0, last_io);
   LAUNCH_CHECK();
   if (e->ev_mode == 1) {
     /* the step's records in (cell, seq) order for the replay and the reverse sweep */
@@ -3181,10 +3523,20 @@ int lm_build_work_list(LmEngine *e, void *stream) {
   const int ncell = e->n_runs * e->nc;
   int32_t *hist = e->work + ncell;
   const dim3 g((unsigned)((ncell + WL_BLOCK * WL_PER - 1) / (WL_BLOCK * WL_PER)));
+  /* the cost sum: an 8-byte aligned slot after the two work counters */
+  unsigned long long *sum = nullptr;
+  long long workers = 0;
+  if (e->col_cost && e->col_work_per_cu > 0) {
+    sum = (unsigned long long *)(e->work + (((size_t)ncell + LM_WORK_BINS + 2 + 1) & ~(size_t)1));
+    const long long gw = (long long)e->col_work_per_cu * device_cus();
+    workers = gw < ncell ? gw : ncell;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), (int32_t *)sum, 0, 2);
+    hipLaunchKernelGGL(k_wl_cost_sum, dim3(g.x), dim3(WL_BLOCK), 0, S(stream), w, sum);
+  }
   hipLaunchKernelGGL(k_fill_i32, dim3(LM_WORK_BINS / 256), dim3(256), 0, S(stream), hist, 0, LM_WORK_BINS);
-  hipLaunchKernelGGL(k_wl_hist, g, dim3(WL_BLOCK), 0, S(stream), w, hist);
+  hipLaunchKernelGGL(k_wl_hist, g, dim3(WL_BLOCK), 0, S(stream), w, hist, sum, workers);
   hipLaunchKernelGGL(k_wl_scan, dim3(1), dim3(SCAN_BLOCK), 0, S(stream), hist);
-  hipLaunchKernelGGL(k_wl_scatter, g, dim3(WL_BLOCK), 0, S(stream), w, hist, e->work);
+  hipLaunchKernelGGL(k_wl_scatter, g, dim3(WL_BLOCK), 0, S(stream), w, hist, e->work, sum, workers);
   LAUNCH_CHECK();
   return 0;
 }
@@ -3741,23 +4093,9 @@ static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w
   }
   const bool tab = E != nullptr, inj = rnd_stream != nullptr;
   /* main pass at the 256-member capacity, larger cells queued for the 1024-member pass */
-  if (tab && inj)
-    hipLaunchKernelGGL((k_collision<256, true, true>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
-  else if (tab)
-    hipLaunchKernelGGL((k_collision<256, true, false>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
-  else if (inj)
-    hipLaunchKernelGGL((k_collision<256, false, true>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
-  else
-    hipLaunchKernelGGL((k_collision<256, false, false>), dim3(nc), dim3(WAVE), 0, st, v, PASS_CELLS, 0);
-  const dim3 go(nc < OVF_GRID ? nc : OVF_GRID);
-  if (tab && inj)
-    hipLaunchKernelGGL((k_collision<1024, true, true>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
-  else if (tab)
-    hipLaunchKernelGGL((k_collision<1024, true, false>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
-  else if (inj)
-    hipLaunchKernelGGL((k_collision<1024, false, true>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
-  else
-    hipLaunchKernelGGL((k_collision<1024, false, false>), go, dim3(WAVE), 0, st, v, PASS_OVF, 0);
+  const ColLists main_io = {nullptr, nullptr, w.ovf_list, w.small}, ovf_io = {w.ovf_list, w.small, nullptr, nullptr};
+  launch_collision<256>(v, tab, inj, dim3(nc), st, PASS_CELLS, 0, main_io);
+  launch_collision<1024>(v, tab, inj, dim3(nc < OVF_GRID ? nc : OVF_GRID), st, PASS_OVF, 0, ovf_io);
   LAUNCH_CHECK();
   return 0;
 }

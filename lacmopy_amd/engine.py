@@ -9,6 +9,8 @@ export.
 """
 import ctypes as C
 import math
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -89,12 +91,13 @@ class Engine:
         mk("scan_buf", (L + 1,), i32)
         mk("scan_part", ((L + 4095) // 4096 + 1,), i32)
         mk("err", (4,), i32)
-        mk("ovf_list", (nr * nc,), i32)
-        mk("ovf_cnt", (2,), i32)
+        mk("ovf_list", (2 * nr * nc,), i32)
+        mk("ovf_cnt", (3,), i32)
         mk("rnd_off", (nr * nc + 2,), i64)
         # (run, cell) work list of the persistent passes (rebuilt by every reindex while one is on; starts as the
         # identity order), its size-bin scratch and two work counters
-        mk("work", (nr * nc + WORK_BINS + 2,), i32)
+        mk("work", (nr * nc + WORK_BINS + 6,), i32)
+        mk("col_cost", (nr * nc,), i32)  # last collision step per cell (ticks): the work list's order
         self.buf["work"][:nr * nc].copy_(torch.arange(nr * nc, dtype=torch.int32))
         self.E_tab = None
         self.rnd_stream = None
@@ -112,7 +115,7 @@ class Engine:
                 getattr(e, f)[b] = self.buf[f"{f}{b}"].data_ptr()
         for f in ("dmxi", "away", "col_step_dev", "scal", "mat", "cc", "dTh", "dRv", "dml", "grid_vel", "flux", "prof_rv",
                   "prof_Th", "t_relax", "colsum", "no_cols", "water_removed", "n_equal_xi", "rem_cnt",
-                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off", "work", "mig_cnt",
+                  "rem_slot", "scan_buf", "scan_part", "err", "ovf_list", "ovf_cnt", "rnd_off", "work", "mig_cnt", "col_cost",
                   "mig_slot"):
             setattr(e, f, self.buf[f].data_ptr())
         e.rem_cap = int(rem_cap)
@@ -279,7 +282,9 @@ class Engine:
         check(self.L.lm_reindex(C.byref(self.e), self.stream))
 
     def build_work_list(self):
-        """Orders the (run, cell) work list of the persistent passes by the current home-segment sizes."""
+        """Orders the (run, cell) work list of the persistent passes by decreasing cost: each cell's last
+        collision step (the col_cost buffer, kept unless LmEngine.col_cost is set to None) or else its current
+        home-segment size."""
         check(self.L.lm_build_work_list(C.byref(self.e), self.stream))
 
     def advection_increments(self, second):
@@ -570,10 +575,14 @@ def host_layout(runs, nx, nz, n_per_run):
     hid = np.zeros(n, dtype=np.int32)
     hcell = np.zeros(n, dtype=np.int32)
     seg = np.zeros(len(runs) * (nc + 1) + 1, dtype=np.int32)
-    base = 0
-    for r, (pos, vel, cells, m_w, m_s, xi, active) in enumerate(runs):
+    bases = np.concatenate([[0], np.cumsum(np.asarray(n_per_run, dtype=np.int64))])
+
+    def one(r):
+        pos, vel, cells, m_w, m_s, xi, active = runs[r]
+        base = int(bases[r])
         N = int(n_per_run[r])
-        assert len(xi) == N, "SIP count differs from the engine's allocation"
+        if len(xi) != N:
+            raise ValueError("SIP count differs from the engine's allocation")
         active = np.asarray(active, dtype=bool)
         cells = np.asarray(cells, dtype=np.int64)
         ids = np.arange(N)
@@ -597,7 +606,11 @@ def host_layout(runs, nx, nz, n_per_run):
         hcell[base + len(order):base + N] = -(2 + cells[0][dead])
         counts = np.bincount(key[active], minlength=nc)
         seg[r * (nc + 1):r * (nc + 1) + nc + 1] = base + np.concatenate([[0], np.cumsum(counts)])
-        base += N
+
+    # runs fill disjoint slices: laid out on host threads
+    with ThreadPoolExecutor(max(1, min(16, len(runs), os.cpu_count() or 1))) as ex:
+        for f in [ex.submit(one, r) for r in range(len(runs))]:
+            f.result()
     seg[-1] = n_tot
     return host, hid, hcell, seg
 

@@ -7,7 +7,7 @@ ID order) and every FP64 state value must be identical.
 import numpy as np
 import pytest
 
-from lacmopy_amd import load_kernel_data_Ecol
+from lacmopy_amd import load_kernel_data_Ecol, rng
 from lacmopy_amd.setup_case import init_profiles, make_case
 from oracle import oracle as orc
 
@@ -189,10 +189,12 @@ def test_batched_runs_match_single(gpu):
     assert nc1.sum() > 0
 
 
+@pytest.mark.parametrize("spec_events", [0, 2])
 @pytest.mark.parametrize("kernel", ["Long_Bott", "const"])
-def test_collision_many_events(gpu, kernel):
-    """Dense rain-like cells (many events per cell): exercises the first-event passes and the
-    anti-diagonal continuation of the collision kernel; Philox draws, bit-exact vs the oracle."""
+def test_collision_many_events(gpu, kernel, spec_events):
+    """Dense rain-like cells (many events per cell): exercises the candidate-queue passes and the
+    anti-diagonal continuation of the collision kernel (spec_events = 2: after the second event of a cell);
+    Philox draws, bit-exact vs the oracle."""
     if kernel == "const":
         E, lnR0, lnf, nb = 0.9, 0.0, 1.0, 0
     else:
@@ -207,6 +209,7 @@ def test_collision_many_events(gpu, kernel):
     parts = tuple(parts)
     pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
     eng = _engine(grid, parts, cfg, E, lnR0, lnf, 99)
+    eng.e.spec_events = spec_events
     no_cols = np.zeros(2, dtype=np.int64)
     for step in range(3):
         orc.collision_all_cells(xi, m_w, m_s, vel, grid.temperature.copy(), cells, grid.no_cells,
@@ -302,30 +305,50 @@ def test_one_crowded_cell_takes_the_overflow_passes(gpu):
     _assert_same(eng.counters(0)[0], no_cols, "no_cols")
 
 
+@pytest.mark.parametrize("order", ["cost", "size"])
 @pytest.mark.parametrize("work_per_cu,chunk", [(1, 0), (1, 1), (4, 3)])
-def test_persistent_work_list_passes_match_oracle(gpu, work_per_cu, chunk):
+def test_persistent_work_list_passes_match_oracle(gpu, work_per_cu, chunk, order):
     """Persistent collision and Delta m_l passes over the (run, cell) work list: 6 runs of 8x8 cells (384
     work items, more than one per workgroup at one workgroup per compute unit); run 0 stays bit-exact with
-    the oracle over three advection steps, the other runs equal the one-workgroup-per-cell launch, and
-    after each reindex the work list is a permutation of the cells ordered by decreasing home-segment size."""
+    the oracle over three advection steps, the other runs equal the one-workgroup-per-cell launch, and the
+    work list is a permutation of the cells ordered by decreasing cost: the last collision step's ticks
+    per cell for the cells above half the expected pass length (col_cost kept; monotone up to the bins of 16
+    per octave), then the home-segment size."""
     E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
     cfg = _config(E, lnR0, lnf, nb)
     grid, *parts = _case(nx=8, nz=8, spc=24, seed=23)
     parts = _prep(grid, parts)
     ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 3, 3)
     eng = _engine(grid, parts, cfg, E, lnR0, lnf, 3, n_runs=6)
+    if order == "size":
+        eng.e.col_cost = None
     eng.set_launch_shape(0, 0, 0, work_per_cu, work_per_cu, chunk, chunk)
     base = _engine(grid, parts, cfg, E, lnR0, lnf, 3, n_runs=6)
     ncell = 6 * 64
     for step in range(3):
         eng.adv_steps(1)
         base.adv_steps(1)
+        if order == "cost":
+            eng.build_work_list()  # from the costs of the step's collision passes, as they are now
         work = eng.buf["work"][:ncell].cpu().numpy()
         assert np.array_equal(np.sort(work), np.arange(ncell)), "work list is not a permutation of the cells"
         seg = eng.buf[f"seg{int(eng.e.cur)}"].cpu().numpy()
         r, c = work // 64, work % 64
         size = seg[r * 65 + c + 1] - seg[r * 65 + c]
-        assert np.all(np.diff(size) <= 0), "work list not ordered by decreasing home-segment size"
+        if order == "size":
+            assert np.all(np.diff(size) <= 0), "work list not ordered by decreasing home-segment size"
+        else:
+            cost = eng.buf["col_cost"].cpu().numpy().view(np.uint32).astype(np.float64)[work]
+            assert cost.max() > 0, "no collision costs recorded"
+            # a head of the cells above a cost threshold, by decreasing cost (bins of 16 per octave), then the
+            # rest by decreasing size: some split k has every head cost above every tail cost
+            head_min = np.minimum.accumulate(cost)
+            tail_max = np.maximum.accumulate(cost[::-1])[::-1]
+            ok = [k for k in range(len(cost) + 1)
+                  if (k == 0 or k == len(cost) or head_min[k - 1] > tail_max[k])
+                  and np.all(np.diff(size[k:]) <= 0)
+                  and np.all(cost[1:k] <= cost[:k - 1] * (17.0 / 16.0) + 16.0)]
+            assert ok, "work list is not a cost-ordered head followed by a size-ordered tail"
     for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
         _assert_same(a, b, nm)
     s, m = eng.export_grid(0)
@@ -369,3 +392,78 @@ def test_launch_shapes_and_autotune_leave_results_unchanged(gpu):
     ch = res["chosen"]
     assert ch["part_block"] in (64, 128, 256, 512) and ch["part_spt"] in (1, 2)
     assert ch["col_work_per_cu"] in (0, 16, 32) and ch["dml_work_per_cu"] in (0, 32, 64)
+
+
+def _with_large_drops(grid, parts, seed, per_cell=(0, 7), R_um=(60.0, 320.0)):
+    """Turns a few SIPs of every cell into drizzle drops (radius R_um, multiplicity / 200, falling at roughly
+    their terminal speed): every row bound of the cell widens, as in the collision-active state."""
+    pos, vel, cells, m_w, m_s, xi, act = [np.array(a, copy=True) for a in parts]
+    rs = np.random.default_rng(seed)
+    key = cells[0] * int(grid.no_cells[1]) + cells[1]
+    for c in np.unique(key[act]):
+        mem = np.flatnonzero((key == c) & act)
+        k = min(int(rs.integers(per_cell[0], per_cell[1] + 1)), len(mem) // 4)
+        pick = rs.choice(mem, k, replace=False)
+        R = rs.uniform(R_um[0], R_um[1], k)
+        m_w[pick] = 4.0 / 3.0 * np.pi * 1000.0 * (R * 1.0E-6) ** 3 * 1.0E18
+        xi[pick] = xi[pick] / 200.0
+        vel[1, pick] -= 0.008 * R
+    return pos, vel, cells, m_w, m_s, xi, act
+
+
+@pytest.mark.parametrize("spec_events", [0, 3])
+@pytest.mark.parametrize("kernel", ["Long_Bott", "Hall_Bott"])
+def test_cells_with_large_drops_match_oracle(gpu, kernel, spec_events):
+    """Collision-active cells (~90 members, up to 7 drops of 60-320 um among cloud droplets): the guard pass
+    leaves the drops' pairs to the per-pair pass, the candidate queue is carried from event to event, and
+    cells past spec_events events (3: most event cells) finish with the filtered anti-diagonal schedule.  The
+    all-cells drop-in (reference layout, member lists) for one collision step and the engine for two advection
+    steps (autotuned capacities) stay bit-identical to the oracle."""
+    from lacmopy_amd.collision.all_or_nothing import collision_step_Ecol_grid_R_all_cells_2D_multicomp_np as step
+    E, lnR0, lnf, nb = load_kernel_data_Ecol(kernel)
+    grid, *parts = _case(nx=6, nz=6, spc=90, seed=23, R_drop=12.0, sigma_drop=1.3)
+    parts = _prep(grid, parts)
+    parts = _with_large_drops(grid, parts, 5)
+    pos, vel, cells, m_w, m_s, xi, act = parts
+    T = grid.temperature.copy()
+    o_xi, o_mw, o_ms, o_nc = xi.copy(), m_w.copy(), m_s.copy(), np.zeros(2, dtype=np.int64)
+    orc.collision_all_cells(o_xi, o_mw, o_ms, vel, T, cells, grid.no_cells, 0.5 / grid.volume_cell, E, nb, lnR0, lnf,
+                            o_nc, seed=3, run=0, step=1)
+    assert o_nc[0] > 200  # many events: several cells take the anti-diagonal schedule
+    g_xi, g_mw, g_ms, g_nc = xi.copy(), m_w.copy(), m_s.copy(), np.zeros(2, dtype=np.int64)
+    rng.seed(3, 1)
+    step(g_xi, g_mw, g_ms, vel, T, cells, grid.no_cells, 0.5 / grid.volume_cell, E, nb, lnR0, lnf, g_nc, "AS")
+    _assert_same(g_xi, o_xi, "xi (one collision step)")
+    _assert_same(g_mw, o_mw, "m_w (one collision step)")
+    _assert_same(g_ms, o_ms, "m_s (one collision step)")
+    _assert_same(g_nc, o_nc, "no_cols (one collision step)")
+    cfg = _config(E, lnR0, lnf, nb)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 6, 2)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 6)
+    eng.e.spec_events = spec_events
+    eng.autotune()
+    eng.adv_steps(2)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, nm)
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    assert no_cols[0] > 400
+
+
+@pytest.mark.parametrize("spc", [200, 300])
+def test_collision_capacity_tiers_match_oracle(gpu, spc):
+    """Main collision pass at 160 members: cells of ~200 members go to the 256-member pass, cells of ~300 on to
+    the 1024-member pass; one advection step stays bit-identical to the oracle."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(nx=4, nz=4, spc=spc, seed=31)
+    parts = _prep(grid, parts)
+    counts = np.bincount(parts[2][0] * 4 + parts[2][1], minlength=16)
+    assert counts.min() > 160 and (counts.max() <= 256 if spc == 200 else counts.min() > 256)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 17, 1)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 17)
+    eng.set_launch_shape(col_cap=160)
+    eng.adv_steps(1)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, nm)
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    assert no_cols.sum() > 0
