@@ -143,7 +143,7 @@ typedef struct LmEngine {
   /*
    * Launch shape (0 = default; set by an autotuner, e.g. lacmopy_amd.Engine.autotune): threads per
    * block (64..512, a multiple of 64) and SIPs per thread (1 or 2) of the particle kernel, and the
-   * member capacity of the main collision pass (160, 256 or 512; larger cells take the overflow pass).
+   * member capacity of the main collision pass (160, 256, 320 or 512; larger cells take the overflow pass).
    */
   int32_t part_block, part_spt, col_cap;
   int32_t tape_mode; /* taped substep of the adjoint: 0 = two passes, (T, p, S) then (m_w, m_s); 1 = one pass */

@@ -817,6 +817,14 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
  * larger B only rejects less); NaN or inf never rejects.  X only decreases
  * with events; Rc can grow, so the bounds are recomputed after each event.
  */
+/*
+ * Quads the candidate queue of a pass of member capacity CAP holds (entries lane + e * WAVE of the evaluating
+ * wave, e < qcap_of(CAP) / WAVE).  A queue must take at least the quads the guard pass can never reject --
+ * those with a zero byte, 1 - (255/256)^4 = 1.55% of a cell's n^2/8 quads, 127 at 256 members -- or every
+ * cell of that size overflows into the anti-diagonal schedule.
+ */
+__host__ __device__ constexpr int qcap_of(int cap) { return cap <= 160 ? 2 * WAVE : (cap <= 256 ? 4 * WAVE : 8 * WAVE); }
+
 /* Largest multiplicity and radius and the velocity ranges over the members outside big[0..m) (wave-wide). */
 struct CellMax {
   double X, Rc, ulo, uhi, wlo, whi;
@@ -904,7 +912,7 @@ __device__ static inline double row_kept(int p, int n, int st) {
  * list sBig[0..*sNbig): a few members of the largest radii (drops among cloud droplets), whose pairs the
  * per-pair pass tests instead.  One large drop widens the radius and velocity ranges of the whole cell and
  * with them every row's bound; the list is used when the expected work of the guard pass's queue drops by
- * more than the per-pair tests cost (quads of nq_all in total; a queue over QCAP quads costs a pass over all).
+ * more than the per-pair tests cost (quads of nq_all in total; a queue over qcap_of(CAP) quads costs a pass over all).
  * Any list is exact: every pair is covered by its row's bound or by the per-pair pass.
  */
 template <int CAP>
@@ -924,6 +932,7 @@ __device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n
   }
   for (int off = WAVE / 2; off > 0; off >>= 1) kept0 += __shfl_xor(kept0, off);
   const double over = 1.25 * nq_all; /* a queue overflow: the anti-diagonal schedule over all pairs */
+  const double q_full = 0.8 * qcap_of(CAP); /* expected kept quads that risk an overflow */
   /* the floor: quads holding a zero byte, which no bound rejects (1 - (255/256)^4 of all) */
   const double floor_kept = 0.0155 * nq_all;
   int best_m = 0;
@@ -953,7 +962,7 @@ __device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n
       }
       __syncthreads();
     }
-    double best = kept0 + (kept0 > 100.0 ? over : 0.0);
+    double best = kept0 + (kept0 > q_full ? over : 0.0);
     for (int m = 2; m <= BIG_MAX; m *= 2) {
       const CellMax Mm = cell_max(n, sX, sR, sU, sW, sRank, m);
       double kept = 0.0;
@@ -965,7 +974,7 @@ __device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n
         kept += row_kept(p, n, st);
       }
       for (int off = WAVE / 2; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
-      const double cost = kept + 0.18 * n * m + (kept > 100.0 ? over : 0.0);
+      const double cost = kept + 0.18 * n * m + (kept > q_full ? over : 0.0);
       if (cost < best) {
         best = cost;
         best_m = m;
@@ -1011,9 +1020,7 @@ __device__ static inline bool pair_may_fire8(const LmPhys &P, double dtdV, doubl
 /* pair number of (i, j) in the reference's row-major loop */
 __device__ static inline int pair_cnt(int i, int j, int n) { return i * n - i * (i + 1) / 2 + (j - i - 1); }
 
-#define QCAP 128 /* quads the candidate queue holds: entries lane and lane + WAVE of the evaluating wave */
 #define KEY_STALE 0xFFFFFFFFFFFFFFFEull /* queue entry to evaluate (keys of real pairs are below 2^63) */
-static_assert(QCAP == 2 * WAVE, "the queue evaluation takes two entries per lane");
 
 /* State of one cell's collision step in LDS (id order) plus slot-indexed globals. */
 
@@ -1168,8 +1175,9 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   __shared__ int sIND[CAP], sSL[CAP], sQ[CAP + 1];
   __shared__ float sB[CAP];
   __shared__ unsigned char sTG[CAP];
-  __shared__ int sQueue[QCAP];
-  __shared__ unsigned long long sQkey[QCAP]; /* each entry's first firing pair (key), ~0: none, KEY_STALE */
+  constexpr int QC = qcap_of(CAP), NQ = QC / WAVE;
+  __shared__ int sQueue[QC];
+  __shared__ unsigned long long sQkey[QC]; /* each entry's first firing pair (key), ~0: none, KEY_STALE */
   __shared__ int sQn;
   __shared__ int sBig[BIG_MAX], sNbig;
   __shared__ unsigned char sIsBig[CAP]; /* rank in the list of largest radii the guard pass leaves out, or 0 */
@@ -1283,7 +1291,9 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   bool fresh = true; /* row bounds describe the current state (no event yet) */
   const int spec_events = v.spec_events > 0 ? v.spec_events : SPEC_MAX_EVENTS;
   while (n_events < spec_events) {
-    unsigned long long kq0 = ~0ull, kq1 = ~0ull; /* this pass's keys of queue entries lane, lane + 64 */
+    unsigned long long kq[NQ]; /* this pass's keys of queue entries lane + e * WAVE */
+#pragma unroll
+    for (int e = 0; e < NQ; ++e) kq[e] = ~0ull;
     int nq = 0;
     /* parallel pass: first firing pair with cnt >= k_next under the current state.  The first quad holding
        pair k_next: the pair after the last event (or pair 0); quad order is row-major pair order. */
@@ -1316,7 +1326,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
          with the guard pass, row-parallel (lane l takes rows l, l + 64, ...; per quad the guard word and an
          integer compare of its valid pairs' smallest leading byte with the row's bound sTG: no Philox, no
          memory access) and the per-pair pass over the pairs of the listed members; then kept across events
-         (an event changes two members only, see below).  A queue over QCAP quads hands the cell to the
+         (an event changes two members only, see below).  A queue over QC quads hands the cell to the
          anti-diagonal schedule. */
       if (build) {
         if (lane == 0) sQn = 0;
@@ -1361,10 +1371,10 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
                   keep = true; /* the bound rejects (almost) nothing: every quad holds a valid pair */
                 if (keep) {
                   const int pos = atomicAdd(&sQn, 1);
-                  if (pos < QCAP) {
-                  sQueue[pos] = (i << 16) | b;
-                  sQkey[pos] = KEY_STALE;
-                }
+                  if (pos < QC) {
+                    sQueue[pos] = (i << 16) | b;
+                    sQkey[pos] = KEY_STALE;
+                  }
                 }
               }
             }
@@ -1383,10 +1393,10 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
             const uint32_t G = lm_guard_byte(lm_guard_word(gkey, (uint32_t)i, (uint32_t)d), (uint32_t)d);
             if (pair_may_fire8(P, dtdV, Eb, sX, sR, sU, sW, i, j, G)) {
               const int pos = atomicAdd(&sQn, 1);
-              if (pos < QCAP) {
-              sQueue[pos] = (i << 16) | (d >> 2);
-              sQkey[pos] = KEY_STALE;
-            }
+              if (pos < QC) {
+                sQueue[pos] = (i << 16) | (d >> 2);
+                sQkey[pos] = KEY_STALE;
+              }
             }
           }
         }
@@ -1396,31 +1406,22 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
       nq = sQn;
       /* too many candidates (the bounds reject little and events are many): the anti-diagonal schedule, exact
          from any k_next, finishes the cell */
-      if (nq > QCAP) break;
-      {
-        /* each lane evaluates queue entries lane and lane + 64; the row filter of eval_quad needs bounds of
-           the current state (fresh: no event since the queue was built) */
-        if (lane < nq) {
-          kq0 = sQkey[lane];
-          if (kq0 == KEY_STALE) {
-            const int pk = sQueue[lane], qi = pk >> 16, qb = pk & 0xffff;
-            kq0 = eval_quad<CAP, TAB, INJ>(P, v, cc, qi, qb, k_next,
-                                           lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
-                                           key_step, key_cell, fresh);
-            sQkey[lane] = kq0;
+      if (nq > QC) break;
+      /* each lane evaluates queue entries lane + e * WAVE; the row filter of eval_quad needs bounds of the
+         current state (fresh: no event since the queue was built) */
+#pragma unroll
+      for (int e = 0; e < NQ; ++e) {
+        const int q = lane + e * WAVE;
+        if (q < nq) {
+          kq[e] = sQkey[q];
+          if (kq[e] == KEY_STALE) {
+            const int pk = sQueue[q], qi = pk >> 16, qb = pk & 0xffff;
+            kq[e] = eval_quad<CAP, TAB, INJ>(P, v, cc, qi, qb, k_next,
+                                             lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
+                                             key_step, key_cell, fresh);
+            sQkey[q] = kq[e];
           }
-          best = kq0 < best ? kq0 : best;
-        }
-        if (lane + WAVE < nq) {
-          kq1 = sQkey[lane + WAVE];
-          if (kq1 == KEY_STALE) {
-            const int pk = sQueue[lane + WAVE], qi = pk >> 16, qb = pk & 0xffff;
-            kq1 = eval_quad<CAP, TAB, INJ>(P, v, cc, qi, qb, k_next,
-                                           lm_guard_word(gkey, (uint32_t)qi, (uint32_t)(4 * qb)), roff, key_run,
-                                           key_step, key_cell, fresh);
-            sQkey[lane + WAVE] = kq1;
-          }
-          best = kq1 < best ? kq1 : best;
+          best = kq[e] < best ? kq[e] : best;
         }
       }
       __syncthreads();
@@ -1474,29 +1475,34 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
          * or ej).  The undecided pairs of ei and ej -- (ei, y) and (ej, y) for y > ej, (x, ej) for ei < x < ej --
          * are tested again with their guard bytes on the new state and their quads queued.
          */
-        const int pk0 = lane < nq ? sQueue[lane] : 0, pk1 = lane + WAVE < nq ? sQueue[lane + WAVE] : 0;
-        const bool al0 = (lane < nq) & (kq0 != ~0ull), al1 = (lane + WAVE < nq) & (kq1 != ~0ull);
-        /* a kept quad's key stays exact unless the quad holds a pair of ei or ej (pairs (i, 4b - i .. 4b + 3 - i)):
-           its pairs before the key still do not fire and the key's pair still fires */
-        const int i_0 = pk0 >> 16, j_0 = 4 * (pk0 & 0xffff) - i_0, i_1 = pk1 >> 16, j_1 = 4 * (pk1 & 0xffff) - i_1;
-        const bool t0 = (i_0 == ei) | (i_0 == ej) | ((unsigned)(ei - j_0) < 4u) | ((unsigned)(ej - j_0) < 4u);
-        const bool t1 = (i_1 == ei) | (i_1 == ej) | ((unsigned)(ei - j_1) < 4u) | ((unsigned)(ej - j_1) < 4u);
-        const unsigned long long nk0 = t0 ? KEY_STALE : kq0, nk1 = t1 ? KEY_STALE : kq1;
-        const unsigned long long b0 = __ballot(al0), b1 = __ballot(al1);
+        int pk[NQ];
+        bool al[NQ];
+        unsigned long long bal[NQ];
         const unsigned long long below = (1ull << lane) - 1ull;
-        const int c0 = __popcll(b0);
+#pragma unroll
+        for (int e = 0; e < NQ; ++e) {
+          const int q = lane + e * WAVE;
+          pk[e] = q < nq ? sQueue[q] : 0;
+          al[e] = (q < nq) & (kq[e] != ~0ull);
+          /* a kept quad's key stays exact unless the quad holds a pair of ei or ej (pairs (i, 4b - i .. 4b + 3
+             - i)): its pairs before the key still do not fire and the key's pair still fires */
+          const int qi = pk[e] >> 16, qj = 4 * (pk[e] & 0xffff) - qi;
+          const bool t = (qi == ei) | (qi == ej) | ((unsigned)(ei - qj) < 4u) | ((unsigned)(ej - qj) < 4u);
+          if (t) kq[e] = KEY_STALE;
+          bal[e] = __ballot(al[e]);
+        }
         __syncthreads();
-        if (al0) {
-          const int q = __popcll(b0 & below);
-          sQueue[q] = pk0;
-          sQkey[q] = nk0;
+        int kept_n = 0;
+#pragma unroll
+        for (int e = 0; e < NQ; ++e) {
+          if (al[e]) {
+            const int q = kept_n + __popcll(bal[e] & below);
+            sQueue[q] = pk[e];
+            sQkey[q] = kq[e];
+          }
+          kept_n += __popcll(bal[e]);
         }
-        if (al1) {
-          const int q = c0 + __popcll(b1 & below);
-          sQueue[q] = pk1;
-          sQkey[q] = nk1;
-        }
-        if (lane == 0) sQn = c0 + __popcll(b1);
+        if (lane == 0) sQn = kept_n;
         __syncthreads();
         const int la = n - 1 - ej, lb = ej - ei - 1;
         for (int t = lane; t < 2 * la + lb; t += WAVE) {
@@ -1515,7 +1521,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
           const uint32_t G = lm_guard_byte(lm_guard_word(gkey, (uint32_t)i, (uint32_t)d), (uint32_t)d);
           if (pair_may_fire8(P, dtdV, Eb, sX, sR, sU, sW, i, j, G)) {
             const int pos = atomicAdd(&sQn, 1);
-            if (pos < QCAP) {
+            if (pos < QC) {
               sQueue[pos] = (i << 16) | (d >> 2);
               sQkey[pos] = KEY_STALE;
             }
@@ -1523,7 +1529,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
         }
         __syncthreads();
         fresh = false;
-        if (sQn > QCAP) break; /* the anti-diagonal schedule takes over */
+        if (sQn > QC) break; /* the anti-diagonal schedule takes over */
       }
     }
   }
@@ -3298,9 +3304,10 @@ static int check_engine(const LmEngine *e) {
   if (e->rem_cap < 1 || e->rem_cap > LM_REM_CAP_MAX || !e->rem_slot || !e->rem_cnt)
     return set_err(LM_ERR_ARG, "rem_cap must be in 1..LM_REM_CAP_MAX (1024) with rem_slot and rem_cnt set");
   if (e->part_block < 0 || e->part_block > PART_BLOCK_MAX || (e->part_block % WAVE) || e->part_spt < 0 ||
-      e->part_spt > 2 || (e->col_cap && e->col_cap != 160 && e->col_cap != 256 && e->col_cap != 512))
+      e->part_spt > 2 ||
+      (e->col_cap && e->col_cap != 160 && e->col_cap != 256 && e->col_cap != 320 && e->col_cap != 512))
     return set_err(LM_ERR_ARG, "bad launch shape (block 64..512 in steps of 64, 1 or 2 SIPs per thread, "
-                               "collision capacity 160, 256 or 512)");
+                               "collision capacity 160, 256, 320 or 512)");
   if (e->col_work_per_cu < 0 || e->dml_work_per_cu < 0 || e->col_work_chunk < 0 || e->dml_work_chunk < 0 ||
       e->col_work_per_cu > 256 || e->dml_work_per_cu > 256 || e->col_work_chunk > 4096 || e->dml_work_chunk > 4096 ||
       (e->part_wpe && e->part_wpe != 5 && e->part_wpe != 6 && e->part_wpe != 8) || e->tape_mode < 0 ||
@@ -3472,6 +3479,8 @@ int lm_collision_step(LmEngine *e, void *stream) {
       launch_collision<160>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
     else if (cap <= 256)
       launch_collision<256>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
+    else if (cap <= 320)
+      launch_collision<320>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
     else
       launch_collision<512>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
   }

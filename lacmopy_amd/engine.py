@@ -400,7 +400,7 @@ class Engine:
     def launch_shape(self):
         return {k: int(getattr(self.e, k)) for k in self.SHAPE_KEYS}
 
-    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), wpes=(0, 5, 6, 8), col_caps=(160, 256, 512),
+    def autotune(self, blocks=(64, 128, 256, 512), spts=(1, 2), wpes=(0, 5, 6, 8), col_caps=(160, 256, 320, 512),
                  col_work=((0, 0), (16, 0), (16, 1), (16, 4), (32, 0), (32, 4)),
                  dml_work=((0, 0), (32, 0), (32, 16), (64, 0), (64, 16)), reps=3):
         """Times candidate launch shapes on the engine's current state with HIP events on its stream and
