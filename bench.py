@@ -16,7 +16,8 @@ torch.cuda.synchronize().  Legs (--legs, default all):
               64 runs, then 900 s with collisions (every run its own draws) before the timed steps;
               eager and graph, kernel times, and the plain run in the same state.  The headline `value`
               is this leg's graph-replay rate (the 64-seed batch in its stated regime, hipGraph step)
-  * adjoint:  cost of the adjoint window after the spin-up and its finite-difference check
+  * adjoint:  cost of the adjoint window after the spin-up and its finite-difference check (icmw75: from the
+              reference's SinSIP initial state computed on the GPU, mode-parameter gradient through its map)
   * box:      the Golovin box model, 50 seeds x 3600 s (the oracle's time for one box beside it)
   * stress:   the 150x150 Poisson(256) 16-seed batch
   * init:     GPU initial states of 64 seeds
@@ -223,19 +224,35 @@ def spin_up(eng, steps, collisions):
         f"{time.perf_counter() - t:.1f} s")
 
 
-def adjoint_cost(eng, cases, start, window, every, check=0, case="icmw75", spinup=0, spinup_collisions=False):
-    """Adjoint of a `window`-step window of one run of `case` from the state `start` (a snapshot after
-    `spinup` steps): wall time of the plain forward window, of the recording forward (events + checkpoints
-    every `every` steps) and of one gradient (taped re-runs + reverse sweep), for the cloud-droplet-number
-    objective.  Without collisions before the window the window-start multiplicities and dry masses are
-    still the SIP initialisation map's, and the gradient with respect to the six aerosol-mode parameters
-    follows by the chain rule through that map.  check > 0: also compare the gradient with central finite
-    differences on `check` random droplets (multiplicity, h = 1e-3 relative) and on the six aerosol-mode
-    parameters (fourth order, h = 1e-3)."""
+def sinsip_engine(seed, spc_modes=(52, 76), seed_sim=1001):
+    """One run of the reference's own initial state on 75x75 cells, computed on the GPU (lacmopy_amd.initialize,
+    init.py:439-1236) with SinSIP counts no_spcm = spc_modes (52, 76: about 128 SIPs per cell, twice the
+    configuration's (26, 38)), in an engine with the icmw75 step (Long table, dt 1 s, 10 substeps, 2 collision
+    steps).  Returns (engine, initial state, init config)."""
+    from lacmopy_amd import load_kernel_data_Ecol
+    from lacmopy_amd.engine import Engine
+    from lacmopy_amd.initialize import initialize_batch, reference_config
+    icfg = reference_config(no_spcm=np.array(spc_modes), seed_SIP_gen=seed)
+    st = initialize_batch(icfg, [seed])[0]
+    E, lnR0, lnf, _ = load_kernel_data_Ecol("Long_Bott")
+    c = CASES["icmw75"]
+    step = dict(solute_type="AS", dt_adv=c["dt"], dt_col=c["dt"] / 2, scale_dt_cond=c["sdc"], no_col_per_adv=2,
+                no_iter_impl_mass=3, g_set=9.80665, act_collisions=True, act_relaxation=True, seed_sim=seed_sim,
+                R_kernel_low_log=lnR0, bin_factor_R_log=lnf)
+    return Engine.from_initial_states([st], step, E), st, icfg
+
+
+def adjoint_cost(eng, start, window, every, mode_map=None, check=0, case="icmw75", state_desc=""):
+    """Adjoint of a `window`-step window of one run of `case` from the state `start`: wall time of the plain
+    forward window, of the recording forward (events + checkpoints every `every` steps) and of one gradient
+    (taped re-runs + reverse sweep), for the cloud-droplet-number objective.  mode_map(theta) -> (xi, m_s) by
+    SIP id: the initialisation map from the six aerosol-mode parameters to the window-start multiplicities and
+    dry masses (valid when nothing collided before the window); the gradient with respect to the parameters
+    follows by the chain rule through it.  check > 0: also compare the gradient with central differences of the
+    window on `check` random droplets (multiplicity, h = 1e-3 relative) and on the six parameters (h = 1e-7),
+    each run's cell membership fingerprinted after every step (adjoint.membership_fingerprint)."""
     import torch
     from lacmopy_amd.adjoint import AdjointWindow
-    from lacmopy_amd.setup_case import aerosol_sips
-    c = CASES[case]
     n1 = eng.n_tot
     eng.restore(start)
     eng.adv_steps(2)  # warm-up of the collision path
@@ -259,24 +276,21 @@ def adjoint_cost(eng, cases, start, window, every, check=0, case="icmw75", spinu
     lam_xi, lam_ms = w.gradient(w_nc=1.0, w_lwp=0.0)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    c = CASES[case]
     res = {"workload": f"one {case} run ({c['nx']}x{c['nz']} cells, {n1} SIPs), {window}-step window (dt "
-                       f"{c['dt']:g} s, {2 * c['sdc']} substeps, 2 collision steps) after a {spinup}-step spin-up "
-                       f"{'with' if spinup_collisions else 'without'} collisions; fixed collision events; "
-                       f"checkpoints every {every} steps",
+                       f"{c['dt']:g} s, {2 * c['sdc']} substeps, 2 collision steps) from {state_desc}; fixed "
+                       f"collision events; checkpoints every {every} steps",
            "sips": n1, "window_steps": window, "checkpoint_every": every, "events": w.n_events,
            "J_nc": float(J[0][0]), "J_lwp": float(J[1][0]),
            "forward_s": t_fwd, "record_s": t_rec, "gradient_s": t_grad,
            "cost_ratio": (t_rec + t_grad) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
            "tape_gb": sum(t.numel() * t.element_size() for t in w.tape.values()) / 1e9}
-    if not spinup_collisions:
-        # aerosol-mode parameters (N, r_med, sigma of both modes) -> window-start xi, m_s -> J_nc
+    if mode_map is not None:
         eng.restore(w.start)
         slots = w.slot_map()
         cur = int(eng.e.cur)
-        theta = torch.tensor([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6], dtype=torch.float64, device="cuda",
-                             requires_grad=True)
-        xi_m, ms_m = aerosol_sips(theta, cases[0][2])
-        # the map restated in torch rounds differently from make_case's numpy in the last bits
+        theta = torch.tensor(THETA0, dtype=torch.float64, device="cuda", requires_grad=True)
+        xi_m, ms_m = mode_map(theta)
         res["mode_map_rel_dev_from_window_start"] = max(
             float(((xi_m.detach() - eng.buf[f"xi{cur}"][slots]).abs() / xi_m.detach().abs()).max()),
             float(((ms_m.detach() - eng.buf[f"m_s{cur}"][slots]).abs() / ms_m.detach().abs()).max()))
@@ -288,57 +302,84 @@ def adjoint_cost(eng, cases, start, window, every, check=0, case="icmw75", spinu
                                    "window-start state is not the initialisation map's; per-SIP gradients only")
     log(f"adjoint: {res}")
     if check:
-        res["fd_check"] = adjoint_fd_check(w, eng, cases if not spinup_collisions else None, lam_xi, check)
+        res["fd_check"] = adjoint_fd_check(w, eng, mode_map, check)
         log(f"adjoint fd check: {res['fd_check']}")
     del w
     torch.cuda.empty_cache()
     return res
 
 
-def adjoint_fd_check(w, eng, cases, lam_xi, n_check):
+THETA0 = (60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6)  # N [m^-3], r_med [um], sigma of both modes (config.py:106-114)
+
+
+def adjoint_fd_check(w, eng, mode_map, n_check, h_sip=1.0E-3, h_mode=1.0E-7):
+    """Central differences of the window's J_nc against the adjoint: `n_check` random droplets (multiplicity,
+    h_sip relative) and, through mode_map, the six mode parameters (h_mode relative).  Every run's cell
+    membership is fingerprinted after every step; a run in which some SIP ended a step in another cell than in
+    the unperturbed run carries a piecewise-constant jump that is no part of the derivative, and is reported
+    but left out of the maxima."""
     import torch
-    from lacmopy_amd.setup_case import aerosol_sips
+    from lacmopy_amd.adjoint import membership_changes
     eps = np.finfo(np.float64).eps
+    eng.restore(w.start)
     slots = w.slot_map()
     cur = int(eng.e.cur)
     xi0 = eng.buf[f"xi{cur}"][slots].clone()
     ms0 = eng.buf[f"m_s{cur}"][slots].clone()
     drop = ((eng.buf[f"cell{int(eng.e.ccur)}"][slots] >= 0) & (eng.buf[f"m_w{cur}"][slots] > 4.19E3)).cpu().numpy()
-    J0 = float(np.sum(w.run(xi0, ms0)[0]))
-    out = {"sips": [], "modes": []}
+    fp0 = []
+    J0 = float(np.sum(w.run(xi0, ms0, membership=fp0)[0]))
+    lam_xi, _ = w.gradient(w_nc=1.0, w_lwp=0.0)
+    out = {"h_sip": h_sip, "h_mode": h_mode, "sips": [], "modes": []}
+
+    def J_run(x, m):
+        fp = []
+        J = float(np.sum(w.run(x, m, membership=fp)[0]))
+        return J, membership_changes(fp, fp0)
+
     rng = np.random.default_rng(11)
     for gid in rng.choice(np.flatnonzero(drop), n_check, replace=False):
-        x, h = float(xi0[gid]), 1.0E-3
-        Jpm = []
+        x = float(xi0[gid])
+        Jpm, ch = [], 0
         for sgn in (1.0, -1.0):
             p = xi0.clone()
-            p[gid] = x * (1.0 + sgn * h)
-            Jpm.append(float(np.sum(w.run(p, ms0)[0])))
-        fd = (Jpm[0] - Jpm[1]) / (2 * h * x)
+            p[gid] = x * (1.0 + sgn * h_sip)
+            J, c = J_run(p, ms0)
+            Jpm.append(J)
+            ch += c
+        fd = (Jpm[0] - Jpm[1]) / (2 * h_sip * x)
         a = float(lam_xi[gid])
         out["sips"].append({"gid": int(gid), "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300),
-                            "floor_rel": 256 * eps * abs(J0) / (2 * h * x) / max(abs(fd), 1e-300)})
-    out["max_rel_sips"] = max(s["rel"] for s in out["sips"])
-    # mode parameters through the initialisation map (fourth-order quotient, h = 1e-3)
-    draws = cases[0][2] if cases is not None else None
-    if draws is not None:
-        theta0 = np.array([60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6])
-        theta = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
-        xi_m, ms_m = aerosol_sips(theta, draws)
+                            "floor_rel": 256 * eps * abs(J0) / (2 * h_sip * x) / max(abs(fd), 1e-300),
+                            "membership_changed_steps": ch})
+    kept = [s["rel"] for s in out["sips"] if s["membership_changed_steps"] == 0]
+    out["max_rel_sips"] = max(kept) if kept else None
+    out["sips_with_membership_change"] = len(out["sips"]) - len(kept)
+    if mode_map is not None:
+        theta = torch.tensor(THETA0, dtype=torch.float64, device="cuda", requires_grad=True)
+        xi_m, ms_m = mode_map(theta)
         w.run(xi_m.detach(), ms_m.detach())
         g_xi, g_ms = w.gradient(w_nc=1.0, w_lwp=0.0)
         (grad,) = torch.autograd.grad((xi_m * g_xi).sum() + (ms_m * g_ms).sum(), theta)
+        fp0 = []
+        Jm0 = float(np.sum(w.run(xi_m.detach(), ms_m.detach(), membership=fp0)[0]))
         for k in range(6):
-            def J_at(f):
-                th = theta0.copy()
-                th[k] *= 1.0 + f
-                xx, mm = aerosol_sips(torch.tensor(th, dtype=torch.float64, device="cuda"), draws)
-                return float(np.sum(w.run(xx, mm)[0]))
-            h = 1.0E-3
-            fd = (8.0 * (J_at(h) - J_at(-h)) - (J_at(2 * h) - J_at(-2 * h))) / (12 * h * theta0[k])
+            Jpm, ch = [], 0
+            for sgn in (1.0, -1.0):
+                th = np.array(THETA0)
+                th[k] *= 1.0 + sgn * h_mode
+                xx, mm = mode_map(torch.tensor(th, dtype=torch.float64, device="cuda"))
+                J, c = J_run(xx.detach(), mm.detach())
+                Jpm.append(J)
+                ch += c
+            fd = (Jpm[0] - Jpm[1]) / (2 * h_mode * THETA0[k])
             a = float(grad[k])
-            out["modes"].append({"k": k, "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300)})
-        out["max_rel_modes"] = max(m["rel"] for m in out["modes"])
+            out["modes"].append({"k": k, "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300),
+                                 "floor_rel": 256 * eps * abs(Jm0) / (2 * h_mode * THETA0[k]) / max(abs(fd), 1e-300),
+                                 "membership_changed_steps": ch})
+        kept = [m["rel"] for m in out["modes"] if m["membership_changed_steps"] == 0]
+        out["max_rel_modes"] = max(kept) if kept else None
+        out["modes_with_membership_change"] = len(out["modes"]) - len(kept)
     return out
 
 
@@ -605,6 +646,10 @@ def main():
     ap.add_argument("--adj-check", type=int, default=20,
                     help="droplets of the finite-difference check of the adjoint (plus the six aerosol-mode "
                          "parameters); 0: no check")
+    ap.add_argument("--adj-init", default=None, choices=["sinsip", "synthetic"],
+                    help="start state of the icmw75 adjoint window: the reference's SinSIP initialisation computed "
+                         "on the GPU (about 128 SIPs per cell; default) or the synthetic Poisson sampler; the "
+                         "mode-parameter gradient follows the chosen state's initialisation map")
     ap.add_argument("--init-seeds", type=int, default=64, help="seeds of the timed GPU initialisation")
     ap.add_argument("--stress-runs", type=int, default=16, help="seeds of the stress150 batch leg")
     ap.add_argument("--copy-probe", action="store_true",
@@ -669,9 +714,10 @@ def main():
         log(f"plain early: {r}")
         del eng1
 
-    # ---- spin-up of one run: the collision-active state and the adjoint window start
+    # ---- spin-up of one run: the collision-active state (and the synthetic adjoint window start)
+    adj_init = args.adj_init or ("sinsip" if icmw else "synthetic")
     spin = None
-    if {"drizzle", "adjoint"} & set(legs):
+    if "drizzle" in legs or ("adjoint" in legs and adj_init == "synthetic"):
         spin, n1, cases1 = build_engine(1, args.spc, args.seed, case=args.case, kernel=args.kernel)
         spin_up(spin, args.spinup, adj_col)
         spun = spin.snapshot()
@@ -714,11 +760,34 @@ def main():
         log(f"collision-active: {drz}")
 
     if "adjoint" in legs:
-        out["adjoint"] = adjoint_cost(spin, cases1, spun, args.adj_window, args.adj_every, args.adj_check,
-                                      case=args.case, spinup=args.spinup, spinup_collisions=adj_col)
+        col_desc = "with" if adj_col else "without"
+        if adj_init == "sinsip":
+            from lacmopy_amd.initialize import sinsip_mode_map
+            if spin is not None:
+                del spin, spun
+                spin = None
+                torch.cuda.empty_cache()
+            t = time.perf_counter()
+            aeng, ast, aicfg = sinsip_engine(3711, seed_sim=args.seed)
+            log(f"adjoint: SinSIP initial state ({aeng.n_tot} SIPs) in {time.perf_counter() - t:.1f} s")
+            spin_up(aeng, args.spinup, False)
+            out["adjoint"] = adjoint_cost(
+                aeng, aeng.snapshot(), args.adj_window, args.adj_every,
+                mode_map=lambda th: sinsip_mode_map(th, ast, aicfg), check=args.adj_check, case=args.case,
+                state_desc=f"the reference's SinSIP initial state (computed on the GPU, seed_SIP_gen 3711, no_spcm "
+                           f"(52, 76)) after a {args.spinup}-step spin-up without collisions")
+            del aeng
+        else:
+            from lacmopy_amd.setup_case import aerosol_sips
+            draws = cases1[0][2]
+            out["adjoint"] = adjoint_cost(
+                spin, spun, args.adj_window, args.adj_every,
+                mode_map=None if adj_col else (lambda th: aerosol_sips(th, draws)), check=args.adj_check,
+                case=args.case, state_desc=f"the synthetic start after a {args.spinup}-step spin-up {col_desc} "
+                                           f"collisions")
     if spin is not None:
         del spin, spun
-        torch.cuda.empty_cache()
+    torch.cuda.empty_cache()
 
     if "box" in legs and icmw:
         out["box_model"] = box_model_leg(args.seed)

@@ -33,7 +33,7 @@ import torch
 
 from ._native import LM_EVENT_BYTES, LmAdjoint, check
 
-__all__ = ["AdjointWindow", "WindowObjective", "window_objective"]
+__all__ = ["AdjointWindow", "WindowObjective", "window_objective", "membership_fingerprint", "membership_changes"]
 
 
 def _ptr(t):
@@ -152,11 +152,13 @@ class AdjointWindow:
         return J_nc, J_lwp
 
     # -------------------------------------------------------------- forward
-    def run(self, xi0=None, ms0=None, record=None):
+    def run(self, xi0=None, ms0=None, record=None, membership=None):
         """Runs the window from its start state with xi / m_s of the SIPs replaced (tensors by global id, or
         None to keep them).  The first run records the collision events (record=True); every later run
         re-applies them as fixed events.  Keeps checkpoints every K steps and the end state for gradient().
-        Returns (J_nc, J_lwp) per run."""
+        membership: a list to which the run appends, after every advection step, the fingerprint of the SIPs'
+        cell membership (membership_fingerprint); runs of perturbed inputs whose lists equal the unperturbed
+        run's kept every SIP in the same cell at the end of every step.  Returns (J_nc, J_lwp) per run."""
         eng = self.eng
         if self.start is None:
             self.start = eng.snapshot()
@@ -180,7 +182,12 @@ class AdjointWindow:
         try:
             for s0 in range(0, self.n_steps, self.K):
                 self.ckpt.append(eng.snapshot())
-                eng.adv_steps(min(self.K, self.n_steps - s0))
+                if membership is None:
+                    eng.adv_steps(min(self.K, self.n_steps - s0))
+                else:
+                    for _ in range(min(self.K, self.n_steps - s0)):
+                        eng.adv_steps(1)
+                        membership.append(membership_fingerprint(eng))
             eng.check()
         finally:
             self._unbind()
@@ -233,6 +240,27 @@ class AdjointWindow:
                 continue
             if not torch.equal(eng.buf[name], t):
                 raise RuntimeError(f"taped re-run diverged from the recorded window in {name}")
+
+
+def membership_fingerprint(eng):
+    """Two int64 sums over the SIPs of all runs, sum (cell + 7) (id mod 1000 + 1) and sum (cell + 7)
+    (id div 1000 + 1), with cell the engine's global cell code (run and cell; negative once inactive) and id
+    the SIP's id in its run: equal for two states whose SIPs sit in the same cells (a device tensor; no host
+    synchronisation).  One SIP in another cell changes the first sum."""
+    cur, ccur = int(eng.e.cur), int(eng.e.ccur)
+    n = eng.n_tot
+    cell = eng.buf[f"cell{ccur}"][:n].long() + 7
+    ids = eng.buf[f"id{cur}"][:n].long()
+    return torch.stack([(cell * (ids % 1000 + 1)).sum(), (cell * (ids // 1000 + 1)).sum()])
+
+
+def membership_changes(a, b):
+    """Steps at whose end the fingerprints of two runs (lists from AdjointWindow.run(membership=...)) differ."""
+    if len(a) != len(b):
+        raise ValueError("fingerprints of windows of different lengths")
+    if not a:
+        return 0
+    return int((torch.stack(a) != torch.stack(b)).any(dim=1).sum())
 
 
 class WindowObjective(torch.autograd.Function):

@@ -236,6 +236,57 @@ def _grid_of_run(config, Lr, F):
     return grid
 
 
+def mode_parameters(config):
+    """(N [m^-3], r_med [um], sigma) of every mode with no_spcm > 0, flattened: the parameters of
+    sinsip_mode_map at the configuration's values (config.py:106-114 DNC0, mu_R, sigma_R)."""
+    no_spcm = np.atleast_1d(np.asarray(config["no_spcm"]))
+    idx = np.nonzero(no_spcm)[0]
+    cols = [np.atleast_1d(np.asarray(config[k], dtype=np.float64))[idx] for k in ("DNC0", "mu_R", "sigma_R")]
+    return np.stack(cols, axis=1).reshape(-1)
+
+
+def sinsip_mode_map(theta, state, config):
+    """Multiplicities and dry masses of a SinSIP initial state as functions of the aerosol-mode parameters.
+
+    theta: float64 torch tensor (N, r_med, sigma per mode with no_spcm > 0, as mode_parameters orders them; it
+    may require grad).  The SinSIP scheme (generation_SIP_ensemble.py:147-244, 334-404) draws each SIP's dry
+    mass m uniformly inside a logarithmic bin fixed by r_critmin and kappa alone and gives it the multiplicity
+    xi = pdf(m; mu_m_log, sigma_m_log) * (bin width) * (real particles per cell of its mode), with
+    pdf_lognormal of distributions.py:71-97, mu_m_log = ln(4/3 pi 2163 r_med^3) (init.py:130-131, the NaCl dry
+    density whatever the solute) and sigma_m_log = 3 ln sigma.  With the draws held, the dry masses do not
+    depend on theta and
+
+        xi(theta) = xi_0 * (N / N_0) * pdf(m; theta) / pdf(m; theta_0),
+
+    equal to the state's own xi at theta_0 (the configuration's values).  Held as well: the set of bins kept by
+    the fixed threshold eta * max weight (a bin leaves or joins the ensemble only where its weight crosses the
+    threshold), the level's real-particle count rint(ceil(V N) rho_d / rho_d0) taken as proportional to N
+    (its rounding is 1e-10 relative at these counts) and the level densities, which the saturation adjustment
+    of the levels below feeds back on.  The weak threshold raises some weights to eta * max weight, a
+    different function of theta, and is not covered.  Returns (xi, m_s) in the state's SIP order."""
+    if config.get("eta_threshold", "fix") != "fix":
+        raise ValueError("sinsip_mode_map covers the fixed SinSIP threshold (eta_threshold 'fix') only")
+    dev = theta.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    th0 = torch.as_tensor(mode_parameters(config), **f64).view(-1, 3)
+    th = theta.view(-1, 3)
+    if th.shape != th0.shape:
+        raise ValueError(f"theta holds {th.shape[0]} modes, the configuration {th0.shape[0]}")
+    mode = torch.as_tensor(np.asarray(state.modes), dtype=torch.long, device=dev)
+    m = torch.as_tensor(np.asarray(state.m_s), **f64)
+    xi0 = torch.as_tensor(np.asarray(state.xi), **f64)
+    ln_m = torch.log(m)
+    c = math.log(4.0 * math.pi / 3.0 * 2163.0)
+
+    def log_pdf(t):  # up to the terms of m alone, which cancel in the ratio
+        mu = c + 3.0 * torch.log(t[:, 1])
+        sg = 3.0 * torch.log(t[:, 2])
+        return -0.5 * ((ln_m - mu[mode]) / sg[mode]) ** 2 - torch.log(sg[mode])
+
+    xi = xi0 * (th[:, 0] / th0[:, 0])[mode] * torch.exp(log_pdf(th) - log_pdf(th0))
+    return xi, m
+
+
 def initialize_grid_and_particles_SinSIP(config):
     """init.py:439-1236: the initial grid and SIPs of seed config['seed_SIP_gen'], computed on the GPU, and the
     reference's files of t = 0 (grid and particles, modes_0.npy, no_rpcm_scale_factors_lvl_wise.npy) in

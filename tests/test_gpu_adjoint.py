@@ -269,3 +269,65 @@ def test_gradient_against_cpu_tangent_linear_reference(gpu, collisions):
     print(f"adjoint vs CPU tangent-linear (collisions {collisions}, {w.n_events} events): J rel diff "
           f"{abs(J_gpu[0] - J_cpu[0]) / J_cpu[0]:.1e}; largest |lambda.v - dJ| / sum|lambda_k v_k| over 16 "
           f"derivatives {worst:.1e}")
+
+
+def test_mode_gradient_through_the_sinsip_map(gpu):
+    """A GPU-initialised SinSIP state (lacmopy_amd.initialize, the reference's configuration on 6 x 75 cells),
+    30 steps without collisions, then a 10-step window with collisions: the gradient of both objectives with
+    respect to the six aerosol-mode parameters, by the chain rule through initialize.sinsip_mode_map (the
+    multiplicities of the SinSIP ensemble as functions of N, r_med and sigma), against central differences of
+    the window at h = 1e-7 relative with xi from the map.  Runs whose SIPs ended any step in another cell
+    than in the unperturbed run are not compared (a piecewise-constant membership change is no part of the
+    derivative; none occurred at this h so far); tolerance 1e-4 relative plus the difference quotient's noise
+    floor 256 eps |J| / (2 h |theta|)."""
+    from init_cases import init_config
+    from lacmopy_amd.adjoint import membership_changes
+    from lacmopy_amd.engine import Engine
+    from lacmopy_amd.initialize import initialize_batch, mode_parameters, sinsip_mode_map
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    icfg = init_config(nx=6, nz=75)
+    st = initialize_batch(icfg, [3711])[0]
+    step = dict(solute_type="AS", dt_adv=1.0, dt_col=0.5, scale_dt_cond=5, no_col_per_adv=2, no_iter_impl_mass=3,
+                g_set=9.80665, act_collisions=True, act_relaxation=True, seed_sim=41, R_kernel_low_log=lnR0,
+                bin_factor_R_log=lnf)
+    eng = Engine.from_initial_states([st], step, E)
+    eng.e.act_collisions = 0
+    eng.adv_steps(30)
+    eng.e.act_collisions = 1
+    w = AdjointWindow(eng, 10, checkpoint_every=5)
+    theta0 = mode_parameters(icfg)
+    theta = torch.tensor(theta0, dtype=torch.float64, device="cuda", requires_grad=True)
+    xi_m, ms_m = sinsip_mode_map(theta, st, icfg)
+    # no collisions before the window: its start holds the initial multiplicities and dry masses
+    slots = w.slot_map()
+    cur = int(eng.e.cur)
+    assert torch.equal(xi_m.detach(), eng.buf[f"xi{cur}"][slots]) and torch.equal(ms_m, eng.buf[f"m_s{cur}"][slots])
+    fp0 = []
+    J0 = w.run(xi_m.detach(), ms_m, record=True, membership=fp0)
+    grads = []
+    for q in range(2):
+        lam_xi, lam_ms = w.gradient(w_nc=float(q == 0), w_lwp=float(q == 1))
+        (gq,) = torch.autograd.grad((xi_m * lam_xi).sum() + (ms_m * lam_ms).sum(), theta, retain_graph=True)
+        grads.append(gq.cpu().numpy())
+    h = 1.0E-7
+    compared, seen = 0, []
+    for k in range(6):
+        Jpm, changed = [], 0
+        for sgn in (1.0, -1.0):
+            th = theta0.copy()
+            th[k] *= 1.0 + sgn * h
+            x, m = sinsip_mode_map(torch.tensor(th, dtype=torch.float64, device="cuda"), st, icfg)
+            fp = []
+            Jpm.append([float(np.sum(j)) for j in w.run(x, m, membership=fp)])
+            changed += membership_changes(fp, fp0)
+        if changed:
+            continue
+        compared += 1
+        for q in range(2):
+            fd = (Jpm[0][q] - Jpm[1][q]) / (2 * h * theta0[k])
+            _close(float(grads[q][k]), fd, float(np.sum(J0[q])), theta0[k], h, 1.0E-4, f"d J[{q}] / d theta[{k}]",
+                   seen)
+    assert compared >= 4, f"only {compared} of 6 parameters kept the membership"
+    assert any(abs(g) > 0 for g in grads[0]) and w.n_events >= 0
+    print(f"SinSIP mode map: {compared} of 6 parameters compared, {len(seen)} derivatives resolved above the "
+          f"floor, max rel. deviation {max(seen, default=0):.1e}; {w.n_events} fixed events")
