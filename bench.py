@@ -314,10 +314,12 @@ THETA0 = (60.0E6, 0.02, 1.4, 40.0E6, 0.075, 1.6)  # N [m^-3], r_med [um], sigma 
 
 def adjoint_fd_check(w, eng, mode_map, n_check, h_sip=1.0E-3, h_mode=1.0E-7):
     """Central differences of the window's J_nc against the adjoint: `n_check` random droplets (multiplicity,
-    h_sip relative) and, through mode_map, the six mode parameters (h_mode relative).  Every run's cell
-    membership is fingerprinted after every step; a run in which some SIP ended a step in another cell than in
-    the unperturbed run carries a piecewise-constant jump that is no part of the derivative, and is reported
-    but left out of the maxima."""
+    h_sip relative) and, through mode_map, the six mode parameters (h_mode relative).  Each comparison carries
+    the quotient's noise floor 256 eps |J| / (2 h |x| |fd|) (floor_rel: one perturbed trajectory rounds
+    differently from the other) and the number of steps at whose end some SIP sat in another cell than in the
+    unperturbed run (cell membership fingerprinted after every step: such a run carries piecewise-constant
+    jumps that are no part of the derivative).  The maxima are taken over the comparisons resolved above their
+    floor (floor_rel < 1e-3); for the droplets also without membership change."""
     import torch
     from lacmopy_amd.adjoint import membership_changes
     eps = np.finfo(np.float64).eps
@@ -349,12 +351,14 @@ def adjoint_fd_check(w, eng, mode_map, n_check, h_sip=1.0E-3, h_mode=1.0E-7):
             ch += c
         fd = (Jpm[0] - Jpm[1]) / (2 * h_sip * x)
         a = float(lam_xi[gid])
-        out["sips"].append({"gid": int(gid), "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300),
-                            "floor_rel": 256 * eps * abs(J0) / (2 * h_sip * x) / max(abs(fd), 1e-300),
+        scale = max(abs(fd), abs(a), 1e-300)
+        out["sips"].append({"gid": int(gid), "adjoint": a, "fd": fd, "rel": abs(a - fd) / scale,
+                            "floor_rel": 256 * eps * abs(J0) / (2 * h_sip * x) / scale,
                             "membership_changed_steps": ch})
-    kept = [s["rel"] for s in out["sips"] if s["membership_changed_steps"] == 0]
-    out["max_rel_sips"] = max(kept) if kept else None
-    out["sips_with_membership_change"] = len(out["sips"]) - len(kept)
+    res_ = [s["rel"] for s in out["sips"] if s["membership_changed_steps"] == 0 and s["floor_rel"] < 1e-3]
+    out["sips_resolved"] = len(res_)
+    out["max_rel_sips_resolved"] = max(res_) if res_ else None
+    out["sips_with_membership_change"] = sum(s["membership_changed_steps"] > 0 for s in out["sips"])
     if mode_map is not None:
         theta = torch.tensor(THETA0, dtype=torch.float64, device="cuda", requires_grad=True)
         xi_m, ms_m = mode_map(theta)
@@ -374,12 +378,14 @@ def adjoint_fd_check(w, eng, mode_map, n_check, h_sip=1.0E-3, h_mode=1.0E-7):
                 ch += c
             fd = (Jpm[0] - Jpm[1]) / (2 * h_mode * THETA0[k])
             a = float(grad[k])
-            out["modes"].append({"k": k, "adjoint": a, "fd": fd, "rel": abs(a - fd) / max(abs(fd), 1e-300),
-                                 "floor_rel": 256 * eps * abs(Jm0) / (2 * h_mode * THETA0[k]) / max(abs(fd), 1e-300),
+            scale = max(abs(fd), abs(a), 1e-300)
+            out["modes"].append({"k": k, "adjoint": a, "fd": fd, "rel": abs(a - fd) / scale,
+                                 "floor_rel": 256 * eps * abs(Jm0) / (2 * h_mode * THETA0[k]) / scale,
                                  "membership_changed_steps": ch})
-        kept = [m["rel"] for m in out["modes"] if m["membership_changed_steps"] == 0]
-        out["max_rel_modes"] = max(kept) if kept else None
-        out["modes_with_membership_change"] = len(out["modes"]) - len(kept)
+        res_ = [m["rel"] for m in out["modes"] if m["floor_rel"] < 1e-3]
+        out["modes_resolved"] = len(res_)
+        out["max_rel_modes_resolved"] = max(res_) if res_ else None
+        out["modes_with_membership_change"] = sum(m["membership_changed_steps"] > 0 for m in out["modes"])
     return out
 
 

@@ -105,10 +105,12 @@ typedef struct LmEngine {
   int32_t *scan_buf;  /* n_runs*(2nc+1)+1 */
   int32_t *scan_part; /* scan partials */
   int32_t *err;       /* 4 error words on the device */
-  int32_t *ovf_list;  /* 2*n_runs*nc: cells over the fast-path LDS capacity (collision: list A in the first
-                         half, its cells over 256 members in the second) */
-  int32_t *ovf_cnt;   /* 3: cells queued for the large-capacity collision [0] and Delta m_l [1] passes, and for
-                         the collision's 1024-member pass after its 256-member pass [2] */
+  int32_t *ovf_list;  /* 3*n_runs*nc (2*n_runs*nc without big_ws): cells over the fast-path LDS capacity
+                         (collision: list A in the first third, its cells over 256 members in the second, cells
+                         over 1024 members in the third) */
+  int32_t *ovf_cnt;   /* 4 (3 without big_ws): cells queued for the large-capacity collision [0] and Delta m_l
+                         [1] passes, for the collision's 1024-member pass after its 256-member pass [2] and for
+                         its pass over cells of any size [3] */
   /* injected uniform stream (parity mode; NULL = Philox) */
   const double *rnd_stream;
   int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */
@@ -186,10 +188,18 @@ typedef struct LmEngine {
      collision passes; with it the work list starts with the cells whose cost exceeds half the expected
      length of a persistent collision pass (their cost order), then the others by home-segment size */
   uint32_t *col_cost;
+  /*
+   * Collision pass over cells of more than 1024 members (NULL: such a cell is an error, LM_ERR_CELL_CAP):
+   * big_blocks workgroups, each with its own region of lm_big_ws_bytes(big_cap) bytes of big_ws holding the
+   * per-member state of one cell at a time; cells of up to big_cap (2..LM_BIG_CELL_MAX) members.
+   */
+  void *big_ws;
+  int32_t big_cap, big_blocks;
 } LmEngine;
 
-#define LM_MIG_CAP_MAX 64
-#define LM_SPEC_EVENTS_MAX 1024 /* largest migrant list per cell */
+#define LM_MIG_CAP_MAX 64       /* largest migrant list per cell */
+#define LM_SPEC_EVENTS_MAX 1024 /* largest spec_events */
+#define LM_BIG_CELL_MAX 46340   /* members of a cell the collision passes take at most (32-bit pair numbers) */
 
 #define LM_WORK_BINS 1024 /* home-segment sizes are binned up to LM_WORK_BINS - 1 for the work-list order */
 
@@ -273,6 +283,9 @@ const char *lm_version(void);
 const char *lm_last_error_message(void);
 /* sizeof(LmEngine) and byte offsets of selected fields (for binding checks). */
 int64_t lm_engine_sizeof(void);
+/* Bytes of one workgroup's region of LmEngine.big_ws for cells of up to cap members (no counterpart in the
+   reference, which keeps a cell's members in NumPy arrays of any size, all_or_nothing.py:811-846). */
+int64_t lm_big_ws_bytes(int32_t cap);
 void lm_engine_offsets(int64_t *out, int32_t n);
 /* Physical constants as the kernels use them (for the constant-parity test). */
 void lm_get_constants(double *out, int32_t n);

@@ -64,6 +64,7 @@ class LmEngine(C.Structure):
         ("col_work_per_cu", _i32), ("dml_work_per_cu", _i32), ("work", _p),
         ("col_work_chunk", _i32), ("dml_work_chunk", _i32), ("part_wpe", _i32), ("pad4", _i32),
         ("mig_cnt", _p), ("mig_slot", _p), ("mig_cap", _i32), ("spec_events", _i32), ("col_cost", _p),
+        ("big_ws", _p), ("big_cap", _i32), ("big_blocks", _i32),
     ]
 
 
@@ -94,7 +95,8 @@ LM_EVENT_BYTES = 80  # sizeof(LmEvent): 6 int32 + 7 double
 OFFSET_FIELDS = ["n_tot", "solute", "dt", "E", "seed", "x", "id", "dmxi", "scal",
                  "grid_vel", "no_cols", "rem_cap", "scan_buf", "rnd_stream", "rnd_len", "away", "col_step_dev",
                  "ev_mode", "ev", "run_base", "tape_part", "ccd", "part_block", "E_max", "vel_run_stride",
-                 "col_work_per_cu", "work", "dml_work_chunk", "part_wpe", "mig_cnt", "mig_cap", "spec_events", "col_cost"]
+                 "col_work_per_cu", "work", "dml_work_chunk", "part_wpe", "mig_cnt", "mig_cap", "spec_events", "col_cost",
+                 "big_ws", "big_cap"]
 
 EXPORTED = [
     "lm_version", "lm_last_error_message", "lm_engine_sizeof", "lm_engine_offsets",
@@ -107,7 +109,7 @@ EXPORTED = [
     "lm_derived_fields", "lm_ensemble_stats", "lm_build_slot_of", "lm_objective", "lm_adj_seed",
     "lm_adj_adv_steps", "lm_init_par_sizeof", "lm_init_level_profile", "lm_init_level_sips", "lm_init_positions",
     "lm_box_minmax", "lm_box_stats", "lm_mean_sem", "lm_collision_all_cells", "lm_collision_all_cells_check",
-    "lm_collision_all_cells_noalloc",
+    "lm_collision_all_cells_noalloc", "lm_big_ws_bytes",
 ]
 
 _lib = None
@@ -131,6 +133,8 @@ def lib():
     L.lm_version.restype = C.c_char_p
     L.lm_last_error_message.restype = C.c_char_p
     L.lm_engine_sizeof.restype = _i64
+    L.lm_big_ws_bytes.argtypes = [_i32]
+    L.lm_big_ws_bytes.restype = _i64
     L.lm_engine_offsets.argtypes = [C.POINTER(_i64), _i32]
     L.lm_get_constants.argtypes = [C.POINTER(_d), _i32]
     L.lm_num_constants.restype = _i32

@@ -30,6 +30,7 @@
 #define WAVE 64
 #define DML_CAP 512 /* member capacity of the gather kernels (Delta m_l, reindex) */
 #define OVF_GRID 1024 /* blocks of the grid-strided overflow passes */
+#define REF_BIG_BLOCKS 4 /* workgroups (and workspace regions) of the reference-layout step's pass over large cells */
 #define PART_BLOCK_MAX 512 /* largest block of the particle kernel */
 #define TAPE_BLOCK 256     /* block of the taped particle kernel (its dual Newton needs the registers of one wave per SIMD) */
 
@@ -317,6 +318,47 @@ __device__ static void rank_members(int n, const int *s_id, const unsigned char 
   }
 }
 
+/*
+ * Members of global cell gc in ascending id for a cell of any size -- the path of the rare cells over a pass's
+ * LDS capacity.  The calling lane merges the members of the neighbour segments of tb: after the reindex each
+ * home segment holds its SIPs in ascending id, so its members of gc form an ascending sub-sequence, and ids
+ * are unique within a run.  Calls f(slot) once per member, in ascending id; O(9 n) work on one lane.  Returns
+ * the member count.
+ */
+template <class F>
+__device__ static int merge_members(int gc, const int32_t *cell, const int32_t *ids, const NbrTab *tb, F &&f) {
+  int pos[9], end[9], hid[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    pos[t] = tb->start[t];
+    end[t] = pos[t] + (tb->pre[t + 1] - tb->pre[t]);
+    while (pos[t] < end[t] && cell[pos[t]] != gc) ++pos[t];
+    hid[t] = pos[t] < end[t] ? ids[pos[t]] : 0x7fffffff;
+  }
+  int n = 0;
+  for (;;) {
+    int best = 0x7fffffff, bt = -1, slot = 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+      if (hid[t] < best) {
+        best = hid[t];
+        bt = t;
+        slot = pos[t];
+      }
+    if (bt < 0) break;
+    f(slot);
+    ++n;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+      if (t == bt) {
+        ++pos[t];
+        while (pos[t] < end[t] && cell[pos[t]] != gc) ++pos[t];
+        hid[t] = pos[t] < end[t] ? ids[pos[t]] : 0x7fffffff;
+      }
+  }
+  return n;
+}
+
 __global__ void k_fill_i32(int32_t *a, int32_t value, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) a[t] = value;
@@ -366,6 +408,8 @@ struct KView {
      are mlist[moff[c] .. moff[c+1]), in any order */
   const int32_t *mlist, *moff;
   const int32_t *work; /* (run, cell) work list of the persistent passes */
+  char *big_ws;        /* per-workgroup regions of the pass over cells of any size (NULL: off) */
+  int big_cap;         /* members per region */
   int32_t *work_ctr;   /* its 2 counters: collision [0], Delta m_l [1] */
   const double *rnd_stream;
   long long *rnd_off;
@@ -464,6 +508,8 @@ static KView make_view(const LmEngine *e) {
   v.spec_events = e->spec_events > 0 ? e->spec_events : SPEC_MAX_EVENTS;
   v.col_cost = e->col_cost;
   v.work = e->work;
+  v.big_ws = (char *)e->big_ws;
+  v.big_cap = e->big_cap;
   v.work_ctr = e->work ? e->work + (size_t)e->n_runs * e->nc + LM_WORK_BINS : nullptr;
   v.rnd_stream = e->rnd_stream;
   v.rnd_off = (long long *)e->rnd_off;
@@ -823,7 +869,49 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
  * those with a zero byte, 1 - (255/256)^4 = 1.55% of a cell's n^2/8 quads, 127 at 256 members -- or every
  * cell of that size overflows into the anti-diagonal schedule.
  */
-__host__ __device__ constexpr int qcap_of(int cap) { return cap <= 160 ? 2 * WAVE : (cap <= 256 ? 4 * WAVE : 8 * WAVE); }
+__host__ __device__ constexpr int qcap_of(int cap) {
+  return cap <= 0 ? 8 * WAVE : (cap <= 160 ? 2 * WAVE : (cap <= 256 ? 4 * WAVE : 8 * WAVE));
+}
+
+/*
+ * Region of the pass over cells of any size (capacity 0): the per-member arrays the LDS passes keep in LDS, in
+ * global memory, cap members.
+ */
+struct BigWs {
+  double *X, *R, *U, *W;
+  int32_t *IND, *SL, *Q;
+  float *B;
+  unsigned char *TG, *IsBig;
+};
+__host__ __device__ constexpr size_t big_up(size_t b) { return (b + 255) & ~(size_t)255; }
+__host__ __device__ constexpr size_t big_stride(int cap) {
+  return 4 * big_up(8 * (size_t)cap) + 2 * big_up(4 * (size_t)cap) + big_up(4 * ((size_t)cap + 1)) +
+         big_up(4 * (size_t)cap) + 2 * big_up((size_t)cap);
+}
+__device__ static inline BigWs big_carve(char *b, int cap) {
+  BigWs w;
+  size_t o = 0;
+  w.X = (double *)(b + o);
+  o += big_up(8 * (size_t)cap);
+  w.R = (double *)(b + o);
+  o += big_up(8 * (size_t)cap);
+  w.U = (double *)(b + o);
+  o += big_up(8 * (size_t)cap);
+  w.W = (double *)(b + o);
+  o += big_up(8 * (size_t)cap);
+  w.IND = (int32_t *)(b + o);
+  o += big_up(4 * (size_t)cap);
+  w.SL = (int32_t *)(b + o);
+  o += big_up(4 * (size_t)cap);
+  w.Q = (int32_t *)(b + o);
+  o += big_up(4 * ((size_t)cap + 1));
+  w.B = (float *)(b + o);
+  o += big_up(4 * (size_t)cap);
+  w.TG = (unsigned char *)(b + o);
+  o += big_up((size_t)cap);
+  w.IsBig = (unsigned char *)(b + o);
+  return w;
+}
 
 /* Largest multiplicity and radius and the velocity ranges over the members outside big[0..m) (wave-wide). */
 struct CellMax {
@@ -1169,30 +1257,79 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   const int r = gc / nc, c = gc % nc;
   const int lane = threadIdx.x;
 
-  /* LDS: xi, R, u, w, kernel index and slot of each member, in id order; the
-     gather scratch lives inside the u/w arrays until the ranks are known. */
-  __shared__ double sX[CAP], sR[CAP], sU[CAP], sW[CAP];
-  __shared__ int sIND[CAP], sSL[CAP], sQ[CAP + 1];
-  __shared__ float sB[CAP];
-  __shared__ unsigned char sTG[CAP];
+  /* xi, R, u, w, kernel index and slot of each member, in id order: in LDS, where the gather scratch lives
+     inside the u/w arrays until the ranks are known, or (CAP 0: cells of any size) in the workgroup's region
+     of the global workspace big_ws */
+  constexpr bool BIG = CAP == 0;
+  constexpr int LC = BIG ? 1 : CAP;
+  __shared__ double lX[LC], lR[LC], lU[LC], lW[LC];
+  __shared__ int lIND[LC], lSL[LC], lQ[LC + 1];
+  __shared__ float lB[LC];
+  __shared__ unsigned char lTG[LC], lIsBig[LC];
+  double *sX = lX, *sR = lR, *sU = lU, *sW = lW;
+  int *sIND = lIND, *sSL = lSL, *sQ = lQ;
+  float *sB = lB;
+  unsigned char *sTG = lTG, *sIsBig = lIsBig; /* sIsBig: rank in the list of largest radii the guard pass
+                                                  leaves out, or 0 */
+  if constexpr (BIG) {
+    const BigWs b = big_carve(v.big_ws + (size_t)blockIdx.x * big_stride(v.big_cap), v.big_cap);
+    sX = b.X;
+    sR = b.R;
+    sU = b.U;
+    sW = b.W;
+    sIND = b.IND;
+    sSL = b.SL;
+    sQ = b.Q;
+    sB = b.B;
+    sTG = b.TG;
+    sIsBig = b.IsBig;
+  }
   constexpr int QC = qcap_of(CAP), NQ = QC / WAVE;
   __shared__ int sQueue[QC];
   __shared__ unsigned long long sQkey[QC]; /* each entry's first firing pair (key), ~0: none, KEY_STALE */
   __shared__ int sQn;
   __shared__ int sBig[BIG_MAX], sNbig;
-  __shared__ unsigned char sIsBig[CAP]; /* rank in the list of largest radii the guard pass leaves out, or 0 */
   __shared__ NbrTab tb;
   __shared__ int sSeq;
-  int *sSlot = (int *)sU;
+  int *sSlot = (int *)lU;
   int *sId = sSlot + CAP;
-  int *sRank = (int *)sW;
+  int *sRank = (int *)lW;
   unsigned char *sSub = (unsigned char *)(sRank + CAP);
 
   /* members: from the home segment and the migrant list (slots in id order straight into sSL), or, with
-     an incomplete list, gathered from the 3x3 home segments and ranked by id */
+     an incomplete list, gathered from the 3x3 home segments and ranked by id; cells of any size: merged in
+     id order from the 3x3 home segments by one lane */
   int n;
   bool fast;
-  if (v.mlist) {
+  if constexpr (BIG) {
+    fast = true;
+    if (v.mlist) {
+      /* the cell's member list, sorted by SIP index (rank by counting: the rare cells of this pass) */
+      const int o0 = v.moff[gc];
+      n = v.moff[gc + 1] - o0;
+      if (n <= v.big_cap)
+        for (int p = lane; p < n; p += WAVE) {
+          const int k = v.mlist[o0 + p];
+          int rk = 0;
+          for (int q = 0; q < n; ++q) rk += v.mlist[o0 + q] < k;
+          sSL[rk] = k;
+        }
+    } else {
+      nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
+      int m = 0;
+      if (lane == 0) {
+        const int cap = v.big_cap;
+        int cnt = 0;
+        merge_members(gc, v.cell, v.id, &tb, [&](int k) {
+          if (cnt < cap) sSL[cnt] = k;
+          ++cnt;
+        });
+        m = cnt;
+      }
+      n = __shfl(m, 0);
+    }
+    __syncthreads();
+  } else if (v.mlist) {
     /* the cell's member list, sorted by SIP index (the reference's mask order, all_or_nothing.py:812-829) */
     const int o0 = v.moff[gc];
     n = v.moff[gc + 1] - o0;
@@ -1216,7 +1353,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
     nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
     n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
   }
-  if (n > CAP) {
+  if (n > (BIG ? v.big_cap : CAP)) {
     if (lane == 0) {
       if (!io.out_list) {
         raise_err(v.err, LM_ERR_CELL_CAP, gc);
@@ -2092,7 +2229,10 @@ __device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
   if (n > CAP) {
     if (threadIdx.x == 0) {
       if (from_ovf) {
-        raise_err(v.err, LM_ERR_CELL_CAP, gc);
+        /* over every LDS capacity: one lane sums in ascending id straight from the segments */
+        double sum = 0.0;
+        merge_members(gc, v.cell, v.id, &tb, [&](int k) { sum += v.dmxi[k]; });
+        v.dml[gc] = sum * 1.0E-18;
       } else {
         const int q = atomicAdd(&v.ovf_cnt[1], 1);
         v.ovf_list[q] = gc;
@@ -2144,7 +2284,13 @@ __global__ __launch_bounds__(WAVE) void k_rl(KView v) {
   nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   if (n > DML_CAP) {
-    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+    /* over the LDS capacity: one lane sums in ascending id straight from the segments */
+    if (threadIdx.x == 0) {
+      double acc = 0.0;
+      merge_members(gc, v.cell, v.id, &tb, [&](int k) { acc += v.m_w[k] * v.xi[k]; });
+      double *S = v.scal + (size_t)r * 10 * nc;
+      S[5 * nc + c] = acc * (1.0E-18 * S[8 * nc + c]);
+    }
     return;
   }
   __syncthreads();
@@ -2179,14 +2325,48 @@ __global__ __launch_bounds__(WAVE) void k_derived(KView v, double *out) {
   __shared__ NbrTab tb;
   nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DERIV_CAP);
+  const double *C = v.cc + (size_t)gc * CC_N;
   if (n > DERIV_CAP) {
-    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
-    return;
-  }
+    /* over the LDS capacity: one lane runs the eleven ordered sums straight from the segments */
+    if (threadIdx.x == 0) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0, a6 = 0.0, a7 = 0.0, a8 = 0.0, a9 = 0.0,
+             a10 = 0.0;
+      merge_members(gc, v.cell, v.id, &tb, [&](int k) {
+        double R, ws, rho;
+        lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], C[CC_T], C[CC_RHOW], R, ws, rho);
+        const double mw = v.m_w[k] * v.xi[k], x = v.xi[k];
+        if (R < 0.5) {
+          a0 += mw;
+          a3 += x;
+        } else if (R < 25.) {
+          a1 += mw;
+          a4 += x;
+          a8 += x * R;
+          a9 += x * (R * R);
+          a10 += x * (R * (R * R));
+        } else {
+          a2 += mw;
+          a5 += x;
+        }
+        a6 += x;
+        a7 += x * R;
+      });
+      sSum[0] = a0;
+      sSum[1] = a1;
+      sSum[2] = a2;
+      sSum[3] = a3;
+      sSum[4] = a4;
+      sSum[5] = a5;
+      sSum[6] = a6;
+      sSum[7] = a7;
+      sSum[8] = a8;
+      sSum[9] = a9;
+      sSum[10] = a10;
+    }
+  } else {
   __syncthreads();
   rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
-  const double *C = v.cc + (size_t)gc * CC_N;
   for (int m = threadIdx.x; m < n; m += WAVE) {
     const int k = sSlot[m], p = sRank[m];
     double R, ws, rho;
@@ -2216,6 +2396,7 @@ __global__ __launch_bounds__(WAVE) void k_derived(KView v, double *out) {
       }
     }
     sSum[q] = acc;
+  }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -2665,15 +2846,22 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
   __shared__ NbrTab tb;
   nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
-  if (n > DML_CAP) {
-    if (lane == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
-    return;
-  }
-  __syncthreads();
-  rank_members(n, sId, sSub, &tb, sRank);
-  __syncthreads();
   const int dst0 = sc[c];
-  for (int m = lane; m < n; m += WAVE) copy_sip(v, sSlot[m], dst0 + sRank[m], gc);
+  if (n > DML_CAP) {
+    /* over the LDS capacity: one lane merges the members in id order, parking each source slot in the cell
+       array of its destination (this cell's range of the new buffer), then the wave copies */
+    if (lane == 0) {
+      int idx = 0;
+      merge_members(gc, v.cell, v.id, &tb, [&](int k) { v.cell2[dst0 + idx++] = k; });
+    }
+    __syncthreads();
+    for (int m = lane; m < n; m += WAVE) copy_sip(v, v.cell2[dst0 + m], dst0 + m, gc);
+  } else {
+    __syncthreads();
+    rank_members(n, sId, sSub, &tb, sRank);
+    __syncthreads();
+    for (int m = lane; m < n; m += WAVE) copy_sip(v, sSlot[m], dst0 + sRank[m], gc);
+  }
   /* SIPs that died in this home segment since the last reindex, in slot order */
   int dead = 0;
   const int ddst = sc[nc + 1 + c];
@@ -3299,6 +3487,8 @@ static int check_engine(const LmEngine *e) {
   if (e->ev_mode < 0 || e->ev_mode > 2) return set_err(LM_ERR_ARG, "bad ev_mode (0 off, 1 record, 2 replay)");
   if (e->mig_cnt && (!e->mig_slot || e->mig_cap < 1 || e->mig_cap > LM_MIG_CAP_MAX))
     return set_err(LM_ERR_ARG, "migrant lists need mig_slot and mig_cap in 1..LM_MIG_CAP_MAX (64)");
+  if (e->big_ws && (e->big_cap < 2 || e->big_cap > LM_BIG_CELL_MAX || e->big_blocks < 1 || e->big_blocks > 4096))
+    return set_err(LM_ERR_ARG, "big_ws needs big_cap in 2..LM_BIG_CELL_MAX (46340) and big_blocks in 1..4096");
   if (e->spec_events < 0 || e->spec_events > LM_SPEC_EVENTS_MAX)
     return set_err(LM_ERR_ARG, "spec_events must be 0 (default) or 1..LM_SPEC_EVENTS_MAX (1024)");
   if (e->rem_cap < 1 || e->rem_cap > LM_REM_CAP_MAX || !e->rem_slot || !e->rem_cnt)
@@ -3392,6 +3582,8 @@ const char *lm_last_error_message(void) { return g_errmsg; }
 
 int64_t lm_engine_sizeof(void) { return (int64_t)sizeof(LmEngine); }
 
+int64_t lm_big_ws_bytes(int32_t cap) { return cap < 2 || cap > LM_BIG_CELL_MAX ? -1 : (int64_t)big_stride(cap); }
+
 void lm_engine_offsets(int64_t *out, int32_t n) {
   const int64_t o[] = {
       (int64_t)offsetof(LmEngine, n_tot),     (int64_t)offsetof(LmEngine, solute),
@@ -3412,6 +3604,7 @@ void lm_engine_offsets(int64_t *out, int32_t n) {
       (int64_t)offsetof(LmEngine, mig_cap),
       (int64_t)offsetof(LmEngine, spec_events),
       (int64_t)offsetof(LmEngine, col_cost),
+      (int64_t)offsetof(LmEngine, big_ws), (int64_t)offsetof(LmEngine, big_cap),
   };
   const int m = (int)(sizeof(o) / sizeof(o[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = o[i];
@@ -3464,7 +3657,7 @@ int lm_collision_step(LmEngine *e, void *stream) {
   /* list A: cells over the main pass's capacity; list B (second half of ovf_list, counter ovf_cnt[2]): cells
      over 256 members, after a main pass below 256 */
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt, 0, 1);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 2, 0, 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 2, 0, e->big_ws ? 2 : 1);
   KView v = make_view(e);
   const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
   /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
@@ -3494,7 +3687,16 @@ int lm_collision_step(LmEngine *e, void *stream) {
     last_io.in_list = e->ovf_list + ncell;
     last_io.in_cnt = e->ovf_cnt + 2;
   }
+  if (e->big_ws) {
+    /* cells over 1024 members on to list C, then the pass over cells of any size */
+    last_io.out_list = e->ovf_list + 2 * (size_t)ncell;
+    last_io.out_cnt = e->ovf_cnt + 3;
+  }
   launch_collision<1024>(v, tab, inj, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), PASS_OVF, 0, last_io);
+  if (e->big_ws) {
+    const ColLists big_io = {e->ovf_list + 2 * (size_t)ncell, e->ovf_cnt + 3, nullptr, nullptr};
+    launch_collision<0>(v, tab, inj, dim3(e->big_blocks), S(stream), PASS_OVF, 0, big_io);
+  }
   LAUNCH_CHECK();
   if (e->ev_mode == 1) {
     /* the step's records in (cell, seq) order for the replay and the reverse sweep */
@@ -3993,8 +4195,8 @@ struct RefWork {
   int32_t *list;     /* n: SIP indices grouped by cell */
   double *rho;       /* n: pre-step density of each SIP (events only) */
   double *cc;        /* nc * CC_N: cell temperature and water density */
-  int32_t *ovf_list; /* nc */
-  int32_t *small;    /* 8: ovf_cnt[2], err[4], step counter, pad */
+  int32_t *ovf_list; /* 2 nc: cells over 256 members, then cells over 1024 */
+  int32_t *small;    /* 8: ovf_cnt[2], err[4], step counter, count of cells over 1024 members */
   long long *n_eq;   /* 1: near-equal collections */
   long long *rnd_off; /* nc + 2: injected-stream offsets */
 };
@@ -4002,7 +4204,7 @@ struct RefWork {
 static constexpr size_t ref_up(size_t b) { return (b + 255) & ~(size_t)255; }
 static constexpr size_t ref_bytes(long long n, long long nc) {
   return ref_up(4 * (nc + 1)) + ref_up(4 * nc) + ref_up(4 * n) + ref_up(8 * n) + ref_up(8 * CC_N * nc) +
-         ref_up(4 * nc) + ref_up(4 * 8) + ref_up(8) + ref_up(8 * (nc + 2));
+         ref_up(8 * nc) + ref_up(4 * 8) + ref_up(8) + ref_up(8 * (nc + 2));
 }
 
 static RefWork ref_carve(char *base, long long n, long long nc) {
@@ -4020,7 +4222,7 @@ static RefWork ref_carve(char *base, long long n, long long nc) {
   w.cc = (double *)(base + o);
   o += up(8 * CC_N * nc);
   w.ovf_list = (int32_t *)(base + o);
-  o += up(4 * nc);
+  o += up(8 * nc);
   w.small = (int32_t *)(base + o);
   o += up(4 * 8);
   w.n_eq = (long long *)(base + o);
@@ -4046,7 +4248,8 @@ static int ref_args(long long n_sips, const double *xis, const double *m_w, cons
 static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w, double *m_s, const double *vel,
                     const double *grid_T, const int64_t *cells, int nx, int nz, double dt_over_dV, const double *E,
                     int nb, double lnR0, double lnf, double E_const, double E_max, int64_t *no_cols, int solute,
-                    uint64_t seed, uint32_t step, const double *rnd_stream, long long rnd_len, hipStream_t st) {
+                    uint64_t seed, uint32_t step, const double *rnd_stream, long long rnd_len, hipStream_t st,
+                    char *big_ws = nullptr, int big_cap = 0) {
   const int nc = nx * nz;
   const long long N = n_sips;
   const unsigned gN = (unsigned)((N + 255) / 256), gC = (unsigned)((nc + 255) / 256);
@@ -4055,6 +4258,7 @@ static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w
   hipLaunchKernelGGL(k_fill_i32, dim3((nc + 1 + 255) / 256), dim3(256), 0, st, w.cnt, 0, nc + 1);
   hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, st, w.fill, 0, nc);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, w.small, 0, 6); /* overflow counters, error word */
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, w.small + 7, 0, 1);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)w.n_eq, 0, 2);
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, (uint32_t *)(w.small + 6), step);
   hipLaunchKernelGGL(k_ref_cc, dim3(gC), dim3(256), 0, st, nc, grid_T, w.cc);
@@ -4095,21 +4299,30 @@ static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w
   v.rnd_stream = rnd_stream;
   v.rnd_len = rnd_len;
   v.rnd_off = w.rnd_off;
+  v.big_ws = big_ws;
+  v.big_cap = big_cap;
   if (rnd_stream) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)(w.rnd_off + nc), 0, 4);
     hipLaunchKernelGGL(k_ref_count_pairs, dim3(gC), dim3(256), 0, st, nc, w.cnt, w.rnd_off);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.rnd_off, nc);
   }
   const bool tab = E != nullptr, inj = rnd_stream != nullptr;
-  /* main pass at the 256-member capacity, larger cells queued for the 1024-member pass */
-  const ColLists main_io = {nullptr, nullptr, w.ovf_list, w.small}, ovf_io = {w.ovf_list, w.small, nullptr, nullptr};
+  /* main pass at the 256-member capacity, larger cells queued for the 1024-member pass, and with a workspace
+     cells over 1024 members for the pass over cells of any size */
+  const ColLists main_io = {nullptr, nullptr, w.ovf_list, w.small};
+  const ColLists ovf_io = {w.ovf_list, w.small, big_ws ? w.ovf_list + nc : nullptr, big_ws ? w.small + 7 : nullptr};
   launch_collision<256>(v, tab, inj, dim3(nc), st, PASS_CELLS, 0, main_io);
   launch_collision<1024>(v, tab, inj, dim3(nc < OVF_GRID ? nc : OVF_GRID), st, PASS_OVF, 0, ovf_io);
+  if (big_ws) {
+    const ColLists big_io = {w.ovf_list + nc, w.small + 7, nullptr, nullptr};
+    launch_collision<0>(v, tab, inj, dim3(REF_BIG_BLOCKS), st, PASS_OVF, 0, big_io);
+  }
   LAUNCH_CHECK();
   return 0;
 }
 
-/* the allocating form's workspace, per device, grown on demand */
+/* the allocating form's workspace, per device, grown on demand: the step's scratch, then REF_BIG_BLOCKS regions
+   for cells of over 1024 members */
 static char *g_ref_ws[64];
 static size_t g_ref_ws_bytes[64];
 
@@ -4125,7 +4338,9 @@ int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return set_err(LM_ERR_ARG, "device index out of range");
-  const size_t need = ref_bytes(n_sips, (long long)nx * nz);
+  const int big_cap = n_sips < 2 ? 2 : (n_sips > LM_BIG_CELL_MAX ? LM_BIG_CELL_MAX : (int)n_sips);
+  const size_t base_bytes = ref_bytes(n_sips, (long long)nx * nz);
+  const size_t need = base_bytes + (size_t)REF_BIG_BLOCKS * big_stride(big_cap);
   if (g_ref_ws_bytes[dev] < need) {
     /* a larger workspace: the previous one may still be in use by queued work on this stream */
     HIP_TRY(hipStreamSynchronize(S(stream)));
@@ -4139,7 +4354,8 @@ int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s
   const RefWork w = ref_carve(g_ref_ws[dev], n_sips, (long long)nx * nz);
   g_ref_err[dev][0] = w.small + 2;
   return ref_step(w, n_sips, xis, m_w, m_s, vel, grid_T, cells, nx, nz, dt_over_dV, E_col, nb, lnR0, lnf, E_const,
-                  E_max, no_cols, solute, seed, step, rnd_stream, rnd_len, S(stream));
+                  E_max, no_cols, solute, seed, step, rnd_stream, rnd_len, S(stream), g_ref_ws[dev] + base_bytes,
+                  big_cap);
 }
 
 /*
@@ -4202,7 +4418,9 @@ int lm_collision_all_cells_check(int32_t no_alloc, void *stream) {
   HIP_TRY(hipStreamSynchronize(S(stream)));
   char msg[200];
   if (h[0] == LM_ERR_CELL_CAP)
-    snprintf(msg, sizeof msg, "cell %d holds more than 1024 SIPs (the all-cells collision kernels' capacity)", h[1]);
+    snprintf(msg, sizeof msg,
+             "cell %d holds more SIPs than the all-cells collision step takes (1024 without allocation, else "
+             "LM_BIG_CELL_MAX)", h[1]);
   else if (h[0] == LM_ERR_RND_SHORT)
     snprintf(msg, sizeof msg, "the injected uniform stream is shorter than the step's pairs");
   else

@@ -471,14 +471,32 @@ __global__ __launch_bounds__(WAVE) void k_objective(KView v, double width, doubl
   __shared__ NbrTab tb;
   nbr_table(gc, nc, v.nx, v.nz, v.seg + (size_t)r * (nc + 1), v.away + (size_t)r * nc, &tb);
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
+  const double *C = v.cc + (size_t)gc * CC_N;
   if (n > DML_CAP) {
-    if (threadIdx.x == 0) raise_err(v.err, LM_ERR_CELL_CAP, gc);
+    /* over the LDS capacity: one lane runs both compensated sums straight from the segments, ascending id */
+    if (threadIdx.x == 0) {
+      double h0 = 0.0, l0 = 0.0, h1 = 0.0, l1 = 0.0;
+      merge_members(gc, v.cell, v.id, &tb, [&](int k) {
+        double R, ws, rho;
+        lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], C[CC_T], C[CC_RHOW], R, ws, rho);
+        double t, e;
+        lm_two_sum(h0, v.xi[k] * v.m_w[k], &t, &e);
+        h0 = t;
+        l0 += e;
+        lm_two_sum(h1, v.xi[k] * cloud_indicator(R, width), &t, &e);
+        h1 = t;
+        l1 += e;
+      });
+      out[gc] = h0;
+      out[(size_t)ncell + gc] = h1;
+      out[(size_t)2 * ncell + gc] = l0;
+      out[(size_t)3 * ncell + gc] = l1;
+    }
     return;
   }
   __syncthreads();
   rank_members(n, sId, sSub, &tb, sRank);
   __syncthreads();
-  const double *C = v.cc + (size_t)gc * CC_N;
   for (int m = threadIdx.x; m < n; m += WAVE) {
     const int k = sSlot[m];
     double R, ws, rho;

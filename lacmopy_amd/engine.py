@@ -22,6 +22,8 @@ CC_N, CC_TP = 16, 15  # per-cell constant table: width, slot of the pre-update c
 WORK_BINS = 1024  # LM_WORK_BINS: size bins of the work-list order (include/lacmo_amd.h)
 REM_CAP_MAX = 1024  # LM_REM_CAP_MAX: ground removals per run and substep (include/lacmo_amd.h)
 MIG_CAP = 32  # migrant-list entries per cell (LmEngine.mig_cap, at most LM_MIG_CAP_MAX = 64)
+BIG_CAP = 46340  # members of a cell the collision step takes (LmEngine.big_cap = LM_BIG_CELL_MAX)
+BIG_BLOCKS = 4  # workgroups of the collision pass over cells of more than 1024 members, one workspace region each
 
 
 def _ptr(t):
@@ -91,8 +93,8 @@ class Engine:
         mk("scan_buf", (L + 1,), i32)
         mk("scan_part", ((L + 4095) // 4096 + 1,), i32)
         mk("err", (4,), i32)
-        mk("ovf_list", (2 * nr * nc,), i32)
-        mk("ovf_cnt", (3,), i32)
+        mk("ovf_list", (3 * nr * nc,), i32)
+        mk("ovf_cnt", (4,), i32)
         mk("rnd_off", (nr * nc + 2,), i64)
         # (run, cell) work list of the persistent passes (rebuilt by every reindex while one is on; starts as the
         # identity order), its size-bin scratch and two work counters
@@ -120,6 +122,11 @@ class Engine:
             setattr(e, f, self.buf[f].data_ptr())
         e.rem_cap = int(rem_cap)
         e.mig_cap = MIG_CAP
+        # the collision pass over cells of more than 1024 members: BIG_BLOCKS regions of its member state
+        ws = int(self.L.lm_big_ws_bytes(BIG_CAP))
+        self.buf["big_ws"] = torch.empty(BIG_BLOCKS * ws, dtype=torch.uint8, device=self.device)
+        e.big_ws = self.buf["big_ws"].data_ptr()
+        e.big_cap, e.big_blocks = BIG_CAP, BIG_BLOCKS
         e.vel_run_stride = 2 * (self.nx + 1) * (self.nz + 1)
         e.e_const_mode = 1
         e.E_const = 0.5

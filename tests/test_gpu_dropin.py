@@ -204,11 +204,14 @@ def test_collision_all_cells_np(gpu, kernel):
     assert o_nc.sum() > 0 and N > 0
 
 
-@pytest.mark.parametrize("variant", ["device_tensors", "injected_stream", "crowded_cell"])
+@pytest.mark.parametrize("variant", ["device_tensors", "injected_stream", "crowded_cell", "cell_of_2000",
+                                     "cell_of_2000_injected"])
 def test_collision_all_cells_on_device_tensors(gpu, variant):
     """The all-cells drop-in on CUDA tensors of the reference's shapes updates them where they are (same
     storage, no copies) and matches the oracle bit for bit; with an injected uniform stream (consumed in the
-    reference's order); with one cell of 700 SIPs (the 1024-member pass)."""
+    reference's order); with one cell of 700 SIPs (the 1024-member pass); with one cell of 2000 SIPs (the pass
+    over cells of any size, its members ranked by SIP index in global memory), with Philox draws and with an
+    injected stream."""
     import torch
     from lacmopy_amd.collision.all_or_nothing import collision_step_Ecol_grid_R_all_cells_2D_multicomp_np as step
     E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
@@ -217,13 +220,15 @@ def test_collision_all_cells_on_device_tensors(gpu, variant):
     cells = cells.copy()
     if variant == "crowded_cell":
         cells[:, :700] = np.array([[2], [5]])
+    if variant.startswith("cell_of_2000"):
+        cells[:, 300:2300] = np.array([[2], [5]])
     cells[1, ::23] = -1
     N = len(xi)
     vel = vel.copy()
     vel[1] -= np.random.default_rng(4).uniform(0.0, 3.0, N)
     T = grid.temperature.copy()
     rnd = None
-    if variant == "injected_stream":
+    if variant in ("injected_stream", "cell_of_2000_injected"):
         act = cells[1] >= 0
         cnt = np.bincount(cells[0, act] * 12 + cells[1, act], minlength=72)
         rnd = np.random.default_rng(8).random(int((cnt * (cnt - 1) // 2).sum()))

@@ -305,6 +305,58 @@ def test_one_crowded_cell_takes_the_overflow_passes(gpu):
     _assert_same(eng.counters(0)[0], no_cols, "no_cols")
 
 
+@pytest.mark.parametrize("k", [1400, 2600])
+def test_cells_over_1024_members_take_the_any_size_passes(gpu, k):
+    """One cell holding about k SIPs among cells of ~30 -- over every LDS capacity: the collision step's
+    1024-member pass queues it for the pass over cells of any size (its members merged in id order from the
+    3x3 home segments into a global workspace), and the Delta m_l, r_l and derived-field passes sum it on one
+    lane straight from the segments.  Two advection steps, then r_l and the derived fields: bit-identical to
+    the oracle."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, pos, vel, cells, m_w, m_s, xi, act = _case(nx=4, nz=4, spc=30, seed=19)
+    rs = np.random.default_rng(5)
+    src = rs.integers(0, len(xi), k)
+    extra_pos = np.array([rs.uniform(20.0, 40.0, k), rs.uniform(20.0, 40.0, k)])
+    parts = (np.concatenate([pos, extra_pos], 1), np.concatenate([vel, vel[:, src]], 1),
+             np.concatenate([cells, np.ones((2, k), dtype=np.int64)], 1), np.concatenate([m_w, m_w[src]]),
+             np.concatenate([m_s, m_s[src]]), np.concatenate([xi, xi[src]]), np.concatenate([act, np.ones(k, bool)]))
+    parts = _prep(grid, parts)
+    counts = np.bincount(parts[2][0] * 4 + parts[2][1], minlength=16)
+    assert counts[5] > max(1024, k) and np.median(counts) < 60
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 8, 2)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 8)
+    eng.adv_steps(2)
+    got = eng.export_particles(0)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], got, ref_parts):
+        _assert_same(a, b, nm)
+    s, m = eng.export_grid(0)
+    _assert_same(s[:8], scal[:8], "grid_scalar_fields")
+    _assert_same(eng.counters(0)[0], no_cols, "no_cols")
+    assert no_cols.sum() > 0
+    pos2, vel2, cells2, m_w2, m_s2, xi2, act2 = got
+    assert np.bincount(cells2[0, act2] * 4 + cells2[1, act2], minlength=16).max() > 1024
+    eng.update_r_l()
+    s, _ = eng.export_grid(0)
+    ref = np.zeros((4, 4))
+    orc.lib().or_update_grid_r_l(len(xi2), orc.P(m_w2), orc.P(xi2), orc.P(cells2), orc.P(act2.view(np.uint8)), 4, 4,
+                                 orc.P(ref), orc.P(s[8].copy()))
+    _assert_same(s[5], ref, "r_l")
+    from lacmopy_amd.evaluation import derived_fields
+    d = derived_fields(eng)
+    _assert_same(d[0], orc.derived_fields(m_w2, m_s2, xi2, cells2, act2, s[0].copy(), s[8].copy(),
+                                          grid.volume_cell), "derived fields")
+    # the adjoint's objective sums (compensated, per cell in ascending id; the crowded cell on one lane):
+    # J_lwp is the correctly rounded total of xi m_w times its constant
+    import math
+    from lacmopy_amd.adjoint import AdjointWindow
+    J_nc, J_lwp = AdjointWindow(eng, 1).objectives()
+    e = eng.e
+    c_lwp = 1.0E-18 / ((e.x_max - e.x_min) * (e.V_cell / (e.dx * e.dz)))
+    want = c_lwp * math.fsum((xi2 * m_w2)[act2])
+    assert abs(J_lwp[0] - want) <= 2e-16 * abs(want), (J_lwp[0], want)
+
+
 @pytest.mark.parametrize("order", ["cost", "size"])
 @pytest.mark.parametrize("work_per_cu,chunk", [(1, 0), (1, 1), (4, 3)])
 def test_persistent_work_list_passes_match_oracle(gpu, work_per_cu, chunk, order):
