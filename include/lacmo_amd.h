@@ -485,9 +485,9 @@ int lm_box_check(void *stream);
  * constant efficiency E_const (Ecol_const); E_max the table's largest entry (0: unknown, the pair filter then
  * evaluates every pair).  no_cols (2) int64 += (single, multiple) collections.  Draws keyed (seed, run 0, step,
  * cell) as lm_collision_step's, or rnd_stream (rnd_len values, consumed in the reference's order).  Cells of
- * up to 1024 members.  Stream-ordered, no host synchronisation; the library keeps a device workspace (grown
- * on demand).  Errors of the kernels (a cell over capacity, a short stream) are reported by
- * lm_collision_all_cells_check(0, stream).
+ * up to LM_BIG_CELL_MAX members (those over 1024 in a global workspace).  Stream-ordered, no host
+ * synchronisation; the library keeps a device workspace (grown on demand).  Errors of the kernels (a cell over
+ * capacity, a short stream) are reported by lm_collision_all_cells_check(0, stream).
  */
 int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s, const double *vel,
                            const double *grid_T, const int64_t *cells, int32_t nx, int32_t nz, double dt_over_dV,
@@ -499,7 +499,8 @@ int lm_collision_all_cells(int64_t n_sips, double *xis, double *m_w, double *m_s
  * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
  * arguments and results; its scratch (member lists, densities, the per-cell table, counters) is one fixed
  * device array of the library (about 17 MB per GPU, held until the process ends), so it takes at most
- * LM_NOALLOC_MAX_SIPS SIPs on at most LM_NOALLOC_MAX_CELLS cells (larger calls fail with LM_ERR_ARG) and
+ * LM_NOALLOC_MAX_SIPS SIPs on at most LM_NOALLOC_MAX_CELLS cells (larger calls fail with LM_ERR_ARG), cells
+ * of at most 1024 members (larger ones: LM_ERR_CELL_CAP; there is no workspace for the any-size pass), and
  * calls that overlap on one GPU race on it.  Kernel errors: lm_collision_all_cells_check(1, stream).
  */
 #define LM_NOALLOC_MAX_SIPS (1 << 20)
