@@ -283,6 +283,24 @@ const char *lm_version(void);
 const char *lm_last_error_message(void);
 /* sizeof(LmEngine) and byte offsets of selected fields (for binding checks). */
 int64_t lm_engine_sizeof(void);
+/*
+ * One run's SIPs between the reference's layout on the device and the engine's: the device form of the
+ * drop-in boundary of integrate_adv_step_np and the subloop functions (integration.py:944-1294, 1298-1611),
+ * whose arrays are pos, vel (2, n) float64, cells (2, n) int64, m_w, m_s, xi (n) float64 and active_ids (n)
+ * bytes (0/1) in SIP-index order.  run occupies the engine's slots [base, base + n).  Load: the engine's
+ * order (active SIPs by cell, then index; inactive SIPs after them by index), its segment table, no
+ * direction bits or migrants; an active SIP outside the grid is an error (LM_ERR_ARG through
+ * lm_engine_check).  Store: the arrays scattered back by index, cells from the engine's cell codes
+ * (inactive: cells[1] = -1), and when not NULL rel_pos (2, n) of the active SIPs as integration.py:545-549
+ * computes it and T_p (n), the cell temperature each SIP saw in the last particle substep
+ * (integration.py:489-507).  Stream-ordered; the load uses the engine's idle buffers as scratch.
+ */
+int lm_load_reference_layout(LmEngine *e, int32_t run, int64_t base, int64_t n, const double *pos,
+                             const double *vel, const int64_t *cells, const double *m_w, const double *m_s,
+                             const double *xi, const uint8_t *active_ids, void *stream);
+int lm_store_reference_layout(LmEngine *e, int32_t run, int64_t base, int64_t n, double *pos, double *vel,
+                              int64_t *cells, double *rel_pos, double *m_w, double *m_s, double *xi,
+                              uint8_t *active_ids, double *T_p, void *stream);
 /* Bytes of one workgroup's region of LmEngine.big_ws for cells of up to cap members (no counterpart in the
    reference, which keeps a cell's members in NumPy arrays of any size, all_or_nothing.py:811-846). */
 int64_t lm_big_ws_bytes(int32_t cap);

@@ -109,7 +109,7 @@ EXPORTED = [
     "lm_derived_fields", "lm_ensemble_stats", "lm_build_slot_of", "lm_objective", "lm_adj_seed",
     "lm_adj_adv_steps", "lm_init_par_sizeof", "lm_init_level_profile", "lm_init_level_sips", "lm_init_positions",
     "lm_box_minmax", "lm_box_stats", "lm_mean_sem", "lm_collision_all_cells", "lm_collision_all_cells_check",
-    "lm_collision_all_cells_noalloc", "lm_big_ws_bytes",
+    "lm_collision_all_cells_noalloc", "lm_big_ws_bytes", "lm_load_reference_layout", "lm_store_reference_layout",
 ]
 
 _lib = None
@@ -135,6 +135,8 @@ def lib():
     L.lm_engine_sizeof.restype = _i64
     L.lm_big_ws_bytes.argtypes = [_i32]
     L.lm_big_ws_bytes.restype = _i64
+    L.lm_load_reference_layout.argtypes = [E, _i32, _i64, _i64] + [_p] * 7 + [_p]
+    L.lm_store_reference_layout.argtypes = [E, _i32, _i64, _i64] + [_p] * 9 + [_p]
     L.lm_engine_offsets.argtypes = [C.POINTER(_i64), _i32]
     L.lm_get_constants.argtypes = [C.POINTER(_d), _i32]
     L.lm_num_constants.restype = _i32

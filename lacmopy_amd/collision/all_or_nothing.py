@@ -47,31 +47,42 @@ def _stream(torch):
 def _rnd(torch, rnd, n):
     if rnd is None:
         return None
-    r = np.ascontiguousarray(rnd, dtype=np.float64).reshape(-1)
-    if len(r) < n * (n - 1) // 2:
-        raise ValueError(f"rnd holds {len(r)} uniforms, the step needs {n * (n - 1) // 2}")
-    return _dev(torch, r)
+    if isinstance(rnd, torch.Tensor):
+        r = rnd.reshape(-1).to(device="cuda", dtype=torch.float64).contiguous()
+    else:
+        r = _dev(torch, np.ascontiguousarray(rnd, dtype=np.float64).reshape(-1))
+    if r.numel() < n * (n - 1) // 2:
+        raise ValueError(f"rnd holds {r.numel()} uniforms, the step needs {n * (n - 1) // 2}")
+    return r
 
 
-def _finish(torch, L, no_cols, nc_dev):
+def _finish(torch, L, no_cols, nc_dev, pairs=()):
+    """Error check of the box kernels, the collision counters added to no_cols, and the device results copied
+    back into the caller's NumPy arrays (CUDA tensors were updated where they are)."""
     check(L.lm_box_check(_stream(torch)))
-    no_cols[...] = no_cols + nc_dev.cpu().numpy().reshape(2)
+    if isinstance(no_cols, torch.Tensor):
+        no_cols += nc_dev.reshape(no_cols.shape).to(no_cols.device)
+    else:
+        no_cols[...] = no_cols + nc_dev.cpu().numpy().reshape(2)
+    for host, dev in pairs:
+        if not isinstance(host, torch.Tensor):
+            host[...] = dev.cpu().numpy()
 
 
 def collision_step_Golovin_np(xis, masses, dt_over_dV, no_cols, *, rnd=None):
-    """all_or_nothing.py:109-185: Golovin kernel K = b (m_i + m_j), box model."""
+    """all_or_nothing.py:109-185: Golovin kernel K = b (m_i + m_j), box model.  NumPy arrays (copied to the GPU
+    and back) or CUDA float64 tensors (updated where they are)."""
     torch = _torch()
     L = lib()
-    n = len(xis)
-    x, m = _dev(torch, xis), _dev(torch, masses)
+    n = int(xis.shape[0])
+    f64 = torch.float64
+    x, m = _as_device(torch, xis, f64, "xis", (n,)), _as_device(torch, masses, f64, "masses", (n,))
     off = _dev(torch, [0, n], np.int32)
     nc = torch.zeros(2, dtype=torch.int64, device="cuda")
     r = _rnd(torch, rnd, n)
     check(L.lm_golovin_step(1, _p(off), _p(x), _p(m), float(dt_over_dV), _p(nc), rng.get_seed(), rng.take_steps(1),
                             _p(r), _stream(torch)))
-    _finish(torch, L, no_cols, nc)
-    xis[...] = x.cpu().numpy()
-    masses[...] = m.cpu().numpy()
+    _finish(torch, L, no_cols, nc, ((xis, x), (masses, m)))
 
 
 collision_step_Golovin = collision_step_Golovin_np
@@ -79,22 +90,22 @@ collision_step_Golovin = collision_step_Golovin_np
 
 def collision_step_Ecol_grid_R_np(xis, masses, radii, vel, mass_densities, dt_over_dV, E_col_grid, no_kernel_bins,
                                   R_kernel_low_log, bin_factor_R_log, no_cols, *, rnd=None):
-    """all_or_nothing.py:269-390: hydrodynamic kernel with tabulated E_col(R1, R2), box model, |v| 1-D."""
+    """all_or_nothing.py:269-390: hydrodynamic kernel with tabulated E_col(R1, R2), box model, |v| 1-D.  NumPy
+    arrays or CUDA float64 tensors (updated where they are)."""
     torch = _torch()
     L = lib()
-    n = len(xis)
-    x, m, R, v, rho = (_dev(torch, a) for a in (xis, masses, radii, vel, mass_densities))
-    E = _dev(torch, E_col_grid)
+    n = int(xis.shape[0])
+    x, m, R, v, rho = (_as_device(torch, a, torch.float64, nm, (n,)) for a, nm in
+                       ((xis, "xis"), (masses, "masses"), (radii, "radii"), (vel, "vel"),
+                        (mass_densities, "mass_densities")))
+    E, _ = _device_table(torch, E_col_grid)
     off = _dev(torch, [0, n], np.int32)
     nc = torch.zeros(2, dtype=torch.int64, device="cuda")
     r = _rnd(torch, rnd, n)
     check(L.lm_ecol_box_step(1, _p(off), _p(x), _p(m), _p(R), _p(v), _p(rho), float(dt_over_dV), _p(E),
                              int(no_kernel_bins), float(R_kernel_low_log), float(bin_factor_R_log), _p(nc),
                              rng.get_seed(), rng.take_steps(1), _p(r), 0, _stream(torch)))
-    _finish(torch, L, no_cols, nc)
-    xis[...] = x.cpu().numpy()
-    masses[...] = m.cpu().numpy()
-    radii[...] = R.cpu().numpy()
+    _finish(torch, L, no_cols, nc, ((xis, x), (masses, m), (radii, R)))
 
 
 collision_step_Ecol_grid_R = collision_step_Ecol_grid_R_np
@@ -103,21 +114,21 @@ collision_step_Ecol_grid_R = collision_step_Ecol_grid_R_np
 def _box2d(xis, m_w, m_s, radii, vel, mass_densities, dt_over_dV, E_col_grid, E_const, nb, lnR0, lnf, no_cols, rnd):
     torch = _torch()
     L = lib()
-    n = len(xis)
-    x, mw, ms, R, rho = (_dev(torch, a) for a in (xis, m_w, m_s, radii, mass_densities))
-    vx, vz = _dev(torch, vel[0]), _dev(torch, vel[1])
-    E = _dev(torch, E_col_grid) if E_col_grid is not None else None
+    n = int(xis.shape[0])
+    f64 = torch.float64
+    x, mw, ms, R, rho = (_as_device(torch, a, f64, nm, (n,)) for a, nm in
+                         ((xis, "xis"), (m_w, "m_w"), (m_s, "m_s"), (radii, "radii"),
+                          (mass_densities, "mass_densities")))
+    v = _as_device(torch, vel, f64, "vel", (2, n))
+    vx, vz = v[0], v[1]  # contiguous rows of the (2, n) array
+    E = _device_table(torch, E_col_grid)[0] if E_col_grid is not None else None
     off = _dev(torch, [0, n], np.int32)
     nc = torch.zeros(2, dtype=torch.int64, device="cuda")
     r = _rnd(torch, rnd, n)
     check(L.lm_box2d_step(1, _p(off), _p(x), _p(mw), _p(ms), _p(R), _p(vx), _p(vz), _p(rho), float(dt_over_dV),
                           _p(E), float(E_const), int(nb), float(lnR0), float(lnf), _p(nc), rng.get_seed(),
                           rng.take_steps(1), _p(r), _stream(torch)))
-    _finish(torch, L, no_cols, nc)
-    xis[...] = x.cpu().numpy()
-    m_w[...] = mw.cpu().numpy()
-    m_s[...] = ms.cpu().numpy()
-    radii[...] = R.cpu().numpy()
+    _finish(torch, L, no_cols, nc, ((xis, x), (m_w, mw), (m_s, ms), (radii, R)))
 
 
 def collision_step_Ecol_grid_R_2D_multicomp_np(xis, m_w, m_s, radii, vel, mass_densities, dt_over_dV, E_col_grid,

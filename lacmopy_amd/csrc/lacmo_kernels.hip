@@ -2876,6 +2876,140 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* reference layout <-> engine layout on the device                          */
+/* ------------------------------------------------------------------------ */
+/*
+ * One run's SIPs from the reference's arrays (pos, vel (2, N), cells (2, N) int64, m_w, m_s, xi (N),
+ * active_ids (N) bytes) into the engine's order: active SIPs by (cell, index), then the inactive ones by
+ * index (the graveyard).  A counting scatter into per-cell lists, each list ranked by index; the graveyard
+ * by an exclusive scan of the inactive flags.
+ */
+__global__ void k_lay_count(long long N, const int64_t *cells, const uint8_t *active, int nx, int nz, int32_t *cnt,
+                            int32_t *dead, int32_t *err) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const long long cx = cells[k], cz = cells[N + k];
+  if (active[k]) {
+    dead[k] = 0;
+    if (cx < 0 || cx >= nx || cz < 0 || cz >= nz) {
+      raise_err(err, LM_ERR_ARG, (int)k);
+      return;
+    }
+    atomicAdd(&cnt[cx * nz + cz], 1);
+  } else {
+    dead[k] = 1;
+    if (cx < 0 || cx >= nx) raise_err(err, LM_ERR_ARG, (int)k);
+  }
+}
+
+__global__ void k_lay_scatter(long long N, const int64_t *cells, const uint8_t *active, int nx, int nz,
+                              const int32_t *off, int32_t *fill, int32_t *list) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N || !active[k]) return;
+  const long long cx = cells[k], cz = cells[N + k];
+  if (cx < 0 || cx >= nx || cz < 0 || cz >= nz) return;
+  const int c = (int)(cx * nz + cz);
+  list[off[c] + atomicAdd(&fill[c], 1)] = (int32_t)k;
+}
+
+/* the engine's slot s of SIP k of the reference arrays */
+__device__ static inline void lay_put(const KView &v, long long N, long long s, int k, int cellv, const double *pos,
+                                      const double *vel, const double *m_w, const double *m_s, const double *xi) {
+  v.x[s] = pos[k];
+  v.z[s] = pos[N + k];
+  v.u[s] = vel[k];
+  v.w[s] = vel[N + k];
+  v.m_w[s] = m_w[k];
+  v.m_s[s] = m_s[k];
+  v.xi[s] = xi[k];
+  v.id[s] = k;
+  v.cell[s] = cellv;
+}
+
+#define LAY_CAP 1024
+/* one wave per cell of run r: its members in ascending index (ranked by counting; from LDS up to LAY_CAP) */
+__global__ __launch_bounds__(WAVE) void k_lay_cell(KView v, int r, long long N, long long base, const double *pos,
+                                                   const double *vel, const double *m_w, const double *m_s,
+                                                   const double *xi, const int32_t *off, const int32_t *list) {
+  __shared__ int sk[LAY_CAP];
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int o0 = off[c], n = off[c + 1] - o0;
+  const bool lds = n <= LAY_CAP;
+  if (lds)
+    for (int p = lane; p < n; p += WAVE) sk[p] = list[o0 + p];
+  __syncthreads();
+  for (int p = lane; p < n; p += WAVE) {
+    const int k = lds ? sk[p] : list[o0 + p];
+    int rk = 0;
+    if (lds)
+      for (int q = 0; q < n; ++q) rk += sk[q] < k;
+    else
+      for (int q = 0; q < n; ++q) rk += list[o0 + q] < k;
+    lay_put(v, N, base + o0 + rk, k, r * v.nc + c, pos, vel, m_w, m_s, xi);
+  }
+  if (lane == 0) v.seg[(size_t)r * (v.nc + 1) + c] = (int32_t)(base + o0);
+}
+
+/* the inactive SIPs after the cells, ascending index (dead: exclusive scan of the inactive flags) */
+__global__ void k_lay_dead(KView v, int r, long long N, long long base, long long n_tot, const double *pos,
+                           const double *vel, const int64_t *cells, const uint8_t *active, const double *m_w,
+                           const double *m_s, const double *xi, const int32_t *off, const int32_t *dead) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = v.nc;
+  if (k == 0) {
+    v.seg[(size_t)r * (nc + 1) + nc] = (int32_t)(base + off[nc]);
+    v.seg[(size_t)v.n_runs * (nc + 1)] = (int32_t)n_tot;
+  }
+  if (k >= N || active[k]) return;
+  lay_put(v, N, base + off[nc] + dead[k], (int)k, -(2 + (int)cells[k]), pos, vel, m_w, m_s, xi);
+}
+
+/* engine layout of run r -> the reference's arrays (scatter by id); rel_pos and T_p as integration.py:489-550
+   leaves them (either may be NULL) */
+__global__ void k_lay_store(KView v, int r, long long N, long long base, double *pos, double *vel, int64_t *cells,
+                            double *rel_pos, double *m_w, double *m_s, double *xi, uint8_t *active, double *T_p,
+                            const int32_t *cell_prev) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const long long s = base + t;
+  const int k = v.id[s], c = v.cell[s], nc = v.nc, nz = v.nz;
+  if (k < 0 || k >= N) { /* not this run's ids: the engine holds another layout */
+    raise_err(v.err, LM_ERR_ARG, (int)t);
+    return;
+  }
+  const double x = v.x[s], z = v.z[s];
+  pos[k] = x;
+  pos[N + k] = z;
+  vel[k] = v.u[s];
+  vel[N + k] = v.w[s];
+  m_w[k] = v.m_w[s];
+  m_s[k] = v.m_s[s];
+  xi[k] = v.xi[s];
+  if (c >= 0) {
+    const int loc = c - r * nc, cx = loc / nz, cz = loc % nz; /* a code of this run: 0 <= loc < nc */
+    cells[k] = cx;
+    cells[N + k] = cz;
+    active[k] = 1;
+    if (rel_pos) {
+      rel_pos[k] = (x - v.x_min) / v.dx - (double)cx;
+      rel_pos[N + k] = (z - v.z_min) / v.dz - (double)cz;
+    }
+  } else {
+    cells[k] = -c - 2;
+    cells[N + k] = -1;
+    active[k] = 0;
+  }
+  if (T_p) {
+    const int prev = cell_prev[s];
+    const int loc = prev >= 0 ? prev - r * nc : (-prev - 2) * nz + (nz - 1);
+    if (loc >= 0 && loc < nc)
+      T_p[k] = v.cc[((size_t)r * nc + loc) * CC_N + CC_TP];
+    else
+      raise_err(v.err, LM_ERR_ARG, (int)t);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* all-cells collision step on the reference's arrays                       */
 /* ------------------------------------------------------------------------ */
 /* cell temperature and water density of the per-cell table (the slots the collision step reads) */
@@ -3892,6 +4026,63 @@ int lm_update_r_l(LmEngine *e, void *stream) {
   if (rc) return rc;
   KView v = make_view(e);
   hipLaunchKernelGGL(k_rl, dim3(e->n_runs * e->nc), dim3(WAVE), 0, S(stream), v);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_load_reference_layout(LmEngine *e, int32_t run, int64_t base, int64_t n, const double *pos,
+                             const double *vel, const int64_t *cells, const double *m_w, const double *m_s,
+                             const double *xi, const uint8_t *active_ids, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (run < 0 || run >= e->n_runs) return set_err(LM_ERR_ARG, "run index out of range");
+  if (base < 0 || n < 0 || base + n > e->n_tot) return set_err(LM_ERR_ARG, "run slots outside the engine");
+  if (n > 0 && (!pos || !vel || !cells || !m_w || !m_s || !xi || !active_ids))
+    return set_err(LM_ERR_ARG, "array is NULL");
+  const int nc = e->nc;
+  KView v = make_view(e);
+  /* scratch: counts (scan_buf, nc + 1 with the total), cursors (ovf_list), the cell lists (the idle id
+     buffer), the inactive prefix (the idle x buffer, N + 1 ints) */
+  int32_t *cnt = e->scan_buf, *fill = e->ovf_list, *list = e->id[1 - e->cur];
+  int32_t *dead = (int32_t *)e->x[1 - e->cur];
+  const unsigned gN = (unsigned)((n + 255) / 256), gC = (unsigned)((nc + 1 + 255) / 256);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, S(stream), cnt, 0, nc + 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gC), dim3(256), 0, S(stream), fill, 0, nc);
+  if (n > 0)
+    hipLaunchKernelGGL(k_lay_count, dim3(gN), dim3(256), 0, S(stream), (long long)n, cells, active_ids, e->nx, e->nz,
+                       cnt, dead, e->err);
+  hipLaunchKernelGGL(k_scan1_i32, dim3(1), dim3(1024), 0, S(stream), cnt, nc);
+  hipLaunchKernelGGL(k_scan1_i32, dim3(1), dim3(1024), 0, S(stream), dead, (int)n);
+  if (n > 0)
+    hipLaunchKernelGGL(k_lay_scatter, dim3(gN), dim3(256), 0, S(stream), (long long)n, cells, active_ids, e->nx,
+                       e->nz, cnt, fill, list);
+  hipLaunchKernelGGL(k_lay_cell, dim3(nc), dim3(WAVE), 0, S(stream), v, (int)run, (long long)n, (long long)base, pos,
+                     vel, m_w, m_s, xi, cnt, list);
+  hipLaunchKernelGGL(k_lay_dead, dim3(gN > 0 ? gN : 1), dim3(256), 0, S(stream), v, (int)run, (long long)n,
+                     (long long)base, (long long)e->n_tot, pos, vel, cells, active_ids, m_w, m_s, xi, cnt, dead);
+  /* every SIP sits in its home cell: no direction bits, no migrants */
+  hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, S(stream), e->away + (size_t)run * nc, 0, nc);
+  if (e->mig_cnt)
+    hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, S(stream), e->mig_cnt + (size_t)run * nc, 0,
+                       nc);
+  LAUNCH_CHECK();
+  return 0;
+}
+
+int lm_store_reference_layout(LmEngine *e, int32_t run, int64_t base, int64_t n, double *pos, double *vel,
+                              int64_t *cells, double *rel_pos, double *m_w, double *m_s, double *xi,
+                              uint8_t *active_ids, double *T_p, void *stream) {
+  int rc = check_engine(e);
+  if (rc) return rc;
+  if (run < 0 || run >= e->n_runs) return set_err(LM_ERR_ARG, "run index out of range");
+  if (base < 0 || n < 0 || base + n > e->n_tot) return set_err(LM_ERR_ARG, "run slots outside the engine");
+  if (n > 0 && (!pos || !vel || !cells || !m_w || !m_s || !xi || !active_ids))
+    return set_err(LM_ERR_ARG, "array is NULL");
+  if (n == 0) return 0;
+  KView v = make_view(e);
+  hipLaunchKernelGGL(k_lay_store, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, (int)run,
+                     (long long)n, (long long)base, pos, vel, cells, rel_pos, m_w, m_s, xi, active_ids, T_p,
+                     (const int32_t *)e->cell[1 - e->ccur]);
   LAUNCH_CHECK();
   return 0;
 }

@@ -206,12 +206,15 @@ class Engine:
             e.E = None
             e.E_max = 0.0
         else:
-            E = np.ascontiguousarray(E_col_grid, dtype=np.float64)
-            self.E_tab = torch.from_numpy(E).to(self.device)
+            if isinstance(E_col_grid, torch.Tensor):
+                self.E_tab = E_col_grid.to(device=self.device, dtype=torch.float64).contiguous()
+            else:
+                self.E_tab = torch.from_numpy(np.ascontiguousarray(E_col_grid, dtype=np.float64)).to(self.device)
+            E = self.E_tab
             e.E = self.E_tab.data_ptr()
             # bound for the collision kernel's first filter (disabled for non-finite or negative tables)
-            e.E_max = float(E.max()) if np.isfinite(E).all() and E.min() >= 0.0 else 0.0
-            e.nb = E.shape[0]
+            e.E_max = float(E.max()) if bool(torch.isfinite(E).all()) and float(E.min()) >= 0.0 else 0.0
+            e.nb = int(E.shape[0])
             e.e_const_mode = 0
             e.lnR0 = float(R_kernel_low_log)
             e.lnf = float(bin_factor_R_log)
@@ -220,24 +223,22 @@ class Engine:
         """The dry-air mass flux (one field, the same for all runs) and the face velocities derived from it:
         velocity (2, nx+1, nz+1) for every run, or for run `run` only (runs initialised apart divide the flux
         by their own dry-density profile, init.py:1179-1182)."""
-        v = torch.from_numpy(np.ascontiguousarray(velocity, dtype=np.float64))
+        v = _as_f64(velocity)
         if run is None:
             self.buf["grid_vel"].copy_(v.expand_as(self.buf["grid_vel"]))
         else:
             self.buf["grid_vel"][run].copy_(v)
-        self.buf["flux"].copy_(torch.from_numpy(np.ascontiguousarray(mass_flux_air_dry, dtype=np.float64)))
+        self.buf["flux"].copy_(_as_f64(mass_flux_air_dry))
 
     def set_relaxation(self, run, init_profile_r_v, init_profile_Theta, relaxation_time_profile):
-        self.buf["prof_rv"][run].copy_(torch.from_numpy(np.asarray(init_profile_r_v, dtype=np.float64)))
-        self.buf["prof_Th"][run].copy_(torch.from_numpy(np.asarray(init_profile_Theta, dtype=np.float64)))
-        self.buf["t_relax"].copy_(torch.from_numpy(np.asarray(relaxation_time_profile, dtype=np.float64)))
+        self.buf["prof_rv"][run].copy_(_as_f64(init_profile_r_v))
+        self.buf["prof_Th"][run].copy_(_as_f64(init_profile_Theta))
+        self.buf["t_relax"].copy_(_as_f64(relaxation_time_profile))
 
     def set_grid(self, run, grid_scalar_fields, grid_mat_prop):
-        """(10,nx,nz) and (7,nx,nz) in the index order of integration.py:366-386."""
-        s = np.ascontiguousarray(grid_scalar_fields, dtype=np.float64).reshape(10, self.nc)
-        m = np.ascontiguousarray(grid_mat_prop, dtype=np.float64).reshape(7, self.nc)
-        self.buf["scal"][run].copy_(torch.from_numpy(s))
-        self.buf["mat"][run].copy_(torch.from_numpy(m))
+        """(10,nx,nz) and (7,nx,nz) in the index order of integration.py:366-386 (arrays or tensors)."""
+        self.buf["scal"][run].copy_(_as_f64(grid_scalar_fields).reshape(10, self.nc))
+        self.buf["mat"][run].copy_(_as_f64(grid_mat_prop).reshape(7, self.nc))
 
     def init_cells(self):
         check(self.L.lm_engine_init_cells(C.byref(self.e), self.stream))
@@ -255,11 +256,42 @@ class Engine:
         self.buf["away"].zero_()  # every SIP sits in its home cell
         self.buf["mig_cnt"].zero_()
 
+    def load_particles_device(self, run, pos, vel, cells, m_w, m_s, xi, active_ids):
+        """One run from the reference's layout in device memory -- CUDA tensors pos, vel (2, N) float64, cells
+        (2, N) int64, m_w, m_s, xi (N) float64, active_ids (N) bool -- reordered into the engine's layout on the
+        device (lm_load_reference_layout); no host copy."""
+        N = self.n_per_run[run]
+        t = _ref_tensors(N, pos=pos, vel=vel, cells=cells, m_w=m_w, m_s=m_s, xi=xi, active_ids=active_ids)
+        check(self.L.lm_load_reference_layout(C.byref(self.e), int(run), int(self.base[run]), N, *t, self.stream))
+
+    def store_particles_device(self, run, pos, vel, cells, m_w, m_s, xi, active_ids, rel_pos=None, T_p=None):
+        """The inverse of load_particles_device into the caller's CUDA tensors (updated in place), with rel_pos
+        (2, N) of the active SIPs and T_p (N) when given (lm_store_reference_layout)."""
+        N = self.n_per_run[run]
+        t = _ref_tensors(N, pos=pos, vel=vel, cells=cells, rel_pos=rel_pos, m_w=m_w, m_s=m_s, xi=xi,
+                         active_ids=active_ids, T_p=T_p)
+        check(self.L.lm_store_reference_layout(C.byref(self.e), int(run), int(self.base[run]), N, *t, self.stream))
+
+    def export_grid_into(self, run, grid_scalar_fields, grid_mat_prop):
+        """The grid fields of one run copied into the caller's arrays or tensors (in place)."""
+        for name, dst in (("scal", grid_scalar_fields), ("mat", grid_mat_prop)):
+            if dst is None:
+                continue
+            src = self.buf[name][run].reshape(tuple(dst.shape))
+            if isinstance(dst, torch.Tensor):
+                dst.copy_(src)
+            else:
+                dst[...] = src.cpu().numpy()
+
     def set_counters(self, run, no_cols=None, water_removed=None):
         if no_cols is not None:
-            self.buf["no_cols"][run].copy_(torch.as_tensor(np.asarray(no_cols, dtype=np.int64)))
+            self.buf["no_cols"][run].copy_(no_cols.reshape(2) if isinstance(no_cols, torch.Tensor) else
+                                           torch.as_tensor(np.asarray(no_cols, dtype=np.int64)))
         if water_removed is not None:
-            self.buf["water_removed"][run] = float(np.asarray(water_removed).reshape(-1)[0])
+            if isinstance(water_removed, torch.Tensor):
+                self.buf["water_removed"][run].copy_(water_removed.reshape(-1)[0])
+            else:
+                self.buf["water_removed"][run] = float(np.asarray(water_removed).reshape(-1)[0])
 
     def set_rnd_stream(self, stream_values):
         """Inject the uniform stream (1-D float64, consumed in the reference's order) or None for Philox."""
@@ -314,7 +346,7 @@ class Engine:
         sl = slice(run * self.nc, (run + 1) * self.nc)
         for name, a in (("dTh", delta_Theta_ad), ("dRv", delta_r_v_ad), ("dml", delta_m_l)):
             if a is not None:
-                self.buf[name][sl].copy_(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).reshape(-1)))
+                self.buf[name][sl].copy_(_as_f64(a).reshape(-1))
 
     def export_increments(self, run):
         """(delta_Theta_ad, delta_r_v_ad, delta_m_l) of one run as (nx, nz) arrays."""
@@ -544,6 +576,35 @@ class Engine:
         self.check()
         return (self.buf["no_cols"][run].cpu().numpy().copy(),
                 np.array([float(self.buf["water_removed"][run].item())]))
+
+
+def _as_f64(a):
+    """A float64 tensor of an array (host) or of a tensor (kept where it is)."""
+    if isinstance(a, torch.Tensor):
+        return a.to(torch.float64)
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64))
+
+
+_REF_SPEC = {"pos": (torch.float64, 2), "vel": (torch.float64, 2), "cells": (torch.int64, 2), "m_w": (torch.float64, 1),
+             "m_s": (torch.float64, 1), "xi": (torch.float64, 1), "active_ids": (torch.bool, 1),
+             "rel_pos": (torch.float64, 2), "T_p": (torch.float64, 1)}
+
+
+def _ref_tensors(N, **arrays):
+    """Device pointers of the reference-layout tensors, in argument order, after checking device, dtype, shape
+    and contiguity (None -> a null pointer)."""
+    out = []
+    for name, t in arrays.items():
+        if t is None:
+            out.append(C.c_void_p(0))
+            continue
+        dtype, rows = _REF_SPEC[name]
+        shape = (2, N) if rows == 2 else (N,)
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != dtype or not t.is_contiguous() \
+                or tuple(t.shape) != shape:
+            raise TypeError(f"{name}: a contiguous CUDA {dtype} tensor of shape {shape} is required")
+        out.append(C.c_void_p(t.data_ptr()))
+    return out
 
 
 def host_material_properties(T, p, r_v, rho_d):

@@ -57,7 +57,14 @@ def _engine(grid_no_cells, grid_ranges, grid_steps, grid_volume_cell, p_ref, p_r
 
 
 def _check_ids(id_list, N):
-    if id_list is not None and (len(id_list) != N or not np.array_equal(id_list, np.arange(N))):
+    if id_list is None:
+        return
+    if _on_device(id_list):
+        import torch
+        ok = id_list.shape[0] == N and bool(torch.equal(id_list.long(), torch.arange(N, device=id_list.device)))
+    else:
+        ok = len(id_list) == N and np.array_equal(id_list, np.arange(N))
+    if not ok:
         raise ValueError("id_list must be np.arange(N), as integration.py:1937 builds it")
 
 
@@ -79,14 +86,25 @@ def _set_kernel(eng, E_col_grid, R_kernel_low_log, bin_factor_R_log):
         eng.set_collision_kernel(E_col_grid, R_kernel_low_log, bin_factor_R_log)
 
 
+def _on_device(a):
+    """Whether a drop-in argument is a CUDA tensor (the device path: no host copies of the SIP arrays)."""
+    import torch
+    return isinstance(a, torch.Tensor) and a.device.type == "cuda"
+
+
 def _upload(eng, grid_scalar_fields, grid_mat_prop, grid_velocity, grid_mass_flux_air_dry, pos, vel, cells, m_w,
             m_s, xi, active_ids, no_cols, water_removed):
+    """The step's inputs into the engine: NumPy arrays through the host layout, or CUDA tensors of the
+    reference's shapes and dtypes reordered on the device (Engine.load_particles_device)."""
     shape = (2, eng.nx + 1, eng.nz + 1)
     eng.set_flow(grid_velocity if grid_velocity is not None else np.zeros(shape),
                  grid_mass_flux_air_dry if grid_mass_flux_air_dry is not None else np.zeros(shape))
     eng.set_grid(0, grid_scalar_fields, grid_mat_prop)
     eng.init_cells()
-    eng.load_particles([(pos, vel, cells, m_w, m_s, xi, active_ids)])
+    if _on_device(xi):
+        eng.load_particles_device(0, pos, vel, cells, m_w, m_s, xi, active_ids)
+    else:
+        eng.load_particles([(pos, vel, cells, m_w, m_s, xi, active_ids)])
     eng.set_counters(0, no_cols, water_removed)
     eng.e.seed = rng.get_seed()
     eng.col_step = rng.get_step()
@@ -101,6 +119,10 @@ def _rel_pos(pos, cells, rel_pos, active_ids, grid_ranges, grid_steps):
 
 def _download(eng, grid_scalar_fields, grid_mat_prop, pos, vel, cells, rel_pos, m_w, m_s, xi, active_ids, T_p,
               delta_m_l, no_cols, water_removed, grid_ranges, grid_steps, particles_moved=True):
+    if _on_device(xi):
+        _download_device(eng, grid_scalar_fields, grid_mat_prop, pos, vel, cells, rel_pos, m_w, m_s, xi,
+                         active_ids, T_p, delta_m_l, no_cols, water_removed, particles_moved)
+        return
     p, v, c, mw, ms, x, a = eng.export_particles(0)
     m_w[...] = mw
     xi[...] = x
@@ -127,6 +149,36 @@ def _download(eng, grid_scalar_fields, grid_mat_prop, pos, vel, cells, rel_pos, 
     rng.set_step(eng.col_step)
 
 
+def _download_device(eng, grid_scalar_fields, grid_mat_prop, pos, vel, cells, rel_pos, m_w, m_s, xi, active_ids, T_p,
+                     delta_m_l, no_cols, water_removed, particles_moved):
+    """_download into CUDA tensors: the SIP arrays scattered back on the device (Engine.store_particles_device),
+    the grid, increments and counters copied device to device (a host array among them is filled from the
+    device)."""
+    import torch
+    eng.check()
+    moved = particles_moved
+    eng.store_particles_device(0, pos, vel, cells, m_w, m_s, xi, active_ids,
+                               rel_pos=rel_pos if moved and rel_pos is not None else None,
+                               T_p=T_p if moved and T_p is not None else None)
+    if grid_scalar_fields is not None:
+        eng.export_grid_into(0, grid_scalar_fields, grid_mat_prop)
+    if delta_m_l is not None:
+        src = eng.buf["dml"][:eng.nc].reshape(eng.nx, eng.nz)
+        if isinstance(delta_m_l, torch.Tensor):
+            delta_m_l.copy_(src)
+        else:
+            delta_m_l[...] = src.cpu().numpy()
+    for dst, src in ((no_cols, eng.buf["no_cols"][0]), (water_removed, eng.buf["water_removed"][0:1])):
+        if dst is None:
+            continue
+        if isinstance(dst, torch.Tensor):
+            dst.copy_(src.reshape(dst.shape))
+        else:
+            dst[...] = src.cpu().numpy().reshape(np.shape(dst))
+    eng.check()
+    rng.set_step(eng.col_step)
+
+
 # ------------------------------------------------------------------ helpers
 def compute_relaxation_time_profile(z):
     """integration.py:291-311: relaxation time 300 s * exp(z / 200 m) (ICMW 2012 case 1)."""
@@ -150,14 +202,20 @@ def compute_divergence_upwind_np(field, flux_field, grid_no_cells, grid_steps,
     """integration.py:150-286 on the GPU (lm_divergence_upwind).
 
     Limited third-order upwind divergence of field * flux; boundary type per
-    axis 0 = periodic, 1 = fixed.  Returns a new (Nx, Nz) array.
+    axis 0 = periodic, 1 = fixed.  Returns a new (Nx, Nz) array (a CUDA tensor for a CUDA-tensor field).
     """
     import ctypes as C
     import torch
     from ._native import check, lib
     nx, nz = int(grid_no_cells[0]), int(grid_no_cells[1])
-    f = torch.from_numpy(np.ascontiguousarray(field, dtype=np.float64)).cuda()
-    F = torch.from_numpy(np.ascontiguousarray(flux_field, dtype=np.float64)).cuda()
+    on_dev = _on_device(field)
+
+    def dev(a):
+        if _on_device(a):
+            return a.to(torch.float64).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+    f, F = dev(field), dev(flux_field)
     if tuple(f.shape) != (nx, nz) or tuple(F.shape) != (2, nx + 1, nz + 1):
         raise ValueError("field must be (Nx, Nz) and flux_field (2, Nx+1, Nz+1)")
     out = torch.empty_like(f)
@@ -165,7 +223,7 @@ def compute_divergence_upwind_np(field, flux_field, grid_no_cells, grid_steps,
     st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(lib().lm_divergence_upwind(C.c_void_p(f.data_ptr()), C.c_void_p(F.data_ptr()), C.c_void_p(out.data_ptr()),
                                      1, nx, nz, float(grid_steps[0]), float(grid_steps[1]), bc[0], bc[1], st))
-    return out.cpu().numpy()
+    return out if on_dev else out.cpu().numpy()
 
 
 compute_divergence_upwind = compute_divergence_upwind_np
@@ -181,9 +239,8 @@ def propagate_grid_subloop_step_np(grid_scalar_fields, grid_mat_prop, p_ref, p_r
     eng.init_cells()
     eng.set_increments(0, delta_Theta_ad, delta_r_v_ad, delta_m_l)
     eng.grid_substep()
-    s, m = eng.export_grid(0)
-    grid_scalar_fields[...] = s
-    grid_mat_prop[...] = m
+    eng.check()
+    eng.export_grid_into(0, grid_scalar_fields, grid_mat_prop)
 
 
 propagate_grid_subloop_step = propagate_grid_subloop_step_np
@@ -194,7 +251,7 @@ def propagate_particles_subloop_step_np(grid_scalar_fields, grid_mat_prop, grid_
                                         water_removed, id_list, active_ids, T_p, solute_type, delta_m_l, dt_sub,
                                         dt_sub_pos, no_iter_impl_mass, g_set):
     """integration.py:944-1089: T_p, implicit condensation with Delta m_l, drag, positions and cells."""
-    N = len(xi)
+    N = int(xi.shape[0])
     _check_ids(id_list, N)
     ranges, steps = np.asarray(grid_ranges), np.asarray(grid_steps)
     eng = _engine(grid_no_cells, ranges, steps, steps[0] * steps[1], 1.0E5, 1.0E-5, N)
@@ -218,7 +275,7 @@ def integrate_subloop_n_steps_np(grid_scalar_fields, grid_mat_prop, grid_velocit
                                  delta_Theta_ad, delta_r_v_ad, dt_sub, dt_sub_pos, no_cond_steps, no_col_steps,
                                  no_iter_impl_mass, g_set, act_collisions):
     """integration.py:1095-1294: one subloop of no_cond_steps substeps with its collision steps."""
-    N = len(xi)
+    N = int(xi.shape[0])
     _check_ids(id_list, N)
     eng = _engine(grid_no_cells, grid_ranges, grid_steps, grid_volume_cell, p_ref, p_ref_inv, N)
     _set_step_params(eng, solute_type, 2 * dt_sub, dt_sub, 0.5 * dt_sub, dt_col_over_dV, 1, 1, act_collisions,
@@ -252,7 +309,7 @@ def integrate_adv_step_np(grid_scalar_fields, grid_mat_prop, grid_velocity, grid
                           no_iter_impl_mass, g_set, E_col_grid, no_kernel_bins, R_kernel_low_log, bin_factor_R_log,
                           no_cols):
     """integration.py:1298-1611: one advection step of particles and grid (lm_adv_steps)."""
-    N = len(xi)
+    N = int(xi.shape[0])
     _check_ids(id_list, N)
     eng = _engine(grid_no_cells, grid_ranges, grid_steps, grid_volume_cell, p_ref, p_ref_inv, N)
     _set_step_params(eng, solute_type, dt, dt_sub, dt_sub_half, dt_col_over_dV, scale_dt_cond, no_col_per_adv,
@@ -369,7 +426,7 @@ def simulate_interval(grid_scalar_fields, grid_mat_prop, grid_velocity, grid_mas
     updated in place at its end.  T_p of the first dump is the caller's T_p
     if no step has run yet.
     """
-    N = len(xi)
+    N = int(xi.shape[0])
     _check_ids(id_list, N)
     eng = _engine(grid_no_cells, grid_ranges, grid_steps, grid_volume_cell, p_ref, p_ref_inv, N)
     _set_step_params(eng, solute_type, dt, dt_sub, dt_sub_half, dt_col / grid_volume_cell, scale_dt_cond,

@@ -477,3 +477,97 @@ def test_generate_field_frame_data_avg_from_files(gpu, tmp_path):
     _assert_same(got[0], om, "mean")
     _assert_same(got[1], osd, "spread")
     assert list(got[2]) == [0, 300] and len(got[3]) == 12
+
+
+def test_step_dropins_on_device_tensors(gpu):
+    """integrate_adv_step_np, integrate_subloop_n_steps_np, propagate_particles_subloop_step_np,
+    propagate_grid_subloop_step_np and compute_divergence_upwind_np called with CUDA tensors of the
+    reference's shapes and dtypes: the SIP arrays are reordered into the engine's layout and back on the
+    device (no host copy of them) and updated where they are (same storage); every output equals the NumPy
+    call's bit for bit (the NumPy path is checked against the oracle above)."""
+    import torch
+    from lacmopy_amd import integration as itg
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    grid, *parts = _case(nx=8, nz=8, spc=24, seed=23)
+    parts = _prep(grid, parts)
+    N = len(parts[5])
+    prof = init_profiles(grid)
+
+    def state():
+        pos, vel, cells, m_w, m_s, xi, active = [np.array(a, copy=True) for a in parts]
+        rel = np.zeros((2, N))
+        for d in (0, 1):
+            rel[d][active] = (pos[d][active] - grid.ranges[d][0]) / grid.steps[d] - cells[d][active]
+        return dict(scal=grid.scalar_fields(), mat=grid.mat_prop(), pos=pos, vel=vel, cells=cells, rel=rel, m_w=m_w,
+                    m_s=m_s, xi=xi, active=active, T_p=np.ones(N), wr=np.zeros(1), nc=np.zeros(2, dtype=np.int64),
+                    dml=np.zeros((8, 8)), dTh=np.full((8, 8), 1e-4), dRv=np.full((8, 8), -1e-7), ids=np.arange(N))
+
+    def to_dev(h):
+        return {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in h.items()}
+
+    def calls(a):
+        dt, dt_sub = 1.0, 0.1
+        rng.seed(31)
+        itg.integrate_adv_step_np(a["scal"], a["mat"], grid.velocity, grid.mass_flux_air_dry, grid.p_ref,
+                                  grid.p_ref_inv, grid.no_cells, grid.ranges, grid.steps, grid.volume_cell, a["pos"],
+                                  a["vel"], a["cells"], a["rel"], a["m_w"], a["m_s"], a["xi"], "AS", a["wr"], a["ids"],
+                                  a["active"], a["T_p"], a["dml"], dt, dt_sub, 0.5 * dt_sub, 0.5 / grid.volume_cell, 5, 2,
+                                  True, True, prof[0], prof[1], prof[2], 3, 9.80665, E, nb, lnR0, lnf, a["nc"])
+        itg.integrate_subloop_n_steps_np(a["scal"], a["mat"], grid.velocity, grid.no_cells, grid.ranges, grid.steps,
+                                         grid.volume_cell, grid.p_ref, grid.p_ref_inv, a["pos"], a["vel"], a["cells"],
+                                         a["rel"], a["m_w"], a["m_s"], a["xi"], "AS", 0.5 / grid.volume_cell, E, nb,
+                                         lnR0, lnf, a["nc"], a["wr"], a["ids"], a["active"], a["T_p"], a["dml"],
+                                         a["dTh"], a["dRv"], dt_sub, dt_sub, 3, 4, 3, 9.80665, True)
+        itg.propagate_particles_subloop_step_np(a["scal"], a["mat"], grid.velocity, grid.no_cells, grid.ranges,
+                                                grid.steps, a["pos"], a["vel"], a["cells"], a["rel"], a["m_w"],
+                                                a["m_s"], a["xi"], a["wr"], a["ids"], a["active"], a["T_p"], "AS",
+                                                a["dml"], dt_sub, dt_sub, 3, 9.80665)
+        itg.propagate_grid_subloop_step_np(a["scal"], a["mat"], grid.p_ref, grid.p_ref_inv, a["dTh"], a["dRv"],
+                                           a["dml"], grid.volume_cell)
+        return itg.compute_divergence_upwind_np(a["scal"][2], grid.mass_flux_air_dry, grid.no_cells, grid.steps)
+
+    h = state()
+    d = to_dev(state())
+    ptrs = {k: t.data_ptr() for k, t in d.items()}
+    div_h = calls(h)
+    div_d = calls(d)
+    assert isinstance(div_d, torch.Tensor) and div_d.is_cuda
+    assert {k: t.data_ptr() for k, t in d.items()} == ptrs, "a tensor was replaced instead of updated"
+    for k in h:
+        _assert_same(d[k].cpu().numpy(), h[k], k)
+    _assert_same(div_d.cpu().numpy(), div_h, "divergence")
+    assert h["nc"].sum() > 0
+
+
+def test_box_steps_on_device_tensors(gpu):
+    """The Golovin, Ecol_grid_R and two-component cell steps on CUDA tensors: updated where they are (same
+    storage), no_cols a device tensor added to in place, results equal the NumPy calls' bit for bit."""
+    import torch
+    from lacmopy_amd.collision import all_or_nothing as aon
+    n = 180
+    rs = np.random.default_rng(21)
+    xi = rs.uniform(1e5, 1e8, n)
+    m = rs.lognormal(np.log(1e7), 1.5, n)
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    rho = np.full(n, 1000.0)
+    R = (0.75 / np.pi * m / rho) ** (1 / 3)
+    vel1 = np.array([orc.lib().or_terminal_velocity_Beard(r) for r in R])
+    vel2 = np.array([rs.normal(0, 0.5, n), -vel1])
+    m_s = rs.uniform(1e-3, 1.0, n)
+
+    def run(dev):
+        t = (lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()) if dev else (lambda a: np.array(a, copy=True))
+        a = dict(x1=t(xi), m1=t(m), x2=t(xi), m2=t(m), R2=t(R), x3=t(xi), mw3=t(m), ms3=t(m_s), R3=t(R),
+                 nc=t(np.zeros(2, dtype=np.int64)))
+        rng.seed(6, 1)
+        aon.collision_step_Golovin_np(a["x1"], a["m1"], 1.0, a["nc"])
+        aon.collision_step_Ecol_grid_R_np(a["x2"], a["m2"], a["R2"], t(vel1), t(rho), 1.0, E, nb, lnR0, lnf, a["nc"])
+        aon.collision_step_Ecol_grid_R_2D_multicomp_np(a["x3"], a["mw3"], a["ms3"], a["R3"], t(vel2), t(rho), 1.0,
+                                                       E, nb, lnR0, lnf, a["nc"])
+        return a
+
+    h = run(False)
+    d = run(True)
+    for k in h:
+        _assert_same(d[k].cpu().numpy(), h[k], k)
+    assert h["nc"].sum() > 0
