@@ -645,6 +645,9 @@ def main():
                          "(default 7200; stress150: 3000, with collisions)")
     ap.add_argument("--col-steps", type=int, default=900,
                     help="advection steps with collisions between the spin-up and the collision-active timing")
+    ap.add_argument("--eager-prep", action="store_true",
+                    help="run the collision-active leg's preparation steps as eager launches instead of graph "
+                         "replays (same state; for kernel tracers that record graph launches poorly)")
     ap.add_argument("--adj-window", type=int, default=None,
                     help="advection steps of the adjoint window (default 600 for both cases)")
     ap.add_argument("--adj-every", type=int, default=None,
@@ -743,7 +746,10 @@ def main():
             eng.adv_steps(1)
             drz["autotune"] = tune(eng)
         # the runs diverge under their own draws for col_steps more steps before the timing
-        eng.adv_steps_graph(args.col_steps)
+        if args.eager_prep:
+            eng.adv_steps(args.col_steps)
+        else:
+            eng.adv_steps_graph(args.col_steps)
         torch.cuda.synchronize()
         eng.check()
         eng.drop_graph()
