@@ -108,9 +108,10 @@ typedef struct LmEngine {
   int32_t *ovf_list;  /* 3*n_runs*nc (2*n_runs*nc without big_ws): cells over the fast-path LDS capacity
                          (collision: list A in the first third, its cells over 256 members in the second, cells
                          over 1024 members in the third) */
-  int32_t *ovf_cnt;   /* 4 (3 without big_ws): cells queued for the large-capacity collision [0] and Delta m_l
-                         [1] passes, for the collision's 1024-member pass after its 256-member pass [2] and for
-                         its pass over cells of any size [3] */
+  int32_t *ovf_cnt;   /* 288: cells queued for the large-capacity collision [0] and Delta m_l [1] passes, for
+                         the collision's 1024-member pass after its 256-member pass [2] and for its pass over
+                         cells of any size [3], for the Delta m_l 1024-member pass after its 256-member pass
+                         [4]; [32..287]: the collision's 256-member pass's claim counters, one per 128 B */
   /* injected uniform stream (parity mode; NULL = Philox) */
   const double *rnd_stream;
   int64_t *rnd_off; /* n_runs*nc+1 scratch for stream offsets */
@@ -419,8 +420,9 @@ int lm_graph_destroy(void *graph);
 /*
  * Kernel timing with HIP events on the launch stream (no reference
  * counterpart; used by bench.py).  While enabled, every launch of the
- * particle kernel (kind 0), the collision kernel (1), the Delta m_l kernel (2)
- * and the reindex scatter (3) is bracketed by events; lm_prof_collect
+ * particle kernel (kind 0), the collision kernel's main pass (1), the Delta
+ * m_l kernel (2), the reindex scatter (3) and the collision kernel's overflow
+ * passes (4) is bracketed by events; lm_prof_collect
  * synchronises and returns the summed milliseconds and launch counts per
  * kind, then resets.  At most max_events launches are recorded.
  */

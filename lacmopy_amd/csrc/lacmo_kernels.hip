@@ -671,12 +671,30 @@ __device__ static inline CellSource cell_source(int mode, int chunk, int ctr) {
   return s;
 }
 
-/* next global cell of this workgroup, or -1 (PASS_OVF: from ovf_list[0 .. *ovf_cnt)) */
+/*
+ * next global cell of this workgroup, or -1 (PASS_OVF: from ovf_list[0 .. *ovf_cnt), grid-strided, or, with
+ * grab counters, claimed one entry at a time: the workgroups share GRAB_CTRS counters GRAB_STRIDE ints apart,
+ * one cache line each (workgroup b the counter b mod G, G = min(GRAB_CTRS, gridDim.x), so consecutive
+ * workgroups, which the dispatcher places on different XCDs, spread their atomics over several lines); counter
+ * x hands out entries x, x + G, x + 2G, ...  Every counter has a workgroup, the counters only grow, so each
+ * entry is taken exactly once and every workgroup exits.  The counters must be zero at the launch.)
+ */
+#define GRAB_CTRS 8
+#define GRAB_STRIDE 32
 __device__ static inline int next_cell(const KView &v, CellSource &s, const int32_t *ovf_list,
-                                       const int32_t *ovf_cnt) {
+                                       const int32_t *ovf_cnt, int32_t *grab = nullptr) {
   const int it = s.it++;
   if (s.mode == PASS_CELLS) return it == 0 ? (int)blockIdx.x : -1;
   const int n = v.n_runs * v.nc;
+  if (s.mode == PASS_OVF && grab) {
+    const int G = (int)gridDim.x < GRAB_CTRS ? (int)gridDim.x : GRAB_CTRS;
+    const int x = (int)blockIdx.x % G;
+    int k = 0;
+    if (threadIdx.x == 0) k = atomicAdd(grab + GRAB_STRIDE * x, 1);
+    k = __builtin_amdgcn_readfirstlane(k);
+    const long long idx = x + (long long)G * k;
+    return idx < *ovf_cnt ? ovf_list[idx] : -1;
+  }
   if (s.mode == PASS_OVF || s.chunk <= 0) {
     const int idx = (int)blockIdx.x + it * (int)gridDim.x;
     if (s.mode == PASS_OVF) return idx < *ovf_cnt ? ovf_list[idx] : -1;
@@ -779,17 +797,28 @@ __device__ static inline double pair_p(const LmPhys &P, const KView &v, const Co
   return xi_max * v.dt_col_over_dV * lmp_kernel_hydro(P, cc.sR[i], cc.sR[j], Ec, sqrt(du * du + dw * dw));
 }
 
+/* global state of the two members of a pair: masses and pre-step densities (one round of loads) */
+struct PairMass {
+  double mw_i, mw_j, ms_i, ms_j, rho_i, rho_j;
+};
+__device__ static inline PairMass load_pair(const KView &v, const CollCell &cc, const double *rho_g, int i, int j) {
+  const int ki = cc.sSL[i], kj = cc.sSL[j];
+  const PairMass m = {v.m_w[ki], v.m_w[kj], v.m_s[ki], v.m_s[kj], rho_g[ki], rho_g[kj]};
+  return m;
+}
+
 /*
  * Apply the event of a firing pair (all_or_nothing.py:577-632): multiple
  * collection if p_crit > 1, else a single collection (with the near-equal xi
  * split).  m_w, m_s live in global memory (slot-indexed), the pre-step
- * density in rho_g.  Returns 1 for a multiple, 2 for a single, 3 for a
- * near-equal single event.
+ * density in rho_g; M holds the pair's values of both, loaded before.
+ * Returns 1 for a multiple, 2 for a single, 3 for a near-equal single event.
  */
-__device__ static inline int apply_event(const LmPhys &P, const KView &v, const CollCell &cc, const double *rho_g,
+__device__ static inline int apply_event(const LmPhys &P, const KView &v, const CollCell &cc, const PairMass &M,
                                          int i, int j, double p_crit) {
   int imin, imax;
-  if (cc.sX[i] <= cc.sX[j]) {
+  const bool i_min = cc.sX[i] <= cc.sX[j];
+  if (i_min) {
     imin = i;
     imax = j;
   } else {
@@ -798,8 +827,9 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   }
   const double xi_min = cc.sX[imin], xi_max = cc.sX[imax];
   const int kmin = cc.sSL[imin], kmax = cc.sSL[imax];
-  const double mw_min = v.m_w[kmin], mw_max = v.m_w[kmax];
-  const double ms_min = v.m_s[kmin], ms_max = v.m_s[kmax];
+  const double mw_min = i_min ? M.mw_i : M.mw_j, mw_max = i_min ? M.mw_j : M.mw_i;
+  const double ms_min = i_min ? M.ms_i : M.ms_j, ms_max = i_min ? M.ms_j : M.ms_i;
+  const double rho_min = i_min ? M.rho_i : M.rho_j;
   const bool table = !v.e_const_mode;
   if (v.ev_mode == 1) {
     const int kind = p_crit > 1.0 ? 1 : ((xi_max - xi_min) / xi_max < 1.0E-5 ? 3 : 2);
@@ -810,7 +840,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
     const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
     v.m_w[kmin] = mwn;
     v.m_s[kmin] = msn;
-    const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+    const double Rn = lmp_radius(P, (mwn + msn), rho_min);
     cc.sR[imin] = Rn;
     cc.sX[imax] = xi_max - xi_col;
     if (table) cc.sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
@@ -825,7 +855,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
     v.m_s[kmin] = msn;
     v.m_w[kmax] = mwn;
     v.m_s[kmax] = msn;
-    const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+    const double Rn = lmp_radius(P, (mwn + msn), rho_min);
     cc.sR[imin] = Rn;
     cc.sR[imax] = Rn;
     const double xmax_new = 0.5 * 0.7 * xi_ges;
@@ -841,7 +871,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
   v.m_w[kmin] = mwn;
   v.m_s[kmin] = msn;
-  const double Rn = lmp_radius(P, (mwn + msn), rho_g[kmin]);
+  const double Rn = lmp_radius(P, (mwn + msn), rho_min);
   cc.sR[imin] = Rn;
   cc.sX[imax] = xi_max - xi_min;
   if (table) cc.sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
@@ -1247,12 +1277,14 @@ __device__ static inline unsigned long long eval_quad(const LmPhys &P, const KVi
 struct ColLists {
   const int32_t *in_list, *in_cnt;
   int32_t *out_list, *out_cnt;
+  int32_t *in_grab; /* GRAB_CTRS zeroed counters: claim in_list's entries one at a time (NULL: grid-strided) */
 };
 
 template <int CAP, bool TAB, bool INJ>
 __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) {
   const LmPhys &P = c_P;
   __syncthreads(); /* the LDS of the previous cell of this block is free */
+  const unsigned long long t_cell = v.col_cost ? wall_clock64() : 0ull; /* the cell's cost: gather included */
   const int nc = v.nc;
   const int r = gc / nc, c = gc % nc;
   const int lane = threadIdx.x;
@@ -1365,7 +1397,6 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
     return;
   }
   if (n < 2) return;
-  const unsigned long long t_cell = v.col_cost ? wall_clock64() : 0ull;
   if (lane == 0) sSeq = 0;
   __syncthreads();
   if (!fast) {
@@ -1586,8 +1617,10 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
     const int ei = (int)((best >> 16) & 0xffff), ej = (int)(best & 0xffff);
     const int ecnt = (int)(best >> 32);
     if (lane == 0) {
+      /* the table entry and the two members' masses and densities in one round of loads */
       const double Ec = use_tab ? v.E[sIND[ei] * v.nb + sIND[ej]] : v.E_const;
-      const int kind = apply_event(P, v, cc, rho_g, ei, ej, pair_p(P, v, cc, ei, ej, Ec));
+      const PairMass M = load_pair(v, cc, rho_g, ei, ej);
+      const int kind = apply_event(P, v, cc, M, ei, ej, pair_p(P, v, cc, ei, ej, Ec));
       if (kind == 1)
         ++c_multi;
       else
@@ -1728,7 +1761,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
         else
           fires = pair_fires(p_crit, G, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d);
         if (fires) {
-          const int kind = apply_event(P, v, cc, rho_g, i, j, p_crit);
+          const int kind = apply_event(P, v, cc, load_pair(v, cc, rho_g, i, j), i, j, p_crit);
           if (kind == 1)
             ++c_multi;
           else
@@ -1767,7 +1800,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void 
                                                                                            ColLists io) {
   CellSource src = cell_source(mode, chunk, 0);
   for (;;) {
-    const int gc = next_cell(v, src, io.in_list, io.in_cnt);
+    const int gc = next_cell(v, src, io.in_list, io.in_cnt, io.in_grab);
     if (gc < 0) return;
     collide_cell<CAP, TAB, INJ>(v, gc, io);
   }
@@ -2113,11 +2146,12 @@ __global__ __launch_bounds__(WAVE) void k_water(KView v) {
 /*
  * Delta m_l per cell: sum of dm*xi over the cell's members in ascending id
  * (integration.py:915-938), then *1e-18.  CAP bounds the members held in LDS;
- * the CAP = 256 pass (small LDS, high occupancy) queues larger cells in the
- * overflow list (counter ovf_cnt[1]) for the CAP = 1024 pass.
+ * a larger cell is appended to out_list (counter *out_cnt) for a pass of
+ * larger capacity, or, with no out_list, summed by one lane straight from the
+ * segments.
  */
 template <int CAP>
-__device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
+__device__ static void dml_cell(const KView &v, int gc, int32_t *out_list, int32_t *out_cnt) {
   const int nc = v.nc;
   __syncthreads(); /* the LDS of the previous cell of this block is free */
   const int r = gc / nc;
@@ -2228,14 +2262,14 @@ __device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
   const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
   if (n > CAP) {
     if (threadIdx.x == 0) {
-      if (from_ovf) {
+      if (!out_list) {
         /* over every LDS capacity: one lane sums in ascending id straight from the segments */
         double sum = 0.0;
         merge_members(gc, v.cell, v.id, &tb, [&](int k) { sum += v.dmxi[k]; });
         v.dml[gc] = sum * 1.0E-18;
       } else {
-        const int q = atomicAdd(&v.ovf_cnt[1], 1);
-        v.ovf_list[q] = gc;
+        const int q = atomicAdd(out_cnt, 1);
+        out_list[q] = gc;
       }
     }
     return;
@@ -2263,13 +2297,124 @@ __device__ static void dml_cell(const KView &v, int gc, int from_ovf) {
   }
 }
 
+/* cells from the pass's source (PASS_OVF: in_list[0 .. *in_cnt)); larger cells on to out_list (NULL: summed by
+   one lane) */
 template <int CAP>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_dml(KView v, int mode, int chunk) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_dml(KView v, int mode, int chunk,
+                                                                                     const int32_t *in_list,
+                                                                                     const int32_t *in_cnt,
+                                                                                     int32_t *out_list,
+                                                                                     int32_t *out_cnt) {
   CellSource src = cell_source(mode, chunk, 1);
   for (;;) {
-    const int gc = next_cell(v, src, v.ovf_list, v.ovf_cnt + 1);
+    const int gc = next_cell(v, src, in_list, in_cnt);
     if (gc < 0) return;
-    dml_cell<CAP>(v, gc, mode == PASS_OVF);
+    dml_cell<CAP>(v, gc, out_list, out_cnt);
+  }
+}
+
+/*
+ * Delta m_l of two cells per wavefront, 32 lanes each, for cells whose migrant list is complete and whose home
+ * segment and list hold at most DML2_CAP entries: dml_cell's sum over the home segment in slot order (= id
+ * order, non-members adding +0.0) merged by id with the list's members (each slot once), with both cells'
+ * loads, ranks and sequential sums in the same instructions.  Other cells go to out_list (counter *out_cnt)
+ * for dml_cell.  Needs the migrant lists (mig_cnt) and mig_cap <= 32.
+ */
+#define DML2_CAP 224
+#define DML2_L 32
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void k_dml2(KView v, int32_t *out_list,
+                                                                                      int32_t *out_cnt) {
+  constexpr int L = DML2_L, G = WAVE / DML2_L, PER = DML2_CAP / DML2_L;
+  __shared__ double sVal[G][DML2_CAP]; /* home ids (as int) until the ranks are known, then the ordered values */
+  __shared__ int sMk[G][L], sMid[G][L];
+  const int lane = threadIdx.x, g = lane / L, gl = lane % L;
+  int *sHid = (int *)sVal[g];
+  const int nc = v.nc, ncell = v.n_runs * nc;
+  const int gc = (int)blockIdx.x * G + g;
+  int mc = 0, s0 = 0, T0 = 0;
+  bool fast = false;
+  if (gc < ncell) {
+    const int r = gc / nc, c = gc % nc;
+    const int32_t *seg = v.seg + (size_t)r * (nc + 1);
+    mc = v.mig_cnt[gc];
+    s0 = seg[c];
+    T0 = seg[c + 1] - s0;
+    fast = mc <= v.mig_cap && T0 + mc <= DML2_CAP;
+    if (!fast && gl == 0) {
+      const int q = atomicAdd(out_cnt, 1);
+      out_list[q] = gc;
+    }
+  }
+  /* home slots: value (+0.0 for a slot whose SIP left) and id; the list's entries: one per lane */
+  double ov[PER];
+  int oid[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int t = gl + q * L;
+    ov[q] = 0.0;
+    oid[q] = 0;
+    if (fast && t < T0) {
+      const int k = s0 + t;
+      const double d = v.dmxi[k];
+      ov[q] = v.cell[k] == gc ? d : 0.0;
+      oid[q] = v.id[k];
+    }
+  }
+  int mk = -1, mid_ = 0;
+  double mval = 0.0;
+  bool mem = false;
+  if (fast && gl < mc) {
+    mk = v.mig_slot[(size_t)gc * v.mig_cap + gl];
+    mem = v.cell[mk] == gc;
+    mval = v.dmxi[mk];
+    mid_ = v.id[mk];
+    sMk[g][gl] = mk;
+  }
+  __syncthreads();
+  if (mem)
+    for (int e = 0; e < gl; ++e)
+      if (sMk[g][e] == mk) mem = false;
+  const unsigned gb = (unsigned)(__ballot(mem) >> (g * L));
+  const int nm = __popc(gb);
+  if (mem) sMid[g][__popc(gb & ((1u << gl) - 1u))] = mid_;
+#pragma unroll
+  for (int q = 0; q < PER; ++q)
+    if (fast && gl + q * L < T0) sHid[gl + q * L] = oid[q];
+  __syncthreads();
+  /* home entry: its index plus the members of the list with smaller ids; list member: its rank among them plus
+     the home entries with smaller ids (ascending in the segment) */
+  int opos[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    int lo = 0;
+    if (fast && gl + q * L < T0)
+      for (int e = 0; e < nm; ++e) lo += sMid[g][e] < oid[q];
+    opos[q] = gl + q * L + lo;
+  }
+  int rk = 0;
+  if (mem) {
+    for (int e = 0; e < nm; ++e) rk += sMid[g][e] < mid_;
+    int lo = 0, hi = T0;
+    while (lo < hi) {
+      const int m2 = (lo + hi) >> 1;
+      if (sHid[m2] < mid_)
+        lo = m2 + 1;
+      else
+        hi = m2;
+    }
+    rk += lo;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q)
+    if (fast && gl + q * L < T0) sVal[g][opos[q]] = ov[q];
+  if (mem) sVal[g][rk] = mval;
+  __syncthreads();
+  if (fast && gl == 0) {
+    const int tot = T0 + nm;
+    double sum = 0.0;
+    for (int p = 0; p < tot; ++p) sum += sVal[g][p];
+    v.dml[gc] = sum * 1.0E-18;
   }
 }
 
@@ -3792,6 +3937,8 @@ int lm_collision_step(LmEngine *e, void *stream) {
      over 256 members, after a main pass below 256 */
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt, 0, 1);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 2, 0, e->big_ws ? 2 : 1);
+  hipLaunchKernelGGL(k_fill_i32, dim3(GRAB_CTRS * GRAB_STRIDE / 64), dim3(64), 0, S(stream), e->ovf_cnt + GRAB_STRIDE,
+                     0, GRAB_CTRS * GRAB_STRIDE);
   KView v = make_view(e);
   const bool tab = !e->e_const_mode, inj = e->rnd_stream != nullptr;
   /* member capacity of the main pass from the mean SIPs per cell (mean + 2.5 sigma of a Poisson count): a
@@ -3811,25 +3958,31 @@ int lm_collision_step(LmEngine *e, void *stream) {
     else
       launch_collision<512>(v, tab, inj, grid, S(stream), mode, e->col_work_chunk, main_io);
   }
-  ColLists last_io = {e->ovf_list, e->ovf_cnt, nullptr, nullptr};
-  if (cap <= 160) {
-    /* the 256-member pass over list A at its own occupancy (a few resident workgroups per compute unit,
-       grid-strided), the larger cells on to list B */
-    const ColLists mid_io = {e->ovf_list, e->ovf_cnt, e->ovf_list + ncell, e->ovf_cnt + 2};
-    const long long gm = 12LL * device_cus();
-    launch_collision<256>(v, tab, inj, dim3((unsigned)(gm < ncell ? gm : ncell)), S(stream), PASS_OVF, 0, mid_io);
-    last_io.in_list = e->ovf_list + ncell;
-    last_io.in_cnt = e->ovf_cnt + 2;
-  }
-  if (e->big_ws) {
-    /* cells over 1024 members on to list C, then the pass over cells of any size */
-    last_io.out_list = e->ovf_list + 2 * (size_t)ncell;
-    last_io.out_cnt = e->ovf_cnt + 3;
-  }
-  launch_collision<1024>(v, tab, inj, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), PASS_OVF, 0, last_io);
-  if (e->big_ws) {
-    const ColLists big_io = {e->ovf_list + 2 * (size_t)ncell, e->ovf_cnt + 3, nullptr, nullptr};
-    launch_collision<0>(v, tab, inj, dim3(e->big_blocks), S(stream), PASS_OVF, 0, big_io);
+  {
+    /* the overflow passes, timed apart from the main pass */
+    ProfScope ps(e, 4, S(stream));
+    ColLists last_io = {e->ovf_list, e->ovf_cnt, nullptr, nullptr};
+    if (cap <= 160) {
+      /* the 256-member pass over list A at its own occupancy (a few resident workgroups per compute unit,
+         each claiming the next cell of the list: the costs of these cells spread widely), the larger cells on
+         to list B */
+      const ColLists mid_io = {e->ovf_list, e->ovf_cnt, e->ovf_list + ncell, e->ovf_cnt + 2,
+                               e->ovf_cnt + GRAB_STRIDE};
+      const long long gm = 12LL * device_cus();
+      launch_collision<256>(v, tab, inj, dim3((unsigned)(gm < ncell ? gm : ncell)), S(stream), PASS_OVF, 0, mid_io);
+      last_io.in_list = e->ovf_list + ncell;
+      last_io.in_cnt = e->ovf_cnt + 2;
+    }
+    if (e->big_ws) {
+      /* cells over 1024 members on to list C, then the pass over cells of any size */
+      last_io.out_list = e->ovf_list + 2 * (size_t)ncell;
+      last_io.out_cnt = e->ovf_cnt + 3;
+    }
+    launch_collision<1024>(v, tab, inj, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), S(stream), PASS_OVF, 0, last_io);
+    if (e->big_ws) {
+      const ColLists big_io = {e->ovf_list + 2 * (size_t)ncell, e->ovf_cnt + 3, nullptr, nullptr};
+      launch_collision<0>(v, tab, inj, dim3(e->big_blocks), S(stream), PASS_OVF, 0, big_io);
+    }
   }
   LAUNCH_CHECK();
   if (e->ev_mode == 1) {
@@ -3990,14 +4143,30 @@ int lm_particle_substep(LmEngine *e, double dt_sub, double dt_sub_pos, void *str
   {
     ProfScope ps(e, 2, S(stream));
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), e->ovf_cnt + 1, 0, 1);
-    dim3 grid;
-    const int mode = main_pass(e, e->dml_work_per_cu, e->dml_work_chunk, 1, &grid, S(stream));
-    if (cell_capacity(e) <= 256)
-      hipLaunchKernelGGL(k_dml<256>, grid, dim3(WAVE), 0, S(stream), v, mode, e->dml_work_chunk);
-    else
-      hipLaunchKernelGGL(k_dml<512>, grid, dim3(WAVE), 0, S(stream), v, mode, e->dml_work_chunk);
-    hipLaunchKernelGGL(k_dml<1024>, dim3(ncell < OVF_GRID ? ncell : OVF_GRID), dim3(WAVE), 0, S(stream), v,
-                       PASS_OVF, 0);
+    const dim3 ogrid(ncell < OVF_GRID ? ncell : OVF_GRID);
+    int32_t *list1 = e->ovf_list, *cnt1 = e->ovf_cnt + 1;
+    if (e->mig_cnt && e->mig_cap <= DML2_L && e->dml_work_per_cu <= 0 && cell_capacity(e) <= DML2_CAP) {
+      /* two cells per wavefront; the others through the 256- and 1024-member passes (second list: ovf_list +
+         ncell, counter ovf_cnt[4]) */
+      int32_t *list2 = e->ovf_list + ncell, *cnt2 = e->ovf_cnt + 4;
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, S(stream), cnt2, 0, 1);
+      hipLaunchKernelGGL(k_dml2, dim3((unsigned)((ncell + WAVE / DML2_L - 1) / (WAVE / DML2_L))), dim3(WAVE), 0,
+                         S(stream), v, list1, cnt1);
+      hipLaunchKernelGGL(k_dml<256>, ogrid, dim3(WAVE), 0, S(stream), v, PASS_OVF, 0, list1, cnt1, list2, cnt2);
+      hipLaunchKernelGGL(k_dml<1024>, ogrid, dim3(WAVE), 0, S(stream), v, PASS_OVF, 0, list2, cnt2, nullptr,
+                         nullptr);
+    } else {
+      dim3 grid;
+      const int mode = main_pass(e, e->dml_work_per_cu, e->dml_work_chunk, 1, &grid, S(stream));
+      if (cell_capacity(e) <= 256)
+        hipLaunchKernelGGL(k_dml<256>, grid, dim3(WAVE), 0, S(stream), v, mode, e->dml_work_chunk, list1, cnt1,
+                           list1, cnt1);
+      else
+        hipLaunchKernelGGL(k_dml<512>, grid, dim3(WAVE), 0, S(stream), v, mode, e->dml_work_chunk, list1, cnt1,
+                           list1, cnt1);
+      hipLaunchKernelGGL(k_dml<1024>, ogrid, dim3(WAVE), 0, S(stream), v, PASS_OVF, 0, list1, cnt1, nullptr,
+                         nullptr);
+    }
   }
   hipLaunchKernelGGL(k_water, dim3(e->n_runs), dim3(WAVE), 0, S(stream), v);
   LAUNCH_CHECK();

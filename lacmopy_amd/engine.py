@@ -94,7 +94,7 @@ class Engine:
         mk("scan_part", ((L + 4095) // 4096 + 1,), i32)
         mk("err", (4,), i32)
         mk("ovf_list", (3 * nr * nc,), i32)
-        mk("ovf_cnt", (4,), i32)
+        mk("ovf_cnt", (288,), i32)
         mk("rnd_off", (nr * nc + 2,), i64)
         # (run, cell) work list of the persistent passes (rebuilt by every reindex while one is on; starts as the
         # identity order), its size-bin scratch and two work counters
@@ -522,15 +522,16 @@ class Engine:
             self.buf[n].copy_(t)
         self.e.cur, self.e.ccur, self.e.col_step = snap["cur"], snap["ccur"], snap["col_step"]
 
-    KINDS = ("particles", "collision", "dml", "reindex_scatter")
+    KINDS = ("particles", "collision", "dml", "reindex_scatter", "collision_overflow")
 
     def prof_enable(self, max_events=4096):
         check(self.L.lm_prof_enable(C.byref(self.e), int(max_events)))
 
     def prof_collect(self):
-        ms = (C.c_double * 4)()
-        cnt = (C.c_int32 * 4)()
-        check(self.L.lm_prof_collect(C.byref(self.e), ms, cnt, 4))
+        nk = len(self.KINDS)
+        ms = (C.c_double * nk)()
+        cnt = (C.c_int32 * nk)()
+        check(self.L.lm_prof_collect(C.byref(self.e), ms, cnt, nk))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KINDS)}
 
     def prof_disable(self):
