@@ -1280,6 +1280,132 @@ struct ColLists {
   int32_t *in_grab; /* GRAB_CTRS zeroed counters: claim in_list's entries one at a time (NULL: grid-strided) */
 };
 
+/*
+ * Members of a cell with a complete migrant list, staged straight into the pass's arrays (the home segment's
+ * loads issued together with the list's, the state of each member loaded with its cell and id rather than after
+ * the ranking): every lane loads the cell, id, xi, u, w, m_w and m_s of its home slots (up to HOME_CHUNKS * 64
+ * of them) and of its list entry, the members keep slot order (= id order) merged by id with the list's
+ * members (each slot once), and each member's xi, u, w, radius, kernel index and slot go to its position.
+ * Returns the member count (nothing staged above CAP), or -1 when the list is incomplete or the home segment
+ * longer than the lanes hold (the caller takes the general path).  sM: 3 * LM_MIG_CAP_MAX ints, sH: CAP ints of
+ * scratch.
+ */
+template <int CAP, bool TAB>
+__device__ static int fast_stage(const LmPhys &P, const KView &v, int gc, double T, double rho_w_T, double *sX,
+                                 double *sU, double *sW, double *sR, int *sIND, int *sSL, int *sM, int *sH) {
+  constexpr int HC = CAP / WAVE + 1; /* home chunks a lane holds */
+  const int lane = threadIdx.x;
+  const int nc = v.nc, r = gc / nc, c = gc % nc;
+  const int mc = v.mig_cnt[gc];
+  const int32_t *seg = v.seg + (size_t)r * (nc + 1);
+  const int s0 = seg[c], T0 = seg[c + 1] - s0;
+  if (mc > v.mig_cap || T0 > HC * WAVE) return -1;
+  /* one round: the home slots' and the list entry's state (entries past T0 / mc are not read) */
+  int hcell[HC], hid[HC];
+  double hx[HC], hu[HC], hw[HC], hmw[HC], hms[HC];
+#pragma unroll
+  for (int q = 0; q < HC; ++q) {
+    const int t = lane + q * WAVE;
+    hcell[q] = -1;
+    hid[q] = 0;
+    hx[q] = hu[q] = hw[q] = hmw[q] = hms[q] = 0.0;
+    if (t < T0) {
+      const int k = s0 + t;
+      hcell[q] = v.cell[k];
+      hid[q] = v.id[k];
+      hx[q] = v.xi[k];
+      hu[q] = v.u[k];
+      hw[q] = v.w[k];
+      hmw[q] = v.m_w[k];
+      hms[q] = v.m_s[k];
+    }
+  }
+  int mk = -1;
+  if (lane < mc) mk = v.mig_slot[(size_t)gc * v.mig_cap + lane];
+  int mcell = -1, mid_ = 0;
+  double mx = 0.0, mu = 0.0, mw = 0.0, mmw = 0.0, mms = 0.0;
+  if (mk >= 0) {
+    mcell = v.cell[mk];
+    mid_ = v.id[mk];
+    mx = v.xi[mk];
+    mu = v.u[mk];
+    mw = v.w[mk];
+    mmw = v.m_w[mk];
+    mms = v.m_s[mk];
+  }
+  /* home members: rank among them by slot order */
+  int hpos[HC];
+  int n_own = 0;
+#pragma unroll
+  for (int q = 0; q < HC; ++q) {
+    const bool mem = hcell[q] == gc;
+    const unsigned long long bal = __ballot(mem);
+    hpos[q] = mem ? n_own + __popcll(bal & lane_mask_lt()) : -1;
+    n_own += __popcll(bal);
+  }
+  /* list members, each slot once (an earlier entry with the same slot wins), ranked by id */
+  int *sMk = sM, *sMid = sM + LM_MIG_CAP_MAX;
+  if (lane < mc) sMk[lane] = mk;
+  __syncthreads();
+  bool mmem = mk >= 0 && mcell == gc;
+  if (mmem)
+    for (int e = 0; e < lane; ++e)
+      if (sMk[e] == mk) mmem = false;
+  const unsigned long long mbal = __ballot(mmem);
+  const int nm = __popcll(mbal);
+  const int n = n_own + nm;
+  if (n > CAP) {
+    __syncthreads();
+    return n;
+  }
+  if (mmem) sMid[__popcll(mbal & lane_mask_lt())] = mid_;
+  /* home members' ids in rank order, for the list members' positions */
+#pragma unroll
+  for (int q = 0; q < HC; ++q)
+    if (hpos[q] >= 0) sH[hpos[q]] = hid[q];
+  __syncthreads();
+  /* position in id order: a home member's rank plus the list members with smaller ids; a list member's rank
+     among them plus the home members with smaller ids (ascending in sH) */
+#pragma unroll
+  for (int q = 0; q < HC; ++q) {
+    if (hpos[q] < 0) continue;
+    int lo = 0;
+    for (int e = 0; e < nm; ++e) lo += sMid[e] < hid[q];
+    const int p = hpos[q] + lo;
+    double R, ws, rho;
+    lmp_R_ws_rho(P, v.solute, hmw[q], hms[q], T, rho_w_T, R, ws, rho);
+    sX[p] = hx[q];
+    sU[p] = hu[q];
+    sW[p] = hw[q];
+    sR[p] = R;
+    sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
+    sSL[p] = s0 + lane + q * WAVE;
+  }
+  if (mmem) {
+    int rk = 0;
+    for (int e = 0; e < nm; ++e) rk += sMid[e] < mid_;
+    int lo = 0, hi = n_own;
+    while (lo < hi) {
+      const int m2 = (lo + hi) >> 1;
+      if (sH[m2] < mid_)
+        lo = m2 + 1;
+      else
+        hi = m2;
+    }
+    const int p = rk + lo;
+    double R, ws, rho;
+    lmp_R_ws_rho(P, v.solute, mmw, mms, T, rho_w_T, R, ws, rho);
+    sX[p] = mx;
+    sU[p] = mu;
+    sW[p] = mw;
+    sR[p] = R;
+    sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
+    sSL[p] = mk;
+  }
+  __syncthreads();
+  return n;
+}
+
 template <int CAP, bool TAB, bool INJ>
 __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) {
   const LmPhys &P = c_P;
@@ -1333,6 +1459,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
      id order from the 3x3 home segments by one lane */
   int n;
   bool fast;
+  bool staged = false; /* members' state already in the arrays (fast_stage) */
   if constexpr (BIG) {
     fast = true;
     if (v.mlist) {
@@ -1378,7 +1505,15 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
       __syncthreads();
     }
   } else {
-    n = v.mig_cnt ? fast_members(v, gc, sSL, CAP, sSlot, sId, (int *)sW) : -1;
+    n = -1;
+    if constexpr (CAP > 0 && CAP <= 256) {
+      /* the queue's storage is free until the first pass */
+      if (v.mig_cnt)
+        n = fast_stage<CAP, TAB>(P, v, gc, v.cc[(size_t)gc * CC_N + CC_T], v.cc[(size_t)gc * CC_N + CC_RHOW], sX,
+                                 sU, sW, sR, sIND, sSL, (int *)sQkey, sQ);
+    }
+    staged = n >= 0;
+    if (!staged) n = v.mig_cnt ? fast_members(v, gc, sSL, CAP, sSlot, sId, (int *)sW) : -1;
     fast = n >= 0;
   }
   if (!fast) {
@@ -1408,7 +1543,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
 
   const double *C = v.cc + (size_t)gc * CC_N;
   const double T = C[CC_T], rho_w_T = C[CC_RHOW];
-  for (int p = lane; p < n; p += WAVE) {
+  for (int p = lane; p < n && !staged; p += WAVE) {
     const int k = sSL[p];
     sX[p] = v.xi[k];
     sU[p] = v.u[k];
