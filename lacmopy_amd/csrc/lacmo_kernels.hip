@@ -739,6 +739,8 @@ struct CollCell {
   int gc;   /* global cell */
   int *seq; /* LDS: events of this cell so far (event log) */
   const float *sB; /* LDS: row bounds of the pass filter */
+  double T, rho_w_T; /* the cell's temperature and water density at it */
+  uint32_t *chg;     /* one bit per member: its masses changed in this step (its pre-step density saved) */
 };
 
 /* global SIP id of slot k: runs occupy the slot ranges [run_base[r], run_base[r+1]) */
@@ -797,14 +799,42 @@ __device__ static inline double pair_p(const LmPhys &P, const KView &v, const Co
   return xi_max * v.dt_col_over_dV * lmp_kernel_hydro(P, cc.sR[i], cc.sR[j], Ec, sqrt(du * du + dw * dw));
 }
 
-/* global state of the two members of a pair: masses and pre-step densities (one round of loads) */
+/* member m's bit of the changed set (plain read; set by chg_set, atomically, any number of lanes at once) */
+__device__ static inline bool chg_get(const uint32_t *chg, int m) { return (chg[m >> 5] >> (m & 31)) & 1u; }
+__device__ static inline void chg_set(uint32_t *chg, int m) { atomicOr(chg + (m >> 5), 1u << (m & 31)); }
+
+/* the density of all_or_nothing.py:822-827 (compute_R_p_w_s_rho_p of the cell before its loop) from a member's
+   masses */
+__device__ static inline double pre_density(const LmPhys &P, const KView &v, const CollCell &cc, double mw, double ms) {
+  const double m_p = mw + ms;
+  const double w_s = ms / m_p;
+  return lmp_density_solution(P, v.solute, w_s, cc.T, cc.rho_w_T);
+}
+
+/*
+ * Global state of the two members of a pair: masses and pre-step densities (one round of loads).  A member
+ * no event of this step has changed still holds its pre-step masses, and its density is computed from them;
+ * a changed member's pre-step density was saved in rho_g (slot-indexed) by the event that first changed it.
+ */
 struct PairMass {
   double mw_i, mw_j, ms_i, ms_j, rho_i, rho_j;
 };
-__device__ static inline PairMass load_pair(const KView &v, const CollCell &cc, const double *rho_g, int i, int j) {
+__device__ static inline PairMass load_pair(const LmPhys &P, const KView &v, const CollCell &cc, const double *rho_g,
+                                            int i, int j) {
   const int ki = cc.sSL[i], kj = cc.sSL[j];
-  const PairMass m = {v.m_w[ki], v.m_w[kj], v.m_s[ki], v.m_s[kj], rho_g[ki], rho_g[kj]};
+  const bool ci = chg_get(cc.chg, i), cj = chg_get(cc.chg, j);
+  PairMass m = {v.m_w[ki], v.m_w[kj], v.m_s[ki], v.m_s[kj], ci ? rho_g[ki] : 0.0, cj ? rho_g[kj] : 0.0};
+  if (!ci) m.rho_i = pre_density(P, v, cc, m.mw_i, m.ms_i);
+  if (!cj) m.rho_j = pre_density(P, v, cc, m.mw_j, m.ms_j);
   return m;
+}
+
+/* before member m (slot k) first takes new masses: keep its pre-step density for later events of the step */
+__device__ static inline void keep_pre_density(const KView &v, const CollCell &cc, int m, int k, double rho) {
+  if (!chg_get(cc.chg, m)) {
+    v.dmxi[k] = rho;
+    chg_set(cc.chg, m);
+  }
 }
 
 /*
@@ -829,7 +859,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   const int kmin = cc.sSL[imin], kmax = cc.sSL[imax];
   const double mw_min = i_min ? M.mw_i : M.mw_j, mw_max = i_min ? M.mw_j : M.mw_i;
   const double ms_min = i_min ? M.ms_i : M.ms_j, ms_max = i_min ? M.ms_j : M.ms_i;
-  const double rho_min = i_min ? M.rho_i : M.rho_j;
+  const double rho_min = i_min ? M.rho_i : M.rho_j, rho_max = i_min ? M.rho_j : M.rho_i;
   const bool table = !v.e_const_mode;
   if (v.ev_mode == 1) {
     const int kind = p_crit > 1.0 ? 1 : ((xi_max - xi_min) / xi_max < 1.0E-5 ? 3 : 2);
@@ -838,6 +868,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   if (p_crit > 1.0) {
     const double xi_col = p_crit * xi_min;
     const double mwn = mw_min + p_crit * mw_max, msn = ms_min + p_crit * ms_max;
+    keep_pre_density(v, cc, imin, kmin, rho_min);
     v.m_w[kmin] = mwn;
     v.m_s[kmin] = msn;
     const double Rn = lmp_radius(P, (mwn + msn), rho_min);
@@ -851,6 +882,8 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
     const double xi_ges = xi_min + xi_max;
     const double mwn = 2.0 * (xi_min * mw_min + xi_max * mw_max) / xi_ges;
     const double msn = 2.0 * (xi_min * ms_min + xi_max * ms_max) / xi_ges;
+    keep_pre_density(v, cc, imin, kmin, rho_min);
+    keep_pre_density(v, cc, imax, kmax, rho_max);
     v.m_w[kmin] = mwn;
     v.m_s[kmin] = msn;
     v.m_w[kmax] = mwn;
@@ -869,6 +902,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
     return 3;
   }
   const double mwn = mw_min + mw_max, msn = ms_min + ms_max;
+  keep_pre_density(v, cc, imin, kmin, rho_min);
   v.m_w[kmin] = mwn;
   v.m_s[kmin] = msn;
   const double Rn = lmp_radius(P, (mwn + msn), rho_min);
@@ -912,11 +946,12 @@ struct BigWs {
   int32_t *IND, *SL, *Q;
   float *B;
   unsigned char *TG, *IsBig;
+  uint32_t *Chg;
 };
 __host__ __device__ constexpr size_t big_up(size_t b) { return (b + 255) & ~(size_t)255; }
 __host__ __device__ constexpr size_t big_stride(int cap) {
   return 4 * big_up(8 * (size_t)cap) + 2 * big_up(4 * (size_t)cap) + big_up(4 * ((size_t)cap + 1)) +
-         big_up(4 * (size_t)cap) + 2 * big_up((size_t)cap);
+         big_up(4 * (size_t)cap) + 2 * big_up((size_t)cap) + big_up(4 * (((size_t)cap + 31) / 32));
 }
 __device__ static inline BigWs big_carve(char *b, int cap) {
   BigWs w;
@@ -940,6 +975,8 @@ __device__ static inline BigWs big_carve(char *b, int cap) {
   w.TG = (unsigned char *)(b + o);
   o += big_up((size_t)cap);
   w.IsBig = (unsigned char *)(b + o);
+  o += big_up((size_t)cap);
+  w.Chg = (uint32_t *)(b + o);
   return w;
 }
 
@@ -1365,22 +1402,24 @@ __device__ static int fast_stage(const LmPhys &P, const KView &v, int gc, double
     if (hpos[q] >= 0) sH[hpos[q]] = hid[q];
   __syncthreads();
   /* position in id order: a home member's rank plus the list members with smaller ids; a list member's rank
-     among them plus the home members with smaller ids (ascending in sH) */
+     among them plus the home members with smaller ids (ascending in sH).  The masses go to sU / sW at that
+     position until the radii are known: the radius and kernel index then take ceil(n / 64) rounds instead of
+     one per home chunk plus one for the list */
+  int hp[HC];
 #pragma unroll
   for (int q = 0; q < HC; ++q) {
+    hp[q] = -1;
     if (hpos[q] < 0) continue;
     int lo = 0;
     for (int e = 0; e < nm; ++e) lo += sMid[e] < hid[q];
     const int p = hpos[q] + lo;
-    double R, ws, rho;
-    lmp_R_ws_rho(P, v.solute, hmw[q], hms[q], T, rho_w_T, R, ws, rho);
+    hp[q] = p;
     sX[p] = hx[q];
-    sU[p] = hu[q];
-    sW[p] = hw[q];
-    sR[p] = R;
-    sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
+    sU[p] = hmw[q];
+    sW[p] = hms[q];
     sSL[p] = s0 + lane + q * WAVE;
   }
+  int mp = -1;
   if (mmem) {
     int rk = 0;
     for (int e = 0; e < nm; ++e) rk += sMid[e] < mid_;
@@ -1393,14 +1432,29 @@ __device__ static int fast_stage(const LmPhys &P, const KView &v, int gc, double
         hi = m2;
     }
     const int p = rk + lo;
-    double R, ws, rho;
-    lmp_R_ws_rho(P, v.solute, mmw, mms, T, rho_w_T, R, ws, rho);
+    mp = p;
     sX[p] = mx;
-    sU[p] = mu;
-    sW[p] = mw;
+    sU[p] = mmw;
+    sW[p] = mms;
+    sSL[p] = mk;
+  }
+  __syncthreads();
+  for (int p = lane; p < n; p += WAVE) {
+    double R, ws, rho;
+    lmp_R_ws_rho(P, v.solute, sU[p], sW[p], T, rho_w_T, R, ws, rho);
     sR[p] = R;
     sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
-    sSL[p] = mk;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < HC; ++q)
+    if (hp[q] >= 0) {
+      sU[hp[q]] = hu[q];
+      sW[hp[q]] = hw[q];
+    }
+  if (mp >= 0) {
+    sU[mp] = mu;
+    sW[mp] = mw;
   }
   __syncthreads();
   return n;
@@ -1424,11 +1478,13 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   __shared__ int lIND[LC], lSL[LC], lQ[LC + 1];
   __shared__ float lB[LC];
   __shared__ unsigned char lTG[LC], lIsBig[LC];
+  __shared__ uint32_t lChg[(LC + 31) / 32];
   double *sX = lX, *sR = lR, *sU = lU, *sW = lW;
   int *sIND = lIND, *sSL = lSL, *sQ = lQ;
   float *sB = lB;
   unsigned char *sTG = lTG, *sIsBig = lIsBig; /* sIsBig: rank in the list of largest radii the guard pass
                                                   leaves out, or 0 */
+  uint32_t *sChg = lChg;
   if constexpr (BIG) {
     const BigWs b = big_carve(v.big_ws + (size_t)blockIdx.x * big_stride(v.big_cap), v.big_cap);
     sX = b.X;
@@ -1441,6 +1497,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
     sB = b.B;
     sTG = b.TG;
     sIsBig = b.IsBig;
+    sChg = b.Chg;
   }
   constexpr int QC = qcap_of(CAP), NQ = QC / WAVE;
   __shared__ int sQueue[QC];
@@ -1533,6 +1590,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   }
   if (n < 2) return;
   if (lane == 0) sSeq = 0;
+  for (int w = lane; w < (n + 31) / 32; w += WAVE) sChg[w] = 0u;
   __syncthreads();
   if (!fast) {
     rank_members(n, sId, sSub, &tb, sRank);
@@ -1571,7 +1629,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   }
   __syncthreads();
 
-  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n, gc, &sSeq, sB};
+  const CollCell cc = {sX, sR, sU, sW, sIND, sSL, n, gc, &sSeq, sB, T, rho_w_T, sChg};
   const bool use_tab = TAB;
   const long long roff = INJ ? v.rnd_off[gc] : 0;
   const double dtdV = v.dt_col_over_dV, pi = P.pi;
@@ -1581,13 +1639,12 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
   __syncthreads();
   const uint32_t key_run = (uint32_t)r, key_step = *v.col_step_dev, key_cell = (uint32_t)c;
   const uint32_t gkey = INJ ? 0u : lm_guard_key(v.seed, key_run, key_step, key_cell);
-  double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step density of each member */
+  const double *rho_g = v.dmxi; /* slot-indexed scratch: pre-step densities of the changed members */
   const int nQ = sQ[n - 1];
   long long c_single = 0, c_multi = 0, c_eq = 0;
   int k_next = 0;  /* first pair (row-major number) not yet decided */
   unsigned long long prev_event = 0; /* key of the last applied event */
   int n_events = 0;
-  bool rho_ready = false;
   bool finished = false;
 
   bool build = true; /* the candidate queue is built from the row bounds by the first pass */
@@ -1737,24 +1794,12 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
       finished = true;
       break;
     }
-    if (!rho_ready) {
-      /* first event of the cell: the pre-step densities from the still unchanged masses */
-      for (int p = lane; p < n; p += WAVE) {
-        const int k = sSL[p];
-        double R, ws, rho;
-        lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
-        rho_g[k] = rho;
-      }
-      rho_ready = true;
-      __threadfence_block();
-      __syncthreads();
-    }
     const int ei = (int)((best >> 16) & 0xffff), ej = (int)(best & 0xffff);
     const int ecnt = (int)(best >> 32);
     if (lane == 0) {
       /* the table entry and the two members' masses and densities in one round of loads */
       const double Ec = use_tab ? v.E[sIND[ei] * v.nb + sIND[ej]] : v.E_const;
-      const PairMass M = load_pair(v, cc, rho_g, ei, ej);
+      const PairMass M = load_pair(P, v, cc, rho_g, ei, ej);
       const int kind = apply_event(P, v, cc, M, ei, ej, pair_p(P, v, cc, ei, ej, Ec));
       if (kind == 1)
         ++c_multi;
@@ -1839,18 +1884,6 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
     }
   }
 
-  if (!finished && !rho_ready) {
-    /* the pre-step densities (the candidate queue overflowed before the cell's first event) */
-    for (int p = lane; p < n; p += WAVE) {
-      const int k = sSL[p];
-      double R, ws, rho;
-      lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
-      rho_g[k] = rho;
-    }
-    rho_ready = true;
-    __threadfence_block();
-    __syncthreads();
-  }
   if (!finished) {
     /* many events or candidates: anti-diagonal schedule over the pairs with cnt >= k_next */
     int tag0 = -1, tag1 = -1;
@@ -1896,7 +1929,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
         else
           fires = pair_fires(p_crit, G, aw, v.seed, key_run, key_step, key_cell, (uint32_t)i, (uint32_t)d);
         if (fires) {
-          const int kind = apply_event(P, v, cc, load_pair(v, cc, rho_g, i, j), i, j, p_crit);
+          const int kind = apply_event(P, v, cc, load_pair(P, v, cc, rho_g, i, j), i, j, p_crit);
           if (kind == 1)
             ++c_multi;
           else
