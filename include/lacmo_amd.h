@@ -441,6 +441,17 @@ int lm_engine_check(LmEngine *e, void *stream);
 int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, double *out, void *stream);
 
 /*
+ * Check of the collision kernels' fast kernel-table index (the bin of all_or_nothing.py:35-65 from the hardware
+ * float log2 where the bin is certain, else the exact double evaluation): out_fast[k] / out_exact[k] = the fast and
+ * the exact index of R[k] for the table (lnR0, lnf, nb).  If sweep is not NULL (4 doubles): sweep[0] = the largest
+ * |hardware log2(x) - log2(x)| over every float x of the fast path's range [2^-14, 2^16), sweep[1] = the bound the
+ * fast path assumes for it, sweep[2] = the resulting margin in t, sweep[3] = the floats swept.  Test probe; no
+ * counterpart in the reference.
+ */
+int lm_kernel_index_check(int64_t n, const double *R, double lnR0, double lnf, int32_t nb, int32_t *out_fast,
+                          int32_t *out_exact, double *sweep, void *stream);
+
+/*
  * Batched box model: one collision step in each of n_boxes independent boxes
  * (collision/all_or_nothing.py:109-185 collision_step_Golovin_np).  Box b
  * holds SIPs [off[b], off[b+1]) of xis/masses (at most 1024); counters

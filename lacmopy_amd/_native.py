@@ -110,6 +110,7 @@ EXPORTED = [
     "lm_adj_adv_steps", "lm_init_par_sizeof", "lm_init_level_profile", "lm_init_level_sips", "lm_init_positions",
     "lm_box_minmax", "lm_box_stats", "lm_mean_sem", "lm_collision_all_cells", "lm_collision_all_cells_check",
     "lm_collision_all_cells_noalloc", "lm_big_ws_bytes", "lm_load_reference_layout", "lm_store_reference_layout",
+    "lm_kernel_index_check",
 ]
 
 _lib = None
@@ -174,6 +175,7 @@ def lib():
     L.lm_ecol_box_step.argtypes = [_i32, _p, _p, _p, _p, _p, _p, _d, _p, _i32, _d, _d, _p,
                                    C.c_uint64, C.c_uint32, _p, _i32, _p]
     L.lm_math_probe.argtypes = [_i32, _i64, _p, _p, _p, _p]
+    L.lm_kernel_index_check.argtypes = [_i64, _p, _d, _d, _i32, _p, _p, _p, _p]
     L.lm_box_minmax.argtypes = [_i32, _i64, _p, _p, _p, _p]
     L.lm_box_stats.argtypes = [_i32, _i32, _p, _i64, _p, _p, _p, _p, _p, _i32, _d, _p, _p, _p, _p, _p]
     L.lm_mean_sem.argtypes = [_p, _i32, _i64, _p, _p, _p]
@@ -191,7 +193,8 @@ def lib():
                  "lm_graph_launch", "lm_graph_destroy", "lm_derived_fields", "lm_ensemble_stats",
                  "lm_build_slot_of", "lm_objective", "lm_adj_seed", "lm_adj_adv_steps", "lm_init_level_profile",
                  "lm_init_level_sips", "lm_init_positions", "lm_box_minmax", "lm_box_stats", "lm_mean_sem",
-                 "lm_collision_all_cells", "lm_collision_all_cells_check", "lm_collision_all_cells_noalloc"]:
+                 "lm_collision_all_cells", "lm_collision_all_cells_check", "lm_collision_all_cells_noalloc",
+                 "lm_kernel_index_check"]:
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -421,9 +421,46 @@ struct KView {
   const long long *run_base;
   double *tape_part, *ccd;
   int32_t *tape_part_i;
+  double ki_inv_lnf, ki_margin; /* kernel_index_fast: 1 / lnf and the bound on its error in t (set by ki_setup) */
 };
 
 #define SPEC_MAX_EVENTS 48 /* default of LmEngine.spec_events */
+
+/*
+ * Kernel-table index without the double log where the answer is certain.  The index is
+ * clamp(trunc(t), 0, nb - 1) with t = (ln R - lnR0) / lnf + 0.5 (all_or_nothing.py:35-65, lmp_kernel_index).
+ * kernel_index_fast takes ln R from the hardware log2 of R rounded to float: |ln(float(R)) - ln R| <= 2^-24
+ * (1 + 2^-23), the hardware log2 errs by at most KI_LOG2_ERR (absolute, on every float of [KI_RMIN, KI_RMAX):
+ * checked exhaustively on the GPU by lm_kernel_index_check), and the double operations of both evaluations
+ * add below 1e-12 in ln R and 1e-9 in t.  So |t_approx - t| < ki_margin, and a t_approx farther than that from
+ * every integer truncates (after the clamp) like t; otherwise, and outside [KI_RMIN, KI_RMAX) or for NaN, the
+ * exact evaluation runs.
+ */
+#define KI_LOG2_ERR 4.0e-6
+#define KI_RMIN 0x1p-14
+#define KI_RMAX 0x1p16
+static void ki_setup(KView &v) {
+  const double e_ln = 0x1.000002p-24 + KI_LOG2_ERR * 0.6931471805599453 + 1e-12;
+  v.ki_inv_lnf = 0.0;
+  v.ki_margin = 1.0; /* fast path off */
+  if (v.lnf > 0.0 && v.lnf < 1e300) {
+    v.ki_inv_lnf = 1.0 / v.lnf;
+    const double m = 2.0 * e_ln * v.ki_inv_lnf + 1e-9;
+    if (m < 0.25) v.ki_margin = m;
+  }
+}
+
+__device__ static inline int kernel_index_fast(double R, const KView &v) {
+  if (R >= KI_RMIN && R < KI_RMAX) {
+    const double L = (double)__builtin_amdgcn_logf((float)R) * 0.6931471805599453;
+    const double ta = (L - v.lnR0) * v.ki_inv_lnf + 0.5;
+    const double fl = floor(ta);
+    const double fr = ta - fl;
+    if (fr > v.ki_margin && fr < 1.0 - v.ki_margin)
+      return fl < 0.0 ? 0 : (fl > (double)(v.nb - 1) ? v.nb - 1 : (int)fl);
+  }
+  return lmp_kernel_index(R, v.lnR0, v.lnf, v.nb);
+}
 
 static KView make_view(const LmEngine *e) {
   KView v;
@@ -453,6 +490,7 @@ static KView make_view(const LmEngine *e) {
   v.E_const = e->E_const;
   v.lnR0 = e->lnR0;
   v.lnf = e->lnf;
+  ki_setup(v);
   v.seed = e->seed;
   v.col_step_dev = e->col_step_dev;
   const int c = e->cur, o = 1 - e->cur;
@@ -874,7 +912,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
     const double Rn = lmp_radius(P, (mwn + msn), rho_min);
     cc.sR[imin] = Rn;
     cc.sX[imax] = xi_max - xi_col;
-    if (table) cc.sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+    if (table) cc.sIND[imin] = kernel_index_fast(Rn, v);
     return 1;
   }
   const double xi_rel_dev = (xi_max - xi_min) / xi_max;
@@ -895,7 +933,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
     cc.sX[imax] = xmax_new;
     cc.sX[imin] = 0.5 * xi_ges - xmax_new;
     if (table) {
-      const int ind = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+      const int ind = kernel_index_fast(Rn, v);
       cc.sIND[imin] = ind;
       cc.sIND[imax] = ind;
     }
@@ -908,7 +946,7 @@ __device__ static inline int apply_event(const LmPhys &P, const KView &v, const 
   const double Rn = lmp_radius(P, (mwn + msn), rho_min);
   cc.sR[imin] = Rn;
   cc.sX[imax] = xi_max - xi_min;
-  if (table) cc.sIND[imin] = lmp_kernel_index(Rn, v.lnR0, v.lnf, v.nb);
+  if (table) cc.sIND[imin] = kernel_index_fast(Rn, v);
   return 2;
 }
 
@@ -1443,7 +1481,7 @@ __device__ static int fast_stage(const LmPhys &P, const KView &v, int gc, double
     double R, ws, rho;
     lmp_R_ws_rho(P, v.solute, sU[p], sW[p], T, rho_w_T, R, ws, rho);
     sR[p] = R;
-    sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
+    sIND[p] = TAB ? kernel_index_fast(R, v) : 0;
   }
   __syncthreads();
 #pragma unroll
@@ -1609,7 +1647,7 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
     double R, ws, rho;
     lmp_R_ws_rho(P, v.solute, v.m_w[k], v.m_s[k], T, rho_w_T, R, ws, rho);
     sR[p] = R;
-    sIND[p] = TAB ? lmp_kernel_index(R, v.lnR0, v.lnf, v.nb) : 0;
+    sIND[p] = TAB ? kernel_index_fast(R, v) : 0;
   }
   /* quads per row: row i holds diagonals d = 2i+1 .. i+n-1, blocks (2i+1)>>2 .. (i+n-1)>>2; sQ[i] = quads
      of the rows before i (wave-wide exclusive scan, 64 rows at a time), sQ[n-1] = all quads */
@@ -3892,6 +3930,40 @@ __global__ void k_math_probe(int which, long long n, const double *x, const doub
   out[k] = r;
 }
 
+/* fast and exact kernel-table indices of the radii R[0..n) (lm_kernel_index_check) */
+__global__ void k_ki_probe(KView v, long long n, const double *R, int32_t *fast, int32_t *exact) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  fast[k] = kernel_index_fast(R[k], v);
+  exact[k] = lmp_kernel_index(R[k], v.lnR0, v.lnf, v.nb);
+}
+
+/* largest |hardware log2(x) - log2(x)| over every float x of [KI_RMIN, KI_RMAX) into out[0] (as ordered bits) */
+__global__ __launch_bounds__(256) void k_log2_sweep(uint32_t lo, uint32_t count, unsigned long long *out) {
+  double e = 0.0;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += gridDim.x * 256u) {
+    const float x = __uint_as_float(lo + i);
+    const double hw = (double)__builtin_amdgcn_logf(x);
+    const double ex = lm_log((double)x) * 1.4426950408889634;
+    const double d = fabs(hw - ex);
+    e = d > e ? d : e;
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const double o = __shfl_xor(e, off);
+    e = o > e ? o : e;
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
+}
+
+__global__ void k_ki_consts(KView v, double *out, uint32_t count) {
+  if (threadIdx.x == 0) {
+    out[0] = 0.0;
+    out[1] = KI_LOG2_ERR;
+    out[2] = v.ki_margin;
+    out[3] = (double)count;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* C ABI                                                                    */
 /* ------------------------------------------------------------------------ */
@@ -4630,6 +4702,30 @@ int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, do
   return 0;
 }
 
+int lm_kernel_index_check(int64_t n, const double *R, double lnR0, double lnf, int32_t nb, int32_t *out_fast,
+                          int32_t *out_exact, double *sweep, void *stream) {
+  if (n < 0 || nb < 1 || (n > 0 && (!R || !out_fast || !out_exact))) return set_err(LM_ERR_ARG, "bad arguments");
+  KView v;
+  memset(&v, 0, sizeof(v));
+  v.lnR0 = lnR0;
+  v.lnf = lnf;
+  v.nb = nb;
+  ki_setup(v);
+  if (n > 0)
+    hipLaunchKernelGGL(k_ki_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S(stream), v, (long long)n, R,
+                       out_fast, out_exact);
+  if (sweep) {
+    float a = (float)KI_RMIN, b = (float)KI_RMAX;
+    uint32_t lo, hi;
+    memcpy(&lo, &a, 4);
+    memcpy(&hi, &b, 4);
+    hipLaunchKernelGGL(k_ki_consts, dim3(1), dim3(64), 0, S(stream), v, sweep, hi - lo);
+    hipLaunchKernelGGL(k_log2_sweep, dim3(4096), dim3(256), 0, S(stream), lo, hi - lo, (unsigned long long *)sweep);
+  }
+  LAUNCH_CHECK();
+  return 0;
+}
+
 static int32_t *g_box_err = nullptr;
 
 static int box_prepare() {
@@ -4807,6 +4903,7 @@ static int ref_step(const RefWork &w, long long n_sips, double *xis, double *m_w
   v.E_const = E_const;
   v.lnR0 = lnR0;
   v.lnf = lnf;
+  ki_setup(v);
   v.E_max = E ? E_max : 0.0;
   v.seed = seed;
   v.col_step_dev = (const uint32_t *)(w.small + 6);

@@ -51,3 +51,43 @@ def test_ieee_ops_device_equals_host(gpu):
     L = np.full_like(x, 1500.0)
     assert np.array_equal(_dev(3, x, L), np.mod(x, L))
     assert np.array_equal(_dev(3, x, L), orc.lm_vec(3, x, L))
+
+
+def test_fast_kernel_index_equals_exact(gpu):
+    """The collision kernels' fast kernel-table index (the bin of a radius from the hardware float log2 where the
+    bin is certain) equals the exact double evaluation: the hardware log2 errs by no more than the bound the fast
+    path assumes on every float of its range, and the two indices agree on random radii, on radii at and within a
+    few ulps and a few margins of every bin edge of both tables, and outside the fast path's range."""
+    import torch
+    from lacmopy_amd import load_kernel_data_Ecol
+    from lacmopy_amd._native import check, lib
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    sweep = torch.zeros(4, dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(17)
+    for name in ["Long_Bott", "Hall_Bott"]:
+        _, lnR0, lnf, nb = load_kernel_data_Ecol(name)
+        parts = [np.exp(rng.uniform(np.log(1e-6), np.log(1e6), 2_000_000))]
+        edges = np.exp(lnR0 + (np.arange(-3, nb + 3) - 0.5) * lnf)
+        for d in range(-8, 9):  # the edges and their neighbours ulp by ulp
+            parts.append(edges + d * np.spacing(edges))
+        for rel in [1e-9, 1e-7, 1e-6, 3e-6, 1e-5, 3e-5, 1e-4]:  # around the fast path's margin (~2.5e-4 * lnf)
+            parts += [edges * (1 + rel), edges * (1 - rel)]
+        lim = np.array([2.0 ** -14, 2.0 ** 16])
+        parts += [lim, np.nextafter(lim, 0), np.nextafter(lim, np.inf),
+                  np.array([0.0, 5e-324, 1e-300, 1e300, np.inf, 1.0, np.exp(lnR0)])]
+        R = torch.as_tensor(np.concatenate(parts), device="cuda")
+        fast = torch.empty(R.numel(), dtype=torch.int32, device="cuda")
+        exact = torch.empty_like(fast)
+        check(lib().lm_kernel_index_check(R.numel(), C.c_void_p(R.data_ptr()), lnR0, lnf, nb,
+                                          C.c_void_p(fast.data_ptr()), C.c_void_p(exact.data_ptr()),
+                                          C.c_void_p(sweep.data_ptr()), s))
+        torch.cuda.synchronize()
+        f, e = fast.cpu().numpy(), exact.cpu().numpy()
+        bad = np.flatnonzero(f != e)
+        assert bad.size == 0, (name, R.cpu().numpy()[bad[:5]], f[bad[:5]], e[bad[:5]])
+        assert e.min() == 0 and e.max() == nb - 1
+        max_err, bound, margin, swept = sweep.cpu().numpy()
+        assert swept == 30 * 2 ** 23
+        assert 0.0 < max_err <= bound, (max_err, bound)
+        assert 0.0 < margin < 1e-3, margin
+        print(f"{name}: hardware log2 error {max_err:.3g} (bound {bound:.3g}), margin in t {margin:.3g}")
