@@ -2164,6 +2164,11 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
   const int r = gc / nc, c = gc % nc;
   const int i = c / v.nz, j = c % v.nz;
   double x = v.x[k], z = v.z[k], u = v.u[k], w = v.w[k];
+  /* quotients by the cell sizes through their reciprocals (lm_div_y: the same values) */
+  const bool fd = lm_div_range(v.dx) & lm_div_range(v.dz);
+  const double ydx = lm_rcp_nr(v.dx), ydz = lm_rcp_nr(v.dz);
+  const auto over_dx = [&](double a) { return fd ? lm_div_y(a, v.dx, ydx) : a / v.dx; };
+  const auto over_dz = [&](double a) { return fd ? lm_div_y(a, v.dz, ydz) : a / v.dz; };
   if (!POS_ONLY) {
     const double *C = v.cc + (size_t)gc * CC_N;
     const double T = C[CC_T];
@@ -2174,9 +2179,7 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
     if (TAPE) {
       dm = newton_dm_tape<SOLUTE, TPART>(P, v, k, n_tot, gc, dt_sub, m_w, m_s, xi, C);
     } else {
-      lmp_R_ws_rho(P, SOLUTE, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
-      const double sigma_p = lmp_sigma_solution(P, SOLUTE, w_s, C[CC_SIGW]);
-      dm = lmp_newton_dm(P, SOLUTE, dt_sub, v.no_iter, m_w, m_s, w_s, R, T, rho, C, sigma_p);
+      dm = lmp_condensation_dm(P, SOLUTE, dt_sub, v.no_iter, m_w, m_s, T, C);
     }
     m_w += dm;
     v.m_w[k] = m_w;
@@ -2184,8 +2187,8 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
     /* integration.py:1075: AS radius and density after the mass update */
     lmp_R_ws_rho(P, 0, m_w, m_s, T, C[CC_RHOW], R, w_s, rho);
     /* grid.py:257-323 with rel_pos of integration.py:545-549 */
-    const double wx = (x - v.x_min) / v.dx - (double)i;
-    const double wy = (z - v.z_min) / v.dz - (double)j;
+    const double wx = over_dx(x - v.x_min) - (double)i;
+    const double wy = over_dz(z - v.z_min) - (double)j;
     const double *U = v.grid_vel + (size_t)r * v.vel_stride, *V = U + (size_t)(v.nx + 1) * (v.nz + 1);
     double uf, vf;
     lmp_interp_vel(i, j, wx, wy, U, V, v.nx, v.nz, uf, vf);
@@ -2201,8 +2204,7 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
       else
         k_dt *= 0.0183333333333 * Re;
     }
-    u = (u + k_dt * uf) / (1.0 + k_dt);
-    w = (w + k_dt * vf - dt_sub * v.g_set) / (1.0 + k_dt);
+    lmp_div2(u + k_dt * uf, w + k_dt * vf - dt_sub * v.g_set, 1.0 + k_dt, u, w);
   }
   /* integration.py:552-630 */
   x += dt_pos * u;
@@ -2222,8 +2224,8 @@ __device__ __forceinline__ static void particle_update(const KView &v, long long
       raise_err(v.err, LM_ERR_REM_CAP, r);
   } else {
     /* integration.py:545-546 */
-    const long long ii = (long long)floor(x / v.dx);
-    const long long jj = (long long)floor(z / v.dz);
+    const long long ii = (long long)floor(over_dx(x));
+    const long long jj = (long long)floor(over_dz(z));
     if (ii < 0 || ii >= v.nx || jj < 0 || jj >= v.nz) {
       raise_err(v.err, LM_ERR_OUT_OF_DOMAIN, (int)k);
       gc_new = gc;

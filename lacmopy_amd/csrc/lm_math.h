@@ -294,6 +294,32 @@ LM_FN double lm_pow_third(double x) {
   return c + (c_lo - c * eps);
 }
 
+/*
+ * Quotients that share a denominator.  gfx950 divides a / b (IEEE, correctly rounded) by the sequence
+ * v_div_scale (a and b), v_rcp_f64, two Newton steps on the reciprocal (y), q = a y, r = fma(-b, q, a),
+ * v_div_fmas (fma(r, y, q)) and v_div_fixup.  With |a| and |b| in [2^-300, 2^300] (lm_div_range) the scale steps
+ * leave a and b alone and the fix-up step leaves the normal quotient alone, so lm_div_nr(a, b, lm_rcp_nr(b))
+ * is that sequence without them: a / b bit for bit, with one reciprocal for every numerator of b.  The host
+ * build (the oracle) never takes this path and divides.
+ */
+LM_FN int lm_div_range(double x) {
+  const double ax = __builtin_fabs(x);
+  return ax >= 0x1p-300 && ax <= 0x1p300;
+}
+#if defined(__HIPCC__)
+LM_FN double lm_rcp_nr(double b) {
+  const double y0 = __builtin_amdgcn_rcp(b);
+  const double y1 = LM_FMA(y0, LM_FMA(-b, y0, 1.0), y0);
+  return LM_FMA(y1, LM_FMA(-b, y1, 1.0), y1);
+}
+LM_FN double lm_div_nr(double a, double b, double y) {
+  const double q = a * y;
+  return LM_FMA(LM_FMA(-b, q, a), y, q);
+}
+/* a / b, given y = lm_rcp_nr(b) for a b in lm_div_range */
+LM_FN double lm_div_y(double a, double b, double y) { return lm_div_range(a) ? lm_div_nr(a, b, y) : a / b; }
+#endif
+
 /* Python / Numba float floor-modulo a % b (fmod, then shift into the sign of b). */
 LM_FN double lm_pymod(double a, double b) {
   if (a > 0.0 && a < b) return a; /* fmod(a, b) = a, already of b's sign */

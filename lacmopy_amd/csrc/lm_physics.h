@@ -90,6 +90,26 @@ LMDT R lmp_sigma_water(R T) {
   const R tau = 1 - T / 647.096;
   return 0.2358 * lm_pow(tau, 1.256) * (1 - 0.625 * tau);
 }
+/* q1 = a1 / b and q2 = a2 / b */
+template <class A1, class A2, class B>
+LMD void lmp_div2(A1 a1, A2 a2, B b, LmCommonT<A1, B> &q1, LmCommonT<A2, B> &q2) {
+  q1 = a1 / b;
+  q2 = a2 / b;
+}
+#if defined(__HIPCC__)
+/* doubles on the device: one reciprocal for both quotients where lm_div_nr is exact (the same values) */
+LMD void lmp_div2(double a1, double a2, double b, double &q1, double &q2) {
+  if (lm_div_range(a1) & lm_div_range(a2) & lm_div_range(b)) {
+    const double y = lm_rcp_nr(b);
+    q1 = lm_div_nr(a1, b, y);
+    q2 = lm_div_nr(a2, b, y);
+  } else {
+    q1 = a1 / b;
+    q2 = a2 / b;
+  }
+}
+#endif
+
 /* microphysics.py:83-100 (the exponent is one_third = RN(1/3), lm_pow_third's) */
 template <class M, class RH>
 LMD LmCommonT<M, RH> lmp_radius(const LmPhys &P, M m, RH rho) {
@@ -128,20 +148,21 @@ LMD LmCommonT<W, SW> lmp_sigma_solution(const LmPhys &P, int solute, W w, SW sig
 
 /*
  * Condensation mass rate gamma and d gamma / d m_w (Fukuta-Walter):
- * microphysics.py:1359-1469 (AS), 1237-1353 (NaCl).
+ * microphysics.py:1359-1469 (AS), 1237-1353 (NaCl), given m_p_inv_SI = 1e18 / (m_w + m_s) and the first
+ * quotient of drho_dm_over_rho, q_rho = (-rho_w(T) m_p_inv_SI) / rho_p (AS) or (-w_s m_p_inv_SI) / rho_p (NaCl).
  */
-template <class MW, class MS, class WS, class RP, class TP, class RH, class SA, class CT, class SG, class G>
-LMD void lmp_mass_rate(const LmPhys &P, int solute, MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p, SA S_amb,
-                       const CT *cc, SG sigma_p, G &gamma, G &dgamma) {
+template <class MW, class MS, class WS, class RP, class TP, class RH, class SA, class CT, class SG, class MI, class QR,
+          class G>
+LMD void lmp_mass_rate_q(const LmPhys &P, int solute, MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p, SA S_amb,
+                         const CT *cc, SG sigma_p, MI m_p_inv_SI, QR q_rho, G &gamma, G &dgamma) {
   const LmCommonT<RP, CT> l_alpha_plus_R_p = 1.0E-6 * (R_p + cc[CC_LALPHA]);
   const LmCommonT<RP, CT> l_beta_plus_R_p = 1.0E-6 * (R_p + cc[CC_LBETA]);
-  const LmCommonT<MW, MS> m_p_inv_SI = 1.0E18 / (m_w + m_s);
   LmCommonT<CT, MW, MS, RH, WS, TP> drho_dm_over_rho;
   if (solute == 0) {
-    drho_dm_over_rho = -cc[CC_RHOW] * m_p_inv_SI / rho_p * w_s * lmp_poly(P.par_rho_deriv_AS, 3, w_s);
+    drho_dm_over_rho = q_rho * w_s * lmp_poly(P.par_rho_deriv_AS, 3, w_s);
   } else {
     const double *p = P.par_sol_dens_NaCl;
-    drho_dm_over_rho = -w_s * m_p_inv_SI / rho_p * (p[1] + 2.0 * p[3] * w_s + p[4] * T_p);
+    drho_dm_over_rho = q_rho * (p[1] + 2.0 * p[3] * w_s + p[4] * T_p);
   }
   const auto dR_p_dm_over_R_p = P.one_third * (m_p_inv_SI - drho_dm_over_rho);
   const auto dR_p_dm = 1.0E-6 * dR_p_dm_over_R_p * R_p;
@@ -165,37 +186,78 @@ LMD void lmp_mass_rate(const LmPhys &P, int solute, MW m_w, MS m_s, WS w_s, RP R
   dgamma = 1.0E-12 * f1f3 * (S_sup * (2.0 * dR_p_dm_over_R_p - f3 * dg1_dm) - dSeq_dm);
 }
 
+template <class MW, class MS, class WS, class RP, class TP, class RH, class SA, class CT, class SG, class G>
+LMD void lmp_mass_rate(const LmPhys &P, int solute, MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p, SA S_amb,
+                       const CT *cc, SG sigma_p, G &gamma, G &dgamma) {
+  const LmCommonT<MW, MS> m_p_inv_SI = 1.0E18 / (m_w + m_s);
+  using QR = LmCommonT<CT, MW, MS, RH, WS>;
+  const QR q_rho = solute == 0 ? QR(-cc[CC_RHOW] * m_p_inv_SI / rho_p) : QR(-w_s * m_p_inv_SI / rho_p);
+  lmp_mass_rate_q(P, solute, m_w, m_s, w_s, R_p, T_p, rho_p, S_amb, cc, sigma_p, m_p_inv_SI, q_rho, gamma, dgamma);
+}
+
 /*
  * Implicit Newton mass update, fixed no_iter iterations with the clamps of
  * integration.py:728-836 (the inner iterations use the AS density, radius
  * and surface tension for both solutes, as the reference does).
  * Returns dm = mass_new - m_w.
  */
-template <class MW, class MS, class WS, class RP, class TP, class RH, class CT, class SG>
-LMD LmCommonT<MW, MS, WS, RP, TP, RH, CT, SG> lmp_newton_dm(const LmPhys &P, int solute, double dt_sub, int no_iter,
-                                                            MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p,
-                                                            const CT *cc, SG sigma_p) {
-  using R = LmCommonT<MW, MS, WS, RP, TP, RH, CT, SG>;
+/* the update after the first rate evaluation (gamma, dgamma) and the later iterations; returns dm */
+template <class R, class MW, class MS, class TP, class CT>
+LMD R lmp_newton_iterate(const LmPhys &P, int solute, double dt_sub, int no_iter, MW m_w, MS m_s, TP T_p,
+                         const CT *cc, R gamma, R dgamma) {
   const R m_w_effl = m_s * (P.w_s_max_AS_inv - 1.0);
   const CT S_amb = cc[CC_S];
-  R gamma, dgamma;
-  lmp_mass_rate(P, solute, m_w, m_s, w_s, R_p, T_p, rho_p, S_amb, cc, sigma_p, gamma, dgamma);
   R d = dt_sub * dgamma;
   R denom_inv = d < 0.9 ? R(1.0 / (1.0 - d)) : R(10.0);
   R mass_new = lm_np_max(m_w_effl, R(m_w + dt_sub * gamma * denom_inv));
   for (int it = 0; it < no_iter - 1; ++it) {
     const R m_p = mass_new + m_s;
-    const R ws = m_s / m_p;
+    R ws, m_p_inv_SI; /* m_s / m_p and the rate's 1e18 / m_p */
+    lmp_div2(m_s, 1.0E18, m_p, ws, m_p_inv_SI);
     const R rho = cc[CC_RHOW] * lmp_poly(P.par_rho_AS, 4, ws);
-    const R Rr = lmp_radius(P, m_p, rho);
+    R r_arg, q_rho; /* the radius's (m_p / (4 pi / 3)) / rho and the rate's drho quotient */
+    lmp_div2(P.four_pi_over_three_inv * m_p, solute == 0 ? R(-cc[CC_RHOW] * m_p_inv_SI) : R(-ws * m_p_inv_SI), rho,
+             r_arg, q_rho);
+    const R Rr = lm_pow_third(r_arg);
     const R sigma = cc[CC_SIGW] * (1.0 + P.par_sigma_AS * ws / (1. - ws));
-    lmp_mass_rate(P, solute, mass_new, m_s, ws, Rr, T_p, rho, S_amb, cc, sigma, gamma, dgamma);
+    lmp_mass_rate_q(P, solute, mass_new, m_s, ws, Rr, T_p, rho, S_amb, cc, sigma, m_p_inv_SI, q_rho, gamma, dgamma);
     d = dt_sub * dgamma;
     denom_inv = d < 0.9 ? R(1.0 / (1.0 - d)) : R(10.0);
     mass_new += (dt_sub * gamma + m_w - mass_new) * denom_inv;
     mass_new = lm_np_max(m_w_effl, mass_new);
   }
   return mass_new - m_w;
+}
+
+template <class MW, class MS, class WS, class RP, class TP, class RH, class CT, class SG>
+LMD LmCommonT<MW, MS, WS, RP, TP, RH, CT, SG> lmp_newton_dm(const LmPhys &P, int solute, double dt_sub, int no_iter,
+                                                            MW m_w, MS m_s, WS w_s, RP R_p, TP T_p, RH rho_p,
+                                                            const CT *cc, SG sigma_p) {
+  using R = LmCommonT<MW, MS, WS, RP, TP, RH, CT, SG>;
+  R gamma, dgamma;
+  lmp_mass_rate(P, solute, m_w, m_s, w_s, R_p, T_p, rho_p, cc[CC_S], cc, sigma_p, gamma, dgamma);
+  return lmp_newton_iterate<R>(P, solute, dt_sub, no_iter, m_w, m_s, T_p, cc, gamma, dgamma);
+}
+
+/*
+ * dm of the condensation substep of one SIP from its masses at the cell's temperature T_p: lmp_R_ws_rho,
+ * lmp_sigma_solution and lmp_newton_dm in one, the quotients of one denominator taken together (lmp_div2).
+ * The same values.
+ */
+LMDT R lmp_condensation_dm(const LmPhys &P, int solute, double dt_sub, int no_iter, R m_w, R m_s, R T_p,
+                           const R *cc) {
+  const R m_p = m_w + m_s;
+  R w_s, m_p_inv_SI;
+  lmp_div2(m_s, 1.0E18, m_p, w_s, m_p_inv_SI);
+  const R rho = lmp_density_solution(P, solute, w_s, T_p, cc[CC_RHOW]);
+  R r_arg, q_rho;
+  lmp_div2(P.four_pi_over_three_inv * m_p, solute == 0 ? R(-cc[CC_RHOW] * m_p_inv_SI) : R(-w_s * m_p_inv_SI), rho,
+           r_arg, q_rho);
+  const R R_p = lm_pow_third(r_arg);
+  const R sigma_p = lmp_sigma_solution(P, solute, w_s, cc[CC_SIGW]);
+  R gamma, dgamma;
+  lmp_mass_rate_q(P, solute, m_w, m_s, w_s, R_p, T_p, rho, cc[CC_S], cc, sigma_p, m_p_inv_SI, q_rho, gamma, dgamma);
+  return lmp_newton_iterate<R>(P, solute, dt_sub, no_iter, m_w, m_s, T_p, cc, gamma, dgamma);
 }
 
 /* grid.py:225-254 with Python negative indexing of the first axis */
