@@ -2001,8 +2001,12 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
 
 /* main pass: one block per cell or persistent blocks over the work list; overflow passes: the queued cells,
    grid-strided */
+/* waves per SIMD the collision passes are compiled for: the 256-member pass's LDS holds 10 workgroups per compute
+   unit, so 3 waves per SIMD (168 VGPRs) use it where 2 (its register count unbounded) would not */
+__host__ __device__ constexpr int col_wpe(int cap) { return cap == 256 ? 3 : 4; }
+
 template <int CAP, bool TAB, bool INJ>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_collision(KView v, int mode, int chunk,
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(col_wpe(CAP)))) void k_collision(KView v, int mode, int chunk,
                                                                                            ColLists io) {
   CellSource src = cell_source(mode, chunk, 0);
   for (;;) {
