@@ -20,6 +20,8 @@ torch.cuda.synchronize().  Legs (--legs, default all):
               reference's SinSIP initial state computed on the GPU, mode-parameter gradient through its map)
   * box:      the Golovin box model, 50 seeds x 3600 s (the oracle's time for one box beside it)
   * stress:   the 150x150 Poisson(256) 16-seed batch
+  * stress_adjoint: the adjoint cost of one 150x150 run: 600-step window (the last 300 s of 1800 s) after a
+              3000-step spin-up with collisions, checkpoints every 10 steps (no finite-difference check)
   * init:     GPU initial states of 64 seeds
   * cpu:      the oracle (C restatement) on the host cores: 5 warm-up, 3 x 60 timed advection steps
               (median of 3) per process, one process per core of the box's share and one alone
@@ -42,7 +44,7 @@ B_SUBSTEP = 104.0    # algorithmic bytes per SIP per particle substep (60 read +
 B_COLLISION = 64.0   # per SIP per collision step (40 read + 24 write)
 B_REINDEX = 140.0    # per SIP per reindex
 HOST_THREADS = 16    # the box's CPU share for one GPU
-LEGS = ("early", "plain", "drizzle", "adjoint", "box", "stress", "init", "cpu")
+LEGS = ("early", "plain", "drizzle", "adjoint", "box", "stress", "stress_adjoint", "init", "cpu")
 
 from lacmopy_amd.setup_case import WORKLOADS as CASES  # noqa: E402
 
@@ -819,6 +821,17 @@ def main():
         out["stress"] = st
         log(f"stress: {st}")
         del eng
+        torch.cuda.empty_cache()
+
+    if "stress_adjoint" in legs and icmw:
+        win, every, spin_steps, spin_col, _ = ADJ_DEFAULTS["stress150"]
+        seng, _, _ = build_engine(1, None, args.seed, case="stress150")
+        spin_up(seng, spin_steps, spin_col)
+        out["stress_adjoint"] = adjoint_cost(
+            seng, seng.snapshot(), win, every, mode_map=None, check=0, case="stress150",
+            state_desc=f"the synthetic start after a {spin_steps}-step spin-up {'with' if spin_col else 'without'} "
+                       f"collisions")
+        del seng
         torch.cuda.empty_cache()
 
     if "init" in legs and icmw:

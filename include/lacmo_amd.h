@@ -436,7 +436,8 @@ int lm_engine_check(LmEngine *e, void *stream);
 /*
  * Test hook (no reference counterpart): elementwise device math of the
  * kernels -- which = 0 exp, 1 log, 2 pow(x,y), 3 x % y (Python float modulo),
- * 4 sqrt, 5 x / y, 6 floor(x / y), 7 x^(1/3 rounded) -- for the host/device bit-parity test.
+ * 4 sqrt, 5 x / y, 6 floor(x / y), 7 x^(1/3 rounded), 8 and 9 x / y through the shared-reciprocal
+ * quotients of the particle kernel (lmp_div2, lm_div_y) -- for the host/device bit-parity test.
  */
 int lm_math_probe(int32_t which, int64_t n, const double *x, const double *y, double *out, void *stream);
 

@@ -91,3 +91,34 @@ def test_fast_kernel_index_equals_exact(gpu):
         assert 0.0 < max_err <= bound, (max_err, bound)
         assert 0.0 < margin < 1e-3, margin
         print(f"{name}: hardware log2 error {max_err:.3g} (bound {bound:.3g}), margin in t {margin:.3g}")
+
+
+def test_shared_reciprocal_quotients_equal_ieee_division(gpu):
+    """The particle kernel's quotients through one refined reciprocal per denominator (the division sequence
+    without its scale and fix-up steps, taken for operands of exponent within +-300, else the full division)
+    equal IEEE a / b bit for bit: dense in the shortcut's range, across the whole exponent range (both paths),
+    and at zeros of either sign, subnormals, infinities and NaNs."""
+    rng = np.random.default_rng(23)
+    n = 2_000_000
+    sign = rng.choice([-1.0, 1.0], (2, n))
+    inside = sign * np.ldexp(rng.uniform(1.0, 2.0, (2, n)), rng.integers(-300, 300, (2, n)))
+    anywhere = sign * np.ldexp(rng.uniform(1.0, 2.0, (2, n)), rng.integers(-1074, 1024, (2, n)))
+    mantissa_edges = sign[:, :8] * np.array([1.0, np.nextafter(2.0, 0), 1.5, np.nextafter(1.0, 2), 1.25, 1.75,
+                                            np.nextafter(1.5, 0), np.nextafter(1.5, 2)])
+    special = np.array([0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308, 1.7976931348623157e308, np.inf,
+                        -np.inf, np.nan, 1.0, -1.0, 2.0 ** -300, 2.0 ** 300, np.nextafter(2.0 ** -300, 0), 3.0])
+    sa, sb = np.meshgrid(special, special)
+    a = np.concatenate([inside[0], anywhere[0], mantissa_edges[0], sa.ravel()])
+    b = np.concatenate([inside[1], anywhere[1], mantissa_edges[1], sb.ravel()])
+    with np.errstate(all="ignore"):
+        want = a / b
+    plain = _dev(5, a, b)  # the device's own division
+
+    def check(got, ref, what):
+        same = (got.view(np.int64) == ref.view(np.int64)) | (np.isnan(got) & np.isnan(ref))
+        bad = np.flatnonzero(~same)
+        assert bad.size == 0, (what, a[bad[:4]], b[bad[:4]], got[bad[:4]], ref[bad[:4]])
+
+    check(plain, want, "device division against IEEE")
+    for which in (8, 9):
+        check(_dev(which, a, b), plain, f"probe {which} against the device division")
