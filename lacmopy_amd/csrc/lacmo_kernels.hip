@@ -1387,7 +1387,7 @@ __device__ static int fast_stage(const LmPhys &P, const KView &v, int gc, double
     if (t < T0) {
       const int k = s0 + t;
       hcell[q] = v.cell[k];
-      hid[q] = v.id[k];
+      if (mc > 0) hid[q] = v.id[k]; /* the ids only place the list's members */
       hx[q] = v.xi[k];
       hu[q] = v.u[k];
       hw[q] = v.w[k];
@@ -2569,7 +2569,7 @@ __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(8))) void 
       const int k = s0 + t;
       const double d = v.dmxi[k];
       ov[q] = v.cell[k] == gc ? d : 0.0;
-      oid[q] = v.id[k];
+      if (mc > 0) oid[q] = v.id[k]; /* the ids only place the list's members */
     }
   }
   int mk = -1, mid_ = 0;
@@ -2973,13 +2973,32 @@ __global__ __launch_bounds__(WAVE) void k_ri_count(KView v) {
   }
   const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
   const int32_t *sr = v.seg + (size_t)r * (nc + 1);
-  __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
-  const int n = count_members(gc, v.cell, &tb);
-  int dead = 0;
-  for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
-    const int k = base + lane;
-    dead += __popcll(__ballot(k < sr[c + 1] && v.cell[k] < 0));
+  int n = 0, dead = 0;
+  const int mc = v.mig_cnt ? v.mig_cnt[gc] : 0;
+  if (v.mig_cnt && mc <= v.mig_cap) {
+    /* a complete migrant list: the home segment's members and dead SIPs in one pass, then the list's
+       members, each slot once (a SIP that entered the cell twice is listed twice) */
+    for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
+      const int k = base + lane;
+      const int cv = k < sr[c + 1] ? v.cell[k] : gc + 1;
+      n += __popcll(__ballot(cv == gc));
+      dead += __popcll(__ballot(cv < 0));
+    }
+    const int k = lane < mc ? v.mig_slot[(size_t)gc * v.mig_cap + lane] : -1;
+    bool mem = k >= 0 && v.cell[k] == gc;
+    for (int e = 0; e < mc; ++e) {
+      const int ke = __shfl(k, e);
+      if (e < lane && ke == k) mem = false;
+    }
+    n += __popcll(__ballot(mem));
+  } else {
+    __shared__ NbrTab tb;
+    nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
+    n = count_members(gc, v.cell, &tb);
+    for (int base = sr[c]; base < sr[c + 1]; base += WAVE) {
+      const int k = base + lane;
+      dead += __popcll(__ballot(k < sr[c + 1] && v.cell[k] < 0));
+    }
   }
   if (lane == 0) {
     v.scan_buf[(size_t)r * L + c] = n;
@@ -3200,24 +3219,31 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
   const int32_t *sc = v.scan_buf + (size_t)r * L;
   __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
   __shared__ unsigned char sSub[DML_CAP];
+  __shared__ int sMig[3 * LM_MIG_CAP_MAX];
   __shared__ NbrTab tb;
-  nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
-  const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
   const int dst0 = sc[c];
-  if (n > DML_CAP) {
-    /* over the LDS capacity: one lane merges the members in id order, parking each source slot in the cell
-       array of its destination (this cell's range of the new buffer), then the wave copies */
-    if (lane == 0) {
-      int idx = 0;
-      merge_members(gc, v.cell, v.id, &tb, [&](int k) { v.cell2[dst0 + idx++] = k; });
-    }
-    __syncthreads();
-    for (int m = lane; m < n; m += WAVE) copy_sip(v, v.cell2[dst0 + m], dst0 + m, gc);
+  /* a complete migrant list: the members in id order from the home segment and the list (no 3x3 gather) */
+  const int nf = v.mig_cnt ? fast_members(v, gc, sRank, DML_CAP, sSlot, sId, sMig) : -1;
+  if (nf >= 0 && nf <= DML_CAP) {
+    for (int m = lane; m < nf; m += WAVE) copy_sip(v, sRank[m], dst0 + m, gc);
   } else {
-    __syncthreads();
-    rank_members(n, sId, sSub, &tb, sRank);
-    __syncthreads();
-    for (int m = lane; m < n; m += WAVE) copy_sip(v, sSlot[m], dst0 + sRank[m], gc);
+    nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
+    const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
+    if (n > DML_CAP) {
+      /* over the LDS capacity: one lane merges the members in id order, parking each source slot in the cell
+         array of its destination (this cell's range of the new buffer), then the wave copies */
+      if (lane == 0) {
+        int idx = 0;
+        merge_members(gc, v.cell, v.id, &tb, [&](int k) { v.cell2[dst0 + idx++] = k; });
+      }
+      __syncthreads();
+      for (int m = lane; m < n; m += WAVE) copy_sip(v, v.cell2[dst0 + m], dst0 + m, gc);
+    } else {
+      __syncthreads();
+      rank_members(n, sId, sSub, &tb, sRank);
+      __syncthreads();
+      for (int m = lane; m < n; m += WAVE) copy_sip(v, sSlot[m], dst0 + sRank[m], gc);
+    }
   }
   /* SIPs that died in this home segment since the last reindex, in slot order */
   int dead = 0;
@@ -3931,6 +3957,13 @@ __global__ void k_math_probe(int which, long long n, const double *x, const doub
     case 4: r = sqrt(a); break;
     case 5: r = a / b; break;
     case 7: r = lm_pow_third(a); break;
+    case 8: {
+      double q1, q2;
+      lmp_div2(a, 1.0, b, q1, q2); /* the shared-reciprocal quotient a / b */
+      r = q1;
+      break;
+    }
+    case 9: r = lm_div_range(b) ? lm_div_y(a, b, lm_rcp_nr(b)) : a / b; break;
     default: r = floor(a / b); break;
   }
   out[k] = r;
