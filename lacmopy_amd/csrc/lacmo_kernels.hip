@@ -1730,49 +1730,91 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
         if (lane == 0) sQn = 0;
         __syncthreads();
         const int nbig = sNbig;
-        /* rows i0 .. n-2 in pairs (i0 + p, n - 2 - p): every lane gets about the same number of quads */
+        /* rows i0 .. n-2 in pairs (i0 + p, n - 2 - p): every lane gets about the same number of quads.  The
+           first and last quad of each row (those with pairs outside [jmin, n-1]) are tested on their own; the
+           interior quads of both rows form one run of quads per lane, so that the wave walks max(interior
+           quads) and not max(first row) + max(second row) */
         const int n_rows = n - 1 - i0;
-        for (int pr = lane; 2 * pr < n_rows; pr += WAVE) {
-#pragma unroll 1
+        for (int pr0 = 0; 2 * pr0 < n_rows; pr0 += WAVE) {
+          const int pr = pr0 + lane;
+          int ir[2], b0r[2], b1r[2], jr[2];
+          uint32_t str[2];
+#pragma unroll
           for (int side = 0; side < 2; ++side) {
             const int i = side == 0 ? i0 + pr : n - 2 - pr;
-            if (side == 1 && i <= i0 + pr) break;
+            ir[side] = i;
+            b0r[side] = 0;
+            b1r[side] = -1; /* no quads */
+            jr[side] = n;
+            str[side] = 0u;
+            if (2 * pr >= n_rows || (side == 1 && i <= i0 + pr)) continue;
             if (sIsBig[i]) continue; /* the per-pair pass takes the rows of the list */
             const int base = i * n - i * (i + 1) / 2;
             const int jmin = (k_next - base + i + 1 > i + 1) ? k_next - base + i + 1 : i + 1;
             if (jmin >= n) continue;
-            const uint32_t st = sTG[i]; /* keep a quad when a valid pair's byte is <= st */
-            const int b0 = (i + jmin) >> 2, b1 = (i + n - 1) >> 2;
-            /* four quads per iteration: their hash chains are independent and overlap in the pipeline */
-            for (int bb = b0; bb <= b1; bb += 4) {
-              uint32_t gq[4];
+            jr[side] = jmin;
+            str[side] = sTG[i]; /* keep a quad when a valid pair's byte is <= st */
+            b0r[side] = (i + jmin) >> 2;
+            b1r[side] = (i + n - 1) >> 2;
+          }
+          /* the first and last quad of each row: bytes of pairs outside [jmin, n-1] set to 0xFF, which no bound
+             below 255 keeps (each such quad holds a valid pair: j = jmin in the first, j = n-1 in the last) */
 #pragma unroll
-              for (int u = 0; u < 4; ++u) gq[u] = lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * (bb + u)));
+          for (int e = 0; e < 4; ++e) {
+            const int side = e >> 1, i = ir[side];
+            const int b = (e & 1) ? b1r[side] : b0r[side];
+            if (b1r[side] < b0r[side] || ((e & 1) && b1r[side] == b0r[side])) continue;
+            uint32_t g = lm_guard_word(gkey, (uint32_t)i, (uint32_t)(4 * b));
 #pragma unroll
-              for (int u = 0; u < 4; ++u) {
-                const int b = bb + u;
-                if (b > b1) break;
-                uint32_t g = gq[u];
-                if ((b == b0) | (b == b1)) {
-                  /* bytes of pairs outside [jmin, n-1] set to 0xFF, which no bound below 255 keeps; the quad
-                     holds at least one valid pair (j = jmin in the first, j = n-1 in the last) */
+            for (int t = 0; t < 4; ++t) {
+              const int j = 4 * b + t - i;
+              if ((j < jr[side]) | (j >= n)) g |= 0xFFu << (8 * t);
+            }
+            const uint32_t st = str[side];
+            /* a byte below st + 1 (exact word test, valid for bounds up to 128); st >= 127 rejects (almost)
+               nothing: every quad holds a valid pair */
+            const bool keep = (st >= 127u) | (((g - 0x01010101u * (st + 1u)) & ~g & 0x80808080u) != 0u);
+            if (keep) {
+              const int pos = atomicAdd(&sQn, 1);
+              if (pos < QC) {
+                sQueue[pos] = (i << 16) | b;
+                sQkey[pos] = KEY_STALE;
+              }
+            }
+          }
+          /* interior quads (every pair valid): b0 + 1 .. b1 - 1 of the first row, then of the second; the guard
+             word's argument key + ((i << 16) | b) * 0x9E3779B9 is linear in the run index t */
+          const int na = b1r[0] - b0r[0] - 1 > 0 ? b1r[0] - b0r[0] - 1 : 0;
+          const int nb2 = b1r[1] - b0r[1] - 1 > 0 ? b1r[1] - b0r[1] - 1 : 0;
+          const int ntot = na + nb2;
+          const uint32_t GR = 0x9E3779B9u;
+          const uint32_t hA = gkey + (((uint32_t)ir[0] << 16) + (uint32_t)(b0r[0] + 1)) * GR;
+          const uint32_t hB = gkey + (((uint32_t)ir[1] << 16) + (uint32_t)(b0r[1] + 1)) * GR - (uint32_t)na * GR;
+          int tmax = ntot;
+          for (int off = WAVE / 2; off > 0; off >>= 1) {
+            const int o = __shfl_xor(tmax, off);
+            tmax = o > tmax ? o : tmax;
+          }
+          /* four quads per iteration: their hash chains are independent and overlap in the pipeline */
+          for (int t0 = 0; t0 < tmax; t0 += 4) {
+            uint32_t gq[4];
 #pragma unroll
-                  for (int t = 0; t < 4; ++t) {
-                    const int j = 4 * b + t - i;
-                    if ((j < jmin) | (j >= n)) g |= 0xFFu << (8 * t);
-                  }
-                }
-                bool keep;
-                if (st < 127u) /* a byte below st + 1 (exact word test, valid for bounds up to 128) */
-                  keep = ((g - 0x01010101u * (st + 1u)) & ~g & 0x80808080u) != 0u;
-                else
-                  keep = true; /* the bound rejects (almost) nothing: every quad holds a valid pair */
-                if (keep) {
-                  const int pos = atomicAdd(&sQn, 1);
-                  if (pos < QC) {
-                    sQueue[pos] = (i << 16) | b;
-                    sQkey[pos] = KEY_STALE;
-                  }
+            for (int u = 0; u < 4; ++u) {
+              const int t = t0 + u;
+              gq[u] = lm_fmix32((t < na ? hA : hB) + (uint32_t)t * GR);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int t = t0 + u;
+              const bool in_a = t < na;
+              const uint32_t st = in_a ? str[0] : str[1], g = gq[u];
+              const bool keep =
+                  (t < ntot) & ((st >= 127u) | (((g - 0x01010101u * (st + 1u)) & ~g & 0x80808080u) != 0u));
+              if (keep) {
+                const int pos = atomicAdd(&sQn, 1);
+                if (pos < QC) {
+                  sQueue[pos] = in_a ? ((ir[0] << 16) | (b0r[0] + 1 + t)) : ((ir[1] << 16) | (b0r[1] + 1 + t - na));
+                  sQkey[pos] = KEY_STALE;
                 }
               }
             }
