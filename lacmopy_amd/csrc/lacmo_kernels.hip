@@ -1093,6 +1093,19 @@ __device__ static inline int row_quads(int p, int n) {
   return p < n - 1 ? ((p + n - 1) >> 2) - ((2 * p + 1) >> 2) + 1 : 0;
 }
 
+/* the guard byte bound of row_bound without its float rounding and exact search: only for estimating how many
+   quads a choice of the exclusion list keeps (the bounds the passes use come from row_bound) */
+__device__ static inline int row_bound_est(const LmPhys &P, double dtdV, double Eb, const CellMax &M, double R,
+                                           double u, double w) {
+  const double du = fmax(u - M.ulo, M.uhi - u), dw = fmax(w - M.wlo, M.whi - w);
+  const double sr = fabs(R) + M.Rc;
+  const double Pb = (((M.X * dtdV) * ((P.pi * sr) * sr)) * Eb) * 1.0E-12;
+  const double B = (Pb * Pb) * (du * du + dw * dw);
+  if (!(B < 1.0)) return 255;
+  const int tg = (int)(256.0 * sqrt(B)) + 1;
+  return (tg < 256 ? tg : 256) - 1;
+}
+
 /* expected quads of row p that the guard pass keeps with bound st (uniform bytes, four per quad) */
 __device__ static inline double row_kept(int p, int n, int st) {
   const double q = 1.0 - (st + 1) * (1.0 / 256.0);
@@ -1130,41 +1143,83 @@ __device__ static void row_bounds(const LmPhys &P, double dtdV, double Eb, int n
   const double floor_kept = 0.0155 * nq_all;
   int best_m = 0;
   if (kept0 > floor_kept + 24.0 && n >= 3 * BIG_MAX) {
-    /* the BIG_MAX largest radii, in decreasing order (ties by position): sRank[p] = 1 + place */
-    for (int k = 0; k < BIG_MAX; ++k) {
-      double br = -1.0;
-      int bp = n;
-      for (int p = lane; p < n; p += WAVE) {
-        const double r = fabs(sR[p]);
-        if (sRank[p] == 0 && (r > br || (r == br && p < bp))) {
-          br = r;
-          bp = p;
-        }
+    /* the BIG_MAX largest radii, in decreasing order (ties by position): sRank[p] = 1 + place.  Which members
+       the list holds only matters for speed (any list is exact); a radius that is NaN is never listed, so the
+       list can come out shorter (nsel) */
+    int nsel = 0;
+    constexpr int RPL = (CAP > 0 && CAP <= 512) ? (CAP + WAVE - 1) / WAVE : 0;
+    if constexpr (RPL > 0) {
+      /* the lane's radii in registers: a pass is a local scan and a wave reduction, no LDS reads or barrier */
+      double rr[RPL];
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        const int p = lane + q * WAVE;
+        rr[q] = p < n ? fabs(sR[p]) : -1.0;
       }
-      for (int off = WAVE / 2; off > 0; off >>= 1) {
-        const double orr = __shfl_xor(br, off);
-        const int op = __shfl_xor(bp, off);
-        if (orr > br || (orr == br && op < bp)) {
-          br = orr;
-          bp = op;
+      for (int k = 0; k < BIG_MAX; ++k) {
+        double br = -1.0;
+        int bp = n;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q)
+          if (rr[q] > br) { /* positions grow with q: a tie keeps the smaller */
+            br = rr[q];
+            bp = lane + q * WAVE;
+          }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+          const double orr = __shfl_xor(br, off);
+          const int op = __shfl_xor(bp, off);
+          if (orr > br || (orr == br && op < bp)) {
+            br = orr;
+            bp = op;
+          }
         }
-      }
-      if (lane == 0) {
-        sRank[bp] = (unsigned char)(k + 1);
-        sBig[k] = bp;
+        if (bp >= n) break;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q)
+          if (lane + q * WAVE == bp) rr[q] = -1.0;
+        if (lane == 0) {
+          sRank[bp] = (unsigned char)(k + 1);
+          sBig[k] = bp;
+        }
+        ++nsel;
       }
       __syncthreads();
+    } else {
+      for (int k = 0; k < BIG_MAX; ++k) {
+        double br = -1.0;
+        int bp = n;
+        for (int p = lane; p < n; p += WAVE) {
+          const double r = fabs(sR[p]);
+          if (sRank[p] == 0 && (r > br || (r == br && p < bp))) {
+            br = r;
+            bp = p;
+          }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+          const double orr = __shfl_xor(br, off);
+          const int op = __shfl_xor(bp, off);
+          if (orr > br || (orr == br && op < bp)) {
+            br = orr;
+            bp = op;
+          }
+        }
+        if (bp >= n) break;
+        if (lane == 0) {
+          sRank[bp] = (unsigned char)(k + 1);
+          sBig[k] = bp;
+        }
+        ++nsel;
+        __syncthreads();
+      }
     }
     double best = kept0 + (kept0 > q_full ? over : 0.0);
-    for (int m = 2; m <= BIG_MAX; m *= 2) {
+    for (int m = 2; m <= BIG_MAX && m <= nsel; m *= 2) {
       const CellMax Mm = cell_max(n, sX, sR, sU, sW, sRank, m);
       double kept = 0.0;
       for (int p = lane; p < n; p += WAVE) {
         const int rk = sRank[p];
         if (rk != 0 && rk <= m) continue;
-        int st;
-        (void)row_bound(P, dtdV, Eb, Mm, sR[p], sU[p], sW[p], st);
-        kept += row_kept(p, n, st);
+        kept += row_kept(p, n, row_bound_est(P, dtdV, Eb, Mm, sR[p], sU[p], sW[p]));
       }
       for (int off = WAVE / 2; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
       const double cost = kept + 0.18 * n * m + (kept > q_full ? over : 0.0);
