@@ -2103,8 +2103,8 @@ __device__ static void collide_cell(const KView &v, int gc, const ColLists &io) 
 __host__ __device__ constexpr int col_wpe(int cap) { return cap == 256 ? 3 : 4; }
 
 template <int CAP, bool TAB, bool INJ>
-__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(col_wpe(CAP)))) void k_collision(KView v, int mode, int chunk,
-                                                                                           ColLists io) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(col_wpe(CAP)))) void k_collision(
+    KView v, int mode, int chunk, ColLists io) {
   CellSource src = cell_source(mode, chunk, 0);
   for (;;) {
     const int gc = next_cell(v, src, io.in_list, io.in_cnt, io.in_grab);
