@@ -11,7 +11,8 @@
     pdf and of m times it over the sampled mass range without bias;
   * the whole initial state of a full-height column: pressure chaining between levels, water and liquid
     budgets against the particles, convergence of the adjustment, S close to 1 in cloud, latent heating
-    against the condensed water, multiplicities against the real particle counts, positions in cells.
+    against the condensed water, multiplicities against the real particle counts, positions in cells;
+  * the adjustment's saturation cap S_init_max: one iteration with and without it.
 """
 import math
 
@@ -202,3 +203,29 @@ def test_initial_column_budgets_and_adjustment():
     assert np.all((o["rel"] >= 0) & (o["rel"] < 1))
     assert np.allclose(o["x"], o["cell_x"] * dx + dx * o["rel"][0], rtol=0, atol=1e-12)
     assert np.allclose(o["z"], o["cell_z"] * dz + dz * o["rel"][1], rtol=0, atol=1e-12)
+
+
+def test_adjustment_caps_the_saturation_of_its_condensation_steps():
+    """init.py:903-905: the saturation adjustment condenses at min(S, S_init_max).  One adjustment iteration of a
+    full-height column with the cap out of reach (S_init_max = 100) and with it at 1.0: levels whose saturation
+    stays below the cap condense exactly alike, and every level where the cap acts condenses less (a smaller
+    ambient saturation in the implicit mass update), in the supersaturated upper column.  A dropped cap makes the
+    two runs equal everywhere; the cap taken as a lower bound (max) makes the capped run condense more."""
+    runs = {}
+    for cap in (100.0, 1.0):
+        cfg = init_config(nx=4, nz=75, S_init_max=cap, iter_cnt_limit=1)
+        g, q = _par(cfg)
+        o = orc.init_SinSIP(oracle_par(q), 3711, g.corners[1][0], cfg["p_0"], g.corners[0][:, 0])
+        runs[cap] = o["lv"]
+    free, capped = runs[100.0], runs[1.0]
+    assert np.all(free[:, 15] == 1) and np.all(capped[:, 15] == 1)
+    m_free, m_cap = free[:, 12], capped[:, 12]
+    same = m_free == m_cap
+    # levels below the cap: bit-identical levels up to the first level the cap changes (the levels above start
+    # from the changed state of the levels below)
+    first = int(np.argmin(same)) if not same.all() else len(same)
+    assert first > 0 and same[:first].all()
+    assert not same.all(), "the cap never acted"
+    # the cap acts where the column is supersaturated: there every level condenses less under it
+    assert free[first, 7] > 1.0 and np.all(free[~same, 7] > 1.0)
+    assert np.all(m_cap[~same] < m_free[~same]), np.flatnonzero(m_cap[~same] >= m_free[~same])
