@@ -111,6 +111,34 @@ def test_adv_steps_bit_exact(gpu, n_steps):
     assert no_cols.sum() > 0, "case should produce collisions"
 
 
+@pytest.mark.parametrize("lists", ["capacity_1", "off"])
+def test_migrant_list_fallbacks_match_oracle(gpu, lists):
+    """The members of a cell come from its home segment and migrant list while the list is complete, else from
+    the 3x3 neighbourhood gather (collision passes, Delta m_l, the reindex's count and scatter).  With lists of
+    one entry most cells that received a SIP overflow theirs and mix both paths; with the lists switched off
+    every kernel gathers.  Three advection steps with collisions stay bit-identical to the oracle."""
+    E, lnR0, lnf, nb = load_kernel_data_Ecol("Long_Bott")
+    cfg = _config(E, lnR0, lnf, nb)
+    grid, *parts = _case(seed=41)
+    parts = _prep(grid, parts)
+    ref_parts, scal, mat, wr, no_cols = _oracle_run(grid, parts, cfg, E, 19, 3)
+    eng = _engine(grid, parts, cfg, E, lnR0, lnf, 19)
+    if lists == "capacity_1":
+        eng.e.mig_cap = 1
+    else:
+        eng.e.mig_cnt = None
+    eng.adv_steps(3)
+    for nm, a, b in zip(["pos", "vel", "cells", "m_w", "m_s", "xi", "active"], eng.export_particles(0), ref_parts):
+        _assert_same(a, b, f"{lists} {nm}")
+    s_, m_ = eng.export_grid(0)
+    _assert_same(s_[:8], scal[:8], f"{lists} grid_scalar_fields")
+    _assert_same(m_, mat, f"{lists} grid_mat_prop")
+    nc_got, wr_got = eng.counters(0)
+    _assert_same(nc_got, no_cols, f"{lists} no_cols")
+    _assert_same(wr_got, wr, f"{lists} water_removed")
+    assert no_cols.sum() > 0
+
+
 @pytest.mark.parametrize("variant", ["NaCl", "stress150", "one_collision_step", "four_collision_steps_no_relaxation"])
 def test_adv_steps_configurations(gpu, variant):
     """Two advection steps, bit-exact vs the oracle, for the other configurations the step supports: NaCl
