@@ -628,6 +628,24 @@ def roofline(ktimes, n_sips, default_workload):
     return roof
 
 
+def roofline_collision(ktimes, n_sips, default_workload):
+    """The collision step's main pass in the early state (where its counters were taken): algorithmic bytes of
+    one launch (64 B per SIP) over its HIP-event duration against HBM, and its VALU issue rate from the
+    committed counter summary (the pass is bound by instruction issue)."""
+    if not ktimes or "collision" not in ktimes:
+        return None
+    ms = ktimes["collision"][0]
+    achieved = n_sips * B_COLLISION / (ms * 1e-3) / 1e9
+    roof = {"leg": "early", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_collision") if default_workload else None,
+            "kernel": "k_collision<160>", "bytes_per_launch": n_sips * B_COLLISION, "launch_ms": ms}
+    valu = profile_value(VALU_PROFILE, "VALU", "k_collision") if default_workload else None
+    if valu:
+        roof["valu"] = {"achieved": valu / (ms * 1e-3), "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
+                        "frac": valu / (ms * 1e-3) / VALU_ISSUE_PEAK}
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -875,6 +893,8 @@ def main():
                    if (head and plain) else None,
                    **out},
         "roofline": roofline(kt, n_head, default_workload) if kt else None,
+        "roofline_collision": roofline_collision(out["early"]["kernel_ms"], out["early"]["sips"], default_workload)
+        if "early" in out else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -656,7 +656,9 @@ __device__ static int fast_members(const KView &v, int gc, int *sSL, int cap, in
 /* ------------------------------------------------------------------------ */
 /* per-cell constants                                                       */
 /* ------------------------------------------------------------------------ */
-__device__ static void fill_cc(const LmPhys &P, const double *scal_r, const double *mat_r, int c, int nc, double *C) {
+/* sigma_w: the water surface tension at the cell's T when the caller has it (the same lmp_sigma_water), else NaN */
+__device__ static void fill_cc(const LmPhys &P, const double *scal_r, const double *mat_r, int c, int nc, double *C,
+                               double sigma_w = NAN) {
   const double T = scal_r[0 * nc + c], p = scal_r[1 * nc + c];
   const double K = mat_r[0 * nc + c], D_v = mat_r[1 * nc + c], L_v = mat_r[2 * nc + c];
   const double e_s = scal_r[7 * nc + c];
@@ -674,7 +676,7 @@ __device__ static void fill_cc(const LmPhys &P, const double *scal_r, const doub
   C[CC_C1] = L_v * L_v / (P.R_v * K * T * T);  /* microphysics.py:1458 */
   C[CC_C2] = P.R_v * T / (D_v * e_s);          /* microphysics.py:1459 */
   C[CC_RHOW] = lmp_density_water(P, T);
-  C[CC_SIGW] = lmp_sigma_water(T);
+  C[CC_SIGW] = sigma_w == sigma_w ? sigma_w : lmp_sigma_water(T);
 }
 
 __global__ void k_init_cc(KView v) {
@@ -2918,11 +2920,12 @@ __global__ __launch_bounds__(256) void k_grid(KView v) {
   M[0 * nc + c] = 4.1868E-3 * (5.69 + 0.017 * (T - 273.15));
   M[1 * nc + c] = 4.01218E-5 * lm_pow(T, 1.94) / p;
   M[2 * nc + c] = 1.0E3 * (2500.82 - 2.358 * (T - 273.0));
-  M[3 * nc + c] = lmp_sigma_water(T);
+  const double sigma_w = lmp_sigma_water(T);
+  M[3 * nc + c] = sigma_w;
   M[4 * nc + c] = P.c_pd * (1 + 0.897 * r_v);
   M[5 * nc + c] = 1.458E-6 * lm_pow(T, 1.5) / (T + 110.4);
   M[6 * nc + c] = rho_d * (1 + r_v);
-  fill_cc(P, S, M, c, nc, v.cc + (size_t)gc * CC_N);
+  fill_cc(P, S, M, c, nc, v.cc + (size_t)gc * CC_N, sigma_w);
 }
 
 /* ------------------------------------------------------------------------ */
