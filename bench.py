@@ -529,7 +529,7 @@ def host_info():
 
 
 def cpu_baseline(seed0, spc, workers, nx=2):
-    """BASELINE protocol for the CPU path: single core (one process alone), then all cores of the share (one
+    """CPU timing protocol: single core (one process alone), then all cores of the share (one
     independent run per core, started together, the reference's one process per run, run_lacmo.sh:7-16);
     each process 5 warm-up + 3 x 60 timed advection steps, median of the 3; plus one step of the full 75x75
     case on one core."""
