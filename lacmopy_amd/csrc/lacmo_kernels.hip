@@ -3294,6 +3294,9 @@ __device__ static inline void copy_sip(const KView &v, int src, int dst, int cel
   v.cell2[dst] = cellv;
 }
 
+/* CAP: the LDS member capacity (larger cells merged by one lane); 256 keeps the kernel at eight waves per SIMD,
+   512 at about five */
+template <int CAP>
 __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
   const int nc = v.nc, L = 2 * nc + 1;
   const int lane = threadIdx.x;
@@ -3317,19 +3320,19 @@ __global__ __launch_bounds__(WAVE) void k_ri_scatter(KView v) {
   const int gc = blockIdx.x, r = gc / nc, c = gc % nc;
   const int32_t *sr = v.seg + (size_t)r * (nc + 1);
   const int32_t *sc = v.scan_buf + (size_t)r * L;
-  __shared__ int sSlot[DML_CAP], sId[DML_CAP], sRank[DML_CAP];
-  __shared__ unsigned char sSub[DML_CAP];
+  __shared__ int sSlot[CAP], sId[CAP], sRank[CAP];
+  __shared__ unsigned char sSub[CAP];
   __shared__ int sMig[3 * LM_MIG_CAP_MAX];
   __shared__ NbrTab tb;
   const int dst0 = sc[c];
   /* a complete migrant list: the members in id order from the home segment and the list (no 3x3 gather) */
-  const int nf = v.mig_cnt ? fast_members(v, gc, sRank, DML_CAP, sSlot, sId, sMig) : -1;
-  if (nf >= 0 && nf <= DML_CAP) {
+  const int nf = v.mig_cnt ? fast_members(v, gc, sRank, CAP, sSlot, sId, sMig) : -1;
+  if (nf >= 0 && nf <= CAP) {
     for (int m = lane; m < nf; m += WAVE) copy_sip(v, sRank[m], dst0 + m, gc);
   } else {
     nbr_table(gc, nc, v.nx, v.nz, sr, v.away + (size_t)r * nc, &tb);
-    const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, DML_CAP);
-    if (n > DML_CAP) {
+    const int n = gather_members(gc, v.cell, v.id, &tb, sSlot, sId, sSub, CAP);
+    if (n > CAP) {
       /* over the LDS capacity: one lane merges the members in id order, parking each source slot in the cell
          array of its destination (this cell's range of the new buffer), then the wave copies */
       if (lane == 0) {
@@ -4431,7 +4434,12 @@ int lm_reindex(LmEngine *e, void *stream) {
   hipLaunchKernelGGL(k_scan_add, dim3(nchunk), dim3(SCAN_BLOCK), 0, S(stream), e->scan_buf, e->scan_part, L);
   {
     ProfScope ps(e, 3, S(stream));
-    hipLaunchKernelGGL(k_ri_scatter, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
+    /* the 256-member build while cells average about 128 members (its occupancy: 1.66 against 1.92 ms for 46 M
+       SIPs in 360,000 cells); the 512-member build for denser grids, where the one-lane merge would take many */
+    if (cell_capacity(e) <= 192)
+      hipLaunchKernelGGL(k_ri_scatter<256>, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
+    else
+      hipLaunchKernelGGL(k_ri_scatter<DML_CAP>, dim3(R * nc + R), dim3(WAVE), 0, S(stream), v);
   }
   /* every SIP is in its home cell again (a fill kernel, not a memset: the step stays graph-capturable) */
   hipLaunchKernelGGL(k_fill_i32, dim3((R * nc + 255) / 256), dim3(256), 0, S(stream), e->away, 0, R * nc);
